@@ -1,0 +1,71 @@
+"""Build the ccfj native parts in-tree (no pip install, no JIT cache).
+
+    python cc-fjpy_amd/build.py
+
+Produces, next to the Python package so they travel with the source tree:
+  ccfj/libccfj.so          HIP kernels + C ABI (include/ccfj.h), gfx950
+  ccfj/_ccfj.*.so          Cython shim linked against libccfj.so
+  ccfj/libccfj_emul.so     host build of the kernel arithmetic (CPU tests)
+"""
+import os
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+PKG = os.path.join(HERE, "ccfj")
+CSRC = os.path.join(HERE, "csrc")
+INCLUDE = os.path.join(ROOT, "include")
+
+ARCH = os.environ.get("CCFJ_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# packed FP32 instructions are kept out of generated code
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HIP_SOURCES = ["fj_kernels.hip", "ccfj_capi.hip", "fj_plain64.hip"]
+
+
+def _run(cmd, **kw):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, **kw)
+
+
+def build_hip():
+    out = os.path.join(PKG, "libccfj.so")
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, "-o", out, *srcs])
+    return out
+
+
+def build_cython():
+    import numpy
+    pyx = os.path.join(PKG, "_ccfj.pyx")
+    cpp = os.path.join(PKG, "_ccfj.cpp")
+    _run([sys.executable, "-m", "cython", "-3", "--cplus", "-o", cpp, pyx])
+    ext = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    out = os.path.join(PKG, "_ccfj" + ext)
+    pyinc = sysconfig.get_paths()["include"]
+    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17",
+          "-DNPY_NO_DEPRECATED_API=NPY_1_7_API_VERSION",
+          "-I", pyinc, "-I", numpy.get_include(), "-I", INCLUDE,
+          "-o", out, cpp, "-L", PKG, "-lccfj", "-Wl,-rpath,$ORIGIN"])
+    os.remove(cpp)
+    return out
+
+
+def build_emul():
+    out = os.path.join(PKG, "libccfj_emul.so")
+    _run(["g++", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c++17", "-I", INCLUDE,
+          "-o", out, os.path.join(CSRC, "fj_host_emul.cpp")])
+    return out
+
+
+def build_all():
+    build_hip()
+    build_cython()
+    build_emul()
+
+
+if __name__ == "__main__":
+    build_all()

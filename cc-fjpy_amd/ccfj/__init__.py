@@ -1,0 +1,244 @@
+"""ccfj -- MI355X-native frequency-Bessel (F-J) transform, drop-in for CC-FJpy.
+
+The public functions keep the names, defaults, argument meaning, dtypes and
+output layout of the reference's Python API (src/ccfj-gpu.pyx:168-313, the
+GPU build; src/ccfj.pyx:148-268, the CPU build):
+
+    fj_noise(uf, r, c, f, fstride=1, itype=1, func=0, num=-1)   -> float32 [nc, nf]
+    fj_earthquake(u, r, c, f, fstride=1, itype=1, func=0, num=-1) -> float32 [nc, nf]
+    MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, fstride=1,
+         itype=1, func=0, num=-1)                                -> float64 [nwin, nc, nf]
+    fj, fh, fhr, fhi (un-normalised single-basis transforms), win,
+    GetStationPairs
+
+`num` selects the GPU (-1 = the current device); the reference's GPU build
+did this through CUDA_VISIBLE_DEVICES (src/ccfj-gpu.pyx:247-248).
+
+All F-J arithmetic runs in the HIP kernels of libccfj.so; this module only
+sorts, casts, slices and combines as the reference's Python layer does.
+There is no CPU fallback: importing the package without its native parts,
+or calling it without a GPU, raises.
+"""
+import os
+import warnings
+
+import numpy as np
+
+try:
+    from . import _ccfj
+except ImportError as exc:  # the native extension is the product; no fallback
+    raise ImportError(
+        "ccfj: native extension not built (run `python cc-fjpy_amd/build.py`): %s" % exc
+    ) from exc
+
+__all__ = ["fj_noise", "fj_earthquake", "MWFJ", "fj", "fh", "fhr", "fhi", "win",
+           "GetStationPairs", "hilbert_rows", "device_count"]
+
+MODE_J = _ccfj.MODE_J
+MODE_Y = _ccfj.MODE_Y
+MODE_J_MINUS_Y = _ccfj.MODE_J_MINUS_Y
+MODE_ABS_J = _ccfj.MODE_ABS_J
+MODE_ABS_H = _ccfj.MODE_ABS_H
+
+_contexts = {}
+
+
+def device_count():
+    return _ccfj.device_count()
+
+
+def _context(num=-1):
+    """Per-device context (stream + cached device workspace)."""
+    dev = -1 if num is None or int(num) < 0 else int(num)
+    ctx = _contexts.get(dev)
+    if ctx is None:
+        ctx = _ccfj.Context(dev)
+        _contexts[dev] = ctx
+    return ctx
+
+
+def _as_f32(a):
+    # the reference casts with np.array(.., dtype=np.float32) only when the
+    # dtype differs (src/ccfj-gpu.pyx:188-195); complex input keeps its real
+    # part and numpy emits ComplexWarning, exactly as there
+    if a.dtype != np.float32:
+        a = np.array(a, dtype=np.float32)
+    return np.ascontiguousarray(a)
+
+
+def _stage(uf, r, c, f, fstride):
+    """Staging of the reference's fj/fh (src/ccfj-gpu.pyx:181-197)."""
+    nr = len(r)
+    nf = len(f)
+    uf = uf[:, ::fstride]
+    uf = uf.reshape(nr * nf)
+    r = _as_f32(np.asarray(r))
+    f = _as_f32(np.asarray(f))
+    uf = _as_f32(uf).reshape(nr, nf)
+    c = _as_f32(np.asarray(c))
+    return uf, r, c, f
+
+
+def _native(mode, itype, normalize, U0, U1, r, c, f, num):
+    nr, nf = len(r), len(f)
+    if itype not in (0, 1):
+        # the reference's native code silently does nothing for other
+        # itype values (src/FJcpu.cpp:403-407); the output stays zero
+        out = np.zeros((len(c), nf), np.float32)
+        if normalize:
+            with np.errstate(invalid="ignore", divide="ignore"):
+                _normalize_columns(out)
+        return out
+    ctx = _context(num)
+    return ctx.transform(mode, int(itype), bool(normalize), U0.reshape(nr, nf),
+                         None if U1 is None else U1.reshape(nr, nf), r, f, c)
+
+
+def _normalize_columns(out):
+    # src/ccfj.pyx:225-226; Python's builtin max over the column
+    for i in range(out.shape[1]):
+        out[:, i] = out[:, i] / max(np.abs(out[:, i]))
+
+
+def fj(uf, r, c, f, fstride=1, itype=1, num=-1):
+    """J0-basis transform of uf (no normalisation), float32 [nc, nf]."""
+    U, r, c, f = _stage(uf, r, c, f, fstride)
+    return _native(MODE_J, itype, False, U, None, r, c, f, num)
+
+
+def fhr(uf, r, c, f, fstride=1, itype=1, num=-1):
+    """J0-basis transform (the GPU build's FP64 "real Hankel" part)."""
+    return fj(uf, r, c, f, fstride, itype, num)
+
+
+def fhi(uf, r, c, f, fstride=1, itype=1, num=-1):
+    """Y0-basis transform of uf (no normalisation), float32 [nc, nf]."""
+    U, r, c, f = _stage(uf, r, c, f, fstride)
+    return _native(MODE_Y, itype, False, U, None, r, c, f, num)
+
+
+def fh(uf, r, c, f, fstride=1, itype=1, nThread=20, num=-1):
+    """Y0-basis transform under the CPU build's name (src/ccfj.pyx:181-198)."""
+    return fhi(uf, r, c, f, fstride, itype, num)
+
+
+def hilbert_rows(x):
+    """Row-wise Hilbert transform with scipy.fftpack's convention, as the
+    reference applies it along frequency (src/ccfj.pyx:217-219):
+    y_k = i sign(k) x_k in Fourier space, with the zero and (even length)
+    Nyquist coefficients set to zero, so hilbert(cos) = -sin.  A complex
+    row is transformed part by part; the float32 store then keeps
+    hilbert(Re x)."""
+    x = np.asarray(x)
+    if np.iscomplexobj(x):
+        x = x.real
+    n = x.shape[-1]
+    X = np.fft.fft(x.astype(np.float64, copy=False), axis=-1)
+    k = np.fft.fftfreq(n) * n
+    h = 1j * np.sign(k)
+    if n % 2 == 0:
+        h[n // 2] = 0.0
+    return np.fft.ifft(X * h, axis=-1).real.astype(np.float32)
+
+
+def fj_noise(uf, r, c, f, fstride=1, itype=1, func=0, num=-1):
+    """F-J of noise cross-correlation spectra (src/ccfj-gpu.pyx:241-268).
+
+    uf: [nr, ncol] spectra (only the real part is used), r: distances (m),
+    c: phase velocities (m/s), f: frequencies, ceil(ncol/fstride) == len(f).
+    func 0: Bessel basis; func 1: Hankel basis, FJ[uf] - FY[Hilbert_f(uf)].
+    Returns the per-frequency normalised float32 [nc, nf] spectrum.
+    """
+    indx = np.argsort(r)
+    r = r[indx]
+    uf = uf[indx]
+    if func == 0:
+        U, rr, cc, ff = _stage(uf, r, c, f, fstride)
+        return _native(MODE_J, itype, True, U, None, rr, cc, ff, num)
+    elif func == 1:
+        ufi = hilbert_rows(uf)
+        U, rr, cc, ff = _stage(uf, r, c, f, fstride)
+        Ui, _, _, _ = _stage(ufi, r, c, f, fstride)
+        return _native(MODE_J_MINUS_Y, itype, True, U, Ui, rr, cc, ff, num)
+    else:
+        print('set func as 0 for Bessel function 1 for Hankel function')
+        return 0
+
+
+def _earthquake_spectra(u, r, f, fstride):
+    indx = np.argsort(r)
+    r = r[indx]
+    u1 = u[indx]
+    uf = np.fft.rfft(u1)
+    uf = uf[:, 0:len(f) * fstride:fstride]
+    return np.real(uf), np.imag(uf), r
+
+
+def fj_earthquake(u, r, c, f, fstride=1, itype=1, func=0, num=-1):
+    """F-J of earthquake / active-source records (src/ccfj-gpu.pyx:270-295).
+
+    u: [nr, npts] time series; the rFFT along time is sliced to
+    0:len(f)*fstride:fstride and then staged like fj (which strides again,
+    as the reference does).  func 0: |FJ[Re] + i FJ[Im]|; func 1:
+    |FJ[Re] - FY[Im] + i (FJ[Im] + FY[Re])|.  Normalised float32 [nc, nf].
+    """
+    ufr, ufi, r = _earthquake_spectra(u, r, f, fstride)
+    if func not in (0, 1):
+        print('set func as 0 for Bessel function 1 for Hankel function')
+        return 0
+    Ur, rr, cc, ff = _stage(ufr, r, c, f, fstride)
+    Ui, _, _, _ = _stage(ufi, r, c, f, fstride)
+    mode = MODE_ABS_J if func == 0 else MODE_ABS_H
+    return _native(mode, itype, True, Ur, Ui, rr, cc, ff, num)
+
+
+def win(npts, Fs, T1, T2, taper=0.8):
+    """Tapered box window (src/ccfj.pyx:148-158): zero outside
+    [floor(T1 Fs), floor(T2 Fs)), sine ramps of floor((n2-n1)(1-taper)/2)
+    samples at both ends."""
+    n1 = max(0, np.floor(T1 * Fs))
+    n2 = min(npts, np.floor(T2 * Fs))
+    delta = np.floor((n2 - n1) * (1 - taper) / 2)
+    i1, i2, d = int(n1), int(n2), int(delta)
+    if 0 <= i1 <= i2 <= npts and 2 * d <= i2 - i1:
+        out = np.ones(npts)
+        out[0:i1] = 0
+        out[i2:int(npts)] = 0
+        if d > 0:
+            ramp = np.sin(np.pi / 2 * np.arange(d) / delta)
+            out[i1:i1 + d] = ramp
+            out[i2 - d:i2] = ramp[::-1]
+        return out
+    # degenerate windows (reversed, out of range, overlapping ramps): the
+    # element-by-element semantics of the reference, negative indices included
+    out = np.ones(npts)
+    out[0:i1] = 0
+    out[i2:int(npts)] = 0
+    for i in range(d):
+        out[int(n1 + i)] = np.sin(np.pi / 2 * i / delta)
+        out[int(n2 - i - 1)] = np.sin(np.pi / 2 * i / delta)
+    return out
+
+
+def MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, fstride=1, itype=1, func=0,
+         num=-1):
+    """Multi-window F-J (src/ccfj-gpu.pyx:297-313): for each window w the
+    traces are tapered by win(npts, Fs, winl[w,j], winr[w,j], taper) (window
+    edges in seconds, original station order) and passed to fj_earthquake.
+    Returns float64 [nwin, nc, nf]."""
+    nr = len(r)
+    npts = len(u[0])
+    out = np.zeros([nwin, len(c), len(f)])
+    for i in range(0, nwin):
+        u0 = np.zeros([nr, npts])
+        for j in range(nr):
+            u0[j, :] = u[j, :] * win(npts, Fs, winl[i, j], winr[i, j], taper)
+        out[i, :, :] = fj_earthquake(u0, r, c, f, fstride=fstride, itype=itype,
+                                     func=func, num=num)
+    return out
+
+
+def GetStationPairs(nsta):
+    """Station pair list i<j, flattened (src/ccfj.pyx:116-123)."""
+    pairs = [(ii, jj) for ii in range(nsta) for jj in range(ii + 1, nsta)]
+    return np.array(pairs, dtype=np.int32).reshape(-1)

@@ -1,0 +1,179 @@
+# cython: language_level=3, boundscheck=False, wraparound=False
+"""Cython shim between the ccfj Python API and libccfj.so (include/ccfj.h).
+
+Mirrors the reference's typed-memoryview shims (src/ccfj-gpu.pyx:75-135
+fj_full / fHr_full / fHi_full, src/ccfj.pyx:95-113 fH_full): float32
+buffers are passed as raw pointers to the C ABI.  Unlike the reference the
+GIL is released during the native call and a non-zero status raises
+RuntimeError carrying ccfj_last_error().
+"""
+import numpy as np
+cimport numpy as cnp
+
+cdef extern from "ccfj.h" nogil:
+    ctypedef struct ccfj_ctx:
+        pass
+    int CCFJ_MODE_J
+    int CCFJ_MODE_Y
+    int CCFJ_MODE_J_MINUS_Y
+    int CCFJ_MODE_ABS_J
+    int CCFJ_MODE_ABS_H
+    int FJ(float *u_f, float *r, float *f, float *out, float *c, int nc, int nr, int nf, int type)
+    int FHr(float *u_f, float *r, float *f, float *out, float *c, int nc, int nr, int nf, int type)
+    int FHi(float *u_f, float *r, float *f, float *out, float *c, int nc, int nr, int nf, int type)
+    int FH(float *u_f, float *r, float *f, float *out, float *c, int nc, int nr, int nf, int type, int nThread)
+    int ccfj_create(int device, ccfj_ctx **ctx)
+    int ccfj_destroy(ccfj_ctx *ctx)
+    const char *ccfj_last_error()
+    int ccfj_device_count()
+    int ccfj_transform(ccfj_ctx *ctx, int mode, int itype, int normalize,
+                       const float *U0, const float *U1, const float *r,
+                       const float *f, const float *c, int nr, int nf, int nc,
+                       float *out)
+    int ccfj_transform_dev(ccfj_ctx *ctx, int mode, int itype, int normalize,
+                           const float *U0, const float *U1, const float *r,
+                           const float *f, const float *c, int nr, int nf, int nc,
+                           float *out, void *stream)
+    int ccfj_synchronize(ccfj_ctx *ctx)
+    int ccfj_set_timing(ccfj_ctx *ctx, int enable)
+    int ccfj_get_timing(ccfj_ctx *ctx, double *kernel_ms, int *launches)
+    int ccfj_plain_fp64(int device, int ybasis, int itype, const float *U,
+                        const float *r, const float *f, const float *c, int nr,
+                        int nf, int nc, float *out, double *kernel_ms)
+
+MODE_J = CCFJ_MODE_J
+MODE_Y = CCFJ_MODE_Y
+MODE_J_MINUS_Y = CCFJ_MODE_J_MINUS_Y
+MODE_ABS_J = CCFJ_MODE_ABS_J
+MODE_ABS_H = CCFJ_MODE_ABS_H
+
+
+cdef _check(int rc, what):
+    if rc != 0:
+        raise RuntimeError("%s failed (%d): %s" % (what, rc, ccfj_last_error().decode()))
+
+
+def device_count():
+    return ccfj_device_count()
+
+
+def fj_full(float[:] U_f, float[:] r, float[:] f, float[:] out, float[:] c,
+            int nc, int nr, int nf, int type):
+    """J0 basis, the reference's fj_full (src/ccfj-gpu.pyx:75-92)."""
+    cdef int rc
+    with nogil:
+        rc = FJ(&U_f[0], &r[0], &f[0], &out[0], &c[0], nc, nr, nf, type)
+    _check(rc, "FJ")
+
+
+def fHr_full(float[:] U_f, float[:] r, float[:] f, float[:] out, float[:] c,
+             int nc, int nr, int nf, int type):
+    """J0 basis, the reference's fHr_full (src/ccfj-gpu.pyx:97-114)."""
+    cdef int rc
+    with nogil:
+        rc = FHr(&U_f[0], &r[0], &f[0], &out[0], &c[0], nc, nr, nf, type)
+    _check(rc, "FHr")
+
+
+def fHi_full(float[:] U_f, float[:] r, float[:] f, float[:] out, float[:] c,
+             int nc, int nr, int nf, int type):
+    """Y0 basis, the reference's fHi_full (src/ccfj-gpu.pyx:118-135)."""
+    cdef int rc
+    with nogil:
+        rc = FHi(&U_f[0], &r[0], &f[0], &out[0], &c[0], nc, nr, nf, type)
+    _check(rc, "FHi")
+
+
+def fH_full(float[:] U_f, float[:] r, float[:] f, float[:] out, float[:] c,
+            int nc, int nr, int nf, int type, int nThread=20):
+    """Y0 basis under the CPU build's name (src/ccfj.pyx:95-113)."""
+    cdef int rc
+    with nogil:
+        rc = FH(&U_f[0], &r[0], &f[0], &out[0], &c[0], nc, nr, nf, type, nThread)
+    _check(rc, "FH")
+
+
+def plain_fp64(int device, int ybasis, int itype, float[:, ::1] U, float[:] r,
+               float[:] f, float[:] c):
+    """Plain FP64 GPU F-J (speed-up baseline).  Returns (out[nc,nf], kernel_ms)."""
+    cdef int nr = U.shape[0], nf = U.shape[1], nc = c.shape[0]
+    out = np.empty((nc, nf), np.float32)
+    cdef float[:, ::1] o = out
+    cdef double ms = 0
+    cdef int rc
+    with nogil:
+        rc = ccfj_plain_fp64(device, ybasis, itype, &U[0, 0], &r[0], &f[0], &c[0],
+                             nr, nf, nc, &o[0, 0], &ms)
+    _check(rc, "ccfj_plain_fp64")
+    return out, ms
+
+
+cdef class Context:
+    """One device, one HIP stream, a grow-only device workspace."""
+    cdef ccfj_ctx *ctx
+    cdef readonly int device
+
+    def __cinit__(self, int device=-1):
+        self.ctx = NULL
+        _check(ccfj_create(device, &self.ctx), "ccfj_create")
+        self.device = device
+
+    def __dealloc__(self):
+        if self.ctx != NULL:
+            ccfj_destroy(self.ctx)
+            self.ctx = NULL
+
+    def transform(self, int mode, int itype, bint normalize, U0, U1, r, f, c):
+        """Host arrays in, float32 [nc, nf] out.  U0/U1: C-contiguous float32
+        [nr, nf]; r, f, c: float32 vectors."""
+        cdef float[:, ::1] u0 = U0
+        cdef float[:, ::1] u1 = U1 if U1 is not None else U0
+        cdef float[::1] rv = r
+        cdef float[::1] fv = f
+        cdef float[::1] cv = c
+        cdef int nr = u0.shape[0], nf = u0.shape[1], nc = cv.shape[0]
+        if rv.shape[0] != nr or fv.shape[0] != nf:
+            raise ValueError("shape mismatch: U %s, r %d, f %d" % ((nr, nf), rv.shape[0], fv.shape[0]))
+        if u1.shape[0] != nr or u1.shape[1] != nf:
+            raise ValueError("U1 shape differs from U0")
+        out = np.zeros((nc, nf), np.float32)
+        cdef float[:, ::1] o = out
+        cdef const float *p0 = &u0[0, 0] if nr > 0 and nf > 0 else NULL
+        cdef const float *p1 = &u1[0, 0] if nr > 0 and nf > 0 else NULL
+        cdef const float *pr = &rv[0] if nr > 0 else NULL
+        cdef const float *pf = &fv[0] if nf > 0 else NULL
+        cdef const float *pc = &cv[0] if nc > 0 else NULL
+        cdef float *po = &o[0, 0] if nc > 0 and nf > 0 else NULL
+        cdef int rc
+        with nogil:
+            rc = ccfj_transform(self.ctx, mode, itype, normalize, p0, p1, pr, pf, pc,
+                                nr, nf, nc, po)
+        _check(rc, "ccfj_transform")
+        return out
+
+    def transform_dev(self, int mode, int itype, bint normalize, size_t U0, size_t U1,
+                      size_t r, size_t f, size_t c, int nr, int nf, int nc, size_t out,
+                      size_t stream=0):
+        """Device pointers (integers), enqueued on `stream` (0 = own stream)."""
+        cdef int rc
+        with nogil:
+            rc = ccfj_transform_dev(self.ctx, mode, itype, normalize, <const float *>U0,
+                                    <const float *>U1, <const float *>r, <const float *>f,
+                                    <const float *>c, nr, nf, nc, <float *>out, <void *>stream)
+        _check(rc, "ccfj_transform_dev")
+
+    def synchronize(self):
+        cdef int rc
+        with nogil:
+            rc = ccfj_synchronize(self.ctx)
+        _check(rc, "ccfj_synchronize")
+
+    def set_timing(self, bint enable):
+        _check(ccfj_set_timing(self.ctx, enable), "ccfj_set_timing")
+
+    def get_timing(self):
+        """(summed main-kernel ms, launches) since the last call."""
+        cdef double ms = 0
+        cdef int n = 0
+        _check(ccfj_get_timing(self.ctx, &ms, &n), "ccfj_get_timing")
+        return ms, n

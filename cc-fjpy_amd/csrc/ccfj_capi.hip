@@ -1,0 +1,362 @@
+// ccfj_capi.hip -- host side of libccfj.so: contexts, workspace, the C ABI.
+//
+// Boundary (reference): the Cython layer hands float32 host buffers to
+// FJ/FH (CPU build, include/FJcpu.h:19-20) or FJ/FHr/FHi (GPU build,
+// include/FJgpu.hh:13-15).  The reference GPU host code allocates, copies,
+// launches and frees on every call (src/FJgpu.cu:390-419).  Here a context
+// keeps a stream and grow-only device buffers; the reference-compatible
+// entry points use one default context per device.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ccfj.h"
+#include "fj_kernels.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(e == hipErrorOutOfMemory ? CCFJ_ENOMEM : CCFJ_EDEVICE, "%s: %s (%d)",
+              what, hipGetErrorString(e), (int)e);
+}
+
+// Fault injection for tests: CCFJ_FAIL_ALLOC_AT=N makes the N-th device
+// allocation of the process (1-based) fail with CCFJ_ENOMEM.
+std::atomic<long> g_alloc_count{0};
+bool injected_alloc_failure() {
+  static long fail_at = [] {
+    const char* s = getenv("CCFJ_FAIL_ALLOC_AT");
+    return s ? atol(s) : 0L;
+  }();
+  long n = ++g_alloc_count;
+  return fail_at > 0 && n == fail_at;
+}
+
+enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_COEF, S_COUNT };
+
+}  // namespace
+
+struct ccfj_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  void* buf[S_COUNT] = {};
+  size_t cap[S_COUNT] = {};
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;  // recorded, unread
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> spare;
+  double timed_ms = 0.0;
+  int timed_launches = 0;
+};
+
+namespace {
+
+int ensure(ccfj_ctx* ctx, Slot s, size_t bytes) {
+  if (bytes == 0) bytes = 1;
+  if (ctx->cap[s] >= bytes) return CCFJ_OK;
+  if (ctx->buf[s]) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    hipFree(ctx->buf[s]);
+    ctx->buf[s] = nullptr;
+    ctx->cap[s] = 0;
+  }
+  if (injected_alloc_failure())
+    return fail(CCFJ_ENOMEM, "device allocation of %zu bytes failed (injected)", bytes);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  ctx->buf[s] = p;
+  ctx->cap[s] = bytes;
+  return CCFJ_OK;
+}
+
+bool two_spectra(int mode) { return mode == CCFJ_MODE_J_MINUS_Y || mode == CCFJ_MODE_ABS_J || mode == CCFJ_MODE_ABS_H; }
+
+int check_args(int mode, int itype, const void* U0, const void* U1, const void* r,
+               const void* f, const void* c, int nr, int nf, int nc, const void* out) {
+  if (mode < CCFJ_MODE_J || mode > CCFJ_MODE_ABS_H) return fail(CCFJ_EINVAL, "unknown mode %d", mode);
+  if (itype != 0 && itype != 1) return fail(CCFJ_EINVAL, "itype must be 0 or 1, got %d", itype);
+  if (nr < 0 || nf < 0 || nc < 0) return fail(CCFJ_EINVAL, "negative size (nr=%d nf=%d nc=%d)", nr, nf, nc);
+  if ((long long)nf * ((nc + 255) / 256) > (1LL << 31) - 8) return fail(CCFJ_EINVAL, "grid too large");
+  if (nr > 0 && nf > 0 && (!U0 || !r || !f)) return fail(CCFJ_EINVAL, "null input pointer");
+  if (nr > 0 && nf > 0 && two_spectra(mode) && !U1) return fail(CCFJ_EINVAL, "mode %d needs U1", mode);
+  if (nf > 0 && nc > 0 && (!c || !out)) return fail(CCFJ_EINVAL, "null c/out pointer");
+  return CCFJ_OK;
+}
+
+int set_device(ccfj_ctx* ctx) {
+  hipError_t e = hipSetDevice(ctx->device);
+  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "hipSetDevice");
+}
+
+// Enqueue prep + main kernel (+ normalisation) on st.  Inputs are device
+// pointers; workspace comes from ctx.
+int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U0,
+               const float* U1, const float* r, const float* f, const float* c, int nr,
+               int nf, int nc, float* out, hipStream_t st) {
+  if (nf == 0 || nc == 0) return CCFJ_OK;
+  hipError_t e;
+  if (nr == 0) {
+    e = hipMemsetAsync(out, 0, sizeof(float) * (size_t)nf * nc, st);
+    if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+  } else {
+    const int NC = fjk::coef_per_station(mode, itype);
+    int rc;
+    if ((rc = ensure(ctx, S_RR, sizeof(double) * nr))) return rc;
+    if ((rc = ensure(ctx, S_HH, sizeof(float) * nr))) return rc;
+    if ((rc = ensure(ctx, S_COEF, sizeof(float) * (size_t)nf * nr * NC))) return rc;
+    fjk::Problem p;
+    p.U0 = U0;
+    p.U1 = two_spectra(mode) ? U1 : U0;
+    p.r = r;
+    p.f = f;
+    p.c = c;
+    p.nr = nr;
+    p.nf = nf;
+    p.nc = nc;
+    p.rr = (double*)ctx->buf[S_RR];
+    p.hh = (float*)ctx->buf[S_HH];
+    p.coef = (float*)ctx->buf[S_COEF];
+    p.out = out;
+    e = fjk::launch_prep(mode, itype, p, st);
+    if (e != hipSuccess) return hip_fail(e, "prep kernel launch");
+    std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+    if (ctx->timing) {
+      if (!ctx->spare.empty()) {
+        ev = ctx->spare.back();
+        ctx->spare.pop_back();
+      } else {
+        if ((e = hipEventCreate(&ev.first)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+        if ((e = hipEventCreate(&ev.second)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+      }
+      hipEventRecord(ev.first, st);
+    }
+    e = fjk::launch_fj(mode, itype, p, st);
+    if (e != hipSuccess) return hip_fail(e, "F-J kernel launch");
+    if (ctx->timing) {
+      hipEventRecord(ev.second, st);
+      ctx->events.push_back(ev);
+    }
+  }
+  if (normalize) {
+    e = fjk::launch_normalize(out, nf, nc, st);
+    if (e != hipSuccess) return hip_fail(e, "normalize kernel launch");
+  }
+  return CCFJ_OK;
+}
+
+std::mutex g_default_mu;
+std::vector<ccfj_ctx*> g_default;  // per device
+
+int default_ctx(ccfj_ctx** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  if ((int)g_default.size() <= dev) g_default.resize(dev + 1, nullptr);
+  if (!g_default[dev]) {
+    int rc = ccfj_create(dev, &g_default[dev]);
+    if (rc) return rc;
+  }
+  *out = g_default[dev];
+  return CCFJ_OK;
+}
+
+int ref_entry(int mode, float* u_f, float* r, float* f, float* out, float* c, int nc,
+              int nr, int nf, int type) {
+  ccfj_ctx* ctx;
+  int rc = default_ctx(&ctx);
+  if (rc) return rc;
+  std::lock_guard<std::mutex> lk(g_default_mu);
+  return ccfj_transform(ctx, mode, type, 0, u_f, nullptr, r, f, c, nr, nf, nc, out);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ccfj_last_error(void) { return g_last_error.c_str(); }
+
+int ccfj_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+int ccfj_create(int device, ccfj_ctx** out) {
+  if (!out) return fail(CCFJ_EINVAL, "null ctx pointer");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0) return fail(CCFJ_EDEVICE, "no HIP GPU available");
+  if (device < 0) {
+    e = hipGetDevice(&device);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  }
+  if (device >= n) return fail(CCFJ_EINVAL, "device %d out of range (%d visible)", device, n);
+  ccfj_ctx* ctx = new ccfj_ctx();
+  ctx->device = device;
+  int rc = set_device(ctx);
+  if (rc) {
+    delete ctx;
+    return rc;
+  }
+  e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return hip_fail(e, "hipStreamCreate");
+  }
+  *out = ctx;
+  return CCFJ_OK;
+}
+
+int ccfj_destroy(ccfj_ctx* ctx) {
+  if (!ctx) return CCFJ_OK;
+  set_device(ctx);
+  hipStreamSynchronize(ctx->stream);
+  for (int s = 0; s < S_COUNT; s++)
+    if (ctx->buf[s]) hipFree(ctx->buf[s]);
+  for (auto& ev : ctx->events) {
+    hipEventDestroy(ev.first);
+    hipEventDestroy(ev.second);
+  }
+  for (auto& ev : ctx->spare) {
+    hipEventDestroy(ev.first);
+    hipEventDestroy(ev.second);
+  }
+  hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CCFJ_OK;
+}
+
+int ccfj_transform_dev(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U0,
+                       const float* U1, const float* r, const float* f, const float* c,
+                       int nr, int nf, int nc, float* out, void* stream) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  int rc = check_args(mode, itype, U0, U1, r, f, c, nr, nf, nc, out);
+  if (rc) return rc;
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  return run_device(ctx, mode, itype, normalize, U0, U1, r, f, c, nr, nf, nc, out, st);
+}
+
+int ccfj_transform(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U0,
+                   const float* U1, const float* r, const float* f, const float* c, int nr,
+                   int nf, int nc, float* out) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  int rc = check_args(mode, itype, U0, U1, r, f, c, nr, nf, nc, out);
+  if (rc) return rc;
+  if (nf == 0 || nc == 0) return CCFJ_OK;
+  if ((rc = set_device(ctx))) return rc;
+  const size_t nu = (size_t)nr * nf;
+  const bool two = two_spectra(mode);
+  if ((rc = ensure(ctx, S_U0, sizeof(float) * nu))) return rc;
+  if (two && (rc = ensure(ctx, S_U1, sizeof(float) * nu))) return rc;
+  if ((rc = ensure(ctx, S_R, sizeof(float) * nr))) return rc;
+  if ((rc = ensure(ctx, S_F, sizeof(float) * nf))) return rc;
+  if ((rc = ensure(ctx, S_C, sizeof(float) * nc))) return rc;
+  if ((rc = ensure(ctx, S_OUT, sizeof(float) * (size_t)nf * nc))) return rc;
+  hipStream_t st = ctx->stream;
+  hipError_t e = hipSuccess;
+  if (nu) {
+    e = hipMemcpyAsync(ctx->buf[S_U0], U0, sizeof(float) * nu, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && two)
+      e = hipMemcpyAsync(ctx->buf[S_U1], U1, sizeof(float) * nu, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(ctx->buf[S_R], r, sizeof(float) * nr, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(ctx->buf[S_F], f, sizeof(float) * nf, hipMemcpyHostToDevice, st);
+  }
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(ctx->buf[S_C], c, sizeof(float) * nc, hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
+  rc = run_device(ctx, mode, itype, normalize, (const float*)ctx->buf[S_U0],
+                  (const float*)ctx->buf[S_U1], (const float*)ctx->buf[S_R],
+                  (const float*)ctx->buf[S_F], (const float*)ctx->buf[S_C], nr, nf, nc,
+                  (float*)ctx->buf[S_OUT], st);
+  if (rc) return rc;
+  e = hipMemcpyAsync(out, ctx->buf[S_OUT], sizeof(float) * (size_t)nf * nc,
+                     hipMemcpyDeviceToHost, st);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync D2H");
+  e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  return CCFJ_OK;
+}
+
+int ccfj_synchronize(ccfj_ctx* ctx) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+int ccfj_set_timing(ccfj_ctx* ctx, int enable) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  ctx->timing = enable != 0;
+  return CCFJ_OK;
+}
+
+int ccfj_get_timing(ccfj_ctx* ctx, double* kernel_ms, int* launches) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  for (auto& ev : ctx->events) {
+    hipError_t e = hipEventSynchronize(ev.second);
+    if (e != hipSuccess) return hip_fail(e, "hipEventSynchronize");
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, ev.first, ev.second);
+    if (e != hipSuccess) return hip_fail(e, "hipEventElapsedTime");
+    ctx->timed_ms += ms;
+    ctx->timed_launches++;
+    ctx->spare.push_back(ev);
+  }
+  ctx->events.clear();
+  if (kernel_ms) *kernel_ms = ctx->timed_ms;
+  if (launches) *launches = ctx->timed_launches;
+  ctx->timed_ms = 0.0;
+  ctx->timed_launches = 0;
+  return CCFJ_OK;
+}
+
+int FJ(float* u_f, float* r, float* f, float* out, float* c, int nc, int nr, int nf,
+       int type) {
+  return ref_entry(CCFJ_MODE_J, u_f, r, f, out, c, nc, nr, nf, type);
+}
+
+int FHr(float* u_f, float* r, float* f, float* out, float* c, int nc, int nr, int nf,
+        int type) {
+  return ref_entry(CCFJ_MODE_J, u_f, r, f, out, c, nc, nr, nf, type);
+}
+
+int FHi(float* u_f, float* r, float* f, float* out, float* c, int nc, int nr, int nf,
+        int type) {
+  return ref_entry(CCFJ_MODE_Y, u_f, r, f, out, c, nc, nr, nf, type);
+}
+
+int FH(float* u_f, float* r, float* f, float* out, float* c, int nc, int nr, int nf,
+       int type, int nThread) {
+  (void)nThread;
+  return ref_entry(CCFJ_MODE_Y, u_f, r, f, out, c, nc, nr, nf, type);
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+/* Generated by gen_fits.py -- do not edit by hand. */
+#pragma once
+#ifndef FJ_HD
+#define FJ_HD
+#endif
+
+/* large argument x >= 8, t = (8/x)^2, float */
+/* max abs error of the fit on (0,1]: 8.30e-11 */
+FJ_HD inline float fit_P0(float t) {
+  float p = 2.0872274757632412e-07f;
+  p = __builtin_fmaf(p, t, -2.072035746898442e-06f);
+  p = __builtin_fmaf(p, t, 2.7344267112284302e-05f);
+  p = __builtin_fmaf(p, t, -0.0010986284596301117f);
+  p = __builtin_fmaf(p, t, 0.9999999999157112f);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 4.50e-11 */
+FJ_HD inline float fit_q0(float t) {
+  float p = -9.308052403636231e-08f;
+  p = __builtin_fmaf(p, t, 7.612796799996811e-07f);
+  p = __builtin_fmaf(p, t, -6.910627331787113e-06f);
+  p = __builtin_fmaf(p, t, 0.00014304877194255163f);
+  p = __builtin_fmaf(p, t, -0.015624999954253075f);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 9.30e-11 */
+FJ_HD inline float fit_P1(float t) {
+  float p = -2.396020151809229e-07f;
+  p = __builtin_fmaf(p, t, 2.4560468976503757e-06f);
+  p = __builtin_fmaf(p, t, -3.516304122374271e-05f);
+  p = __builtin_fmaf(p, t, 0.001831049815192328f);
+  p = __builtin_fmaf(p, t, 1.000000000094397f);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 4.99e-11 */
+FJ_HD inline float fit_q1(float t) {
+  float p = 1.0533367480691048e-07f;
+  p = __builtin_fmaf(p, t, -8.813803538404306e-07f);
+  p = __builtin_fmaf(p, t, 8.448628806224969e-06f);
+  p = __builtin_fmaf(p, t, -0.00020026897267309463f);
+  p = __builtin_fmaf(p, t, 0.04687499994925341f);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 9.01e-10 */
+FJ_HD inline float fit_R0(float t) {
+  float p = -2.820982558380003e-05f;
+  p = __builtin_fmaf(p, t, 0.0001375252158385671f);
+  p = __builtin_fmaf(p, t, -0.0003148555691661709f);
+  p = __builtin_fmaf(p, t, 0.0005059846072811284f);
+  p = __builtin_fmaf(p, t, -0.0008291186708186722f);
+  p = __builtin_fmaf(p, t, 0.0021942019537319547f);
+  p = __builtin_fmaf(p, t, -0.015624870399185526f);
+  p = __builtin_fmaf(p, t, 0.9999999990698373f);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 5.10e-10 */
+FJ_HD inline float fit_R1(float t) {
+  float p = -5.35064546265832e-06f;
+  p = __builtin_fmaf(p, t, 2.603089300202931e-05f);
+  p = __builtin_fmaf(p, t, -6.712250189155052e-05f);
+  p = __builtin_fmaf(p, t, 0.00016461168687600765f);
+  p = __builtin_fmaf(p, t, -0.0007314297629187565f);
+  p = __builtin_fmaf(p, t, 0.015624944443527137f);
+  p = __builtin_fmaf(p, t, 1.0000000005252478f);
+  return p;
+}
+
+/* small argument x < 8, y = (x/8)^2, double */
+/* max abs error of the fit on (0,1]: 8.04e-11 */
+FJ_HD inline double fit_J0s(double y) {
+  double p = -0.005618801482900782;
+  p = __builtin_fma(p, y, 0.07079209695405728);
+  p = __builtin_fma(p, y, -0.5037809434408522);
+  p = __builtin_fma(p, y, 2.625220559644334);
+  p = __builtin_fma(p, y, -10.557357884793742);
+  p = __builtin_fma(p, y, 32.35918804582165);
+  p = __builtin_fma(p, y, -72.81660855664654);
+  p = __builtin_fma(p, y, 113.77757421102329);
+  p = __builtin_fma(p, y, -113.77775682787487);
+  p = __builtin_fma(p, y, 63.99999888465921);
+  p = __builtin_fma(p, y, -15.999999976722172);
+  p = __builtin_fma(p, y, 0.9999999999195268);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 1.21e-10 */
+FJ_HD inline double fit_J1s(double y) {
+  double p = 0.001933119992191311;
+  p = __builtin_fma(p, y, -0.022146255813530545);
+  p = __builtin_fma(p, y, 0.1417755820614683);
+  p = __builtin_fma(p, y, -0.6564575115820559);
+  p = __builtin_fma(p, y, 2.309572547363542);
+  p = __builtin_fma(p, y, -6.06743715646006);
+  p = __builtin_fma(p, y, 11.377627299279547);
+  p = __builtin_fma(p, y, -14.222203592242003);
+  p = __builtin_fma(p, y, 10.66666548155003);
+  p = __builtin_fma(p, y, -3.9999999705566402);
+  p = __builtin_fma(p, y, 0.4999999998789634);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 1.62e-10 */
+FJ_HD inline double fit_Y0s(double y) {
+  double p = 0.011007520614961696;
+  p = __builtin_fma(p, y, -0.13607675855273377);
+  p = __builtin_fma(p, y, 0.9417493466708342);
+  p = __builtin_fma(p, y, -4.732348709939299);
+  p = __builtin_fma(p, y, 18.20275922795894);
+  p = __builtin_fma(p, y, -52.85781250766169);
+  p = __builtin_fma(p, y, 111.22092650701482);
+  p = __builtin_fma(p, y, -159.2993419475141);
+  p = __builtin_fma(p, y, 141.1914151817568);
+  p = __builtin_fma(p, y, -65.8389707815695);
+  p = __builtin_fma(p, y, 11.366785032612231);
+  p = __builtin_fma(p, y, -0.07380429494619678);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 2.41e-10 */
+FJ_HD inline double fit_Y1s(double y) {
+  double p = -0.0037279760680778515;
+  p = __builtin_fma(p, y, 0.04184985392300427);
+  p = __builtin_fma(p, y, -0.2599904328974209);
+  p = __builtin_fma(p, y, 1.1572271941470782);
+  p = __builtin_fma(p, y, -3.8770978479644107);
+  p = __builtin_fma(p, y, 9.589115060130059);
+  p = __builtin_fma(p, y, -16.654007220047983);
+  p = __builtin_fma(p, y, 18.780663548716326);
+  p = __builtin_fma(p, y, -12.104928295564273);
+  p = __builtin_fma(p, y, 3.478315983625302);
+  p = __builtin_fma(p, y, -0.19605709040518532);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 5.06e-12 */
+FJ_HD inline double fit_GJs(double y) {
+  double p = 8.887397533409992e-05;
+  p = __builtin_fma(p, y, -0.0010796865524819253);
+  p = __builtin_fma(p, y, 0.007517728519888622);
+  p = __builtin_fma(p, y, -0.03868451112092575);
+  p = __builtin_fma(p, y, 0.1540241037174691);
+  p = __builtin_fma(p, y, -0.4667509709541983);
+  p = __builtin_fma(p, y, 1.034337151432889);
+  p = __builtin_fma(p, y, -1.5802461351138604);
+  p = __builtin_fma(p, y, 1.5238094742576516);
+  p = __builtin_fma(p, y, -0.7999999987682918);
+  p = __builtin_fma(p, y, 0.16666666666160115);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 1.55e-10 */
+FJ_HD inline double fit_GYs(double y) {
+  double p = -0.010493016794276316;
+  p = __builtin_fma(p, y, 0.12932686955425834);
+  p = __builtin_fma(p, y, -0.8908749111050289);
+  p = __builtin_fma(p, y, 4.447592307093985);
+  p = __builtin_fma(p, y, -16.958832589184595);
+  p = __builtin_fma(p, y, 48.66961558750545);
+  p = __builtin_fma(p, y, -100.72670076329833);
+  p = __builtin_fma(p, y, 140.7051499670384);
+  p = __builtin_fma(p, y, -119.54298053751586);
+  p = __builtin_fma(p, y, 51.0414296530347);
+  p = __builtin_fma(p, y, -6.446088190525072);
+  p = __builtin_fma(p, y, -0.6366197725231723);
+  return p;
+}

@@ -1,0 +1,107 @@
+// fj_host_emul.cpp -- host (CPU) build of the F-J kernel arithmetic.
+//
+// Compiles fj_lane.h with the portable sin/cos/rsq/log path so the CPU test
+// suite can check the kernel algorithm (coefficient preparation, the
+// Euler-Maclaurin / G-difference segment rule, the small/large argument
+// split, compensated accumulation) against the FP64 oracle without a GPU.
+// It is a test build of the kernel code, not a fallback: the ccfj Python
+// package never loads it.
+#include <stdint.h>
+#include <stdlib.h>
+
+#include <vector>
+
+#include "fj_lane.h"
+
+using namespace fjl;
+
+namespace {
+
+template <int MODE>
+void run_mode(int itype, const float* U0, const float* U1, const float* r, const float* f,
+              const float* c, int nr, int nf, int nc, float* out) {
+  const int NS = ModeT<MODE>::nspec;
+  const int NC = coef_count(MODE, itype);
+  std::vector<double> rr(nr);
+  std::vector<float> hh(nr);
+  for (int s = 0; s < nr; s++) {
+    rr[s] = r[s];
+    hh[s] = s > 0 ? r[s] - r[s - 1] : 0.0f;
+  }
+  std::vector<float> cf((size_t)nr * NC);
+#pragma omp parallel for schedule(dynamic, 1) firstprivate(cf)
+  for (int i = 0; i < nf; i++) {
+    for (int s = 0; s < nr; s++) {
+      float gp[2] = {0.f, 0.f}, gc[2];
+      gc[0] = U0[(size_t)s * nf + i];
+      gc[1] = NS > 1 ? U1[(size_t)s * nf + i] : 0.f;
+      if (s > 0) {
+        gp[0] = U0[(size_t)(s - 1) * nf + i];
+        gp[1] = NS > 1 ? U1[(size_t)(s - 1) * nf + i] : 0.f;
+      }
+      const float rp = s > 0 ? r[s - 1] : 0.f, rn = s + 1 < nr ? r[s + 1] : 0.f;
+      float* rec = &cf[(size_t)s * NC];
+      if (itype == 0) {
+        if (NS == 1) station_coef<0, 1>(s, nr, rp, r[s], rn, gp, gc, rec);
+        else station_coef<0, 2>(s, nr, rp, r[s], rn, gp, gc, rec);
+      } else {
+        if (NS == 1) station_coef<1, 1>(s, nr, rp, r[s], rn, gp, gc, rec);
+        else station_coef<1, 2>(s, nr, rp, r[s], rn, gp, gc, rec);
+      }
+    }
+    for (int j = 0; j < nc; j++) {
+      const double q = (double)f[i] / (double)c[j];
+      out[(size_t)j * nf + i] = itype == 0 ? trapz_lane<MODE>(rr.data(), cf.data(), nr, q)
+                                           : linseg_lane<MODE>(rr.data(), hh.data(), cf.data(), nr, q);
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" int fjemul_transform(int mode, int itype, const float* U0, const float* U1,
+                                const float* r, const float* f, const float* c, int nr, int nf,
+                                int nc, float* out) {
+  if (itype != 0 && itype != 1) return CCFJ_EINVAL;
+  if (nr <= 0) {
+    for (size_t k = 0; k < (size_t)nf * nc; k++) out[k] = 0.f;
+    return 0;
+  }
+  switch (mode) {
+    case MODE_J: run_mode<MODE_J>(itype, U0, U1, r, f, c, nr, nf, nc, out); break;
+    case MODE_Y: run_mode<MODE_Y>(itype, U0, U1, r, f, c, nr, nf, nc, out); break;
+    case MODE_J_MINUS_Y: run_mode<MODE_J_MINUS_Y>(itype, U0, U1, r, f, c, nr, nf, nc, out); break;
+    case MODE_ABS_J: run_mode<MODE_ABS_J>(itype, U0, U1, r, f, c, nr, nf, nc, out); break;
+    case MODE_ABS_H: run_mode<MODE_ABS_H>(itype, U0, U1, r, f, c, nr, nf, nc, out); break;
+    default: return CCFJ_EINVAL;
+  }
+  return 0;
+}
+
+// Scalar access to the special-function layer for unit tests.
+// which: 0 J0, 1 J1, 2 Y0, 3 Y1, 4 G_J, 5 G_Y (x > 0).
+extern "C" double fjemul_special(int which, double x) {
+  const double q = x / fjm::kTwoPi;
+  const float xf = (float)x;
+  Bess b;
+  float gj = 0.f, gy = 0.f;
+  if (xf >= fjm::kSplit) {
+    float t;
+    b = fjm::bessel_large<true, true>(xf, fjm::reduce_phase(q, 1.0), &t);
+    fjm::g_large(xf, t, b, &gj, &gy, true, true);
+  } else {
+    double y, L;
+    b = fjm::bessel_small<true, true>(x, &y, &L);
+    fjm::g_small(x, y, L, true, true, &gj, &gy);
+    gy = (float)(gy - fjm::kTwoOverPiD * x);  // unshift
+  }
+  switch (which) {
+    case 0: return b.j0;
+    case 1: return b.j1;
+    case 2: return b.y0;
+    case 3: return b.y1;
+    case 4: return gj;
+    case 5: return gy;
+  }
+  return 0.0;
+}

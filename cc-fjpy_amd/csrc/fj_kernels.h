@@ -1,0 +1,29 @@
+// fj_kernels.h -- interface between the host library and the F-J kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/ccfj.h"
+
+namespace fjk {
+
+// Device pointers of one transform.  U1 is the second spectrum for the
+// two-spectrum modes (Hilbert-transformed spectrum, or imaginary part).
+struct Problem {
+  const float* U0;
+  const float* U1;
+  const float* r;
+  const float* f;
+  const float* c;
+  int nr, nf, nc;
+  double* rr;    // [nr] workspace
+  float* hh;     // [nr] workspace
+  float* coef;   // [nf][nr][NC] workspace
+  float* out;    // [nc][nf]
+};
+
+int coef_per_station(int mode, int itype);
+hipError_t launch_prep(int mode, int itype, const Problem& p, hipStream_t st);
+hipError_t launch_fj(int mode, int itype, const Problem& p, hipStream_t st);
+hipError_t launch_normalize(float* out, int nf, int nc, hipStream_t st);
+
+}  // namespace fjk

@@ -1,0 +1,188 @@
+// fj_kernels.hip -- F-J transform kernels for gfx950 (MI355X).
+//
+// The per-(f,c) arithmetic lives in fj_lane.h (shared with the host build);
+// this file maps it onto the GPU.
+//
+// Data layout in HBM (prepared on the device by k_stations and k_prep):
+//   rr[nr]            double   station distances, ascending
+//   hh[nr]            float    r_s - r_{s-1} (hh[0] = 0)
+//   coef[nf][nr][NC]  float    per (frequency, station) coefficients:
+//       itype 0: A_s of each spectrum          (NC = nspec)
+//       itype 1: alpha_s, beta_s per spectrum  (NC = 2 nspec)
+//   out[nc][nf]       float    (the reference's layout, src/FJcpu.cpp:247)
+//
+// Launch: a workgroup is 256 lanes = 256 consecutive velocities of ONE
+// frequency.  The station data of the loop are then wave-uniform and are
+// read with scalar loads (one NC-float record per station per wave); each
+// lane owns one (f,c) output and keeps the whole station loop in registers,
+// so the main kernels are VALU-bound by construction.  Workgroups are
+// numbered XCD-major so that the workgroups of one frequency share an XCD
+// and its L2.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fj_kernels.h"
+#include "fj_lane.h"
+
+namespace fjk {
+
+using namespace fjl;
+
+constexpr int kBlock = 256;
+
+// Hardware places workgroup w on XCD (w mod 8); tile T = (w mod 8) * per +
+// w / 8 keeps consecutive tiles (same frequency) on one XCD.  Returns false
+// for the padding workgroups of the last round.
+__device__ inline bool tile_of(int ntiles, int* tile) {
+  const int w = blockIdx.x;
+  const int per = (ntiles + 7) >> 3;
+  const int T = (w & 7) * per + (w >> 3);
+  *tile = T;
+  return T < ntiles;
+}
+
+// station arrays: rr (double), hh (float gap)
+__global__ void k_stations(const float* __restrict__ r, int nr, double* __restrict__ rr,
+                           float* __restrict__ hh) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nr) return;
+  rr[s] = (double)r[s];
+  hh[s] = s > 0 ? r[s] - r[s - 1] : 0.0f;
+}
+
+// Coefficients + [nr][nf] -> [nf][nr][NC] transpose through LDS.
+// Tile: 64 stations x 64 frequencies plus one halo station below; the loads
+// are coalesced along f, the stores along s.
+template <int ITYPE, int NSPEC>
+__global__ __launch_bounds__(256) void k_prep(const float* __restrict__ U0,
+                                              const float* __restrict__ U1,
+                                              const float* __restrict__ r, int nr, int nf,
+                                              float* __restrict__ coef) {
+  constexpr int TS = 64, TF = 64, LD = TF + 1;
+  constexpr int NC = ITYPE == 0 ? NSPEC : 2 * NSPEC;
+  __shared__ float tile[NSPEC][TS + 1][LD];
+  const int f0 = blockIdx.x * TF;
+  const int s0 = blockIdx.y * TS;
+  const int lane = threadIdx.x & 63;
+  const int grp = threadIdx.x >> 6;
+  for (int row = grp; row < TS + 1; row += 4) {
+    const int s = s0 - 1 + row;
+    const int f = f0 + lane;
+    const bool ok = s >= 0 && s < nr && f < nf;
+    const size_t idx = ok ? (size_t)s * nf + f : 0;
+    tile[0][row][lane] = ok ? U0[idx] : 0.0f;
+    if (NSPEC > 1) tile[NSPEC - 1][row][lane] = ok ? U1[idx] : 0.0f;
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < TS * TF; it += 256) {
+    const int sl = it & (TS - 1);
+    const int fl = it >> 6;
+    const int s = s0 + sl, f = f0 + fl;
+    if (s >= nr || f >= nf) continue;
+    float gp[NSPEC], gc[NSPEC], rec[NC];
+    for (int k = 0; k < NSPEC; k++) {
+      gp[k] = tile[k][sl][fl];
+      gc[k] = tile[k][sl + 1][fl];
+    }
+    const float rp = s > 0 ? r[s - 1] : 0.0f;
+    const float rn = s + 1 < nr ? r[s + 1] : 0.0f;
+    station_coef<ITYPE, NSPEC>(s, nr, rp, r[s], rn, gp, gc, rec);
+    float* dst = coef + ((size_t)f * nr + s) * NC;
+#pragma unroll
+    for (int k = 0; k < NC; k++) dst[k] = rec[k];
+  }
+}
+
+template <int MODE, int ITYPE>
+__global__ __launch_bounds__(kBlock) void k_fj(const double* __restrict__ rr,
+                                              const float* __restrict__ hh,
+                                              const float* __restrict__ coef,
+                                              const float* __restrict__ fv,
+                                              const float* __restrict__ cv, int nr, int nf,
+                                              int nc, float* __restrict__ out) {
+  constexpr int NC = ITYPE == 0 ? ModeT<MODE>::nspec : 2 * ModeT<MODE>::nspec;
+  const int ncb = (nc + kBlock - 1) / kBlock;
+  int T;
+  if (!tile_of(nf * ncb, &T)) return;
+  const int fi = T / ncb;
+  const int j = (T - fi * ncb) * kBlock + threadIdx.x;
+  if (j >= nc) return;
+  const double q = (double)fv[fi] / (double)cv[j];
+  const float* __restrict__ cf = coef + (size_t)fi * nr * NC;
+  float v;
+  if (ITYPE == 0)
+    v = trapz_lane<MODE>(rr, cf, nr, q);
+  else
+    v = linseg_lane<MODE>(rr, hh, cf, nr, q);
+  out[(size_t)j * nf + fi] = v;
+}
+
+// Per-column normalisation (src/ccfj.pyx:225-226): out[:,i] /= max(|out[:,i]|)
+// with Python's builtin max over the column: a NaN in row 0 propagates, a NaN
+// later never compares greater and is skipped.
+__global__ void k_normalize(float* __restrict__ out, int nf, int nc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf || nc <= 0) return;
+  float m = fabsf(out[i]);
+  for (int j = 1; j < nc; j++) {
+    const float v = fabsf(out[(size_t)j * nf + i]);
+    if (v > m) m = v;
+  }
+  for (int j = 0; j < nc; j++) {
+    const size_t idx = (size_t)j * nf + i;
+    out[idx] = out[idx] / m;
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <int MODE>
+static hipError_t launch_main(int itype, const Problem& p, hipStream_t st) {
+  const int ncb = (p.nc + kBlock - 1) / kBlock;
+  const long ntiles = (long)p.nf * ncb;
+  const long nblk = ((ntiles + 7) / 8) * 8;
+  if (itype == 0)
+    hipLaunchKernelGGL((k_fj<MODE, 0>), dim3((unsigned)nblk), dim3(kBlock), 0, st, p.rr, p.hh,
+                       p.coef, p.f, p.c, p.nr, p.nf, p.nc, p.out);
+  else
+    hipLaunchKernelGGL((k_fj<MODE, 1>), dim3((unsigned)nblk), dim3(kBlock), 0, st, p.rr, p.hh,
+                       p.coef, p.f, p.c, p.nr, p.nf, p.nc, p.out);
+  return hipGetLastError();
+}
+
+template <int ITYPE, int NSPEC>
+static hipError_t launch_prep_t(const Problem& p, hipStream_t st) {
+  dim3 grid((p.nf + 63) / 64, (p.nr + 63) / 64);
+  hipLaunchKernelGGL((k_prep<ITYPE, NSPEC>), grid, dim3(256), 0, st, p.U0, p.U1, p.r, p.nr,
+                     p.nf, p.coef);
+  return hipGetLastError();
+}
+
+int coef_per_station(int mode, int itype) { return coef_count(mode, itype); }
+
+hipError_t launch_prep(int mode, int itype, const Problem& p, hipStream_t st) {
+  hipLaunchKernelGGL(k_stations, dim3((p.nr + 255) / 256), dim3(256), 0, st, p.r, p.nr, p.rr,
+                     p.hh);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const bool two = !(mode == MODE_J || mode == MODE_Y);
+  if (itype == 0) return two ? launch_prep_t<0, 2>(p, st) : launch_prep_t<0, 1>(p, st);
+  return two ? launch_prep_t<1, 2>(p, st) : launch_prep_t<1, 1>(p, st);
+}
+
+hipError_t launch_fj(int mode, int itype, const Problem& p, hipStream_t st) {
+  switch (mode) {
+    case MODE_J: return launch_main<MODE_J>(itype, p, st);
+    case MODE_Y: return launch_main<MODE_Y>(itype, p, st);
+    case MODE_J_MINUS_Y: return launch_main<MODE_J_MINUS_Y>(itype, p, st);
+    case MODE_ABS_J: return launch_main<MODE_ABS_J>(itype, p, st);
+    case MODE_ABS_H: return launch_main<MODE_ABS_H>(itype, p, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_normalize(float* out, int nf, int nc, hipStream_t st) {
+  hipLaunchKernelGGL(k_normalize, dim3((nf + 255) / 256), dim3(256), 0, st, out, nf, nc);
+  return hipGetLastError();
+}
+
+}  // namespace fjk

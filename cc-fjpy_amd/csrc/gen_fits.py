@@ -1,0 +1,130 @@
+"""Generate fj_fits.h: polynomial fits of the Bessel/Struve building blocks.
+
+    python cc-fjpy_amd/csrc/gen_fits.py      # rewrites fj_fits.h (about 1 min)
+
+Every fit is a least-squares Chebyshev fit (Chebyshev nodes, mpmath ground
+truth at 40 digits) converted to monomial form and evaluated by Horner's rule
+with fused multiply-adds.  Nothing is taken from other libraries' tables.
+
+Large argument, x >= 8, variable t = (8/x)^2 in (0,1], evaluated in float:
+  Hankel asymptotic form (Abramowitz & Stegun 9.2.5-9.2.6):
+    J_n(x) = sqrt(2/(pi x)) (P_n cos th_n - Q_n sin th_n)
+    Y_n(x) = sqrt(2/(pi x)) (P_n sin th_n + Q_n cos th_n),  th_n = x - (2n+1) pi/4
+  fit_P0(t) = P_0(x), fit_q0(t) = Q_0(x) x/8, fit_P1, fit_q1 likewise.
+  Struve minus Neumann (A&S 12.1.29-12.1.31), smooth and non-oscillating:
+    fit_R0(t) = (pi/2) x (H_0(x) - Y_0(x)),  fit_R1(t) = (pi/2) (H_1(x) - Y_1(x))
+Small argument, x < 8, variable y = (x/8)^2 in [0,1), evaluated in double
+(the monomial coefficients alternate in sign up to ~1e2, too much
+cancellation for float):
+  J_0 = fit_J0s(y);  J_1 = x fit_J1s(y)
+  Y_0 = (2/pi) ln(x) J_0 + fit_Y0s(y)
+  Y_1 = (2/pi) ln(x) J_1 - 2/(pi x) + x fit_Y1s(y)
+  G_J(x) = int_0^x t J_1(t) dt = x^3 fit_GJs(y)
+  G_Y(x) = int_0^x t Y_1(t) dt = (2/pi) ln(x) G_J(x) + x fit_GYs(y)
+"""
+import os
+
+import mpmath as mp
+import numpy as np
+
+mp.mp.dps = 40
+P = np.polynomial
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def cheb_nodes(n, a, b):
+    k = np.arange(n)
+    return (a + b) / 2 + (b - a) / 2 * np.cos(np.pi * (k + 0.5) / n)
+
+
+def large_truth(x):
+    x = mp.mpf(x)
+    J0, Y0 = mp.besselj(0, x), mp.bessely(0, x)
+    J1, Y1 = mp.besselj(1, x), mp.bessely(1, x)
+    th0, th1 = x - mp.pi / 4, x - 3 * mp.pi / 4
+    s = mp.sqrt(mp.pi * x / 2)
+    z = 8 / x
+    P0 = s * (J0 * mp.cos(th0) + Y0 * mp.sin(th0))
+    Q0 = s * (Y0 * mp.cos(th0) - J0 * mp.sin(th0))
+    P1 = s * (J1 * mp.cos(th1) + Y1 * mp.sin(th1))
+    Q1 = s * (Y1 * mp.cos(th1) - J1 * mp.sin(th1))
+    S0 = mp.struveh(0, x) - Y0
+    S1 = mp.struveh(1, x) - Y1
+    return [P0, Q0 / z, P1, Q1 / z, mp.pi / 2 * x * S0, mp.pi / 2 * S1]
+
+
+def small_truth(x):
+    x = mp.mpf(x)
+    L = 2 / mp.pi * mp.log(x)
+    J0, J1 = mp.besselj(0, x), mp.besselj(1, x)
+    Y0, Y1 = mp.bessely(0, x), mp.bessely(1, x)
+    H0, H1 = mp.struveh(0, x), mp.struveh(1, x)
+    gj = mp.pi * x / 2 * (J1 * H0 - J0 * H1)
+    gy = mp.pi * x / 2 * (Y1 * H0 - Y0 * H1)
+    return [J0, J1 / x, Y0 - L * J0, (Y1 - L * J1 + 2 / (mp.pi * x)) / x,
+            gj / x ** 3, (gy - L * gj) / x]
+
+
+LARGE = [("fit_P0", 4), ("fit_q0", 4), ("fit_P1", 4), ("fit_q1", 4),
+         ("fit_R0", 7), ("fit_R1", 6)]
+SMALL = [("fit_J0s", 11), ("fit_J1s", 10), ("fit_Y0s", 11), ("fit_Y1s", 10),
+         ("fit_GJs", 10), ("fit_GYs", 11)]
+
+
+def fit_all(names, var, truth_fn, npts=160):
+    v = cheb_nodes(npts, 0.0, 1.0)
+    v = v[v > 1e-7]
+    xs = 8 / np.sqrt(v) if var == "t" else 8 * np.sqrt(v)
+    vals = np.array([[float(q) for q in truth_fn(x)] for x in xs])
+    # independent check points
+    rng = np.random.default_rng(1)
+    vc = np.sort(rng.uniform(1e-6, 1.0, 300))
+    xc = 8 / np.sqrt(vc) if var == "t" else 8 * np.sqrt(vc)
+    valc = np.array([[float(q) for q in truth_fn(x)] for x in xc])
+    res = []
+    for j, (name, deg) in enumerate(names):
+        cheb = P.Chebyshev.fit(v, vals[:, j], deg, domain=[0, 1])
+        coef = cheb.convert(kind=P.Polynomial, domain=[0, 1], window=[0, 1]).coef
+        err = float(np.max(np.abs(P.polynomial.polyval(vc, coef) - valc[:, j])))
+        res.append((name, coef, err))
+    return res
+
+
+def emit(fname, coefs, ctype, err, var):
+    fma = "__builtin_fmaf" if ctype == "float" else "__builtin_fma"
+    suf = "f" if ctype == "float" else ""
+    lines = [f"/* max abs error of the fit on (0,1]: {err:.2e} */",
+             f"FJ_HD inline {ctype} {fname}({ctype} {var}) {{",
+             f"  {ctype} p = {float(coefs[-1])!r}{suf};"]
+    for c in coefs[-2::-1]:
+        lines.append(f"  p = {fma}(p, {var}, {float(c)!r}{suf});")
+    lines.append("  return p;")
+    lines.append("}")
+    return "\n".join(lines)
+
+
+def main():
+    large = fit_all(LARGE, "t", large_truth)
+    small = fit_all(SMALL, "y", small_truth)
+    out = ["/* Generated by gen_fits.py -- do not edit by hand. */",
+           "#pragma once",
+           "#ifndef FJ_HD",
+           "#define FJ_HD",
+           "#endif",
+           "",
+           "/* large argument x >= 8, t = (8/x)^2, float */"]
+    for name, coef, err in large:
+        out.append(emit(name, coef, "float", err, "t"))
+    out.append("")
+    out.append("/* small argument x < 8, y = (x/8)^2, double */")
+    for name, coef, err in small:
+        out.append(emit(name, coef, "double", err, "y"))
+    path = os.path.join(HERE, "fj_fits.h")
+    with open(path, "w") as fh:
+        fh.write("\n".join(out) + "\n")
+    for name, coef, err in large + small:
+        print(f"{name:10s} deg {len(coef) - 1:2d} err {err:.2e}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,107 @@
+/*
+ * ccfj.h -- C ABI of the MI355X F-J transform library (libccfj.so).
+ *
+ * Two layers:
+ *   1. Reference-compatible entry points with the argument order and meaning
+ *      of the reference's native F-J functions.  Host pointers, synchronous,
+ *      out[nc][nf] float32, U_f[nr][nf] float32, r ascending in metres.
+ *   2. A context-based API: device selection, cached device workspace, one
+ *      stream per context, the fused multi-spectrum modes that fj_noise and
+ *      fj_earthquake need, device-pointer (stream-ordered) calls, on-device
+ *      per-column normalisation and kernel timing.
+ *
+ * Every function returns 0 on success and a negative CCFJ_E* code on
+ * failure; ccfj_last_error() then describes the failure (per thread).  No
+ * function exits the process (the reference's GPU helper calls exit(),
+ * include/cuda_helper.h:74-84).
+ */
+#ifndef CCFJ_H
+#define CCFJ_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCFJ_OK 0
+#define CCFJ_EINVAL (-1)   /* bad argument */
+#define CCFJ_EDEVICE (-2)  /* no usable GPU / HIP runtime error */
+#define CCFJ_ENOMEM (-3)   /* device allocation failed */
+
+/* Transform modes (how the spectra and bases are combined per (f,c)). */
+#define CCFJ_MODE_J 0          /* FJ[U0]                       J0 basis   */
+#define CCFJ_MODE_Y 1          /* FY[U0]                       Y0 basis   */
+#define CCFJ_MODE_J_MINUS_Y 2  /* FJ[U0] - FY[U1]              fj_noise func=1 (U1 = Hilbert(U0)) */
+#define CCFJ_MODE_ABS_J 3      /* |FJ[U0] + i FJ[U1]|          fj_earthquake func=0 (U0 = Re, U1 = Im) */
+#define CCFJ_MODE_ABS_H 4      /* |FJ[U0]-FY[U1] + i(FJ[U1]+FY[U0])|  fj_earthquake func=1 */
+
+/* ---- 1. reference-compatible entry points --------------------------------
+ * type: 0 = trapezoid, 1 = linear segments.  A type outside {0,1} returns
+ * CCFJ_EINVAL and leaves out untouched (the reference silently does nothing,
+ * src/FJcpu.cpp:403-407).  The current HIP device is used. */
+
+/* Replaces FJ, include/FJgpu.hh:13 (GPU build; J0 basis). */
+int FJ(float *u_f, float *r, float *f, float *out, float *c, int nc, int nr,
+       int nf, int type);
+/* Replaces FHr, include/FJgpu.hh:14 (J0 basis, the GPU build's FP64 variant;
+ * here identical to FJ). */
+int FHr(float *u_f, float *r, float *f, float *out, float *c, int nc, int nr,
+        int nf, int type);
+/* Replaces FHi, include/FJgpu.hh:15 (Y0 basis). */
+int FHi(float *u_f, float *r, float *f, float *out, float *c, int nc, int nr,
+        int nf, int type);
+/* Replaces FH, include/FJcpu.h:20 (CPU build's Y0 basis; nThread ignored). */
+int FH(float *u_f, float *r, float *f, float *out, float *c, int nc, int nr,
+       int nf, int type, int nThread);
+
+/* ---- 2. context API --------------------------------------------------------
+ * No counterpart in the reference (its GPU code allocates, copies and frees
+ * on every call, src/FJgpu.cu:398-417).  A context owns a HIP stream and a
+ * grow-only device workspace on one device; calls on one context are
+ * serialised by the caller (one context per host thread). */
+typedef struct ccfj_ctx ccfj_ctx;
+
+/* device < 0 selects the current HIP device. */
+int ccfj_create(int device, ccfj_ctx **ctx);
+int ccfj_destroy(ccfj_ctx *ctx);
+const char *ccfj_last_error(void);
+/* Number of visible GPUs (0 when none); never fails. */
+int ccfj_device_count(void);
+
+/* Host pointers.  U0, U1: [nr][nf] float32 (U1 only for the two-spectrum
+ * modes, else NULL); r[nr] ascending (metres); f[nf]; c[nc]; out[nc][nf].
+ * normalize != 0 divides every column by its max |value| (src/ccfj.pyx:225).
+ * Synchronous. */
+int ccfj_transform(ccfj_ctx *ctx, int mode, int itype, int normalize,
+                   const float *U0, const float *U1, const float *r,
+                   const float *f, const float *c, int nr, int nf, int nc,
+                   float *out);
+
+/* Device pointers, enqueued on `stream` (a hipStream_t; NULL = the
+ * context's stream).  Returns once enqueued. */
+int ccfj_transform_dev(ccfj_ctx *ctx, int mode, int itype, int normalize,
+                       const float *U0, const float *U1, const float *r,
+                       const float *f, const float *c, int nr, int nf, int nc,
+                       float *out, void *stream);
+
+int ccfj_synchronize(ccfj_ctx *ctx);
+
+/* Kernel timing: when enabled, every main F-J kernel launch is bracketed by
+ * HIP events on its stream; ccfj_get_timing returns the summed kernel time
+ * (ms) and launch count since the last reset. */
+int ccfj_set_timing(ccfj_ctx *ctx, int enable);
+int ccfj_get_timing(ccfj_ctx *ctx, double *kernel_ms, int *launches);
+
+/* Plain single-instance FP64 GPU F-J: one thread per (f,c), serial station
+ * loop, double-precision Bessel (ROCm device library) and Struve
+ * (Zhang & Jin) functions, device buffers allocated, copied and freed per
+ * call -- the structure of the reference's GPU engine
+ * (src/FJgpu.cu:160-313, 390-483) in FP64.  Used as the speed-up baseline.
+ * ybasis: 0 = J0, 1 = Y0.  kernel_ms (may be NULL) receives the kernel time. */
+int ccfj_plain_fp64(int device, int ybasis, int itype, const float *U,
+                    const float *r, const float *f, const float *c, int nr,
+                    int nf, int nc, float *out, double *kernel_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CCFJ_H */

@@ -1,0 +1,27 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+for p in (os.path.join(ROOT, "cc-fjpy_amd"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden", "fjcpu_golden.npz")
+GOLDEN_CASES = ["nr2", "nr3", "edges17", "branches", "cfg1s"]
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and the built libccfj.so")
+
+
+@pytest.fixture(scope="session")
+def golden():
+    d = np.load(GOLDEN)
+    return {k: d[k] for k in d.files}
+
+
+def golden_case(g, name):
+    return tuple(g[f"{name}/{k}"] for k in "Urfc")
