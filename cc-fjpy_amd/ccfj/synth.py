@@ -1,0 +1,82 @@
+"""Seeded synthetic inputs for the F-J workloads (no data files needed).
+
+Each generator returns the arrays the reference's Python API takes, in the
+units it uses (r in metres, c in m/s, f in Hz).  Constants the workload
+descriptions leave open are fixed here.
+
+  noise_single(seed=0)   300 CCFs, r ~ U[5,200] km, nf=512 (0.01-0.5 Hz),
+                         nc=500 (1000-5000 m/s); uf = J0(2 pi f r / c0(f))
+                         + complex N(0, 0.1), c0 = 3000 + 1500 exp(-f/0.08)
+  noise_dense(seed=1)    200 stations in a 300 x 300 km square -> 19,900
+                         pairs (GetStationPairs order, r unsorted),
+                         nf=4096 (0.005-2 Hz), nc=3000 (500-6000 m/s);
+                         three modes, amplitudes 1 / 0.5 / 0.25,
+                         c_m(f) = cinf_m + d_m exp(-f/phi_m), plus noise.
+                         Only the real part of uf is used by the transform,
+                         so `real_only=True` skips the imaginary noise.
+"""
+import numpy as np
+from scipy.special import j0
+
+DENSE_MODES = dict(cinf=(2000.0, 2600.0, 3200.0), delta=(2000.0, 2400.0, 2600.0),
+                   phi=(0.15, 0.3, 0.5), amp=(1.0, 0.5, 0.25))
+
+
+def noise_single(seed=0, nr=300, nf=512, nc=500):
+    rng = np.random.default_rng(seed)
+    r = np.sort(rng.uniform(5e3, 200e3, nr))
+    f = np.linspace(0.01, 0.5, nf)
+    c = np.linspace(1000.0, 5000.0, nc)
+    c0 = 3000.0 + 1500.0 * np.exp(-f / 0.08)
+    uf = j0(2 * np.pi * f[None, :] * r[:, None] / c0[None, :])
+    uf = uf + rng.normal(0, 0.1, uf.shape) + 1j * rng.normal(0, 0.1, uf.shape)
+    return dict(uf=uf.astype(np.complex64), r=r, c=c, f=f, c0=c0)
+
+
+def dense_pairs(seed=1, nsta=200, side=300e3):
+    rng = np.random.default_rng(seed)
+    xy = rng.uniform(0.0, side, (nsta, 2))
+    ii, jj = np.triu_indices(nsta, k=1)      # GetStationPairs order (i<j, row-major)
+    r = np.hypot(*(xy[ii] - xy[jj]).T)
+    return r, rng
+
+
+def dense_phase_velocity(f):
+    m = DENSE_MODES
+    return [ci + d * np.exp(-f / p) for ci, d, p in zip(m["cinf"], m["delta"], m["phi"])]
+
+
+def noise_dense(seed=1, nf=4096, nc=3000, real_only=False, chunk=512, fidx=None):
+    """fidx (optional): keep only these frequency columns (the noise is
+    drawn for the full grid first, so a column is identical either way)."""
+    r, rng = dense_pairs(seed)
+    f = np.linspace(0.005, 2.0, nf)
+    c = np.linspace(500.0, 6000.0, nc)
+    if fidx is not None:
+        full = rng.normal(0, 0.1, (len(r), nf)).astype(np.float32)[:, fidx]
+        f = f[fidx]
+        nf = len(f)
+    cm = dense_phase_velocity(f)
+    nr = len(r)
+    re = np.empty((nr, nf), np.float32)
+    for a in range(0, nr, chunk):
+        rr = r[a:a + chunk, None]
+        acc = np.zeros((len(rr), nf))
+        for amp, cv in zip(DENSE_MODES["amp"], cm):
+            acc += amp * j0(2 * np.pi * f[None, :] * rr / cv[None, :])
+        re[a:a + chunk] = acc
+    if fidx is not None:
+        re += full
+        return dict(uf=re, r=r, c=c, f=f, cm=cm)
+    re += rng.normal(0, 0.1, re.shape).astype(np.float32)
+    if real_only:
+        uf = re
+    else:
+        uf = re + 1j * rng.normal(0, 0.1, re.shape).astype(np.float32)
+        uf = uf.astype(np.complex64)
+    return dict(uf=uf, r=r, c=c, f=f, cm=cm)
+
+
+def term_evals(nr, nf, nc, calls=1):
+    """Reference-equivalent term evaluations: nf * nc * nr per native call."""
+    return float(nr) * nf * nc * calls

@@ -73,9 +73,10 @@ int ensure(ccfj_ctx* ctx, Slot s, size_t bytes) {
   if (bytes == 0) bytes = 1;
   if (ctx->cap[s] >= bytes) return CCFJ_OK;
   if (ctx->buf[s]) {
-    hipError_t e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-    hipFree(ctx->buf[s]);
+    // the buffer may still be read by work on any stream the caller used
+    hipError_t e = hipDeviceSynchronize();
+    if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+    (void)hipFree(ctx->buf[s]);
     ctx->buf[s] = nullptr;
     ctx->cap[s] = 0;
   }
@@ -148,12 +149,12 @@ int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U
         if ((e = hipEventCreate(&ev.first)) != hipSuccess) return hip_fail(e, "hipEventCreate");
         if ((e = hipEventCreate(&ev.second)) != hipSuccess) return hip_fail(e, "hipEventCreate");
       }
-      hipEventRecord(ev.first, st);
+      (void)hipEventRecord(ev.first, st);
     }
     e = fjk::launch_fj(mode, itype, p, st);
     if (e != hipSuccess) return hip_fail(e, "F-J kernel launch");
     if (ctx->timing) {
-      hipEventRecord(ev.second, st);
+      (void)hipEventRecord(ev.second, st);
       ctx->events.push_back(ev);
     }
   }
@@ -232,18 +233,18 @@ int ccfj_create(int device, ccfj_ctx** out) {
 int ccfj_destroy(ccfj_ctx* ctx) {
   if (!ctx) return CCFJ_OK;
   set_device(ctx);
-  hipStreamSynchronize(ctx->stream);
+  (void)hipStreamSynchronize(ctx->stream);
   for (int s = 0; s < S_COUNT; s++)
-    if (ctx->buf[s]) hipFree(ctx->buf[s]);
+    if (ctx->buf[s]) (void)hipFree(ctx->buf[s]);
   for (auto& ev : ctx->events) {
-    hipEventDestroy(ev.first);
-    hipEventDestroy(ev.second);
+    (void)hipEventDestroy(ev.first);
+    (void)hipEventDestroy(ev.second);
   }
   for (auto& ev : ctx->spare) {
-    hipEventDestroy(ev.first);
-    hipEventDestroy(ev.second);
+    (void)hipEventDestroy(ev.first);
+    (void)hipEventDestroy(ev.second);
   }
-  hipStreamDestroy(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CCFJ_OK;
 }

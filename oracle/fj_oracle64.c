@@ -92,18 +92,19 @@ static double panel_integral(double a, double b, double r1, double g1,
     return s * hm;
 }
 
-static double segment_integral(double r1, double r2, double g1, double slope,
-                               double k, int ybasis)
+/* int over [lo0, r2] (lo0 <= r2) of the line anchored at (r1, g1). */
+static double span_integral(double lo0, double r2, double r1, double g1, double slope,
+                            double k, int ybasis)
 {
-    double h = r2 - r1;
+    double h = r2 - lo0;
     if (h <= 0.0) return 0.0;
     double s = 0.0;
-    double lo = r1;
+    double lo = lo0;
     const double wmax = 0.5 / k;  /* panel width limit: 0.5 in x */
     if (ybasis) {
         /* rho*Y0(k rho) has a rho*log(rho) singularity at rho = 0: panels
          * are graded geometrically away from it while x < 0.5. */
-        if (lo == 0.0) {
+        if (lo <= 0.0) {
             double edge = fmin(h, wmax);
             double e = edge;
             for (int g = 0; g < 60; g++) {
@@ -128,6 +129,16 @@ static double segment_integral(double r1, double r2, double g1, double slope,
         s += panel_integral(a, b, r1, g1, slope, k, ybasis);
     }
     return s;
+}
+
+/* Signed segment integral: a reversed segment (r2 < r1, unsorted input at
+ * the C level) integrates from r1 down to r2, as the reference's closed
+ * form does. */
+static double segment_integral(double r1, double r2, double g1, double slope,
+                               double k, int ybasis)
+{
+    if (r2 >= r1) return span_integral(r1, r2, r1, g1, slope, k, ybasis);
+    return -span_integral(r2, r1, r1, g1, slope, k, ybasis);
 }
 
 static double column_entry(const float *U, const float *r, int nr, int nf,

@@ -1,0 +1,310 @@
+"""GPU parity of the HIP F-J path (libccfj.so through the C ABI).
+
+Bars (written into each test):
+  * vs the FP64 oracle: max over entries of |X - X*| / max_c |X*[:, f]|
+    <= 1e-5 (the accuracy target), NaN patterns identical;
+  * vs the FJcpu restatement (bit-exact with the reference): within
+    2 * (FJcpu's own distance to the oracle) + 1e-5 -- FJcpu's float
+    closed form is far from the true integral for itype 1, so it cannot be
+    held to 1e-5 itself (see test_oracle.py);
+  * wrapper semantics (sort, cast, stride, Hilbert, combine, normalise)
+    against the restatement of the reference's Python layer.
+"""
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_CASES, ROOT, golden_case
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def ccfj():
+    import ccfj as m
+    assert m.device_count() >= 1, "no GPU visible"
+    return m
+
+
+@pytest.fixture(scope="module")
+def ctx(ccfj):
+    return ccfj._context(-1)
+
+
+def native(ctx, mode, itype, U0, r, f, c, U1=None, normalize=False):
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    return ctx.transform(mode, itype, normalize, f32(U0), None if U1 is None else f32(U1),
+                         f32(r), f32(f), f32(c))
+
+
+# ---------------------------------------------------------------- golden cases
+@pytest.mark.parametrize("case", GOLDEN_CASES)
+@pytest.mark.parametrize("ybasis", [0, 1])
+@pytest.mark.parametrize("itype", [0, 1])
+def test_golden_vs_oracle_and_fjcpu(ccfj, ctx, golden, case, ybasis, itype):
+    U, r, f, c = golden_case(golden, case)
+    got = native(ctx, ccfj.MODE_Y if ybasis else ccfj.MODE_J, itype, U, r, f, c)
+    ref64 = O.oracle64(U, r, f, c, itype, ybasis)
+    emax, erms, mism = O.column_error(got, ref64)
+    print(f"{case} basis={'Y' if ybasis else 'J'} itype={itype}: emax={emax:.2e} erms={erms:.2e}")
+    assert mism == 0
+    assert emax <= TOL
+    fjcpu = golden[f"{case}/{'fh' if ybasis else 'fj'}{itype}"]
+    e_ref, _, _ = O.column_error(fjcpu, ref64)
+    e_cmp, _, mism2 = O.column_error(got, fjcpu)
+    assert mism2 == 0
+    assert e_cmp <= 2 * e_ref + TOL
+
+
+def test_golden_vs_host_build(ccfj, ctx, golden):
+    """The GPU kernels and the host build of the same code agree up to the
+    hardware sin/cos/rsq/log instructions (~1e-7)."""
+    import ctypes
+    lib = ctypes.CDLL(f"{ROOT}/cc-fjpy_amd/ccfj/libccfj_emul.so")
+    P = ctypes.c_void_p
+    lib.fjemul_transform.argtypes = [ctypes.c_int, ctypes.c_int] + [P] * 5 + [ctypes.c_int] * 3 + [P]
+    for case in GOLDEN_CASES:
+        U, r, f, c = golden_case(golden, case)
+        for mode in (ccfj.MODE_J, ccfj.MODE_Y):
+            for itype in (0, 1):
+                host = np.zeros((len(c), len(f)), np.float32)
+                lib.fjemul_transform(mode, itype, U.ctypes.data, U.ctypes.data, r.ctypes.data,
+                                     f.ctypes.data, c.ctypes.data, len(r), len(f), len(c),
+                                     host.ctypes.data)
+                got = native(ctx, mode, itype, U, r, f, c)
+                emax, _, mism = O.column_error(got, host)
+                assert mism == 0 and emax < 2e-6, (case, mode, itype, emax)
+
+
+# ------------------------------------------------------------- fused modes
+@pytest.mark.parametrize("itype", [0, 1])
+def test_fused_modes_match_single_basis(ccfj, ctx, golden, itype):
+    U, r, f, c = golden_case(golden, "cfg1s")
+    V = np.roll(U, 3, axis=0) * 0.7
+    J0, J1 = native(ctx, ccfj.MODE_J, itype, U, r, f, c), native(ctx, ccfj.MODE_J, itype, V, r, f, c)
+    Y0, Y1 = native(ctx, ccfj.MODE_Y, itype, U, r, f, c), native(ctx, ccfj.MODE_Y, itype, V, r, f, c)
+    J0, J1, Y0, Y1 = (a.astype(np.float64) for a in (J0, J1, Y0, Y1))
+    cases = {
+        ccfj.MODE_J_MINUS_Y: J0 - Y1,
+        ccfj.MODE_ABS_J: np.hypot(J0, J1),
+        ccfj.MODE_ABS_H: np.hypot(J0 - Y1, J1 + Y0),
+    }
+    for mode, want in cases.items():
+        got = native(ctx, mode, itype, U, r, f, c, U1=V)
+        emax, _, mism = O.column_error(got, want)
+        assert mism == 0 and emax < 2e-6, (mode, emax)
+
+
+# ------------------------------------------------------- single-array workload
+@pytest.fixture(scope="module")
+def single():
+    from ccfj import synth
+    return synth.noise_single()
+
+
+def test_noise_single_full_size_vs_oracle(ccfj, single):
+    """300 pairs x 512 f x 500 c, itype 1, func 0 through fj_noise, against
+    the reference's Python layer restated over the FP64 oracle."""
+    d = single
+    got = ccfj.fj_noise(d["uf"], d["r"], d["c"], d["f"], itype=1, func=0)
+    assert got.dtype == np.float32 and got.shape == (500, 512)
+    want = O.fj_noise(d["uf"], d["r"], d["c"], d["f"], itype=1, func=0, engine="fp64")
+    emax, erms, mism = O.column_error(got, want)
+    print(f"noise_single itype1 func0: emax={emax:.2e} erms={erms:.2e}")
+    assert mism == 0 and emax <= TOL
+    # dispersion ridge: the transform peaks at the model phase velocity
+    c = d["c"]
+    pick = c[np.argmax(got, axis=0)]
+    band = (d["f"] > 0.05) & (d["f"] < 0.45)
+    assert np.median(np.abs(pick[band] - d["c0"][band])) < 2 * (c[1] - c[0]) * 5
+
+
+def test_noise_single_vs_fjcpu(ccfj, single):
+    d = single
+    got = ccfj.fj_noise(d["uf"], d["r"], d["c"], d["f"], itype=1, func=0)
+    fjcpu = O.fj_noise(d["uf"], d["r"], d["c"], d["f"], itype=1, func=0, engine="port")
+    ref64 = O.fj_noise(d["uf"], d["r"], d["c"], d["f"], itype=1, func=0, engine="fp64")
+    e_ref = O.column_error(fjcpu, ref64)[0]
+    e_cmp = O.column_error(got, fjcpu)[0]
+    print(f"FJcpu vs oracle {e_ref:.2e}; GPU vs FJcpu {e_cmp:.2e}")
+    assert e_cmp <= 2 * e_ref + TOL
+
+
+@pytest.mark.parametrize("itype", [0, 1])
+@pytest.mark.parametrize("func", [0, 1])
+def test_fj_noise_wrapper_semantics(ccfj, single, itype, func):
+    d = single
+    sub = slice(0, 120)
+    uf, r = d["uf"][sub, ::8], d["r"][sub]
+    perm = np.random.default_rng(7).permutation(len(r))   # unsorted input
+    f = d["f"][::8]
+    got = ccfj.fj_noise(uf[perm], r[perm], d["c"][::5], f, itype=itype, func=func)
+    want = O.fj_noise(uf, r, d["c"][::5], f, itype=itype, func=func, engine="fp64")
+    emax, _, mism = O.column_error(got, want)
+    assert mism == 0 and emax <= TOL, emax
+    assert np.allclose(np.nanmax(np.abs(got), axis=0), 1.0)
+
+
+def test_fj_noise_fstride(ccfj, single):
+    d = single
+    uf, f = d["uf"][:60], d["f"][::3]
+    got = ccfj.fj_noise(uf, d["r"][:60], d["c"][::10], f, fstride=3, itype=0)
+    want = O.fj_noise(uf, d["r"][:60], d["c"][::10], f, fstride=3, itype=0, engine="fp64")
+    assert O.column_error(got, want)[0] <= TOL
+
+
+# --------------------------------------------------------- earthquake / MWFJ
+def _quake(seed=3, nr=48, npts=1024, Fs=20.0):
+    rng = np.random.default_rng(seed)
+    r = np.sort(rng.uniform(2e3, 60e3, nr))
+    t = np.arange(npts) / Fs
+    u = np.zeros((nr, npts))
+    for cm, a in ((3200.0, 1.0), (3800.0, 0.4)):
+        arr = r / cm
+        u += a * np.exp(-((t[None, :] - 5 - arr[:, None]) * 2.0) ** 2) * np.cos(2 * np.pi * 1.0 * (t[None, :] - arr[:, None]))
+    u += 0.05 * rng.standard_normal(u.shape)
+    f = np.arange(1, 120) * Fs / npts
+    c = np.linspace(2000.0, 5000.0, 61)
+    return u.astype(np.float32), r, c, f, Fs
+
+
+@pytest.mark.parametrize("itype", [0, 1])
+@pytest.mark.parametrize("func", [0, 1])
+def test_fj_earthquake_wrapper(ccfj, itype, func):
+    u, r, c, f, _ = _quake()
+    perm = np.random.default_rng(1).permutation(len(r))
+    got = ccfj.fj_earthquake(u[perm], r[perm], c, f, itype=itype, func=func)
+    want = O.fj_earthquake(u, r, c, f, itype=itype, func=func, engine="fp64")
+    emax, _, mism = O.column_error(got, want)
+    assert mism == 0 and emax <= TOL, emax
+
+
+def test_mwfj_wrapper(ccfj):
+    u, r, c, f, Fs = _quake(nr=24, npts=512)
+    nwin = 3
+    v = np.array([2500.0, 3500.0, 4500.0, 5500.0])
+    winl = np.stack([r / v[w + 1] for w in range(nwin)])
+    winr = np.stack([r / v[w] + 8.0 for w in range(nwin)])
+    got = ccfj.MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=0)
+    want = O.MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=0, engine="fp64")
+    assert got.dtype == np.float64 and got.shape == (nwin, len(c), len(f))
+    for w in range(nwin):
+        emax, _, mism = O.column_error(got[w], want[w])
+        assert mism == 0 and emax <= TOL, (w, emax)
+
+
+# ------------------------------------------------------------- edge cases
+def test_edges_and_errors(ccfj, ctx):
+    r = np.array([1000.0, 1000.0, 1000.05, 3000.0, 9000.0], np.float32)
+    U = np.random.default_rng(0).standard_normal((5, 11)).astype(np.float32)
+    f = np.concatenate([[0.0], np.linspace(0.05, 1.0, 10)]).astype(np.float32)
+    c = np.linspace(800, 4000, 7).astype(np.float32)
+    for itype in (0, 1):
+        for mode, yb in ((ccfj.MODE_J, 0), (ccfj.MODE_Y, 1)):
+            got = native(ctx, mode, itype, U, r, f, c)
+            want = O.oracle64(U, r, f, c, itype, yb)
+            emax, _, mism = O.column_error(got, want)
+            assert mism == 0 and emax <= TOL, (itype, mode, emax)
+    # no stations / one station: zeros, as the reference's empty loops
+    assert np.all(native(ctx, ccfj.MODE_J, 1, U[:1], r[:1], f, c) == 0)
+    z = ctx.transform(ccfj.MODE_J, 0, False, np.zeros((0, 11), np.float32), None,
+                      np.zeros(0, np.float32), f, c)
+    assert z.shape == (7, 11) and np.all(z == 0)
+    # itype outside {0,1}: the reference leaves zeros -> NaN after normalising
+    with np.errstate(invalid="ignore"):
+        assert np.all(np.isnan(ccfj.fj_noise(U.T.copy().T, r.astype(np.float64), c, f, itype=2)))
+    # func outside {0,1}: prints and returns 0
+    assert ccfj.fj_noise(U, r, c, f, func=2) == 0
+    # fj_earthquake with fstride > 1 strides twice and fails in reshape
+    with pytest.raises(ValueError):
+        ccfj.fj_earthquake(np.ones((5, 64)), r, c, f[:10], fstride=2)
+    # C-ABI argument errors surface as RuntimeError with the library's text
+    with pytest.raises(RuntimeError, match="mode"):
+        ctx.transform(17, 0, False, U, None, r, f, c)
+
+
+def test_reference_compatible_entry_points(ccfj, ctx, golden):
+    from ccfj import _ccfj
+    U, r, f, c = golden_case(golden, "branches")
+    nr, nf, nc = len(r), len(f), len(c)
+    for itype in (0, 1):
+        for fn, mode in ((_ccfj.fj_full, ccfj.MODE_J), (_ccfj.fHr_full, ccfj.MODE_J),
+                         (_ccfj.fHi_full, ccfj.MODE_Y)):
+            out = np.zeros(nf * nc, np.float32)
+            fn(U.reshape(-1).copy(), r.copy(), f.copy(), out, c.copy(), nc, nr, nf, itype)
+            assert np.array_equal(out.reshape(nc, nf), native(ctx, mode, itype, U, r, f, c))
+    out = np.zeros(nf * nc, np.float32)
+    with pytest.raises(RuntimeError):
+        _ccfj.fj_full(U.reshape(-1).copy(), r.copy(), f.copy(), out, c.copy(), nc, nr, nf, 3)
+
+
+def test_plain_fp64_baseline(ccfj, golden):
+    from ccfj import _ccfj
+    U, r, f, c = golden_case(golden, "cfg1s")
+    for yb in (0, 1):
+        for itype in (0, 1):
+            out, ms = _ccfj.plain_fp64(0, yb, itype, U, r, f, c)
+            emax, _, mism = O.column_error(out, O.oracle64(U, r, f, c, itype, yb))
+            assert mism == 0 and emax < 1e-6, (yb, itype, emax)
+            assert ms > 0
+
+
+def test_alloc_failure_is_reported_not_fatal():
+    code = ("import sys; sys.path.insert(0, %r); import numpy as np, ccfj\n"
+            "try:\n"
+            "    ccfj.fj(np.ones((3, 4), np.float32), np.arange(3.) * 100, np.arange(1., 6.) * 1e3, np.arange(4.) * .1)\n"
+            "    print('NO-ERROR')\n"
+            "except RuntimeError as e:\n"
+            "    print('ERR', e)\n" % f"{ROOT}/cc-fjpy_amd")
+    p = subprocess.run([sys.executable, "-c", code], env={**__import__("os").environ, "CCFJ_FAIL_ALLOC_AT": "1"},
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "ERR" in p.stdout and "injected" in p.stdout
+
+
+# ------------------------------------------------------- dense-array workload
+@pytest.fixture(scope="module")
+def dense():
+    from ccfj import synth
+    d = synth.noise_dense(real_only=True)
+    idx = np.argsort(d["r"])
+    return dict(U=np.ascontiguousarray(d["uf"][idx]), r=d["r"][idx].astype(np.float32),
+                f=d["f"].astype(np.float32), c=d["c"].astype(np.float32))
+
+
+DENSE_COLS = [7, 1500, 4000]
+
+
+@pytest.mark.parametrize("itype", [0, 1])
+@pytest.mark.parametrize("func", [0, 1])
+def test_dense_full_size_sampled_columns(ccfj, ctx, dense, itype, func):
+    """19,900 pairs x 4096 f x 3000 c at full size on the GPU; three
+    frequency columns (low, mid, high) checked against the FP64 oracle."""
+    d = dense
+    if func == 0:
+        got = native(ctx, ccfj.MODE_J, itype, d["U"], d["r"], d["f"], d["c"])
+        want = O.oracle64(d["U"], d["r"], d["f"], d["c"], itype, 0, fidx=DENSE_COLS)
+    else:
+        H = ccfj.hilbert_rows(d["U"])
+        got = native(ctx, ccfj.MODE_J_MINUS_Y, itype, d["U"], d["r"], d["f"], d["c"], U1=H)
+        want = (O.oracle64(d["U"], d["r"], d["f"], d["c"], itype, 0, fidx=DENSE_COLS)
+                - O.oracle64(H, d["r"], d["f"], d["c"], itype, 1, fidx=DENSE_COLS))
+    emax, erms, mism = O.column_error(got[:, DENSE_COLS], want)
+    print(f"dense itype={itype} func={func}: emax={emax:.2e} erms={erms:.2e}")
+    assert mism == 0 and emax <= TOL
+    assert np.all(np.isfinite(got))
+
+
+def test_dense_linearity(ccfj, ctx, dense):
+    """Size-independent property at full size: F[U + 2V] = F[U] + 2 F[V]."""
+    d = dense
+    V = np.roll(d["U"], 1, axis=1)
+    a = native(ctx, ccfj.MODE_J, 1, d["U"] + 2 * V, d["r"], d["f"], d["c"]).astype(np.float64)
+    b = native(ctx, ccfj.MODE_J, 1, d["U"], d["r"], d["f"], d["c"]).astype(np.float64)
+    b += 2 * native(ctx, ccfj.MODE_J, 1, V, d["r"], d["f"], d["c"])
+    assert O.column_error(a, b)[0] < 2 * TOL
