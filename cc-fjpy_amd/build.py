@@ -23,7 +23,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 instructions are kept out of generated code
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-HIP_SOURCES = ["fj_kernels.hip", "ccfj_capi.hip", "fj_plain64.hip"]
+HIP_SOURCES = ["fj_kernels.hip", "ccfj_capi.hip", "fj_plain64.hip", "fj_batched.hip",
+               "ccfj_batch.hip"]
 
 
 def _run(cmd, **kw):

@@ -32,7 +32,7 @@ except ImportError as exc:  # the native extension is the product; no fallback
     ) from exc
 
 __all__ = ["fj_noise", "fj_earthquake", "MWFJ", "fj", "fh", "fhr", "fhi", "win",
-           "GetStationPairs", "hilbert_rows", "device_count"]
+           "GetStationPairs", "hilbert_rows", "device_count", "fj_noise_batch"]
 
 MODE_J = _ccfj.MODE_J
 MODE_Y = _ccfj.MODE_Y
@@ -236,6 +236,48 @@ def MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, fstride=1, itype=1, func=0
         out[i, :, :] = fj_earthquake(u0, r, c, f, fstride=fstride, itype=itype,
                                      func=func, num=num)
     return out
+
+
+def fj_noise_batch(ufs, rs, c, f, fstride=1, itype=1, func=0, num=-1):
+    """fj_noise over many independent sub-arrays in one persistent launch.
+
+    No reference counterpart (the reference transforms one array per call).
+    ufs, rs: sequences of per-sub-array spectra [nr_b, ncol] and distances
+    [nr_b]; all sub-arrays share c and f.  Each sub-array is sorted, staged
+    and combined exactly as fj_noise does.  Returns float32 [nb, nc, nf],
+    each sub-array normalised per frequency column.
+    """
+    if func not in (0, 1):
+        print('set func as 0 for Bessel function 1 for Hankel function')
+        return 0
+    nb = len(ufs)
+    if len(rs) != nb:
+        raise ValueError("ufs and rs differ in length")
+    Us, Hs, Rs = [], [], []
+    cc = ff = None
+    for uf, r in zip(ufs, rs):
+        r = np.asarray(r)
+        indx = np.argsort(r)
+        r = r[indx]
+        uf = np.asarray(uf)[indx]
+        U, rr, cc, ff = _stage(uf, r, c, f, fstride)
+        Us.append(U)
+        Rs.append(rr)
+        if func == 1:
+            Hs.append(_stage(hilbert_rows(uf), r, c, f, fstride)[0])
+    soff = np.zeros(nb + 1, np.int64)
+    soff[1:] = np.cumsum([len(r) for r in Rs])
+    if itype not in (0, 1):
+        out = np.zeros((nb, len(cc), len(ff)), np.float32)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            for b in range(nb):
+                _normalize_columns(out[b])
+        return out
+    U = np.ascontiguousarray(np.concatenate(Us, axis=0))
+    H = np.ascontiguousarray(np.concatenate(Hs, axis=0)) if func == 1 else None
+    R = np.ascontiguousarray(np.concatenate(Rs))
+    mode = MODE_J if func == 0 else MODE_J_MINUS_Y
+    return _context(num).transform_batched(mode, int(itype), True, soff, U, H, R, ff, cc)
 
 
 def GetStationPairs(nsta):

@@ -40,6 +40,18 @@ cdef extern from "ccfj.h" nogil:
     int ccfj_plain_fp64(int device, int ybasis, int itype, const float *U,
                         const float *r, const float *f, const float *c, int nr,
                         int nf, int nc, float *out, double *kernel_ms)
+    ctypedef struct ccfj_batch:
+        pass
+    int ccfj_batch_create(ccfj_ctx *ctx, int mode, int itype, int normalize, int nb,
+                          const long long *soff, int nf, int nc, ccfj_batch **plan)
+    int ccfj_batch_run(ccfj_batch *plan, const float *U0, const float *U1, const float *r,
+                       const float *f, const float *c, float *out, void *stream, int use_graph)
+    int ccfj_batch_tile(ccfj_batch *plan)
+    int ccfj_batch_destroy(ccfj_batch *plan)
+    int ccfj_transform_batched(ccfj_ctx *ctx, int mode, int itype, int normalize, int nb,
+                               const long long *soff, const float *U0, const float *U1,
+                               const float *r, const float *f, const float *c, int nf, int nc,
+                               float *out)
 
 MODE_J = CCFJ_MODE_J
 MODE_Y = CCFJ_MODE_Y
@@ -162,6 +174,31 @@ cdef class Context:
                                     <const float *>c, nr, nf, nc, <float *>out, <void *>stream)
         _check(rc, "ccfj_transform_dev")
 
+    def transform_batched(self, int mode, int itype, bint normalize, soff, U0, U1, r, f, c):
+        """Host arrays: soff int64 [nb+1], U0/U1 float32 [rows, nf], r float32
+        [rows], f [nf], c [nc] -> float32 [nb, nc, nf]."""
+        cdef long long[::1] so = soff
+        cdef float[:, ::1] u0 = U0
+        cdef float[:, ::1] u1 = U1 if U1 is not None else U0
+        cdef float[::1] rv = r
+        cdef float[::1] fv = f
+        cdef float[::1] cv = c
+        cdef int nb = so.shape[0] - 1, nf = fv.shape[0], nc = cv.shape[0]
+        if u0.shape[0] != so[nb] or u0.shape[1] != nf or rv.shape[0] != so[nb]:
+            raise ValueError("batched shapes do not match the offsets")
+        out = np.zeros((nb, nc, nf), np.float32)
+        cdef float[:, :, ::1] o = out
+        cdef int rc
+        with nogil:
+            rc = ccfj_transform_batched(self.ctx, mode, itype, normalize, nb, &so[0],
+                                        &u0[0, 0], &u1[0, 0], &rv[0], &fv[0], &cv[0],
+                                        nf, nc, &o[0, 0, 0])
+        _check(rc, "ccfj_transform_batched")
+        return out
+
+    def batch_plan(self, int mode, int itype, bint normalize, soff, int nf, int nc):
+        return BatchPlan(self, mode, itype, normalize, soff, nf, nc)
+
     def synchronize(self):
         cdef int rc
         with nogil:
@@ -177,3 +214,35 @@ cdef class Context:
         cdef int n = 0
         _check(ccfj_get_timing(self.ctx, &ms, &n), "ccfj_get_timing")
         return ms, n
+
+
+cdef class BatchPlan:
+    """A batched-transform plan on a Context (device pointers, optional
+    HIP-graph replay)."""
+    cdef ccfj_batch *plan
+    cdef Context owner
+
+    def __cinit__(self, Context ctx, int mode, int itype, bint normalize, soff, int nf, int nc):
+        cdef long long[::1] so = np.ascontiguousarray(soff, dtype=np.int64)
+        self.plan = NULL
+        self.owner = ctx
+        _check(ccfj_batch_create(ctx.ctx, mode, itype, normalize, so.shape[0] - 1, &so[0],
+                                 nf, nc, &self.plan), "ccfj_batch_create")
+
+    def __dealloc__(self):
+        if self.plan != NULL:
+            ccfj_batch_destroy(self.plan)
+            self.plan = NULL
+
+    def run(self, size_t U0, size_t U1, size_t r, size_t f, size_t c, size_t out,
+            size_t stream=0, bint use_graph=False):
+        cdef int rc
+        with nogil:
+            rc = ccfj_batch_run(self.plan, <const float *>U0, <const float *>U1,
+                                <const float *>r, <const float *>f, <const float *>c,
+                                <float *>out, <void *>stream, use_graph)
+        _check(rc, "ccfj_batch_run")
+
+    @property
+    def tile(self):
+        return ccfj_batch_tile(self.plan)

@@ -77,6 +77,32 @@ def noise_dense(seed=1, nf=4096, nc=3000, real_only=False, chunk=512, fidx=None)
     return dict(uf=uf, r=r, c=c, f=f, cm=cm)
 
 
+def subarrays(seed=3, nb=4096, nmin=20, nmax=400, nf=512, nc=1024, real_only=True):
+    """Many independent sub-arrays: nr_b ~ U{nmin..nmax}, r sorted in
+    [1, 150] km, f 0.02-1 Hz, c 1000-5000 m/s; two random monotone
+    decreasing dispersion curves per sub-array (cinf in [1800, 3600] m/s,
+    rise in [500, 2500] m/s, decay in [0.1, 0.6] Hz, amplitudes 1 and
+    U[0.3, 0.7]) plus noise sigma 0.1.  Returns lists of per-sub-array
+    spectra and distances, and the shared grids."""
+    rng = np.random.default_rng(seed)
+    f = np.linspace(0.02, 1.0, nf)
+    c = np.linspace(1000.0, 5000.0, nc)
+    ns = rng.integers(nmin, nmax + 1, nb)
+    ufs, rs = [], []
+    for n in ns:
+        r = np.sort(rng.uniform(1e3, 150e3, n))
+        acc = np.zeros((n, nf))
+        for amp in (1.0, rng.uniform(0.3, 0.7)):
+            cv = rng.uniform(1800, 3600) + rng.uniform(500, 2500) * np.exp(-f / rng.uniform(0.1, 0.6))
+            acc += amp * j0(2 * np.pi * f[None, :] * r[:, None] / cv[None, :])
+        acc += rng.normal(0, 0.1, acc.shape)
+        if not real_only:
+            acc = acc + 1j * rng.normal(0, 0.1, acc.shape)
+        ufs.append(acc.astype(np.float32 if real_only else np.complex64))
+        rs.append(r)
+    return dict(ufs=ufs, rs=rs, c=c, f=f, ns=ns)
+
+
 def term_evals(nr, nf, nc, calls=1):
     """Reference-equivalent term evaluations: nf * nc * nr per native call."""
     return float(nr) * nf * nc * calls

@@ -18,9 +18,10 @@
 #include <vector>
 
 #include "../../include/ccfj.h"
+#include "ccfj_internal.h"
 #include "fj_kernels.h"
 
-namespace {
+namespace ccfj_internal {
 
 thread_local std::string g_last_error;
 
@@ -38,6 +39,14 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(e == hipErrorOutOfMemory ? CCFJ_ENOMEM : CCFJ_EDEVICE, "%s: %s (%d)",
               what, hipGetErrorString(e), (int)e);
 }
+
+}  // namespace ccfj_internal
+
+namespace {
+
+using ccfj_internal::fail;
+using ccfj_internal::g_last_error;
+using ccfj_internal::hip_fail;
 
 // Fault injection for tests: CCFJ_FAIL_ALLOC_AT=N makes the N-th device
 // allocation of the process (1-based) fail with CCFJ_ENOMEM.
@@ -80,11 +89,9 @@ int ensure(ccfj_ctx* ctx, Slot s, size_t bytes) {
     ctx->buf[s] = nullptr;
     ctx->cap[s] = 0;
   }
-  if (injected_alloc_failure())
-    return fail(CCFJ_ENOMEM, "device allocation of %zu bytes failed (injected)", bytes);
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, bytes);
-  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  int rc = ccfj_internal::device_alloc(&p, bytes);
+  if (rc) return rc;
   ctx->buf[s] = p;
   ctx->cap[s] = bytes;
   return CCFJ_OK;
@@ -121,10 +128,11 @@ int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U
     if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
   } else {
     const int NC = fjk::coef_per_station(mode, itype);
+    const int nrp = fjk::padded(nr);
     int rc;
-    if ((rc = ensure(ctx, S_RR, sizeof(double) * nr))) return rc;
-    if ((rc = ensure(ctx, S_HH, sizeof(float) * nr))) return rc;
-    if ((rc = ensure(ctx, S_COEF, sizeof(float) * (size_t)nf * nr * NC))) return rc;
+    if ((rc = ensure(ctx, S_RR, sizeof(double) * (nrp + 8)))) return rc;
+    if ((rc = ensure(ctx, S_HH, sizeof(float) * (nrp + 8)))) return rc;
+    if ((rc = ensure(ctx, S_COEF, sizeof(float) * (size_t)nf * nrp * NC))) return rc;
     fjk::Problem p;
     p.U0 = U0;
     p.U1 = two_spectra(mode) ? U1 : U0;
@@ -132,6 +140,7 @@ int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U
     p.f = f;
     p.c = c;
     p.nr = nr;
+    p.nrp = nrp;
     p.nf = nf;
     p.nc = nc;
     p.rr = (double*)ctx->buf[S_RR];
@@ -192,6 +201,22 @@ int ref_entry(int mode, float* u_f, float* r, float* f, float* out, float* c, in
 }
 
 }  // namespace
+
+namespace ccfj_internal {
+
+int device_alloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (injected_alloc_failure())
+    return fail(CCFJ_ENOMEM, "device allocation of %zu bytes failed (injected)", bytes);
+  hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  return CCFJ_OK;
+}
+
+int ctx_device(const ccfj_ctx* ctx) { return ctx->device; }
+hipStream_t ctx_stream(const ccfj_ctx* ctx) { return ctx->stream; }
+
+}  // namespace ccfj_internal
 
 extern "C" {
 

@@ -1,0 +1,17 @@
+// ccfj_internal.h -- helpers shared by the host-side sources of libccfj.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/ccfj.h"
+
+namespace ccfj_internal {
+
+// Record an error message for ccfj_last_error() and return code.
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+// Device allocation honouring the CCFJ_FAIL_ALLOC_AT fault injection.
+int device_alloc(void** p, size_t bytes);
+int ctx_device(const ccfj_ctx* ctx);
+hipStream_t ctx_stream(const ccfj_ctx* ctx);
+
+}  // namespace ccfj_internal

@@ -1,0 +1,222 @@
+// fj_batched.hip -- many independent sub-arrays of uneven size in one launch.
+//
+// No reference counterpart: the reference transforms one array per call
+// (src/ccfj.pyx:202-227, one FJ/FH call each).  Here nb sub-arrays that
+// share the frequency and velocity grids are prepared by two batched prep
+// kernels and evaluated by ONE persistent kernel: a grid sized to the
+// device's resident capacity pulls work items (sub-array, frequency,
+// velocity tile) from an atomic counter, largest sub-arrays first, so the
+// tail of the launch is made of the cheapest items.  The per-(f,c) work is
+// the same lane code as the single-array kernels (fj_lane.h).
+//
+// Layout (device):
+//   U0/U1  f32 [sum nr_b][nf]        rows of all sub-arrays, each sorted by r
+//   r      f32 [sum nr_b]
+//   rr, hh per sub-array b at poff[b], nrp_b + 8 entries each (padded)
+//   coef   per sub-array b at coff[b]: [nf][nrp_b][NC]
+//   out    f32 [nb][nc][nf]
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fj_batched.h"
+#include "fj_lane.h"
+
+namespace fjk {
+
+using namespace fjl;
+
+__global__ void k_stations_b(BatchDev d) {
+  const int b = blockIdx.y;
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t s0 = d.soff[b];
+  const int nr = (int)(d.soff[b + 1] - s0);
+  const int ntot = d.nrp[b] + kUnroll;
+  if (s >= ntot || nr <= 0) return;
+  double* rr = d.rr + d.poff[b];
+  float* hh = d.hh + d.poff[b];
+  const float* r = d.r + s0;
+  if (s < nr) {
+    rr[s] = (double)r[s];
+    hh[s] = s > 0 ? r[s] - r[s - 1] : 0.0f;
+  } else {
+    rr[s] = (double)r[nr - 1];
+    hh[s] = 0.0f;
+  }
+}
+
+template <int ITYPE, int NSPEC>
+__global__ __launch_bounds__(256) void k_prep_b(BatchDev d) {
+  constexpr int TS = 64, TF = 64, LD = TF + 1;
+  constexpr int NC = ITYPE == 0 ? NSPEC : 2 * NSPEC;
+  __shared__ float tile[NSPEC][TS + 1][LD];
+  const int b = blockIdx.z;
+  const int64_t s00 = d.soff[b];
+  const int nr = (int)(d.soff[b + 1] - s00);
+  const int nrp = d.nrp[b];
+  const int nf = d.nf;
+  const int f0 = blockIdx.x * TF;
+  const int s0 = blockIdx.y * TS;
+  if (s0 >= nrp || nr <= 0) return;   // uniform per block: no barrier skipped partially
+  const float* U0 = d.U0 + s00 * nf;
+  const float* U1 = d.U1 + s00 * nf;
+  const float* r = d.r + s00;
+  float* coef = d.coef + d.coff[b];
+  const int lane = threadIdx.x & 63;
+  const int grp = threadIdx.x >> 6;
+  for (int row = grp; row < TS + 1; row += 4) {
+    const int s = s0 - 1 + row;
+    const int f = f0 + lane;
+    const bool ok = s >= 0 && s < nr && f < nf;
+    const size_t idx = ok ? (size_t)s * nf + f : 0;
+    tile[0][row][lane] = ok ? U0[idx] : 0.0f;
+    if (NSPEC > 1) tile[NSPEC - 1][row][lane] = ok ? U1[idx] : 0.0f;
+  }
+  __syncthreads();
+  for (int it = threadIdx.x; it < TS * TF; it += 256) {
+    const int sl = it & (TS - 1);
+    const int fl = it >> 6;
+    const int s = s0 + sl, f = f0 + fl;
+    if (s >= nrp || f >= nf) continue;
+    float* dst = coef + ((size_t)f * nrp + s) * NC;
+    if (s >= nr) {
+#pragma unroll
+      for (int k = 0; k < NC; k++) dst[k] = 0.0f;
+      continue;
+    }
+    float gp[NSPEC], gc[NSPEC], rec[NC];
+    for (int k = 0; k < NSPEC; k++) {
+      gp[k] = tile[k][sl][fl];
+      gc[k] = tile[k][sl + 1][fl];
+    }
+    const float rp = s > 0 ? r[s - 1] : 0.0f;
+    const float rn = s + 1 < nr ? r[s + 1] : 0.0f;
+    station_coef<ITYPE, NSPEC>(s, nr, rp, r[s], rn, gp, gc, rec);
+#pragma unroll
+    for (int k = 0; k < NC; k++) dst[k] = rec[k];
+  }
+}
+
+__global__ void k_reset_counter(unsigned int* counter) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+}
+
+// Persistent F-J over work items (p, fi, cb): p indexes d.order (sub-arrays
+// by descending size), cb a tile of TILE velocities.  Items [0, limit) are
+// processed; every workgroup leaves when the counter passes limit.
+template <int MODE, int ITYPE, int TILE>
+__global__ __launch_bounds__(TILE) void k_fj_b(BatchDev d, unsigned int limit) {
+  constexpr int NC = ITYPE == 0 ? ModeT<MODE>::nspec : 2 * ModeT<MODE>::nspec;
+  __shared__ unsigned int item_s;
+  const int nf = d.nf, nc = d.nc;
+  const unsigned int ncb = (unsigned int)((nc + TILE - 1) / TILE);
+  while (true) {
+    if (threadIdx.x == 0) item_s = atomicAdd(d.counter, 1u);
+    __syncthreads();
+    const unsigned int item = item_s;
+    __syncthreads();   // every lane has read item_s before it is overwritten
+    if (item >= limit) break;
+    const unsigned int cb = item % ncb;
+    const unsigned int pf = item / ncb;
+    const int fi = (int)(pf % (unsigned int)nf);
+    const int b = d.order[pf / (unsigned int)nf];
+    const int j = (int)cb * TILE + (int)threadIdx.x;
+    const int64_t s0 = d.soff[b];
+    const int nr = (int)(d.soff[b + 1] - s0);
+    if (j < nc && nr > 0) {
+      const int nrp = d.nrp[b];
+      const double q = (double)d.f[fi] / (double)d.c[j];
+      const float* __restrict__ cf = d.coef + d.coff[b] + (size_t)fi * nrp * NC;
+      const double* __restrict__ rr = d.rr + d.poff[b];
+      float v;
+      if (ITYPE == 0)
+        v = trapz_lane<MODE>(rr, cf, nrp, q);
+      else
+        v = linseg_lane<MODE>(rr, d.hh + d.poff[b], cf, nrp, q);
+      d.out[((size_t)b * nc + j) * nf + fi] = v;
+    } else if (j < nc) {
+      d.out[((size_t)b * nc + j) * nf + fi] = 0.0f;
+    }
+  }
+}
+
+// per-(sub-array, column) normalisation, builtin-max semantics
+__global__ void k_normalize_b(float* __restrict__ out, int nf, int nc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf || nc <= 0) return;
+  float* o = out + (size_t)blockIdx.y * nc * nf;
+  float m = fabsf(o[i]);
+  for (int j = 1; j < nc; j++) {
+    const float v = fabsf(o[(size_t)j * nf + i]);
+    if (v > m) m = v;
+  }
+  for (int j = 0; j < nc; j++) {
+    const size_t idx = (size_t)j * nf + i;
+    o[idx] = o[idx] / m;
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <int ITYPE, int NSPEC>
+static hipError_t prep_t(const BatchDev& d, int max_nrp, hipStream_t st) {
+  dim3 grid((d.nf + 63) / 64, (max_nrp + 63) / 64, d.nb);
+  hipLaunchKernelGGL((k_prep_b<ITYPE, NSPEC>), grid, dim3(256), 0, st, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep_batched(int mode, int itype, const BatchDev& d, int max_nrp,
+                               hipStream_t st) {
+  const int mt = max_nrp + kUnroll;
+  hipLaunchKernelGGL(k_stations_b, dim3((mt + 255) / 256, d.nb), dim3(256), 0, st, d);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const bool two = !(mode == MODE_J || mode == MODE_Y);
+  if (itype == 0) return two ? prep_t<0, 2>(d, max_nrp, st) : prep_t<0, 1>(d, max_nrp, st);
+  return two ? prep_t<1, 2>(d, max_nrp, st) : prep_t<1, 1>(d, max_nrp, st);
+}
+
+template <int MODE, int ITYPE, int TILE>
+static const void* kernel_ptr() {
+  return reinterpret_cast<const void*>(&k_fj_b<MODE, ITYPE, TILE>);
+}
+
+template <int MODE, int ITYPE>
+static const void* pick(int tile) {
+  switch (tile) {
+    case 64: return kernel_ptr<MODE, ITYPE, 64>();
+    case 128: return kernel_ptr<MODE, ITYPE, 128>();
+    default: return kernel_ptr<MODE, ITYPE, 256>();
+  }
+}
+
+const void* batched_kernel(int mode, int itype, int tile) {
+  switch (mode * 2 + itype) {
+    case MODE_J * 2 + 0: return pick<MODE_J, 0>(tile);
+    case MODE_J * 2 + 1: return pick<MODE_J, 1>(tile);
+    case MODE_Y * 2 + 0: return pick<MODE_Y, 0>(tile);
+    case MODE_Y * 2 + 1: return pick<MODE_Y, 1>(tile);
+    case MODE_J_MINUS_Y * 2 + 0: return pick<MODE_J_MINUS_Y, 0>(tile);
+    case MODE_J_MINUS_Y * 2 + 1: return pick<MODE_J_MINUS_Y, 1>(tile);
+    case MODE_ABS_J * 2 + 0: return pick<MODE_ABS_J, 0>(tile);
+    case MODE_ABS_J * 2 + 1: return pick<MODE_ABS_J, 1>(tile);
+    case MODE_ABS_H * 2 + 0: return pick<MODE_ABS_H, 0>(tile);
+    case MODE_ABS_H * 2 + 1: return pick<MODE_ABS_H, 1>(tile);
+  }
+  return nullptr;
+}
+
+hipError_t launch_batched(const void* kern, int tile, int grid, const BatchDev& d,
+                          unsigned int limit, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, d.counter);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  BatchDev dd = d;
+  void* args[] = {&dd, &limit};
+  return hipLaunchKernel(kern, dim3(grid), dim3(tile), args, 0, st);
+}
+
+hipError_t launch_normalize_batched(float* out, int nb, int nf, int nc, hipStream_t st) {
+  hipLaunchKernelGGL(k_normalize_b, dim3((nf + 255) / 256, nb), dim3(256), 0, st, out, nf, nc);
+  return hipGetLastError();
+}
+
+}  // namespace fjk

@@ -22,13 +22,15 @@ void run_mode(int itype, const float* U0, const float* U1, const float* r, const
               const float* c, int nr, int nf, int nc, float* out) {
   const int NS = ModeT<MODE>::nspec;
   const int NC = coef_count(MODE, itype);
-  std::vector<double> rr(nr);
-  std::vector<float> hh(nr);
+  const int nrp = padded_stations(nr);
+  // station arrays padded as on the device
+  std::vector<double> rr(nrp + kUnroll, (double)r[nr - 1]);
+  std::vector<float> hh(nrp + kUnroll, 0.0f);
   for (int s = 0; s < nr; s++) {
     rr[s] = r[s];
     hh[s] = s > 0 ? r[s] - r[s - 1] : 0.0f;
   }
-  std::vector<float> cf((size_t)nr * NC);
+  std::vector<float> cf((size_t)nrp * NC, 0.0f);
 #pragma omp parallel for schedule(dynamic, 1) firstprivate(cf)
   for (int i = 0; i < nf; i++) {
     for (int s = 0; s < nr; s++) {
@@ -51,8 +53,8 @@ void run_mode(int itype, const float* U0, const float* U1, const float* r, const
     }
     for (int j = 0; j < nc; j++) {
       const double q = (double)f[i] / (double)c[j];
-      out[(size_t)j * nf + i] = itype == 0 ? trapz_lane<MODE>(rr.data(), cf.data(), nr, q)
-                                           : linseg_lane<MODE>(rr.data(), hh.data(), cf.data(), nr, q);
+      out[(size_t)j * nf + i] = itype == 0 ? trapz_lane<MODE>(rr.data(), cf.data(), nrp, q)
+                                           : linseg_lane<MODE>(rr.data(), hh.data(), cf.data(), nrp, q);
     }
   }
 }

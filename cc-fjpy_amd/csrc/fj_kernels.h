@@ -15,13 +15,15 @@ struct Problem {
   const float* f;
   const float* c;
   int nr, nf, nc;
-  double* rr;    // [nr] workspace
-  float* hh;     // [nr] workspace
-  float* coef;   // [nf][nr][NC] workspace
+  int nrp;       // station count padded to the kernels' block (fjl::padded_stations)
+  double* rr;    // [nrp + 8] workspace
+  float* hh;     // [nrp + 8] workspace
+  float* coef;   // [nf][nrp][NC] workspace
   float* out;    // [nc][nf]
 };
 
 int coef_per_station(int mode, int itype);
+int padded(int nr);  // station count padded to the kernels' block
 hipError_t launch_prep(int mode, int itype, const Problem& p, hipStream_t st);
 hipError_t launch_fj(int mode, int itype, const Problem& p, hipStream_t st);
 hipError_t launch_normalize(float* out, int nf, int nc, hipStream_t st);

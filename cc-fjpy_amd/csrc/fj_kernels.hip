@@ -3,10 +3,12 @@
 // The per-(f,c) arithmetic lives in fj_lane.h (shared with the host build);
 // this file maps it onto the GPU.
 //
-// Data layout in HBM (prepared on the device by k_stations and k_prep):
-//   rr[nr]            double   station distances, ascending
-//   hh[nr]            float    r_s - r_{s-1} (hh[0] = 0)
-//   coef[nf][nr][NC]  float    per (frequency, station) coefficients:
+// Data layout in HBM (prepared on the device by k_stations and k_prep;
+// nrp = nr rounded up to the station block of 8, padding = last distance
+// with zero coefficients and zero gaps):
+//   rr[nrp + 8]       double   station distances, ascending
+//   hh[nrp + 8]       float    r_s - r_{s-1} (hh[0] = 0)
+//   coef[nf][nrp][NC] float    per (frequency, station) coefficients:
 //       itype 0: A_s of each spectrum          (NC = nspec)
 //       itype 1: alpha_s, beta_s per spectrum  (NC = 2 nspec)
 //   out[nc][nf]       float    (the reference's layout, src/FJcpu.cpp:247)
@@ -41,13 +43,19 @@ __device__ inline bool tile_of(int ntiles, int* tile) {
   return T < ntiles;
 }
 
-// station arrays: rr (double), hh (float gap)
-__global__ void k_stations(const float* __restrict__ r, int nr, double* __restrict__ rr,
-                           float* __restrict__ hh) {
+// station arrays: rr (double), hh (float gap), padded to ntot entries with
+// the last distance and zero gaps
+__global__ void k_stations(const float* __restrict__ r, int nr, int ntot,
+                           double* __restrict__ rr, float* __restrict__ hh) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= nr) return;
-  rr[s] = (double)r[s];
-  hh[s] = s > 0 ? r[s] - r[s - 1] : 0.0f;
+  if (s >= ntot) return;
+  if (s < nr) {
+    rr[s] = (double)r[s];
+    hh[s] = s > 0 ? r[s] - r[s - 1] : 0.0f;
+  } else {
+    rr[s] = (double)r[nr - 1];
+    hh[s] = 0.0f;
+  }
 }
 
 // Coefficients + [nr][nf] -> [nf][nr][NC] transpose through LDS.
@@ -56,8 +64,8 @@ __global__ void k_stations(const float* __restrict__ r, int nr, double* __restri
 template <int ITYPE, int NSPEC>
 __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ U0,
                                               const float* __restrict__ U1,
-                                              const float* __restrict__ r, int nr, int nf,
-                                              float* __restrict__ coef) {
+                                              const float* __restrict__ r, int nr, int nrp,
+                                              int nf, float* __restrict__ coef) {
   constexpr int TS = 64, TF = 64, LD = TF + 1;
   constexpr int NC = ITYPE == 0 ? NSPEC : 2 * NSPEC;
   __shared__ float tile[NSPEC][TS + 1][LD];
@@ -78,7 +86,13 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ U0,
     const int sl = it & (TS - 1);
     const int fl = it >> 6;
     const int s = s0 + sl, f = f0 + fl;
-    if (s >= nr || f >= nf) continue;
+    if (s >= nrp || f >= nf) continue;
+    float* dst = coef + ((size_t)f * nrp + s) * NC;
+    if (s >= nr) {  // padding stations contribute nothing
+#pragma unroll
+      for (int k = 0; k < NC; k++) dst[k] = 0.0f;
+      continue;
+    }
     float gp[NSPEC], gc[NSPEC], rec[NC];
     for (int k = 0; k < NSPEC; k++) {
       gp[k] = tile[k][sl][fl];
@@ -87,7 +101,6 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ U0,
     const float rp = s > 0 ? r[s - 1] : 0.0f;
     const float rn = s + 1 < nr ? r[s + 1] : 0.0f;
     station_coef<ITYPE, NSPEC>(s, nr, rp, r[s], rn, gp, gc, rec);
-    float* dst = coef + ((size_t)f * nr + s) * NC;
 #pragma unroll
     for (int k = 0; k < NC; k++) dst[k] = rec[k];
   }
@@ -98,7 +111,7 @@ __global__ __launch_bounds__(kBlock) void k_fj(const double* __restrict__ rr,
                                               const float* __restrict__ hh,
                                               const float* __restrict__ coef,
                                               const float* __restrict__ fv,
-                                              const float* __restrict__ cv, int nr, int nf,
+                                              const float* __restrict__ cv, int nrp, int nf,
                                               int nc, float* __restrict__ out) {
   constexpr int NC = ITYPE == 0 ? ModeT<MODE>::nspec : 2 * ModeT<MODE>::nspec;
   const int ncb = (nc + kBlock - 1) / kBlock;
@@ -108,12 +121,12 @@ __global__ __launch_bounds__(kBlock) void k_fj(const double* __restrict__ rr,
   const int j = (T - fi * ncb) * kBlock + threadIdx.x;
   if (j >= nc) return;
   const double q = (double)fv[fi] / (double)cv[j];
-  const float* __restrict__ cf = coef + (size_t)fi * nr * NC;
+  const float* __restrict__ cf = coef + (size_t)fi * nrp * NC;
   float v;
   if (ITYPE == 0)
-    v = trapz_lane<MODE>(rr, cf, nr, q);
+    v = trapz_lane<MODE>(rr, cf, nrp, q);
   else
-    v = linseg_lane<MODE>(rr, hh, cf, nr, q);
+    v = linseg_lane<MODE>(rr, hh, cf, nrp, q);
   out[(size_t)j * nf + fi] = v;
 }
 
@@ -142,26 +155,29 @@ static hipError_t launch_main(int itype, const Problem& p, hipStream_t st) {
   const long nblk = ((ntiles + 7) / 8) * 8;
   if (itype == 0)
     hipLaunchKernelGGL((k_fj<MODE, 0>), dim3((unsigned)nblk), dim3(kBlock), 0, st, p.rr, p.hh,
-                       p.coef, p.f, p.c, p.nr, p.nf, p.nc, p.out);
+                       p.coef, p.f, p.c, p.nrp, p.nf, p.nc, p.out);
   else
     hipLaunchKernelGGL((k_fj<MODE, 1>), dim3((unsigned)nblk), dim3(kBlock), 0, st, p.rr, p.hh,
-                       p.coef, p.f, p.c, p.nr, p.nf, p.nc, p.out);
+                       p.coef, p.f, p.c, p.nrp, p.nf, p.nc, p.out);
   return hipGetLastError();
 }
 
 template <int ITYPE, int NSPEC>
 static hipError_t launch_prep_t(const Problem& p, hipStream_t st) {
-  dim3 grid((p.nf + 63) / 64, (p.nr + 63) / 64);
+  dim3 grid((p.nf + 63) / 64, (p.nrp + 63) / 64);
   hipLaunchKernelGGL((k_prep<ITYPE, NSPEC>), grid, dim3(256), 0, st, p.U0, p.U1, p.r, p.nr,
-                     p.nf, p.coef);
+                     p.nrp, p.nf, p.coef);
   return hipGetLastError();
 }
 
 int coef_per_station(int mode, int itype) { return coef_count(mode, itype); }
 
+int padded(int nr) { return padded_stations(nr); }
+
 hipError_t launch_prep(int mode, int itype, const Problem& p, hipStream_t st) {
-  hipLaunchKernelGGL(k_stations, dim3((p.nr + 255) / 256), dim3(256), 0, st, p.r, p.nr, p.rr,
-                     p.hh);
+  const int ntot = p.nrp + kUnroll;
+  hipLaunchKernelGGL(k_stations, dim3((ntot + 255) / 256), dim3(256), 0, st, p.r, p.nr, ntot,
+                     p.rr, p.hh);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const bool two = !(mode == MODE_J || mode == MODE_Y);

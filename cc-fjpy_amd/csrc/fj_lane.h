@@ -42,6 +42,13 @@ constexpr int MODE_ABS_J = CCFJ_MODE_ABS_J;
 constexpr int MODE_ABS_H = CCFJ_MODE_ABS_H;
 
 constexpr int kPartial = 32;  // stations per float partial sum
+// Station loops run in blocks of kUnroll so the wave-uniform station data of
+// a block are fetched by a few wide scalar loads.  The station arrays are
+// padded to a multiple of kUnroll (plus one spare block for the gap array)
+// with copies of the last distance and zero coefficients, which add nothing.
+constexpr int kUnroll = 8;
+
+FJ_HD inline int padded_stations(int nr) { return (nr + kUnroll - 1) / kUnroll * kUnroll; }
 // A segment [x_a, x_b] uses the G difference when |dx| >= min(kTau,
 // kKappa min(x_a, x_b)), else the Euler-Maclaurin rule.  kTau bounds the
 // rule's truncation (dx^6/30240) for oscillating g; kKappa bounds it near
@@ -51,7 +58,7 @@ constexpr float kTau = 0.5f;
 constexpr float kKappa = 0.25f;
 
 FJ_HD inline bool use_gdiff(float xa, float xb, float dx) {
-  return fabsf(dx) >= fminf(kTau, kKappa * fminf(xa, xb));
+  return dx != 0.0f && fabsf(dx) >= fminf(kTau, kKappa * fminf(xa, xb));
 }
 
 template <int MODE>
@@ -105,46 +112,58 @@ FJ_HD inline void station_coef(int s, int nr, float rp, float rs, float rn,
 
 // ---- itype 0 ---------------------------------------------------------------
 // q = f/c in cycles per metre (double).  cf: coefficient records of one
-// frequency, NC = nspec floats per station.
+// frequency, NC = nspec floats per station; nrp = padded station count.
+template <int MODE>
+FJ_HD inline void trapz_station(double rs, double q, double q2pi, const float* c,
+                                float& pR, float& pI) {
+  using M = ModeT<MODE>;
+  const double xd = q2pi * rs;
+  const float x = (float)xd;
+  float J0 = 0.0f, Y0 = 0.0f;
+  if (x >= kSplit) {
+    bessel0_large<M::J, M::Y>(x, reduce_phase(q, rs), &J0, &Y0);
+  } else {
+    double y, L;
+    Bess b = bessel_small<false, M::Y>(xd, &y, &L);
+    J0 = b.j0;
+    Y0 = b.y0;
+  }
+  if (MODE == MODE_J) {
+    pR = __builtin_fmaf(c[0], J0, pR);
+  } else if (MODE == MODE_Y) {
+    pR = __builtin_fmaf(c[0], Y0, pR);
+  } else if (MODE == MODE_J_MINUS_Y) {
+    pR = __builtin_fmaf(c[0], J0, pR);
+    pR = __builtin_fmaf(-c[1], Y0, pR);
+  } else if (MODE == MODE_ABS_J) {
+    pR = __builtin_fmaf(c[0], J0, pR);
+    pI = __builtin_fmaf(c[1], J0, pI);
+  } else {  // MODE_ABS_H: (FJ[re]-FY[im]) + i (FJ[im]+FY[re])
+    pR = __builtin_fmaf(c[0], J0, pR);
+    pR = __builtin_fmaf(-c[1], Y0, pR);
+    pI = __builtin_fmaf(c[1], J0, pI);
+    pI = __builtin_fmaf(c[0], Y0, pI);
+  }
+}
+
 template <int MODE>
 FJ_HD inline float trapz_lane(const double* __restrict__ rr, const float* __restrict__ cf,
-                              int nr, double q) {
-  using M = ModeT<MODE>;
+                              int nrp, double q) {
+  constexpr int NC = ModeT<MODE>::nspec;
   const double q2pi = q * kTwoPi;
   double accR = 0.0, accI = 0.0;
-  for (int s0 = 0; s0 < nr; s0 += kPartial) {
-    const int s1 = nr < s0 + kPartial ? nr : s0 + kPartial;
+  for (int s0 = 0; s0 < nrp; s0 += kPartial) {
+    const int s1 = nrp < s0 + kPartial ? nrp : s0 + kPartial;
     float pR = 0.0f, pI = 0.0f;
-    for (int s = s0; s < s1; ++s) {
-      const double rs = rr[s];
-      const double xd = q2pi * rs;
-      const float x = (float)xd;
-      float J0 = 0.0f, Y0 = 0.0f;
-      if (x >= kSplit) {
-        bessel0_large<M::J, M::Y>(x, reduce_phase(q, rs), &J0, &Y0);
-      } else {
-        double y, L;
-        Bess b = bessel_small<false, M::Y>(xd, &y, &L);
-        J0 = b.j0;
-        Y0 = b.y0;
-      }
-      if (MODE == MODE_J) {
-        pR = __builtin_fmaf(cf[s], J0, pR);
-      } else if (MODE == MODE_Y) {
-        pR = __builtin_fmaf(cf[s], Y0, pR);
-      } else if (MODE == MODE_J_MINUS_Y) {
-        pR = __builtin_fmaf(cf[2 * s], J0, pR);
-        pR = __builtin_fmaf(-cf[2 * s + 1], Y0, pR);
-      } else if (MODE == MODE_ABS_J) {
-        pR = __builtin_fmaf(cf[2 * s], J0, pR);
-        pI = __builtin_fmaf(cf[2 * s + 1], J0, pI);
-      } else {  // MODE_ABS_H: (FJ[re]-FY[im]) + i (FJ[im]+FY[re])
-        const float a = cf[2 * s], b = cf[2 * s + 1];
-        pR = __builtin_fmaf(a, J0, pR);
-        pR = __builtin_fmaf(-b, Y0, pR);
-        pI = __builtin_fmaf(b, J0, pI);
-        pI = __builtin_fmaf(a, Y0, pI);
-      }
+    for (int sb = s0; sb < s1; sb += kUnroll) {
+      double rv[kUnroll];
+      float cv[kUnroll * NC];
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) rv[k] = rr[sb + k];
+#pragma unroll
+      for (int k = 0; k < kUnroll * NC; k++) cv[k] = cf[sb * NC + k];
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) trapz_station<MODE>(rv[k], q, q2pi, cv + k * NC, pR, pI);
     }
     accR += (double)pR;
     accI += (double)pI;
@@ -183,86 +202,115 @@ FJ_HD inline float gdiff_y(const Ends& a, float xa, const Ends& b, float xb, flo
 
 // cf: NC = 2 nspec floats per station: (alpha, beta) per spectrum.
 template <int MODE>
-FJ_HD inline float linseg_lane(const double* __restrict__ rr, const float* __restrict__ hh,
-                               const float* __restrict__ cf, int nr, double q) {
+struct LinState {
+  Ends pj = {0.f, 0.f, 0.f, 0.f, 0.f}, py = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float xprev = 0.0f;
+  float pR = 0.0f, pI = 0.0f;
+};
+
+// One station of the linear-segment sum.  dxL = k (r_s - r_{s-1}),
+// dxR = k (r_{s+1} - r_s) (0 past the last station), c = alpha/beta record.
+template <int MODE>
+FJ_HD inline void linseg_station(int s, double rs, float dxL, float dxR, double q, double q2pi,
+                                 const float* c, LinState<MODE>& st) {
   using M = ModeT<MODE>;
   constexpr bool NJ = M::J, NY = M::Y;
-  const double q2pi = q * kTwoPi;
-  const float kf = (float)q2pi;
-  Ends pj = {0.f, 0.f, 0.f, 0.f, 0.f}, py = {0.f, 0.f, 0.f, 0.f, 0.f};
-  double accR = 0.0, accI = 0.0;
-  float pR = 0.0f, pI = 0.0f;
-  float dxR = 0.0f, xprev = 0.0f;
-  for (int s = 0; s < nr; ++s) {
-    const double rs = rr[s];
-    const double xd = q2pi * rs;
-    const float x = (float)xd;
-    const float dxL = dxR;                     // k (r_s - r_{s-1})
-    dxR = s + 1 < nr ? kf * hh[s + 1] : 0.0f;  // k (r_{s+1} - r_s)
-    const bool useG_L = s > 0 && use_gdiff(xprev, x, dxL);
-    const bool needG = useG_L || (s + 1 < nr && use_gdiff(x, x + dxR, dxR));
-    Ends cj = {0.f, 0.f, 0.f, 0.f, 0.f}, cy = {0.f, 0.f, 0.f, 0.f, 0.f};
-    Bess b;
-    if (x >= kSplit) {
-      float t;
-      b = bessel_large<NJ, NY>(x, reduce_phase(q, rs), &t);
-      if (needG) g_large(x, t, b, &cj.G, &cy.G, NJ, NY);
+  const double xd = q2pi * rs;
+  const float x = (float)xd;
+  const bool useG_L = s > 0 && use_gdiff(st.xprev, x, dxL);
+  const bool needG = useG_L || use_gdiff(x, x + dxR, dxR);
+  Ends cj = {0.f, 0.f, 0.f, 0.f, 0.f}, cy = {0.f, 0.f, 0.f, 0.f, 0.f};
+  Bess b;
+  if (x >= kSplit) {
+    float t;
+    b = bessel_large<NJ, NY>(x, reduce_phase(q, rs), &t);
+    if (needG) g_large(x, t, b, &cj.G, &cy.G, NJ, NY);
+  } else {
+    double y, L;
+    b = bessel_small<true, NY>(xd, &y, &L);
+    if (needG) g_small(xd, y, L, NJ, NY, &cj.G, &cy.G);
+    cy.sh = 1.0f;
+  }
+  if (NJ) {
+    cj.g = x * b.j1;
+    cj.gp = x * b.j0;
+    cj.g3 = -b.j1 - cj.gp;
+  }
+  if (NY) {
+    cy.g = b.gy;
+    cy.gp = x * b.y0;
+    cy.g3 = -b.y1 - cy.gp;
+  }
+  float Dj = 0.0f, Dy = 0.0f;
+  if (s > 0) {
+    if (useG_L) {
+      const float inv = rcp_f(dxL);
+      if (NJ) Dj = (cj.G - st.pj.G) * inv;
+      if (NY) Dy = gdiff_y(st.py, st.xprev, cy, x, inv);
     } else {
-      double y, L;
-      b = bessel_small<true, NY>(xd, &y, &L);
-      if (needG) g_small(xd, y, L, NJ, NY, &cj.G, &cy.G);
-      cy.sh = 1.0f;
-    }
-    if (NJ) {
-      cj.g = x * b.j1;
-      cj.gp = x * b.j0;
-      cj.g3 = -b.j1 - cj.gp;
-    }
-    if (NY) {
-      cy.g = b.gy;
-      cy.gp = x * b.y0;
-      cy.g3 = -b.y1 - cy.gp;
-    }
-    float Dj = 0.0f, Dy = 0.0f;
-    if (s > 0) {
-      if (useG_L) {
-        const float inv = rcp_f(dxL);
-        if (NJ) Dj = (cj.G - pj.G) * inv;
-        if (NY) Dy = gdiff_y(py, xprev, cy, x, inv);
-      } else {
-        if (NJ) Dj = em_divided(pj, cj, dxL);
-        if (NY) Dy = em_divided(py, cy, dxL);
-      }
-    }
-    xprev = x;
-    // station s contributes alpha*g - beta*D per spectrum
-    if (MODE == MODE_J) {
-      pR += cf[2 * s] * cj.g - cf[2 * s + 1] * Dj;
-    } else if (MODE == MODE_Y) {
-      pR += cf[2 * s] * cy.g - cf[2 * s + 1] * Dy;
-    } else if (MODE == MODE_J_MINUS_Y) {
-      pR += (cf[4 * s] * cj.g - cf[4 * s + 1] * Dj) - (cf[4 * s + 2] * cy.g - cf[4 * s + 3] * Dy);
-    } else if (MODE == MODE_ABS_J) {
-      pR += cf[4 * s] * cj.g - cf[4 * s + 1] * Dj;
-      pI += cf[4 * s + 2] * cj.g - cf[4 * s + 3] * Dj;
-    } else {  // MODE_ABS_H
-      const float jre = cf[4 * s] * cj.g - cf[4 * s + 1] * Dj;
-      const float jim = cf[4 * s + 2] * cj.g - cf[4 * s + 3] * Dj;
-      const float yre = cf[4 * s] * cy.g - cf[4 * s + 1] * Dy;
-      const float yim = cf[4 * s + 2] * cy.g - cf[4 * s + 3] * Dy;
-      pR += jre - yim;
-      pI += jim + yre;
-    }
-    pj = cj;
-    py = cy;
-    if ((s & (kPartial - 1)) == kPartial - 1) {
-      accR += (double)pR;
-      accI += (double)pI;
-      pR = pI = 0.0f;
+      if (NJ) Dj = em_divided(st.pj, cj, dxL);
+      if (NY) Dy = em_divided(st.py, cy, dxL);
     }
   }
-  accR += (double)pR;
-  accI += (double)pI;
+  st.xprev = x;
+  // station s contributes alpha*g - beta*D per spectrum
+  if (MODE == MODE_J) {
+    st.pR += c[0] * cj.g - c[1] * Dj;
+  } else if (MODE == MODE_Y) {
+    st.pR += c[0] * cy.g - c[1] * Dy;
+  } else if (MODE == MODE_J_MINUS_Y) {
+    st.pR += (c[0] * cj.g - c[1] * Dj) - (c[2] * cy.g - c[3] * Dy);
+  } else if (MODE == MODE_ABS_J) {
+    st.pR += c[0] * cj.g - c[1] * Dj;
+    st.pI += c[2] * cj.g - c[3] * Dj;
+  } else {  // MODE_ABS_H
+    const float jre = c[0] * cj.g - c[1] * Dj;
+    const float jim = c[2] * cj.g - c[3] * Dj;
+    const float yre = c[0] * cy.g - c[1] * Dy;
+    const float yim = c[2] * cy.g - c[3] * Dy;
+    st.pR += jre - yim;
+    st.pI += jim + yre;
+  }
+  st.pj = cj;
+  st.py = cy;
+}
+
+// cf: NC = 2 nspec floats per station: (alpha, beta) per spectrum.
+// hh[s] = r_s - r_{s-1}, hh[0] = 0, zero past the last station; it holds
+// nrp + kUnroll entries.
+template <int MODE>
+FJ_HD inline float linseg_lane(const double* __restrict__ rr, const float* __restrict__ hh,
+                               const float* __restrict__ cf, int nrp, double q) {
+  constexpr int NC = 2 * ModeT<MODE>::nspec;
+  const double q2pi = q * kTwoPi;
+  const float kf = (float)q2pi;
+  LinState<MODE> st;
+  double accR = 0.0, accI = 0.0;
+  float dxR = 0.0f;
+  for (int sb = 0; sb < nrp; sb += kUnroll) {
+    double rv[kUnroll];
+    float hv[kUnroll];
+    float cv[kUnroll * NC];
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) rv[k] = rr[sb + k];
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) hv[k] = hh[sb + k + 1];
+#pragma unroll
+    for (int k = 0; k < kUnroll * NC; k++) cv[k] = cf[sb * NC + k];
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) {
+      const float dxL = dxR;
+      dxR = kf * hv[k];
+      linseg_station<MODE>(sb + k, rv[k], dxL, dxR, q, q2pi, cv + k * NC, st);
+    }
+    if (((sb + kUnroll) & (kPartial - 1)) == 0) {
+      accR += (double)st.pR;
+      accI += (double)st.pI;
+      st.pR = st.pI = 0.0f;
+    }
+  }
+  accR += (double)st.pR;
+  accI += (double)st.pI;
   const double inv_k2 = 1.0 / (q2pi * q2pi);
   if (MODE == MODE_ABS_J || MODE == MODE_ABS_H)
     return (float)(sqrt(accR * accR + accI * accI) * inv_k2);

@@ -101,6 +101,35 @@ int ccfj_plain_fp64(int device, int ybasis, int itype, const float *U,
                     const float *r, const float *f, const float *c, int nr,
                     int nf, int nc, float *out, double *kernel_ms);
 
+/* ---- 3. batched sub-arrays -------------------------------------------------
+ * No counterpart in the reference (one array per call there).  nb
+ * independent sub-arrays that share f[nf] and c[nc]; sub-array b owns rows
+ * soff[b] .. soff[b+1]-1 of the concatenated U0/U1 ([rows][nf] float32) and
+ * of r (each sub-array's rows sorted by r).  out: [nb][nc][nf] float32, each
+ * sub-array normalised per column when normalize != 0.  One persistent
+ * kernel processes every (sub-array, frequency, velocity tile) item from a
+ * work queue, largest sub-arrays first; its velocity tile (64/128/256) is
+ * chosen by timing on the first run of a plan. */
+typedef struct ccfj_batch ccfj_batch;
+
+/* Plan (host soff[nb+1]); allocates the plan's workspace on ctx's device. */
+int ccfj_batch_create(ccfj_ctx *ctx, int mode, int itype, int normalize, int nb,
+                      const long long *soff, int nf, int nc, ccfj_batch **plan);
+/* Device pointers, enqueued on `stream` (NULL = the context's stream).
+ * use_graph != 0: the launch sequence is captured into a HIP graph on first
+ * use and replayed while the pointers are unchanged (needs a non-null
+ * stream, i.e. not the legacy default stream). */
+int ccfj_batch_run(ccfj_batch *plan, const float *U0, const float *U1, const float *r,
+                   const float *f, const float *c, float *out, void *stream, int use_graph);
+/* Velocity tile chosen by the autotuner (0 before the first run). */
+int ccfj_batch_tile(ccfj_batch *plan);
+int ccfj_batch_destroy(ccfj_batch *plan);
+/* Host pointers: plan, copy in, run, copy out, free.  Synchronous. */
+int ccfj_transform_batched(ccfj_ctx *ctx, int mode, int itype, int normalize, int nb,
+                           const long long *soff, const float *U0, const float *U1,
+                           const float *r, const float *f, const float *c, int nf, int nc,
+                           float *out);
+
 #ifdef __cplusplus
 }
 #endif
