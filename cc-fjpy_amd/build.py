@@ -24,7 +24,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HIP_SOURCES = ["fj_kernels.hip", "ccfj_capi.hip", "fj_plain64.hip", "fj_batched.hip",
-               "ccfj_batch.hip"]
+               "ccfj_batch.hip", "fj_mwfj.hip", "ccfj_mwfj.hip"]
 
 
 def _run(cmd, **kw):
@@ -35,7 +35,8 @@ def _run(cmd, **kw):
 def build_hip():
     out = os.path.join(PKG, "libccfj.so")
     srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, "-o", out, *srcs])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, "-o", out, *srcs,
+          "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"])
     return out
 
 

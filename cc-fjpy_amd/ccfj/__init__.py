@@ -220,22 +220,60 @@ def win(npts, Fs, T1, T2, taper=0.8):
     return out
 
 
+def _window_descriptions(winl, winr, indx, npts, Fs, taper):
+    """Integer form of win() for every (window, sorted station): zero
+    outside [n1, n2), sine ramps of delta samples.  Windows that win()
+    builds through its degenerate branches (reversed, out of range,
+    overlapping ramps) are returned as explicit rows instead."""
+    T1 = np.asarray(winl, np.float64)[:, indx]
+    T2 = np.asarray(winr, np.float64)[:, indx]
+    a = np.maximum(0.0, np.floor(T1 * Fs))
+    b = np.minimum(float(npts), np.floor(T2 * Fs))
+    d = np.floor((b - a) * (1 - taper) / 2)
+    regular = (a >= 0) & (a <= b) & (b <= npts) & (2 * d <= b - a)
+    n1 = a.astype(np.int32)
+    n2 = b.astype(np.int32)
+    dl = np.where(regular, d, 0).astype(np.int32)
+    xidx = np.full(T1.shape, -1, np.int32)
+    rows = []
+    for w, s in zip(*np.nonzero(~regular)):
+        xidx[w, s] = len(rows)
+        rows.append(win(npts, Fs, T1[w, s], T2[w, s], taper))
+    xrows = np.array(rows, np.float64).reshape(len(rows), npts)
+    return n1, n2, dl, xidx, xrows
+
+
 def MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, fstride=1, itype=1, func=0,
          num=-1):
     """Multi-window F-J (src/ccfj-gpu.pyx:297-313): for each window w the
     traces are tapered by win(npts, Fs, winl[w,j], winr[w,j], taper) (window
-    edges in seconds, original station order) and passed to fj_earthquake.
-    Returns float64 [nwin, nc, nf]."""
+    edges in seconds, original station order) and transformed as
+    fj_earthquake does.  Tapering, the rFFT and the F-J of all windows run on
+    the GPU (itype 0: windows as right-hand sides of one MFMA contraction per
+    frequency).  Returns float64 [nwin, nc, nf]."""
     nr = len(r)
     npts = len(u[0])
-    out = np.zeros([nwin, len(c), len(f)])
-    for i in range(0, nwin):
-        u0 = np.zeros([nr, npts])
-        for j in range(nr):
-            u0[j, :] = u[j, :] * win(npts, Fs, winl[i, j], winr[i, j], taper)
-        out[i, :, :] = fj_earthquake(u0, r, c, f, fstride=fstride, itype=itype,
-                                     func=func, num=num)
-    return out
+    nf, nc = len(f), len(c)
+    if func not in (0, 1):
+        for _ in range(nwin):
+            print('set func as 0 for Bessel function 1 for Hankel function')
+        return np.zeros([nwin, nc, nf])
+    # fj_earthquake slices bins 0:nf*fstride:fstride and fj slices again
+    # before reshape(nr*nf) (src/ccfj.pyx:234, :165-166)
+    ncols = len(range(0, min(nf * fstride, npts // 2 + 1), fstride))
+    ncols2 = len(range(0, ncols, fstride))
+    if ncols2 != nf:
+        raise ValueError("cannot reshape array of size %d into shape (%d,)" % (nr * ncols2, nr * nf))
+    if itype not in (0, 1):
+        return np.full([nwin, nc, nf], np.nan)
+    indx = np.argsort(r)
+    rs = np.asarray(r)[indx]
+    us = np.ascontiguousarray(np.asarray(u)[indx], dtype=np.float64)
+    n1, n2, dl, xidx, xrows = _window_descriptions(np.asarray(winl)[:nwin], np.asarray(winr)[:nwin],
+                                                   indx, npts, Fs, taper)
+    return _context(num).mwfj(int(itype), int(func), int(fstride), us, _as_f32(rs),
+                              _as_f32(np.asarray(f)), _as_f32(np.asarray(c)), n1, n2, dl, xidx,
+                              xrows)
 
 
 def fj_noise_batch(ufs, rs, c, f, fstride=1, itype=1, func=0, num=-1):

@@ -52,6 +52,10 @@ cdef extern from "ccfj.h" nogil:
                                const long long *soff, const float *U0, const float *U1,
                                const float *r, const float *f, const float *c, int nf, int nc,
                                float *out)
+    int ccfj_mwfj(ccfj_ctx *ctx, int itype, int func, int nr, int npts, int nwin, int nf,
+                  int nc, int fstride, const double *u, const float *r, const float *f,
+                  const float *c, const int *n1, const int *n2, const int *delta,
+                  const int *xidx, const double *xrows, int nx, double *out)
 
 MODE_J = CCFJ_MODE_J
 MODE_Y = CCFJ_MODE_Y
@@ -194,6 +198,32 @@ cdef class Context:
                                         &u0[0, 0], &u1[0, 0], &rv[0], &fv[0], &cv[0],
                                         nf, nc, &o[0, 0, 0])
         _check(rc, "ccfj_transform_batched")
+        return out
+
+    def mwfj(self, int itype, int func, int fstride, u, r, f, c, n1, n2, delta, xidx, xrows):
+        """Device multi-window F-J.  u float64 [nr, npts] (rows sorted by r),
+        r/f/c float32, n1/n2/delta/xidx int32 [nwin, nr], xrows float64
+        [nx, npts].  Returns float64 [nwin, nc, nf]."""
+        cdef double[:, ::1] uu = u
+        cdef float[::1] rv = r
+        cdef float[::1] fv = f
+        cdef float[::1] cv = c
+        cdef int[:, ::1] a1 = n1
+        cdef int[:, ::1] a2 = n2
+        cdef int[:, ::1] ad = delta
+        cdef int[:, ::1] ax = xidx
+        cdef double[:, ::1] xr = xrows
+        cdef int nr = uu.shape[0], npts = uu.shape[1], nwin = a1.shape[0]
+        cdef int nf = fv.shape[0], nc = cv.shape[0], nx = xr.shape[0]
+        out = np.empty((nwin, nc, nf), np.float64)
+        cdef double[:, :, ::1] o = out
+        cdef const double *px = &xr[0, 0] if nx > 0 else NULL
+        cdef int rc
+        with nogil:
+            rc = ccfj_mwfj(self.ctx, itype, func, nr, npts, nwin, nf, nc, fstride, &uu[0, 0],
+                           &rv[0], &fv[0], &cv[0], &a1[0, 0], &a2[0, 0], &ad[0, 0], &ax[0, 0],
+                           px, nx, &o[0, 0, 0])
+        _check(rc, "ccfj_mwfj")
         return out
 
     def batch_plan(self, int mode, int itype, bint normalize, soff, int nf, int nc):

@@ -1,0 +1,37 @@
+// fj_mwfj.h -- interface of the multi-window F-J kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fjk {
+
+struct MwfjDev {
+  int nr, npts, nf, nc, nwin;
+  int fstride, nbins;  // rFFT bins = npts/2 + 1
+  int func;
+  int half;            // K rows per basis half (nr rounded up to 32)
+  int nrhs;            // 2 nwin rounded up to 32
+  const double* u;     // [nr][npts] traces, rows sorted by r
+  const int* n1;       // [nwin][nr] window descriptions (sorted station order)
+  const int* n2;
+  const int* dl;
+  const int* xidx;     // [nwin][nr] row of xrows for degenerate windows, else -1
+  const double* xrows; // explicit window rows
+  const float* r;      // [nr] sorted, float32 (the reference's cast)
+  const double* rr;    // [nr] the same in double
+  const float* f;
+  const float* c;
+  double* xw;          // tapered traces of one chunk of windows
+  void* spec;          // rFFT of the chunk, complex double
+  float* B;            // [nf][Kp][nrhs] coefficients (itype 0)
+  float* T;            // [nwin][nf][nc] |O| before normalisation (itype 0)
+  double* out;         // [nwin][nc][nf]
+};
+
+hipError_t launch_window(const MwfjDev& d, int w0, int cw, hipStream_t st);
+hipError_t launch_mwfj_coef(const MwfjDev& d, int w0, int cw, hipStream_t st);
+hipError_t launch_mwfj_mfma(const MwfjDev& d, hipStream_t st);
+hipError_t launch_mwfj_finish(const MwfjDev& d, hipStream_t st);
+hipError_t launch_mwfj_stage(const MwfjDev& d, int wl, float* re, float* im, hipStream_t st);
+hipError_t launch_to_double(const float* a, size_t n, double* out, hipStream_t st);
+
+}  // namespace fjk

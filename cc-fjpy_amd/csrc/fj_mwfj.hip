@@ -1,0 +1,269 @@
+// fj_mwfj.hip -- multi-window F-J (MWFJ) on the device.
+//
+// Reference: MWFJ (src/ccfj.pyx:252-268) builds, for every window w and
+// trace j, win(npts, Fs, winl[w,j], winr[w,j], taper) (src/ccfj.pyx:148-158),
+// multiplies, and calls fj_earthquake per window (src/ccfj.pyx:229-250):
+// rFFT over time, then |FJ[Re] + i FJ[Im]| (func 0) or the Hankel form
+// (func 1), normalised per frequency column, stored as float64.
+//
+// Here every stage runs on the GPU:
+//   k_window        tapered traces in double (window from its integer
+//                   description n1, n2, delta; degenerate windows are passed
+//                   as explicit rows), a chunk of windows at a time
+//   hipFFT D2Z      the rFFT, double precision like numpy's
+//   k_mwfj_coef     bins -> float32 spectra (the reference's cast) ->
+//                   trapezoid coefficients B[f][k][rhs], rhs = (window,
+//                   Re/Im); for func 1 the K axis holds a J half and a Y half
+//   k_mwfj_mfma     itype 0: the windows become right-hand sides of one
+//                   contraction per frequency, O[c][rhs] = sum_k Z(k_c r_k)
+//                   B[k][rhs]; the kernel matrix Z (J0 or Y0) is generated by
+//                   VALU and contracted with v_mfma_f32_32x32x2_f32 (full
+//                   FP32), 32 velocities x 128 right-hand sides per wave
+//   k_mwfj_finish   |Re + i Im| was formed in the MFMA epilogue; per-column
+//                   max, divide, transpose to [nwin][nc][nf], float64
+// itype 1 reuses the single-array kernels per window on the device spectra.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fj_lane.h"
+#include "fj_mwfj.h"
+
+namespace fjk {
+
+using namespace fjl;
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// ---------------------------------------------------------------------------
+// tapered traces: xw[(wl*nr + s)*npts + t], windows w0 .. w0+cw-1
+__global__ void k_window(const double* __restrict__ u, const int* __restrict__ n1,
+                         const int* __restrict__ n2, const int* __restrict__ dl,
+                         const int* __restrict__ xidx, const double* __restrict__ xrows,
+                         int w0, int nr, int npts, double* __restrict__ xw) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  const int wl = blockIdx.z;
+  if (t >= npts) return;
+  const int desc = (w0 + wl) * nr + s;
+  double v;
+  const int xi = xidx[desc];
+  if (xi >= 0) {
+    v = xrows[(size_t)xi * npts + t];
+  } else {
+    const int a = n1[desc], b = n2[desc], d = dl[desc];
+    if (t < a || t >= b) {
+      v = 0.0;
+    } else if (t < a + d) {
+      v = sin(3.141592653589793 / 2 * (double)(t - a) / (double)d);
+    } else if (t >= b - d) {
+      v = sin(3.141592653589793 / 2 * (double)(b - 1 - t) / (double)d);
+    } else {
+      v = 1.0;
+    }
+  }
+  const size_t row = (size_t)wl * nr + s;
+  xw[row * npts + t] = u[(size_t)s * npts + t] * v;
+}
+
+// spectra -> float32 staging -> trapezoid coefficients.
+// B layout [nf][Kp][nrhs]; rhs 2w = Re, 2w+1 = Im of window w.
+// func 0: rows k = s.  func 1: rows s (J half: Re, Im) and half + s
+// (Y half: -Im, Re), so O_re = FJ[Re] - FY[Im], O_im = FJ[Im] + FY[Re].
+__global__ void k_mwfj_coef(const double2* __restrict__ spec, const float* __restrict__ r,
+                            int nr, int w0, int cw, int nf, int fstride, int nbins,
+                            int func, int half, int nrhs, float* __restrict__ B) {
+  const int wl = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  const int f = blockIdx.z;
+  if (wl >= cw) return;
+  const double2 z = spec[((size_t)wl * nr + s) * nbins + (size_t)f * fstride];
+  const float re = (float)z.x, im = (float)z.y;   // the reference's float32 staging
+  const float rs = r[s];
+  double w = 0.0;
+  if (s > 0) w += (double)fmaxf(rs - r[s - 1], 0.1f);
+  if (s + 1 < nr) w += (double)fmaxf(r[s + 1] - rs, 0.1f);
+  const double wr = 0.5 * w * (double)rs;
+  const float are = (float)(wr * (double)re), aim = (float)(wr * (double)im);
+  const int w2 = 2 * (w0 + wl);
+  const int Kp = func ? 2 * half : half;
+  float* row = B + ((size_t)f * Kp + s) * nrhs;
+  row[w2] = are;
+  row[w2 + 1] = aim;
+  if (func) {
+    float* yrow = B + ((size_t)f * Kp + half + s) * nrhs;
+    yrow[w2] = -aim;
+    yrow[w2 + 1] = are;
+  }
+}
+
+// O[c][rhs] = sum_k Z(k_c r_k) B[k][rhs] for one frequency, 128 velocities
+// (4 waves x 32) and 128 right-hand sides (4 MFMA tiles) per workgroup.
+// v_mfma_f32_32x32x2_f32 operand layout (lane l): A[i = l%32][k = l/32],
+// B[k = l/32][n = l%32]; result register v holds row 8(v/4) + 4(l/32) + v%4,
+// column l%32.
+constexpr int kMwC = 128;   // velocities per workgroup
+constexpr int kMwR = 128;   // right-hand sides per workgroup
+constexpr int kMwK = 32;    // K rows per LDS chunk
+
+template <int FUNC>
+__global__ __launch_bounds__(256) void k_mwfj_mfma(const float* __restrict__ B,
+                                                   const double* __restrict__ rr,
+                                                   const float* __restrict__ fv,
+                                                   const float* __restrict__ cv, int nr,
+                                                   int half, int nf, int nc, int nrhs,
+                                                   int nwin, float* __restrict__ T) {
+  __shared__ float Bs[kMwK][kMwR + 4];
+  const int fi = blockIdx.y;
+  const int rg = blockIdx.z;   // right-hand-side group
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int i = lane & 31, kq = lane >> 5;
+  const int crow = blockIdx.x * kMwC + wave * 32 + i;
+  const int cidx = crow < nc ? crow : nc - 1;
+  const double q = (double)fv[fi] / (double)cv[cidx];
+  const double q2pi = q * kTwoPi;
+  const int Kp = FUNC ? 2 * half : half;
+  const float* Bf = B + (size_t)fi * Kp * nrhs + (size_t)rg * kMwR;
+  const int rcols = min(kMwR, nrhs - rg * kMwR);   // multiple of 32
+  floatx16 acc[4];
+#pragma unroll
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc[t][v] = 0.0f;
+
+  for (int k0 = 0; k0 < Kp; k0 += kMwK) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < kMwK * kMwR; e += 256) {
+      const int kr = e / kMwR, cc = e - kr * kMwR;
+      Bs[kr][cc] = cc < rcols ? Bf[(size_t)(k0 + kr) * nrhs + cc] : 0.0f;
+    }
+    __syncthreads();
+    const bool ypart = FUNC && k0 >= half;
+    const int sbase = ypart ? k0 - half : k0;
+#pragma unroll 2
+    for (int kk = 0; kk < kMwK; kk += 2) {
+      int s = sbase + kk + kq;
+      s = s < nr ? s : nr - 1;   // padding rows carry zero coefficients
+      const double rs = rr[s];
+      const double xd = q2pi * rs;
+      const float x = (float)xd;
+      float J0 = 0.0f, Y0 = 0.0f;
+      if (x >= kSplit) {
+        if (ypart) bessel0_large<false, true>(x, reduce_phase(q, rs), &J0, &Y0);
+        else bessel0_large<true, false>(x, reduce_phase(q, rs), &J0, &Y0);
+      } else {
+        double y, L;
+        if (ypart) {
+          Bess b = bessel_small<false, true>(xd, &y, &L);
+          Y0 = b.y0;
+        } else {
+          Bess b = bessel_small<false, false>(xd, &y, &L);
+          J0 = b.j0;
+        }
+      }
+      const float a = ypart ? Y0 : J0;
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        const float b = Bs[kk + kq][t * 32 + i];
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  // epilogue: |Re + i Im| per window; Re in even columns, Im in odd
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      const float mine = acc[t][v];
+      const float other = __shfl_xor(mine, 1);
+      const int m = 8 * (v >> 2) + 4 * kq + (v & 3);
+      const int col = rg * kMwR + t * 32 + i;
+      const int c = blockIdx.x * kMwC + wave * 32 + m;
+      if ((i & 1) == 0 && c < nc && col < 2 * nwin) {
+        const int w = col >> 1;
+        T[((size_t)w * nf + fi) * nc + c] = sqrtf(mine * mine + other * other);
+      }
+    }
+  }
+}
+
+// per (window, frequency): builtin-max normalisation over c, transpose to
+// out[w][c][f] in float64 (the reference's MWFJ output dtype).
+__global__ void k_mwfj_finish(const float* __restrict__ T, int nf, int nc, int nwin,
+                              double* __restrict__ out) {
+  const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+  const int w = blockIdx.y;
+  if (fi >= nf) return;
+  const float* col = T + ((size_t)w * nf + fi) * nc;
+  float m = fabsf(col[0]);
+  for (int c = 1; c < nc; c++) {
+    const float v = fabsf(col[c]);
+    if (v > m) m = v;
+  }
+  for (int c = 0; c < nc; c++) out[((size_t)w * nc + c) * nf + fi] = (double)(col[c] / m);
+}
+
+// float32 staging of one window's spectra for the itype-1 path:
+// re/im [nr][nf]
+__global__ void k_mwfj_stage(const double2* __restrict__ spec, int nr, int wl, int nf,
+                             int fstride, int nbins, float* __restrict__ re,
+                             float* __restrict__ im) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  if (f >= nf) return;
+  const double2 z = spec[((size_t)wl * nr + s) * nbins + (size_t)f * fstride];
+  re[(size_t)s * nf + f] = (float)z.x;
+  im[(size_t)s * nf + f] = (float)z.y;
+}
+
+// float32 [nc][nf] (normalised) -> float64 slice of out
+__global__ void k_to_double(const float* __restrict__ a, size_t n, double* __restrict__ out) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = (double)a[k];
+}
+
+// ---------------------------------------------------------------------------
+hipError_t launch_window(const MwfjDev& d, int w0, int cw, hipStream_t st) {
+  dim3 grid((d.npts + 255) / 256, d.nr, cw);
+  hipLaunchKernelGGL(k_window, grid, dim3(256), 0, st, d.u, d.n1, d.n2, d.dl, d.xidx, d.xrows,
+                     w0, d.nr, d.npts, d.xw);
+  return hipGetLastError();
+}
+
+hipError_t launch_mwfj_coef(const MwfjDev& d, int w0, int cw, hipStream_t st) {
+  dim3 grid((cw + 63) / 64, d.nr, d.nf);
+  hipLaunchKernelGGL(k_mwfj_coef, grid, dim3(64), 0, st, (const double2*)d.spec, d.r, d.nr, w0,
+                     cw, d.nf, d.fstride, d.nbins, d.func, d.half, d.nrhs, d.B);
+  return hipGetLastError();
+}
+
+hipError_t launch_mwfj_mfma(const MwfjDev& d, hipStream_t st) {
+  dim3 grid((d.nc + kMwC - 1) / kMwC, d.nf, (d.nrhs + kMwR - 1) / kMwR);
+  if (d.func)
+    hipLaunchKernelGGL(k_mwfj_mfma<1>, grid, dim3(256), 0, st, d.B, d.rr, d.f, d.c, d.nr,
+                       d.half, d.nf, d.nc, d.nrhs, d.nwin, d.T);
+  else
+    hipLaunchKernelGGL(k_mwfj_mfma<0>, grid, dim3(256), 0, st, d.B, d.rr, d.f, d.c, d.nr,
+                       d.half, d.nf, d.nc, d.nrhs, d.nwin, d.T);
+  return hipGetLastError();
+}
+
+hipError_t launch_mwfj_finish(const MwfjDev& d, hipStream_t st) {
+  dim3 grid((d.nf + 255) / 256, d.nwin);
+  hipLaunchKernelGGL(k_mwfj_finish, grid, dim3(256), 0, st, d.T, d.nf, d.nc, d.nwin, d.out);
+  return hipGetLastError();
+}
+
+hipError_t launch_mwfj_stage(const MwfjDev& d, int wl, float* re, float* im, hipStream_t st) {
+  dim3 grid((d.nf + 255) / 256, d.nr);
+  hipLaunchKernelGGL(k_mwfj_stage, grid, dim3(256), 0, st, (const double2*)d.spec, d.nr, wl,
+                     d.nf, d.fstride, d.nbins, re, im);
+  return hipGetLastError();
+}
+
+hipError_t launch_to_double(const float* a, size_t n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_double, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace fjk

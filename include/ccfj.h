@@ -130,6 +130,25 @@ int ccfj_transform_batched(ccfj_ctx *ctx, int mode, int itype, int normalize, in
                            const float *r, const float *f, const float *c, int nf, int nc,
                            float *out);
 
+/* ---- 4. multi-window F-J on the device ------------------------------------
+ * Replaces the per-window loop of MWFJ (src/ccfj.pyx:252-268): tapering,
+ * the rFFT (double precision) and the F-J of every window run on the GPU;
+ * for itype 0 the windows are right-hand sides of one MFMA contraction per
+ * frequency.  Host pointers; synchronous.
+ *   u[nr][npts]            traces, rows already sorted by r (double)
+ *   r[nr]                  sorted distances (float32, as the reference casts)
+ *   f[nf], c[nc]           float32; bins 0, fstride, ... of the rFFT are used
+ *   n1, n2, delta [nwin][nr]  each window as win() describes it
+ *                          (src/ccfj.pyx:148-158): zero outside [n1, n2),
+ *                          sine ramps of delta samples; sorted station order
+ *   xidx[nwin][nr]         -1, or the row of xrows[nx][npts] holding an
+ *                          explicit window (used for degenerate windows)
+ *   out[nwin][nc][nf]      float64, |..| normalised per (window, column). */
+int ccfj_mwfj(ccfj_ctx *ctx, int itype, int func, int nr, int npts, int nwin, int nf,
+              int nc, int fstride, const double *u, const float *r, const float *f,
+              const float *c, const int *n1, const int *n2, const int *delta,
+              const int *xidx, const double *xrows, int nx, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -184,16 +184,38 @@ def test_fj_earthquake_wrapper(ccfj, itype, func):
     assert mism == 0 and emax <= TOL, emax
 
 
-def test_mwfj_wrapper(ccfj):
+@pytest.mark.parametrize("itype", [0, 1])
+@pytest.mark.parametrize("func", [0, 1])
+def test_mwfj_wrapper(ccfj, itype, func):
+    """Device MWFJ (tapering, double rFFT, windows-as-right-hand-sides MFMA
+    for itype 0) against the reference's per-window loop restated over the
+    FP64 oracle; includes a reversed and an out-of-range window."""
     u, r, c, f, Fs = _quake(nr=24, npts=512)
-    nwin = 3
-    v = np.array([2500.0, 3500.0, 4500.0, 5500.0])
+    nwin = 5
+    v = np.array([2500.0, 3000.0, 3500.0, 4500.0, 5500.0, 6500.0])
     winl = np.stack([r / v[w + 1] for w in range(nwin)])
     winr = np.stack([r / v[w] + 8.0 for w in range(nwin)])
-    got = ccfj.MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=0)
-    want = O.MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=0, engine="fp64")
+    winl[1, 3], winr[1, 3] = 20.0, 10.0       # reversed: all zero
+    winr[2, 5] = 1e3                          # beyond the trace end
+    got = ccfj.MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=itype, func=func)
+    want = O.MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=itype, func=func,
+                  engine="fp64")
     assert got.dtype == np.float64 and got.shape == (nwin, len(c), len(f))
     for w in range(nwin):
+        emax, _, mism = O.column_error(got[w], want[w])
+        assert mism == 0 and emax <= TOL, (w, emax)
+
+
+def test_mwfj_many_windows_mfma_tiles(ccfj):
+    """More than 64 windows: the right-hand sides span two MFMA groups."""
+    u, r, c, f, Fs = _quake(nr=16, npts=256, seed=9)
+    nwin = 70
+    rng = np.random.default_rng(4)
+    t0 = rng.uniform(0, 6, (nwin, 1)) + r[None, :] / 4000.0
+    winl, winr = t0, t0 + rng.uniform(2, 6, (nwin, 1))
+    got = ccfj.MWFJ(u, r, c[::4], f[::3], Fs, nwin, winl, winr, taper=0.8, itype=0)
+    want = O.MWFJ(u, r, c[::4], f[::3], Fs, nwin, winl, winr, taper=0.8, itype=0, engine="fp64")
+    for w in (0, 31, 32, 63, 64, 69):
         emax, _, mism = O.column_error(got[w], want[w])
         assert mism == 0 and emax <= TOL, (w, emax)
 
