@@ -146,23 +146,24 @@ def fj_noise(uf, r, c, f, fstride=1, itype=1, func=0, num=-1):
 
     uf: [nr, ncol] spectra (only the real part is used), r: distances (m),
     c: phase velocities (m/s), f: frequencies, ceil(ncol/fstride) == len(f).
-    func 0: Bessel basis; func 1: Hankel basis, FJ[uf] - FY[Hilbert_f(uf)].
+    func 0: Bessel basis; func 1: Hankel basis, FJ[uf] - FY[Hilbert_f(uf)]
+    with the Hilbert transform taken along frequency on the GPU.
     Returns the per-frequency normalised float32 [nc, nf] spectrum.
     """
     indx = np.argsort(r)
     r = r[indx]
     uf = uf[indx]
-    if func == 0:
-        U, rr, cc, ff = _stage(uf, r, c, f, fstride)
-        return _native(MODE_J, itype, True, U, None, rr, cc, ff, num)
-    elif func == 1:
-        ufi = hilbert_rows(uf)
-        U, rr, cc, ff = _stage(uf, r, c, f, fstride)
-        Ui, _, _, _ = _stage(ufi, r, c, f, fstride)
-        return _native(MODE_J_MINUS_Y, itype, True, U, Ui, rr, cc, ff, num)
-    else:
+    if func not in (0, 1):
         print('set func as 0 for Bessel function 1 for Hankel function')
         return 0
+    nr, nf = len(r), len(f)
+    uf[:, ::fstride].reshape(nr * nf)      # the reference's shape check (ValueError)
+    if itype not in (0, 1):
+        return _native(MODE_J, itype, True, None, None, r, c, f, num)
+    U = _as_f32(np.asarray(uf))
+    return _context(num).fj_noise(int(itype), int(func), int(fstride), U.reshape(nr, -1),
+                                  _as_f32(np.asarray(r)), _as_f32(np.asarray(f)),
+                                  _as_f32(np.asarray(c)))
 
 
 def _earthquake_spectra(u, r, f, fstride):
@@ -231,9 +232,9 @@ def _window_descriptions(winl, winr, indx, npts, Fs, taper):
     b = np.minimum(float(npts), np.floor(T2 * Fs))
     d = np.floor((b - a) * (1 - taper) / 2)
     regular = (a >= 0) & (a <= b) & (b <= npts) & (2 * d <= b - a)
-    n1 = a.astype(np.int32)
-    n2 = b.astype(np.int32)
-    dl = np.where(regular, d, 0).astype(np.int32)
+    n1 = np.ascontiguousarray(a, dtype=np.int32)
+    n2 = np.ascontiguousarray(b, dtype=np.int32)
+    dl = np.ascontiguousarray(np.where(regular, d, 0), dtype=np.int32)
     xidx = np.full(T1.shape, -1, np.int32)
     rows = []
     for w, s in zip(*np.nonzero(~regular)):

@@ -52,6 +52,10 @@ cdef extern from "ccfj.h" nogil:
                                const long long *soff, const float *U0, const float *U1,
                                const float *r, const float *f, const float *c, int nf, int nc,
                                float *out)
+    int ccfj_hilbert_dev(ccfj_ctx *ctx, const float *x, float *y, int rows, int n, void *stream)
+    int ccfj_fj_noise(ccfj_ctx *ctx, int itype, int func, int fstride, const float *uf,
+                      const float *r, const float *f, const float *c, int nr, int ncol,
+                      int nf, int nc, float *out)
     int ccfj_mwfj(ccfj_ctx *ctx, int itype, int func, int nr, int npts, int nwin, int nf,
                   int nc, int fstride, const double *u, const float *r, const float *f,
                   const float *c, const int *n1, const int *n2, const int *delta,
@@ -199,6 +203,33 @@ cdef class Context:
                                         nf, nc, &o[0, 0, 0])
         _check(rc, "ccfj_transform_batched")
         return out
+
+    def fj_noise(self, int itype, int func, int fstride, uf, r, f, c):
+        """uf float32 [nr, ncol] (rows sorted by r), r/f/c float32 ->
+        normalised float32 [nc, nf]."""
+        cdef float[:, ::1] u = uf
+        cdef float[::1] rv = r
+        cdef float[::1] fv = f
+        cdef float[::1] cv = c
+        cdef int nr = u.shape[0], ncol = u.shape[1], nf = fv.shape[0], nc = cv.shape[0]
+        out = np.zeros((nc, nf), np.float32)
+        cdef float[:, ::1] o = out
+        cdef const float *pu = &u[0, 0] if nr > 0 and ncol > 0 else NULL
+        cdef const float *pr = &rv[0] if nr > 0 else NULL
+        cdef const float *pf = &fv[0] if nf > 0 else NULL
+        cdef const float *pc = &cv[0] if nc > 0 else NULL
+        cdef float *po = &o[0, 0] if nc > 0 and nf > 0 else NULL
+        cdef int rc
+        with nogil:
+            rc = ccfj_fj_noise(self.ctx, itype, func, fstride, pu, pr, pf, pc, nr, ncol, nf, nc, po)
+        _check(rc, "ccfj_fj_noise")
+        return out
+
+    def hilbert_dev(self, size_t x, size_t y, int rows, int n, size_t stream=0):
+        cdef int rc
+        with nogil:
+            rc = ccfj_hilbert_dev(self.ctx, <const float *>x, <float *>y, rows, n, <void *>stream)
+        _check(rc, "ccfj_hilbert_dev")
 
     def mwfj(self, int itype, int func, int fstride, u, r, f, c, n1, n2, delta, xidx, xrows):
         """Device multi-window F-J.  u float64 [nr, npts] (rows sorted by r),

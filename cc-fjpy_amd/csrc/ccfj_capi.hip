@@ -20,6 +20,7 @@
 #include "../../include/ccfj.h"
 #include "ccfj_internal.h"
 #include "fj_kernels.h"
+#include "fj_synth.h"
 
 namespace ccfj_internal {
 
@@ -60,7 +61,7 @@ bool injected_alloc_failure() {
   return fail_at > 0 && n == fail_at;
 }
 
-enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_COEF, S_COUNT };
+enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_COEF, S_X0, S_X1, S_COUNT };
 
 }  // namespace
 
@@ -123,7 +124,9 @@ int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U
                int nf, int nc, float* out, hipStream_t st) {
   if (nf == 0 || nc == 0) return CCFJ_OK;
   hipError_t e;
-  if (nr == 0) {
+  if (nr < 2) {
+    // no segment: the reference's sums are empty and stay 0 (even at f = 0,
+    // where a segment would divide by k = 0)
     e = hipMemsetAsync(out, 0, sizeof(float) * (size_t)nf * nc, st);
     if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
   } else {
@@ -214,6 +217,14 @@ int device_alloc(void** p, size_t bytes) {
 }
 
 int ctx_device(const ccfj_ctx* ctx) { return ctx->device; }
+
+int ctx_scratch(ccfj_ctx* ctx, int which, size_t bytes, void** p) {
+  const Slot s = which == 0 ? S_X0 : S_X1;
+  int rc = ensure(ctx, s, bytes);
+  if (rc) return rc;
+  *p = ctx->buf[s];
+  return CCFJ_OK;
+}
 hipStream_t ctx_stream(const ccfj_ctx* ctx) { return ctx->stream; }
 
 }  // namespace ccfj_internal
@@ -326,6 +337,66 @@ int ccfj_transform(ccfj_ctx* ctx, int mode, int itype, int normalize, const floa
   e = hipStreamSynchronize(st);
   if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
   return CCFJ_OK;
+}
+
+int ccfj_transform_items(ccfj_ctx* ctx, int mode, int itype, const float* U0, const float* U1,
+                         const float* r, const float* f, const float* c, int nr, int nf, int nc,
+                         const int* items, int nitems, float* out, void* stream) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  if (mode != CCFJ_MODE_J && mode != CCFJ_MODE_J_MINUS_Y)
+    return fail(CCFJ_EINVAL, "item lists support modes J and J_MINUS_Y, got %d", mode);
+  int rc = check_args(mode, itype, U0, U1, r, f, c, nr, nf, nc, out);
+  if (rc) return rc;
+  if (nitems < 0 || (nitems > 0 && !items)) return fail(CCFJ_EINVAL, "bad item list");
+  if (nr < 1 || nitems == 0) return CCFJ_OK;
+  if ((rc = set_device(ctx))) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  const int NC = fjk::coef_per_station(mode, itype);
+  const int nrp = fjk::padded(nr);
+  if ((rc = ensure(ctx, S_RR, sizeof(double) * (nrp + 8)))) return rc;
+  if ((rc = ensure(ctx, S_HH, sizeof(float) * (nrp + 8)))) return rc;
+  if ((rc = ensure(ctx, S_COEF, sizeof(float) * (size_t)nf * nrp * NC))) return rc;
+  fjk::Problem p;
+  p.U0 = U0;
+  p.U1 = mode == CCFJ_MODE_J_MINUS_Y ? U1 : U0;
+  p.r = r;
+  p.f = f;
+  p.c = c;
+  p.nr = nr;
+  p.nrp = nrp;
+  p.nf = nf;
+  p.nc = nc;
+  p.rr = (double*)ctx->buf[S_RR];
+  p.hh = (float*)ctx->buf[S_HH];
+  p.coef = (float*)ctx->buf[S_COEF];
+  p.out = out;
+  hipError_t e = fjk::launch_prep(mode, itype, p, st);
+  if (e != hipSuccess) return hip_fail(e, "prep kernel launch");
+  e = fjk::launch_fj_items(mode, itype, p.rr, p.hh, p.coef, f, c, items, nitems, nrp, nf, out, st);
+  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "item kernel launch");
+}
+
+int ccfj_synth_fields(ccfj_ctx* ctx, int B, int nr, int nf, const float* r, const float* f,
+                      const float* modes, const float* sigma, unsigned long long seed, float* U,
+                      void* stream) {
+  if (!ctx || !r || !f || !modes || !sigma || !U) return fail(CCFJ_EINVAL, "null argument");
+  if (B < 1 || nr < 1 || nf < 1) return fail(CCFJ_EINVAL, "bad sizes");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  hipError_t e = fjk::launch_synth_fields(r, f, modes, sigma, B, nr, nf, seed, U, st);
+  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "synth fields launch");
+}
+
+int ccfj_synth_labels(ccfj_ctx* ctx, int B, int H, int nf, const float* cgrid, const float* f,
+                      const float* modes, float halfwidth, float amin, float* Y, void* stream) {
+  if (!ctx || !cgrid || !f || !modes || !Y) return fail(CCFJ_EINVAL, "null argument");
+  if (B < 1 || H < 1 || nf < 1) return fail(CCFJ_EINVAL, "bad sizes");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  hipError_t e = fjk::launch_synth_labels(cgrid, f, modes, B, H, nf, halfwidth, amin, Y, st);
+  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "synth labels launch");
 }
 
 int ccfj_synchronize(ccfj_ctx* ctx) {

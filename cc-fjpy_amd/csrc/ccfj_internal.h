@@ -12,6 +12,8 @@ int hip_fail(hipError_t e, const char* what);
 // Device allocation honouring the CCFJ_FAIL_ALLOC_AT fault injection.
 int device_alloc(void** p, size_t bytes);
 int ctx_device(const ccfj_ctx* ctx);
+// Grow-only scratch buffer `which` (0 or 1) of the context, at least bytes.
+int ctx_scratch(ccfj_ctx* ctx, int which, size_t bytes, void** p);
 hipStream_t ctx_stream(const ccfj_ctx* ctx);
 
 }  // namespace ccfj_internal

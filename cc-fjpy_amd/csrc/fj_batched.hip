@@ -122,7 +122,7 @@ __global__ __launch_bounds__(TILE) void k_fj_b(BatchDev d, unsigned int limit) {
     const int j = (int)cb * TILE + (int)threadIdx.x;
     const int64_t s0 = d.soff[b];
     const int nr = (int)(d.soff[b + 1] - s0);
-    if (j < nc && nr > 0) {
+    if (j < nc && nr > 1) {
       const int nrp = d.nrp[b];
       const double q = (double)d.f[fi] / (double)d.c[j];
       const float* __restrict__ cf = d.coef + d.coff[b] + (size_t)fi * nrp * NC;
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(TILE) void k_fj_b(BatchDev d, unsigned int limit) {
       else
         v = linseg_lane<MODE>(rr, d.hh + d.poff[b], cf, nrp, q);
       d.out[((size_t)b * nc + j) * nf + fi] = v;
-    } else if (j < nc) {
+    } else if (j < nc) {   // fewer than two stations: empty sum, as the reference
       d.out[((size_t)b * nc + j) * nf + fi] = 0.0f;
     }
   }

@@ -1,0 +1,139 @@
+// fj_synth.hip -- synthetic multimode noise fields and their ridge labels,
+// generated on the device for training the band-proposal CNN, and the
+// item-list F-J kernel that evaluates only proposed velocity bands.
+//
+// Field of sample b (nr stations, distances r_b sorted, shared f grid):
+//   U_b[s][i] = sum_m A_m J0(2 pi f_i r_s / c_m(f_i)) + sigma_b n(b, s, i)
+//   c_m(f) = cinf_m + d_m exp(-f / phi_m)   (monotone decreasing)
+// n is a standard normal from a counter-based hash (splitmix64 + Box-Muller)
+// of (seed, b, s, i), so a field is reproducible from its seed alone.
+// Label of sample b on the coarse grid (velocity h, frequency i):
+//   1 if |c_h - c_m(f_i)| <= halfwidth for a mode with A_m >= amin, else 0.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fj_lane.h"
+#include "fj_synth.h"
+
+namespace fjk {
+
+using namespace fjl;
+
+__device__ inline uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__device__ inline float gauss(uint64_t key) {
+  const uint64_t a = splitmix64(key), b = splitmix64(key ^ 0xD1B54A32D192ED03ull);
+  const double u1 = ((a >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  const double u2 = (b >> 11) * (1.0 / 9007199254740992.0);
+  return (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+}
+
+// modes: [B][kMaxModes][4] = (cinf, d, phi, amp); amp = 0 for unused modes
+__global__ void k_synth_fields(const float* __restrict__ r, const float* __restrict__ f,
+                               const float* __restrict__ modes, const float* __restrict__ sigma,
+                               int nr, int nf, uint64_t seed, float* __restrict__ U) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = blockIdx.y;
+  const int b = blockIdx.z;
+  if (i >= nf) return;
+  const double fi = f[i];
+  const double rs = r[(size_t)b * nr + s];
+  double acc = 0.0;
+  for (int m = 0; m < kMaxModes; m++) {
+    const float* p = modes + ((size_t)b * kMaxModes + m) * 4;
+    if (p[3] == 0.0f) continue;
+    const double cm = p[0] + p[1] * exp(-fi / p[2]);
+    const double q = fi / cm;                 // cycles per metre
+    const double xd = q * kTwoPi * rs;
+    const float x = (float)xd;
+    float J0 = 0.0f, Y0;
+    if (x >= kSplit) {
+      bessel0_large<true, false>(x, reduce_phase(q, rs), &J0, &Y0);
+    } else {
+      double y, L;
+      J0 = bessel_small<false, false>(xd, &y, &L).j0;
+    }
+    acc += p[3] * J0;
+  }
+  const uint64_t key = seed ^ (((uint64_t)b * 1000003ull + s) * 1000033ull + i);
+  U[((size_t)b * nr + s) * nf + i] = (float)acc + sigma[b] * gauss(key);
+}
+
+__global__ void k_synth_labels(const float* __restrict__ cgrid, const float* __restrict__ f,
+                               const float* __restrict__ modes, int H, int nf, float halfwidth,
+                               float amin, float* __restrict__ Y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int h = blockIdx.y;
+  const int b = blockIdx.z;
+  if (i >= nf) return;
+  const float c = cgrid[h];
+  float lab = 0.0f;
+  for (int m = 0; m < kMaxModes; m++) {
+    const float* p = modes + ((size_t)b * kMaxModes + m) * 4;
+    if (p[3] < amin) continue;
+    const float cm = p[0] + p[1] * expf(-f[i] / p[2]);
+    if (fabsf(c - cm) <= halfwidth) lab = 1.0f;
+  }
+  Y[((size_t)b * H + h) * nf + i] = lab;
+}
+
+// F-J on a list of (frequency, first velocity, count <= 256) items; lanes
+// beyond count idle.  out[c][f] is written only for listed entries.
+template <int MODE, int ITYPE>
+__global__ __launch_bounds__(256) void k_fj_items(const double* __restrict__ rr,
+                                                 const float* __restrict__ hh,
+                                                 const float* __restrict__ coef,
+                                                 const float* __restrict__ fv,
+                                                 const float* __restrict__ cv,
+                                                 const int* __restrict__ items, int nrp, int nf,
+                                                 float* __restrict__ out) {
+  constexpr int NC = ITYPE == 0 ? ModeT<MODE>::nspec : 2 * ModeT<MODE>::nspec;
+  const int* it = items + 3 * (size_t)blockIdx.x;
+  const int fi = it[0], c0 = it[1], cnt = it[2];
+  if ((int)threadIdx.x >= cnt) return;
+  const int j = c0 + threadIdx.x;
+  const double q = (double)fv[fi] / (double)cv[j];
+  const float* __restrict__ cf = coef + (size_t)fi * nrp * NC;
+  const float v = ITYPE == 0 ? trapz_lane<MODE>(rr, cf, nrp, q)
+                             : linseg_lane<MODE>(rr, hh, cf, nrp, q);
+  out[(size_t)j * nf + fi] = v;
+}
+
+hipError_t launch_synth_fields(const float* r, const float* f, const float* modes,
+                               const float* sigma, int B, int nr, int nf, uint64_t seed, float* U,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_synth_fields, dim3((nf + 255) / 256, nr, B), dim3(256), 0, st, r, f, modes,
+                     sigma, nr, nf, seed, U);
+  return hipGetLastError();
+}
+
+hipError_t launch_synth_labels(const float* cgrid, const float* f, const float* modes, int B,
+                               int H, int nf, float halfwidth, float amin, float* Y,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_synth_labels, dim3((nf + 255) / 256, H, B), dim3(256), 0, st, cgrid, f,
+                     modes, H, nf, halfwidth, amin, Y);
+  return hipGetLastError();
+}
+
+hipError_t launch_fj_items(int mode, int itype, const double* rr, const float* hh,
+                           const float* coef, const float* f, const float* c, const int* items,
+                           int nitems, int nrp, int nf, float* out, hipStream_t st) {
+  if (nitems <= 0) return hipSuccess;
+#define FJ_ITEMS(M, I)                                                                         \
+  hipLaunchKernelGGL((k_fj_items<M, I>), dim3(nitems), dim3(256), 0, st, rr, hh, coef, f, c,  \
+                     items, nrp, nf, out)
+  if (mode == MODE_J && itype == 0) FJ_ITEMS(MODE_J, 0);
+  else if (mode == MODE_J && itype == 1) FJ_ITEMS(MODE_J, 1);
+  else if (mode == MODE_J_MINUS_Y && itype == 0) FJ_ITEMS(MODE_J_MINUS_Y, 0);
+  else if (mode == MODE_J_MINUS_Y && itype == 1) FJ_ITEMS(MODE_J_MINUS_Y, 1);
+  else return hipErrorInvalidValue;
+#undef FJ_ITEMS
+  return hipGetLastError();
+}
+
+}  // namespace fjk

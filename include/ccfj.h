@@ -149,6 +149,68 @@ int ccfj_mwfj(ccfj_ctx *ctx, int itype, int func, int nr, int npts, int nwin, in
               const float *c, const int *n1, const int *n2, const int *delta,
               const int *xidx, const double *xrows, int nx, double *out);
 
+/* ---- 5. noise F-J end to end on the device -------------------------------
+ * Row-wise Hilbert transform along frequency with scipy.fftpack's
+ * convention (y = ifft(i sign(k) fft(x)), zero and even-length Nyquist bins
+ * dropped; double precision inside), replacing the reference's Python loop
+ * (src/ccfj.pyx:217-219).  x, y: device [rows][n] float32. */
+int ccfj_hilbert_dev(ccfj_ctx *ctx, const float *x, float *y, int rows, int n, void *stream);
+/* fj_noise after its host-side sort and float32 cast (src/ccfj.pyx:202-227):
+ * uf[nr][ncol] real part, rows sorted by r; columns 0, fstride, ... (nf of
+ * them) are used; func 1 takes the Hilbert transform of the full rows first.
+ * out[nc][nf] normalised.  Host pointers; synchronous. */
+int ccfj_fj_noise(ccfj_ctx *ctx, int itype, int func, int fstride, const float *uf,
+                  const float *r, const float *f, const float *c, int nr, int ncol,
+                  int nf, int nc, float *out);
+
+/* ---- 6. learned coarse-to-fine ----------------------------------------------
+ * No counterpart in the reference.  A small CNN proposes, from a coarse F-J
+ * spectrum, the velocity bands that hold modal energy per frequency; the
+ * dense velocity grid is then evaluated only inside them.  All pointers are
+ * device pointers unless noted; calls are enqueued on the context's stream
+ * (or `stream` where given). */
+
+/* Dense F-J on a list of items (frequency index, first velocity index,
+ * count <= 256), items[n][3] int32 on the device.  out[nc][nf] is written
+ * only at listed entries (not normalised).  Modes J and J_MINUS_Y. */
+int ccfj_transform_items(ccfj_ctx *ctx, int mode, int itype, const float *U0, const float *U1,
+                         const float *r, const float *f, const float *c, int nr, int nf, int nc,
+                         const int *items, int nitems, float *out, void *stream);
+
+/* Synthetic multimode fields: U[B*nr][nf] = sum_m A_m J0(2 pi f r / c_m(f))
+ * + sigma_b * N(0,1), c_m(f) = cinf + d exp(-f/phi); modes[B][4][4] =
+ * (cinf, d, phi, A) with A = 0 for unused modes; r[B][nr] sorted; the noise
+ * is a counter-based hash of (seed, b, s, i). */
+int ccfj_synth_fields(ccfj_ctx *ctx, int B, int nr, int nf, const float *r, const float *f,
+                      const float *modes, const float *sigma, unsigned long long seed, float *U,
+                      void *stream);
+/* Ridge labels on the coarse grid: Y[B][H][nf] = 1 where |cgrid[h] - c_m(f)|
+ * <= halfwidth for a mode with A >= amin. */
+int ccfj_synth_labels(ccfj_ctx *ctx, int B, int H, int nf, const float *cgrid, const float *f,
+                      const float *modes, float halfwidth, float amin, float *Y, void *stream);
+
+/* Band-proposal CNN: conv3x3 1->32->32->32->1 (ReLU between), bf16 weights,
+ * fp32 accumulation, input and output [B][1][H][W]. */
+typedef struct ccfj_cnn ccfj_cnn;
+int ccfj_cnn_param_count(void);
+int ccfj_cnn_create(ccfj_ctx *ctx, int H, int W, int max_batch, unsigned long long seed,
+                    ccfj_cnn **net);
+int ccfj_cnn_destroy(ccfj_cnn *net);
+/* fp32 master parameters, host arrays of ccfj_cnn_param_count() floats;
+ * set also resets the optimizer state. */
+int ccfj_cnn_set_params(ccfj_cnn *net, const float *params);
+int ccfj_cnn_get_params(ccfj_cnn *net, float *params);
+/* logits[B][H][W] (device) from x[B][H][W] (device). */
+int ccfj_cnn_forward(ccfj_cnn *net, const float *x, int B, float *logits, void *stream);
+/* Gradients of the mean binary cross-entropy w.r.t. every parameter (host
+ * grads[param_count]) and the loss (host). */
+int ccfj_cnn_gradients(ccfj_cnn *net, const float *x, const float *target, int B, float *grads,
+                       float *loss);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) at learning rate lr; loss (host,
+ * may be NULL) receives the batch loss before the step. */
+int ccfj_cnn_train_step(ccfj_cnn *net, const float *x, const float *target, int B, float lr,
+                        float *loss);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,13 @@ import sys
 import numpy as np
 import pytest
 
+# torch, when present, must bring up its HIP runtime before libccfj.so is
+# loaded, so that both share one runtime in this process
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (os.path.join(ROOT, "cc-fjpy_amd"), ROOT):
     if p not in sys.path:

@@ -233,7 +233,10 @@ def test_edges_and_errors(ccfj, ctx):
             emax, _, mism = O.column_error(got, want)
             assert mism == 0 and emax <= TOL, (itype, mode, emax)
     # no stations / one station: zeros, as the reference's empty loops
-    assert np.all(native(ctx, ccfj.MODE_J, 1, U[:1], r[:1], f, c) == 0)
+    # (also at f = 0, where a segment would divide by k = 0)
+    for itype in (0, 1):
+        for mode in (ccfj.MODE_J, ccfj.MODE_Y):
+            assert np.all(native(ctx, mode, itype, U[:1], r[:1], f, c) == 0)
     z = ctx.transform(ccfj.MODE_J, 0, False, np.zeros((0, 11), np.float32), None,
                       np.zeros(0, np.float32), f, c)
     assert z.shape == (7, 11) and np.all(z == 0)
@@ -272,7 +275,9 @@ def test_plain_fp64_baseline(ccfj, golden):
         for itype in (0, 1):
             out, ms = _ccfj.plain_fp64(0, yb, itype, U, r, f, c)
             emax, _, mism = O.column_error(out, O.oracle64(U, r, f, c, itype, yb))
-            assert mism == 0 and emax < 1e-6, (yb, itype, emax)
+            # FP64 throughout, but the Zhang-Jin Struve polynomials carry
+            # ~1e-8 relative error, amplified by the closed form's cancellation
+            assert mism == 0 and emax < TOL, (yb, itype, emax)
             assert ms > 0
 
 
