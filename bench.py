@@ -43,6 +43,7 @@ VARIANTS = [(0, 0), (0, 1), (1, 0), (1, 1)]   # (itype, func)
 FLOP_PER_TERM = {0: 34, 1: 92}                # per basis call
 FP32_PEAK = 157.3                             # TFLOP/s, MI355X
 HBM_PEAK = 8000.0                             # GB/s
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_fj_dense_noise_array_pmc.md")
 
 
 def calls(func):
@@ -61,6 +62,21 @@ def parse():
     p.add_argument("--copy-calib", action="store_true",
                    help="only run a device copy of known size (PMC byte-counter calibration)")
     return p.parse_args()
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of the F-J kernel for variant key, from the
+    committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or
+    None when that variant was not profiled."""
+    try:
+        with open(PMC_SUMMARY) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == key:
+                    return int(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def copy_calibration():
@@ -164,7 +180,7 @@ def main():
     dom = kernels[dom_key]
     achieved = dom["tflops_alg"]
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK, 4), "traffic": None,
+                "frac": round(achieved / FP32_PEAK, 4), "traffic": measured_traffic(dom_key),
                 "kernel": f"k_fj ({dom_key})", "avg_ms": dom["avg_ms"]}
     print(f"bench: kernels {json.dumps(kernels)}", file=sys.stderr, flush=True)
 

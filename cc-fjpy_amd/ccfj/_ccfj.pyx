@@ -56,6 +56,30 @@ cdef extern from "ccfj.h" nogil:
     int ccfj_fj_noise(ccfj_ctx *ctx, int itype, int func, int fstride, const float *uf,
                       const float *r, const float *f, const float *c, int nr, int ncol,
                       int nf, int nc, float *out)
+    int ccfj_malloc(ccfj_ctx *ctx, size_t bytes, void **ptr)
+    int ccfj_free(ccfj_ctx *ctx, void *ptr)
+    int ccfj_memcpy(ccfj_ctx *ctx, void *dst, const void *src, size_t bytes, int kind)
+    int ccfj_transform_items(ccfj_ctx *ctx, int mode, int itype, const float *U0, const float *U1,
+                             const float *r, const float *f, const float *c, int nr, int nf, int nc,
+                             const int *items, int nitems, float *out, void *stream)
+    int ccfj_synth_fields(ccfj_ctx *ctx, int B, int nr, int nf, const float *r, const float *f,
+                          const float *modes, const float *sigma, unsigned long long seed, float *U,
+                          void *stream)
+    int ccfj_synth_labels(ccfj_ctx *ctx, int B, int H, int nf, const float *cgrid, const float *f,
+                          const float *modes, float halfwidth, float amin, float *Y, void *stream)
+    ctypedef struct ccfj_cnn:
+        pass
+    int ccfj_cnn_param_count()
+    int ccfj_cnn_create(ccfj_ctx *ctx, int H, int W, int max_batch, unsigned long long seed,
+                        ccfj_cnn **net)
+    int ccfj_cnn_destroy(ccfj_cnn *net)
+    int ccfj_cnn_set_params(ccfj_cnn *net, const float *params)
+    int ccfj_cnn_get_params(ccfj_cnn *net, float *params)
+    int ccfj_cnn_forward(ccfj_cnn *net, const float *x, int B, float *logits, void *stream)
+    int ccfj_cnn_gradients(ccfj_cnn *net, const float *x, const float *target, int B, float *grads,
+                           float *loss)
+    int ccfj_cnn_train_step(ccfj_cnn *net, const float *x, const float *target, int B, float lr,
+                            float *loss)
     int ccfj_mwfj(ccfj_ctx *ctx, int itype, int func, int nr, int npts, int nwin, int nf,
                   int nc, int fstride, const double *u, const float *r, const float *f,
                   const float *c, const int *n1, const int *n2, const int *delta,
@@ -307,3 +331,133 @@ cdef class BatchPlan:
     @property
     def tile(self):
         return ccfj_batch_tile(self.plan)
+
+
+cdef class DeviceArray:
+    """A float32 or int32 array in device memory of a Context."""
+    cdef void *ptr
+    cdef Context owner
+    cdef readonly tuple shape
+    cdef readonly object dtype
+    cdef readonly size_t nbytes
+
+    def __cinit__(self, Context ctx, shape, dtype=np.float32):
+        self.ptr = NULL
+        self.owner = ctx
+        self.shape = tuple(int(n) for n in shape)
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.dtype(np.float32), np.dtype(np.int32)):
+            raise TypeError("DeviceArray holds float32 or int32")
+        self.nbytes = int(np.prod(self.shape)) * 4
+        _check(ccfj_malloc(ctx.ctx, self.nbytes if self.nbytes else 4, &self.ptr), "ccfj_malloc")
+
+    def __dealloc__(self):
+        if self.ptr != NULL:
+            ccfj_free(self.owner.ctx, self.ptr)
+            self.ptr = NULL
+
+    @property
+    def addr(self):
+        return <size_t>self.ptr
+
+    def offset(self, size_t elements):
+        """Address of element `elements` (flat index)."""
+        return <size_t>self.ptr + 4 * elements
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.nbytes != self.nbytes:
+            raise ValueError("size mismatch")
+        cdef const unsigned char[::1] v = a.reshape(-1).view(np.uint8)
+        if self.nbytes:
+            _check(ccfj_memcpy(self.owner.ctx, self.ptr, &v[0], self.nbytes, 0), "ccfj_memcpy")
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, self.dtype)
+        cdef unsigned char[::1] v = out.reshape(-1).view(np.uint8)
+        if self.nbytes:
+            _check(ccfj_memcpy(self.owner.ctx, &v[0], self.ptr, self.nbytes, 1), "ccfj_memcpy")
+        return out
+
+
+def device_array(Context ctx, a, dtype=np.float32):
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return DeviceArray(ctx, a.shape, dtype).upload(a)
+
+
+def transform_items(Context ctx, int mode, int itype, size_t U0, size_t U1, size_t r, size_t f,
+                    size_t c, int nr, int nf, int nc, size_t items, int nitems, size_t out,
+                    size_t stream=0):
+    cdef int rc
+    with nogil:
+        rc = ccfj_transform_items(ctx.ctx, mode, itype, <const float *>U0, <const float *>U1,
+                                  <const float *>r, <const float *>f, <const float *>c, nr, nf, nc,
+                                  <const int *>items, nitems, <float *>out, <void *>stream)
+    _check(rc, "ccfj_transform_items")
+
+
+def synth_fields(Context ctx, int B, int nr, int nf, size_t r, size_t f, size_t modes,
+                 size_t sigma, unsigned long long seed, size_t U):
+    _check(ccfj_synth_fields(ctx.ctx, B, nr, nf, <const float *>r, <const float *>f,
+                             <const float *>modes, <const float *>sigma, seed, <float *>U, NULL),
+           "ccfj_synth_fields")
+
+
+def synth_labels(Context ctx, int B, int H, int nf, size_t cgrid, size_t f, size_t modes,
+                 float halfwidth, float amin, size_t Y):
+    _check(ccfj_synth_labels(ctx.ctx, B, H, nf, <const float *>cgrid, <const float *>f,
+                             <const float *>modes, halfwidth, amin, <float *>Y, NULL),
+           "ccfj_synth_labels")
+
+
+def cnn_param_count():
+    return ccfj_cnn_param_count()
+
+
+cdef class CNN:
+    """Band-proposal CNN on a Context (HIP kernels, bf16 weights)."""
+    cdef ccfj_cnn *net
+    cdef Context owner
+    cdef readonly int H, W, max_batch
+
+    def __cinit__(self, Context ctx, int H, int W, int max_batch, unsigned long long seed=0):
+        self.net = NULL
+        self.owner = ctx
+        self.H, self.W, self.max_batch = H, W, max_batch
+        _check(ccfj_cnn_create(ctx.ctx, H, W, max_batch, seed, &self.net), "ccfj_cnn_create")
+
+    def __dealloc__(self):
+        if self.net != NULL:
+            ccfj_cnn_destroy(self.net)
+            self.net = NULL
+
+    def get_params(self):
+        p = np.empty(ccfj_cnn_param_count(), np.float32)
+        cdef float[::1] v = p
+        _check(ccfj_cnn_get_params(self.net, &v[0]), "ccfj_cnn_get_params")
+        return p
+
+    def set_params(self, params):
+        cdef float[::1] v = np.ascontiguousarray(params, dtype=np.float32)
+        if v.shape[0] != ccfj_cnn_param_count():
+            raise ValueError("parameter count mismatch")
+        _check(ccfj_cnn_set_params(self.net, &v[0]), "ccfj_cnn_set_params")
+
+    def forward(self, size_t x, int B, size_t logits):
+        _check(ccfj_cnn_forward(self.net, <const float *>x, B, <float *>logits, NULL),
+               "ccfj_cnn_forward")
+
+    def gradients(self, size_t x, size_t target, int B):
+        g = np.empty(ccfj_cnn_param_count(), np.float32)
+        cdef float[::1] v = g
+        cdef float loss = 0
+        _check(ccfj_cnn_gradients(self.net, <const float *>x, <const float *>target, B, &v[0],
+                                  &loss), "ccfj_cnn_gradients")
+        return g, loss
+
+    def train_step(self, size_t x, size_t target, int B, float lr):
+        cdef float loss = 0
+        _check(ccfj_cnn_train_step(self.net, <const float *>x, <const float *>target, B, lr,
+                                   &loss), "ccfj_cnn_train_step")
+        return loss

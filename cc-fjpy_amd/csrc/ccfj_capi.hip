@@ -399,6 +399,34 @@ int ccfj_synth_labels(ccfj_ctx* ctx, int B, int H, int nf, const float* cgrid, c
   return e == hipSuccess ? CCFJ_OK : hip_fail(e, "synth labels launch");
 }
 
+int ccfj_malloc(ccfj_ctx* ctx, size_t bytes, void** ptr) {
+  if (!ctx || !ptr) return fail(CCFJ_EINVAL, "null argument");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  return ccfj_internal::device_alloc(ptr, bytes);
+}
+
+int ccfj_free(ccfj_ctx* ctx, void* ptr) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e == hipSuccess && ptr) e = hipFree(ptr);
+  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "hipFree");
+}
+
+int ccfj_memcpy(ccfj_ctx* ctx, void* dst, const void* src, size_t bytes, int kind) {
+  if (!ctx || (bytes && (!dst || !src))) return fail(CCFJ_EINVAL, "null argument");
+  if (kind < 0 || kind > 2) return fail(CCFJ_EINVAL, "bad copy kind %d", kind);
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                        : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, k, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "hipMemcpyAsync");
+}
+
 int ccfj_synchronize(ccfj_ctx* ctx) {
   if (!ctx) return fail(CCFJ_EINVAL, "null context");
   int rc = set_device(ctx);

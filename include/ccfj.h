@@ -18,6 +18,8 @@
 #ifndef CCFJ_H
 #define CCFJ_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -84,6 +86,13 @@ int ccfj_transform_dev(ccfj_ctx *ctx, int mode, int itype, int normalize,
                        float *out, void *stream);
 
 int ccfj_synchronize(ccfj_ctx *ctx);
+
+/* Device memory on the context's device (for callers without their own
+ * allocator).  ccfj_memcpy is synchronous on the context's stream; kind:
+ * 0 host->device, 1 device->host, 2 device->device. */
+int ccfj_malloc(ccfj_ctx *ctx, size_t bytes, void **ptr);
+int ccfj_free(ccfj_ctx *ctx, void *ptr);
+int ccfj_memcpy(ccfj_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
 
 /* Kernel timing: when enabled, every main F-J kernel launch is bracketed by
  * HIP events on its stream; ccfj_get_timing returns the summed kernel time
