@@ -23,8 +23,6 @@ each higher mode U[300, 900] m/s faster, d ~ U[500, 2500], phi ~ U[0.05,
 10^(-SNR/20) with SNR ~ U[0, 20] dB; a label marks coarse cells within one
 cell of a curve whose amplitude is at least 0.15.
 """
-import time
-
 import numpy as np
 
 from . import _ccfj, _context, MODE_J
@@ -137,14 +135,15 @@ class Coarse2Fine:
         return out.download()
 
     # ------------------------------------------------------------ inference
-    def propose_items(self, logit, thresh=0.0, dilate=1):
+    def items_from_mask(self, mask, dilate=1):
         """Items (frequency, first dense velocity, count <= 256) covering the
-        proposed bands of one example (logit [H, W]; thresh 0 = p > 0.5).
-        Returns (items int32 [n, 3], dense terms per station)."""
+        coarse cells of mask [H, W] (dilated by `dilate` cells along velocity);
+        a frequency with no cell falls back to its full column.  Returns
+        (items int32 [n, 3], dense terms per station)."""
         dc = self.c_coarse[1] - self.c_coarse[0]
         items = []
         for i in range(self.W):
-            on = logit[:, i] > thresh
+            on = mask[:, i]
             if dilate:
                 on = np.convolve(on.astype(int), np.ones(2 * dilate + 1, int), "same") > 0
             if not on.any():
@@ -161,6 +160,15 @@ class Coarse2Fine:
                     items.append((i, s, min(256, int(b) - s)))
         items = np.array(items, np.int32).reshape(-1, 3)
         return items, int(items[:, 2].sum())
+
+    def propose_items(self, logit, thresh=0.0, dilate=1):
+        """Bands where the network's logit exceeds thresh (0: p > 0.5)."""
+        return self.items_from_mask(logit > thresh, dilate)
+
+    def coarse_items(self, X, level=0.5, dilate=1):
+        """The same banding without the network: coarse cells whose
+        normalised |F-J| reaches `level`."""
+        return self.items_from_mask(np.abs(X) >= level, dilate)
 
     def dense(self, U_rows, d_r_rows, items=None):
         """Dense F-J [nc_dense, W] of one field (device rows of U and r):
@@ -179,35 +187,53 @@ class Coarse2Fine:
         self.ctx.synchronize()
         return out.download()
 
-    def evaluate(self, n, seed, thresh=0.0, dilate=1, timing=True):
-        """Fresh fields: coarse F-J -> CNN -> banded dense F-J vs the full
-        grid.  Returns a dict with the ridge agreement (fraction of
-        frequencies whose pick is within one dense step of the full grid's),
-        the fraction of dense terms evaluated, and timings."""
+    def evaluate(self, n, seed, thresh=0.0, dilate=1, levels=(), use_net=True):
+        """Fresh fields: coarse F-J -> bands -> banded dense F-J, against the
+        full grid.  Bands come from the network (method "cnn", unless
+        use_net is False) and from thresholding the coarse spectrum at each
+        of `levels` (methods "coarse@level").  Per method: ridge agreement
+        (fraction of frequencies whose |F-J| argmax over velocity is within
+        one dense step of the full grid's), fraction of dense terms
+        evaluated, and main-kernel time (HIP events); plus the full grid's
+        kernel time."""
         rng = np.random.default_rng(seed)
         r, modes, sigma = sample_fields(rng, n, self.nr)
         U, d_r, _ = self._fields(r, modes, sigma, seed * 7919 + 1)
         X = _ccfj.DeviceArray(self.ctx, (n, self.H, self.W))
         self._coarse(U, d_r, n, X, 0)
-        lg = self.logits(X, n)
-        agree, frac, t_full, t_band = [], [], 0.0, 0.0
-        for b in range(n):
-            rows = U.offset(b * self.nr * self.W)
-            rb = d_r.offset(b * self.nr)
-            t0 = time.perf_counter()
-            full = self.dense(rows, rb)
-            t1 = time.perf_counter()
-            items, terms = self.propose_items(lg[b], thresh, dilate)
-            band = self.dense(rows, rb, items)
-            t2 = time.perf_counter()
-            t_full += t1 - t0
-            t_band += t2 - t1
-            pf = np.argmax(np.abs(full), axis=0)
-            pb = np.argmax(np.abs(band), axis=0)
-            agree.append(np.mean(np.abs(pf - pb) <= 1))
-            frac.append(terms / (self.nc_dense * self.W))
-        return dict(ridge_agreement=float(np.mean(agree)), evaluated_fraction=float(np.mean(frac)),
-                    time_full_s=t_full, time_banded_s=t_band, n=n)
+        Xh = X.download()
+        lg = self.logits(X, n) if use_net else None
+        methods = (["cnn"] if use_net else []) + [f"coarse@{v:g}" for v in levels]
+        stats = {m: {"agree": [], "frac": [], "ms": 0.0} for m in methods}
+        full_ms = 0.0
+        self.ctx.set_timing(True)
+        self.ctx.get_timing()
+        try:
+            for b in range(n):
+                rows = U.offset(b * self.nr * self.W)
+                rb = d_r.offset(b * self.nr)
+                full = self.dense(rows, rb)
+                full_ms += self.ctx.get_timing()[0]
+                pf = np.argmax(np.abs(full), axis=0)
+                for m in methods:
+                    if m == "cnn":
+                        items, terms = self.propose_items(lg[b], thresh, dilate)
+                    else:
+                        items, terms = self.coarse_items(Xh[b], float(m.split("@")[1]), dilate)
+                    band = self.dense(rows, rb, items)
+                    stats[m]["ms"] += self.ctx.get_timing()[0]
+                    pb = np.argmax(np.abs(band), axis=0)
+                    stats[m]["agree"].append(np.mean(np.abs(pf - pb) <= 1))
+                    stats[m]["frac"].append(terms / (self.nc_dense * self.W))
+        finally:
+            self.ctx.set_timing(False)
+        out = {"n": n, "full_kernel_ms": full_ms}
+        for m, v in stats.items():
+            out[m] = {"ridge_agreement": float(np.mean(v["agree"])),
+                      "worst_field_agreement": float(np.min(v["agree"])),
+                      "evaluated_fraction": float(np.mean(v["frac"])),
+                      "kernel_ms": v["ms"]}
+        return out
 
     def save(self, path):
         np.save(path, self.net.get_params())

@@ -6,10 +6,10 @@
 // candidate velocity tile (64/128/256 lanes per workgroup) on the first
 // sixteenth of the work items and keeps the fastest.  With use_graph != 0 the
 // whole launch sequence (station arrays, coefficients, counter reset,
-// persistent kernel, normalisation) is captured once into a HIP graph and
-// replayed while the device pointers stay the same.  The work-queue counter
-// is reset by a kernel, not a memset, so the captured graph holds no memset
-// node.
+// persistent kernel, transpose with optional normalisation) is captured
+// once into a HIP graph and replayed while the device pointers stay the
+// same.  The work-queue counter is reset by a kernel, not a memset, so the
+// captured graph holds no memset node.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,6 +19,7 @@
 #include "../../include/ccfj.h"
 #include "ccfj_internal.h"
 #include "fj_batched.h"
+#include "fj_kernels.h"
 #include "fj_lane.h"
 
 using ccfj_internal::fail;
@@ -79,10 +80,9 @@ int enqueue(ccfj_batch* p, hipStream_t st, unsigned int limit, int tile) {
   const void* k = fjk::batched_kernel(p->mode, p->itype, tile);
   e = fjk::launch_batched(k, tile, p->grid[tile_index(tile)], p->d, limit, st);
   if (e != hipSuccess) return hip_fail(e, "batched kernel launch");
-  if (p->normalize) {
-    e = fjk::launch_normalize_batched(p->d.out, p->nb, p->nf, p->nc, st);
-    if (e != hipSuccess) return hip_fail(e, "batched normalize launch");
-  }
+  e = fjk::launch_finish(p->d.T, p->normalize ? p->d.m : nullptr, p->nb, p->nf, p->nc, p->d.out,
+                         st);
+  if (e != hipSuccess) return hip_fail(e, "batched finish launch");
   return CCFJ_OK;
 }
 
@@ -169,7 +169,7 @@ int ccfj_batch_create(ccfj_ctx* ctx, int mode, int itype, int normalize, int nb,
   std::stable_sort(horder.begin(), horder.end(), [&](int a, int b) { return hnrp[a] > hnrp[b]; });
 
   int rc = CCFJ_OK;
-  void *rr, *hh, *coef, *counter;
+  void *rr, *hh, *coef, *counter, *T, *m;
   if (!rc) rc = upload(p, hsoff, &p->d.soff);
   if (!rc) rc = upload(p, hpoff, &p->d.poff);
   if (!rc) rc = upload(p, hcoff, &p->d.coff);
@@ -179,6 +179,8 @@ int ccfj_batch_create(ccfj_ctx* ctx, int mode, int itype, int normalize, int nb,
   if (!rc) rc = alloc(p, sizeof(float) * (pacc + 1), &hh);
   if (!rc) rc = alloc(p, sizeof(float) * (cacc + 1), &coef);
   if (!rc) rc = alloc(p, 64, &counter);
+  if (!rc) rc = alloc(p, sizeof(float) * (size_t)nb * nf * nc, &T);
+  if (!rc) rc = alloc(p, sizeof(float) * (size_t)nb * nf, &m);
   if (rc) {
     ccfj_batch_destroy(p);
     return rc;
@@ -190,6 +192,8 @@ int ccfj_batch_create(ccfj_ctx* ctx, int mode, int itype, int normalize, int nb,
   p->d.hh = (float*)hh;
   p->d.coef = (float*)coef;
   p->d.counter = (unsigned int*)counter;
+  p->d.T = (float*)T;
+  p->d.m = (float*)m;
 
   int cus = 0;
   e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);

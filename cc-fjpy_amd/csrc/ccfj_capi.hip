@@ -61,7 +61,7 @@ bool injected_alloc_failure() {
   return fail_at > 0 && n == fail_at;
 }
 
-enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_COEF, S_X0, S_X1, S_COUNT };
+enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_COEF, S_T, S_M, S_X0, S_X1, S_COUNT };
 
 }  // namespace
 
@@ -117,6 +117,28 @@ int set_device(ccfj_ctx* ctx) {
   return e == hipSuccess ? CCFJ_OK : hip_fail(e, "hipSetDevice");
 }
 
+// With timing on, events around the main kernel of each call (read back by
+// ccfj_get_timing).
+int timing_begin(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t>* ev) {
+  if (!ctx->timing) return CCFJ_OK;
+  hipError_t e;
+  if (!ctx->spare.empty()) {
+    *ev = ctx->spare.back();
+    ctx->spare.pop_back();
+  } else {
+    if ((e = hipEventCreate(&ev->first)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&ev->second)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+  }
+  (void)hipEventRecord(ev->first, st);
+  return CCFJ_OK;
+}
+
+void timing_end(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t> ev) {
+  if (!ctx->timing) return;
+  (void)hipEventRecord(ev.second, st);
+  ctx->events.push_back(ev);
+}
+
 // Enqueue prep + main kernel (+ normalisation) on st.  Inputs are device
 // pointers; workspace comes from ctx.
 int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U0,
@@ -124,15 +146,18 @@ int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U
                int nf, int nc, float* out, hipStream_t st) {
   if (nf == 0 || nc == 0) return CCFJ_OK;
   hipError_t e;
+  int rc;
+  if ((rc = ensure(ctx, S_T, sizeof(float) * (size_t)nf * nc))) return rc;
+  if (normalize && (rc = ensure(ctx, S_M, sizeof(float) * (size_t)nf))) return rc;
+  float* T = (float*)ctx->buf[S_T];
   if (nr < 2) {
     // no segment: the reference's sums are empty and stay 0 (even at f = 0,
     // where a segment would divide by k = 0)
-    e = hipMemsetAsync(out, 0, sizeof(float) * (size_t)nf * nc, st);
-    if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+    e = fjk::launch_fill(T, (size_t)nf * nc, 0.0f, st);
+    if (e != hipSuccess) return hip_fail(e, "fill kernel launch");
   } else {
     const int NC = fjk::coef_per_station(mode, itype);
     const int nrp = fjk::padded(nr);
-    int rc;
     if ((rc = ensure(ctx, S_RR, sizeof(double) * (nrp + 8)))) return rc;
     if ((rc = ensure(ctx, S_HH, sizeof(float) * (nrp + 8)))) return rc;
     if ((rc = ensure(ctx, S_COEF, sizeof(float) * (size_t)nf * nrp * NC))) return rc;
@@ -149,31 +174,17 @@ int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U
     p.rr = (double*)ctx->buf[S_RR];
     p.hh = (float*)ctx->buf[S_HH];
     p.coef = (float*)ctx->buf[S_COEF];
-    p.out = out;
+    p.T = T;
     e = fjk::launch_prep(mode, itype, p, st);
     if (e != hipSuccess) return hip_fail(e, "prep kernel launch");
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
-    if (ctx->timing) {
-      if (!ctx->spare.empty()) {
-        ev = ctx->spare.back();
-        ctx->spare.pop_back();
-      } else {
-        if ((e = hipEventCreate(&ev.first)) != hipSuccess) return hip_fail(e, "hipEventCreate");
-        if ((e = hipEventCreate(&ev.second)) != hipSuccess) return hip_fail(e, "hipEventCreate");
-      }
-      (void)hipEventRecord(ev.first, st);
-    }
+    if ((rc = timing_begin(ctx, st, &ev))) return rc;
     e = fjk::launch_fj(mode, itype, p, st);
     if (e != hipSuccess) return hip_fail(e, "F-J kernel launch");
-    if (ctx->timing) {
-      (void)hipEventRecord(ev.second, st);
-      ctx->events.push_back(ev);
-    }
+    timing_end(ctx, st, ev);
   }
-  if (normalize) {
-    e = fjk::launch_normalize(out, nf, nc, st);
-    if (e != hipSuccess) return hip_fail(e, "normalize kernel launch");
-  }
+  e = fjk::launch_finish(T, normalize ? (float*)ctx->buf[S_M] : nullptr, 1, nf, nc, out, st);
+  if (e != hipSuccess) return hip_fail(e, "finish kernel launch");
   return CCFJ_OK;
 }
 
@@ -348,9 +359,14 @@ int ccfj_transform_items(ccfj_ctx* ctx, int mode, int itype, const float* U0, co
   int rc = check_args(mode, itype, U0, U1, r, f, c, nr, nf, nc, out);
   if (rc) return rc;
   if (nitems < 0 || (nitems > 0 && !items)) return fail(CCFJ_EINVAL, "bad item list");
-  if (nr < 1 || nitems == 0) return CCFJ_OK;
+  if (nitems == 0) return CCFJ_OK;
   if ((rc = set_device(ctx))) return rc;
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  if (nr < 2) {   // no segment: listed entries are 0, as in the full grid
+    hipError_t e = fjk::launch_fj_items(mode, itype, nullptr, nullptr, nullptr, f, c, items,
+                                        nitems, nr, 0, nf, nc, out, st);
+    return e == hipSuccess ? CCFJ_OK : hip_fail(e, "item kernel launch");
+  }
   const int NC = fjk::coef_per_station(mode, itype);
   const int nrp = fjk::padded(nr);
   if ((rc = ensure(ctx, S_RR, sizeof(double) * (nrp + 8)))) return rc;
@@ -369,11 +385,16 @@ int ccfj_transform_items(ccfj_ctx* ctx, int mode, int itype, const float* U0, co
   p.rr = (double*)ctx->buf[S_RR];
   p.hh = (float*)ctx->buf[S_HH];
   p.coef = (float*)ctx->buf[S_COEF];
-  p.out = out;
+  p.T = nullptr;   // the item kernel writes out[c][f] directly
   hipError_t e = fjk::launch_prep(mode, itype, p, st);
   if (e != hipSuccess) return hip_fail(e, "prep kernel launch");
-  e = fjk::launch_fj_items(mode, itype, p.rr, p.hh, p.coef, f, c, items, nitems, nrp, nf, out, st);
-  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "item kernel launch");
+  std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+  if ((rc = timing_begin(ctx, st, &ev))) return rc;
+  e = fjk::launch_fj_items(mode, itype, p.rr, p.hh, p.coef, f, c, items, nitems, nr, nrp, nf, nc,
+                           out, st);
+  if (e != hipSuccess) return hip_fail(e, "item kernel launch");
+  timing_end(ctx, st, ev);
+  return CCFJ_OK;
 }
 
 int ccfj_synth_fields(ccfj_ctx* ctx, int B, int nr, int nf, const float* r, const float* f,

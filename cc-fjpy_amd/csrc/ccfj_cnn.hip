@@ -13,6 +13,7 @@
 #include "../../include/ccfj.h"
 #include "ccfj_internal.h"
 #include "fj_cnn.h"
+#include "fj_kernels.h"
 
 using ccfj_internal::fail;
 using ccfj_internal::hip_fail;
@@ -80,9 +81,9 @@ int forward(ccfj_cnn* n, const float* x, int B, hipStream_t st) {
 // gradients of the mean BCE w.r.t. all parameters into n->g; loss into n->loss
 int backward(ccfj_cnn* n, const float* x, const float* target, int B, hipStream_t st) {
   const int HW = n->H * n->W;
-  hipError_t e = hipMemsetAsync(n->g, 0, sizeof(float) * kOff.total, st);
-  if (e == hipSuccess) e = hipMemsetAsync(n->loss, 0, sizeof(float), st);
-  if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+  hipError_t e = fjk::launch_fill(n->g, kOff.total, 0.0f, st);
+  if (e == hipSuccess) e = fjk::launch_fill(n->loss, 1, 0.0f, st);
+  if (e != hipSuccess) return hip_fail(e, "fill kernel launch");
   // dL/dlogits -> gA (1 channel)
   e = fjk::cnn_bce(n->act[kLayers - 1], target, (int64_t)B * HW, n->gA, n->loss, st);
   if (e != hipSuccess) return hip_fail(e, "bce");
@@ -187,8 +188,8 @@ int ccfj_cnn_set_params(ccfj_cnn* n, const float* params) {
   hipStream_t st = ccfj_internal::ctx_stream(n->ctx);
   const size_t P = kOff.total;
   e = hipMemcpyAsync(n->p, params, sizeof(float) * P, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemsetAsync(n->m, 0, sizeof(float) * P, st);
-  if (e == hipSuccess) e = hipMemsetAsync(n->v, 0, sizeof(float) * P, st);
+  if (e == hipSuccess) e = fjk::launch_fill(n->m, P, 0.0f, st);
+  if (e == hipSuccess) e = fjk::launch_fill(n->v, P, 0.0f, st);
   if (e == hipSuccess) e = fjk::cnn_to_bf16(n->p, (int)P, n->pb, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   n->step = 0;

@@ -7,6 +7,7 @@
 
 #include "../../include/ccfj.h"
 #include "ccfj_internal.h"
+#include "fj_kernels.h"
 #include "fj_mwfj.h"
 
 using ccfj_internal::fail;
@@ -135,8 +136,8 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
     const size_t Kp = (size_t)(func ? 2 : 1) * d.half;
     if ((rc = bufs.get(sizeof(float) * nf * Kp * d.nrhs, &dB))) return rc;
     if ((rc = bufs.get(sizeof(float) * (size_t)nwin * nf * nc, &dT))) return rc;
-    e = hipMemsetAsync(dB, 0, sizeof(float) * nf * Kp * d.nrhs, st);
-    if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+    e = fjk::launch_fill((float*)dB, nf * Kp * d.nrhs, 0.0f, st);
+    if (e != hipSuccess) return hip_fail(e, "fill kernel launch");
     d.B = (float*)dB;
     d.T = (float*)dT;
   } else {

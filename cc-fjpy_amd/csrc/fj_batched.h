@@ -24,6 +24,8 @@ struct BatchDev {
   double* rr;
   float* hh;
   float* coef;
+  float* T;              // [nb][nf][nc] persistent-kernel results
+  float* m;              // [nb][nf] column scales (normalisation)
   float* out;            // [nb][nc][nf]
   unsigned int* counter; // work-queue head
 };
@@ -33,6 +35,5 @@ hipError_t launch_prep_batched(int mode, int itype, const BatchDev& d, int max_n
 const void* batched_kernel(int mode, int itype, int tile);
 hipError_t launch_batched(const void* kern, int tile, int grid, const BatchDev& d,
                           unsigned int limit, hipStream_t st);
-hipError_t launch_normalize_batched(float* out, int nb, int nf, int nc, hipStream_t st);
 
 }  // namespace fjk

@@ -14,7 +14,8 @@
 //   r      f32 [sum nr_b]
 //   rr, hh per sub-array b at poff[b], nrp_b + 8 entries each (padded)
 //   coef   per sub-array b at coff[b]: [nf][nrp_b][NC]
-//   out    f32 [nb][nc][nf]
+//   T      f32 [nb][nf][nc]     persistent-kernel results (coalesced along c)
+//   out    f32 [nb][nc][nf]     formed from T by k_finish (fj_kernels.hip)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -132,26 +133,10 @@ __global__ __launch_bounds__(TILE) void k_fj_b(BatchDev d, unsigned int limit) {
         v = trapz_lane<MODE>(rr, cf, nrp, q);
       else
         v = linseg_lane<MODE>(rr, d.hh + d.poff[b], cf, nrp, q);
-      d.out[((size_t)b * nc + j) * nf + fi] = v;
+      d.T[((size_t)b * nf + fi) * nc + j] = v;
     } else if (j < nc) {   // fewer than two stations: empty sum, as the reference
-      d.out[((size_t)b * nc + j) * nf + fi] = 0.0f;
+      d.T[((size_t)b * nf + fi) * nc + j] = 0.0f;
     }
-  }
-}
-
-// per-(sub-array, column) normalisation, builtin-max semantics
-__global__ void k_normalize_b(float* __restrict__ out, int nf, int nc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nf || nc <= 0) return;
-  float* o = out + (size_t)blockIdx.y * nc * nf;
-  float m = fabsf(o[i]);
-  for (int j = 1; j < nc; j++) {
-    const float v = fabsf(o[(size_t)j * nf + i]);
-    if (v > m) m = v;
-  }
-  for (int j = 0; j < nc; j++) {
-    const size_t idx = (size_t)j * nf + i;
-    o[idx] = o[idx] / m;
   }
 }
 
@@ -212,11 +197,6 @@ hipError_t launch_batched(const void* kern, int tile, int grid, const BatchDev& 
   BatchDev dd = d;
   void* args[] = {&dd, &limit};
   return hipLaunchKernel(kern, dim3(grid), dim3(tile), args, 0, st);
-}
-
-hipError_t launch_normalize_batched(float* out, int nb, int nf, int nc, hipStream_t st) {
-  hipLaunchKernelGGL(k_normalize_b, dim3((nf + 255) / 256, nb), dim3(256), 0, st, out, nf, nc);
-  return hipGetLastError();
 }
 
 }  // namespace fjk

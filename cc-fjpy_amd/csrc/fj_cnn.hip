@@ -113,6 +113,7 @@ __global__ __launch_bounds__(256) void k_conv_dw(const float* __restrict__ x,
   constexpr int NP = CO * CI;                 // (co, ci) pairs
   constexpr int PAIRS = (NP + 255) / 256;
   constexpr int RH = 8, RW = 16;
+  static_assert(NP <= 256 || NP % 256 == 0, "each thread owns whole (co, ci) pairs");
   __shared__ float xs[CI][RH + 2][RW + 2];
   __shared__ float gs[CO][RH][RW];
   const int b = blockIdx.y;

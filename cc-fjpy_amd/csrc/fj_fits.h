@@ -65,6 +65,53 @@ FJ_HD inline float fit_R1(float t) {
   return p;
 }
 
+/* far argument x >= 20, t = (8/x)^2 <= 0.16, float */
+constexpr float kFarX = 20.0f;
+/* max abs error of the fit on (0,0.16]: 2.66e-10 */
+FJ_HD inline float fitf_P0(float t) {
+  float p = 2.6872226653787023e-05f;
+  p = __builtin_fmaf(p, t, -0.00109860259568224f);
+  p = __builtin_fmaf(p, t, 0.999999999732733f);
+  return p;
+}
+/* max abs error of the fit on (0,0.16]: 9.89e-11 */
+FJ_HD inline float fitf_q0(float t) {
+  float p = -6.740746258951341e-06f;
+  p = __builtin_fmaf(p, t, 0.000143039894382146f);
+  p = __builtin_fmaf(p, t, -0.015624999900625058f);
+  return p;
+}
+/* max abs error of the fit on (0,0.16]: 3.15e-10 */
+FJ_HD inline float fitf_P1(float t) {
+  float p = -3.4602122207834955e-05f;
+  p = __builtin_fmaf(p, t, 0.0018310189187639315f);
+  p = __builtin_fmaf(p, t, 1.0000000003164382f);
+  return p;
+}
+/* max abs error of the fit on (0,0.16]: 1.14e-10 */
+FJ_HD inline float fitf_q1(float t) {
+  float p = 8.251415608533157e-06f;
+  p = __builtin_fmaf(p, t, -0.00020025860089406176f);
+  p = __builtin_fmaf(p, t, 0.04687499988512577f);
+  return p;
+}
+/* max abs error of the fit on (0,0.16]: 2.27e-09 */
+FJ_HD inline float fitf_R0(float t) {
+  float p = -0.0006925691349526906f;
+  p = __builtin_fmaf(p, t, 0.0021819717373648033f);
+  p = __builtin_fmaf(p, t, -0.015624529368443808f);
+  p = __builtin_fmaf(p, t, 0.999999997700073f);
+  return p;
+}
+/* max abs error of the fit on (0,0.16]: 3.52e-10 */
+FJ_HD inline float fitf_R1(float t) {
+  float p = 0.0001468112808747115f;
+  p = __builtin_fmaf(p, t, -0.0007300897303956774f);
+  p = __builtin_fmaf(p, t, 0.015624927646223167f);
+  p = __builtin_fmaf(p, t, 1.0000000003552583f);
+  return p;
+}
+
 /* small argument x < 8, y = (x/8)^2, double */
 /* max abs error of the fit on (0,1]: 8.04e-11 */
 FJ_HD inline double fit_J0s(double y) {

@@ -19,13 +19,18 @@ struct Problem {
   double* rr;    // [nrp + 8] workspace
   float* hh;     // [nrp + 8] workspace
   float* coef;   // [nf][nrp][NC] workspace
-  float* out;    // [nc][nf]
+  float* T;      // [nf][nc] main-kernel results (k_finish forms [nc][nf])
 };
 
 int coef_per_station(int mode, int itype);
 int padded(int nr);  // station count padded to the kernels' block
 hipError_t launch_prep(int mode, int itype, const Problem& p, hipStream_t st);
 hipError_t launch_fj(int mode, int itype, const Problem& p, hipStream_t st);
-hipError_t launch_normalize(float* out, int nf, int nc, hipStream_t st);
+// out[b][c][f] = T[b][f][c], divided by the column max |.| when m (workspace
+// [nb][nf]) is given; nb stacked problems of nf x nc.
+hipError_t launch_finish(const float* T, float* m, int nb, int nf, int nc, float* out,
+                         hipStream_t st);
+// p[0..n) = v (a kernel, so that a captured graph holds no memset node)
+hipError_t launch_fill(float* p, size_t n, float v, hipStream_t st);
 
 }  // namespace fjk

@@ -11,7 +11,10 @@
 //   coef[nf][nrp][NC] float    per (frequency, station) coefficients:
 //       itype 0: A_s of each spectrum          (NC = nspec)
 //       itype 1: alpha_s, beta_s per spectrum  (NC = 2 nspec)
-//   out[nc][nf]       float    (the reference's layout, src/FJcpu.cpp:247)
+//   T[nf][nc]         float    main-kernel results, one row per frequency
+//   out[nc][nf]       float    (the reference's layout, src/FJcpu.cpp:247),
+//                              formed from T by k_finish (optionally
+//                              normalised per frequency)
 //
 // Launch: a workgroup is 256 lanes = 256 consecutive velocities of ONE
 // frequency.  The station data of the loop are then wave-uniform and are
@@ -19,7 +22,10 @@
 // lane owns one (f,c) output and keeps the whole station loop in registers,
 // so the main kernels are VALU-bound by construction.  Workgroups are
 // numbered XCD-major so that the workgroups of one frequency share an XCD
-// and its L2.
+// and its L2.  A wave's 64 results are consecutive velocities of one
+// frequency, so they are stored into T[f][c] (one 256-byte segment per wave)
+// rather than strided into out[c][f], where every lane would dirty its own
+// cache line; k_finish transposes through LDS.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -112,7 +118,7 @@ __global__ __launch_bounds__(kBlock) void k_fj(const double* __restrict__ rr,
                                               const float* __restrict__ coef,
                                               const float* __restrict__ fv,
                                               const float* __restrict__ cv, int nrp, int nf,
-                                              int nc, float* __restrict__ out) {
+                                              int nc, float* __restrict__ T_out) {
   constexpr int NC = ITYPE == 0 ? ModeT<MODE>::nspec : 2 * ModeT<MODE>::nspec;
   const int ncb = (nc + kBlock - 1) / kBlock;
   int T;
@@ -127,24 +133,64 @@ __global__ __launch_bounds__(kBlock) void k_fj(const double* __restrict__ rr,
     v = trapz_lane<MODE>(rr, cf, nrp, q);
   else
     v = linseg_lane<MODE>(rr, hh, cf, nrp, q);
-  out[(size_t)j * nf + fi] = v;
+  T_out[(size_t)fi * nc + j] = v;
 }
 
-// Per-column normalisation (src/ccfj.pyx:225-226): out[:,i] /= max(|out[:,i]|)
-// with Python's builtin max over the column: a NaN in row 0 propagates, a NaN
-// later never compares greater and is skipped.
-__global__ void k_normalize(float* __restrict__ out, int nf, int nc) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nf || nc <= 0) return;
-  float m = fabsf(out[i]);
-  for (int j = 1; j < nc; j++) {
-    const float v = fabsf(out[(size_t)j * nf + i]);
-    if (v > m) m = v;
+// Per-frequency scale for the normalisation (src/ccfj.pyx:225-226,
+// out[:,i] /= max(abs(out[:,i]))), with Python's builtin-max semantics over
+// the column: a NaN in row 0 propagates; a later NaN never compares greater
+// and is skipped (fmaxf skips it too).  Block (f, b) reduces row f of
+// T[b] (contiguous), 256 lanes, then wave shuffles.
+__global__ __launch_bounds__(256) void k_rowmax(const float* __restrict__ T, int nf, int nc,
+                                                float* __restrict__ m) {
+  __shared__ float red[4];
+  const int f = blockIdx.x, b = blockIdx.y;
+  const float* row = T + ((size_t)b * nf + f) * nc;
+  float v = 0.0f;   // |values| >= 0, and row 0 is always part of the max
+  for (int j = threadIdx.x; j < nc; j += 256) v = fmaxf(v, fabsf(row[j]));
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float a0 = fabsf(row[0]);
+    const float r = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    m[(size_t)b * nf + f] = a0 != a0 ? a0 : r;
   }
-  for (int j = 0; j < nc; j++) {
-    const size_t idx = (size_t)j * nf + i;
-    out[idx] = out[idx] / m;
+}
+
+// out[b][c][f] = T[b][f][c] (/ m[b][f] when m is given): 64 x 64 tiles
+// through LDS, loads coalesced along c, stores along f.  The division (not a
+// reciprocal product) rounds as the reference's in-place division.
+__global__ __launch_bounds__(256) void k_finish(const float* __restrict__ T,
+                                                const float* __restrict__ m, int nf, int nc,
+                                                float* __restrict__ out) {
+  __shared__ float tile[64][65];
+  const int b = blockIdx.z;
+  const int c0 = blockIdx.x * 64, f0 = blockIdx.y * 64;
+  const size_t plane = (size_t)nf * nc;
+  const float* Tb = T + b * plane;
+  float* ob = out + b * plane;
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  for (int fl = grp; fl < 64; fl += 4) {
+    const int f = f0 + fl, c = c0 + lane;
+    if (f < nf && c < nc) tile[fl][lane] = Tb[(size_t)f * nc + c];
   }
+  __syncthreads();
+  const int f = f0 + lane;
+  const float scale_src = (m && f < nf) ? m[(size_t)b * nf + f] : 1.0f;
+  for (int cl = grp; cl < 64; cl += 4) {
+    const int c = c0 + cl;
+    if (f < nf && c < nc) {
+      const float v = tile[lane][cl];
+      ob[(size_t)c * nf + f] = m ? v / scale_src : v;
+    }
+  }
+}
+
+__global__ void k_fill(float* __restrict__ p, size_t n, float v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -155,10 +201,10 @@ static hipError_t launch_main(int itype, const Problem& p, hipStream_t st) {
   const long nblk = ((ntiles + 7) / 8) * 8;
   if (itype == 0)
     hipLaunchKernelGGL((k_fj<MODE, 0>), dim3((unsigned)nblk), dim3(kBlock), 0, st, p.rr, p.hh,
-                       p.coef, p.f, p.c, p.nrp, p.nf, p.nc, p.out);
+                       p.coef, p.f, p.c, p.nrp, p.nf, p.nc, p.T);
   else
     hipLaunchKernelGGL((k_fj<MODE, 1>), dim3((unsigned)nblk), dim3(kBlock), 0, st, p.rr, p.hh,
-                       p.coef, p.f, p.c, p.nrp, p.nf, p.nc, p.out);
+                       p.coef, p.f, p.c, p.nrp, p.nf, p.nc, p.T);
   return hipGetLastError();
 }
 
@@ -196,8 +242,22 @@ hipError_t launch_fj(int mode, int itype, const Problem& p, hipStream_t st) {
   return hipErrorInvalidValue;
 }
 
-hipError_t launch_normalize(float* out, int nf, int nc, hipStream_t st) {
-  hipLaunchKernelGGL(k_normalize, dim3((nf + 255) / 256), dim3(256), 0, st, out, nf, nc);
+hipError_t launch_finish(const float* T, float* m, int nb, int nf, int nc, float* out,
+                         hipStream_t st) {
+  if (nb <= 0 || nf <= 0 || nc <= 0) return hipSuccess;
+  if (m) {
+    hipLaunchKernelGGL(k_rowmax, dim3(nf, nb), dim3(256), 0, st, T, nf, nc, m);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(k_finish, dim3((nc + 63) / 64, (nf + 63) / 64, nb), dim3(256), 0, st, T,
+                     (const float*)m, nf, nc, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(float* p, size_t n, float v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p, n, v);
   return hipGetLastError();
 }
 

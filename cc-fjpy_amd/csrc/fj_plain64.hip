@@ -130,35 +130,39 @@ extern "C" int ccfj_plain_fp64(int device, int ybasis, int itype, const float* U
       hipMalloc(&dout, sizeof(float) * (size_t)nf * nc) != hipSuccess) {
     rc = CCFJ_ENOMEM;
   } else {
-    hipMemcpy(dU, U, sizeof(float) * nu, hipMemcpyHostToDevice);
-    hipMemcpy(dr, r, sizeof(float) * nr, hipMemcpyHostToDevice);
-    hipMemcpy(df, f, sizeof(float) * nf, hipMemcpyHostToDevice);
-    hipMemcpy(dc, c, sizeof(float) * nc, hipMemcpyHostToDevice);
-    dim3 block(32, 16), grid((nf + 31) / 32, (nc + 15) / 16);
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0, 0);
-    if (ybasis) {
-      if (itype) hipLaunchKernelGGL((k_plain<true, 1>), grid, block, 0, 0, dU, dr, df, dc, nr, nf, nc, dout);
-      else hipLaunchKernelGGL((k_plain<true, 0>), grid, block, 0, 0, dU, dr, df, dc, nr, nf, nc, dout);
-    } else {
-      if (itype) hipLaunchKernelGGL((k_plain<false, 1>), grid, block, 0, 0, dU, dr, df, dc, nr, nf, nc, dout);
-      else hipLaunchKernelGGL((k_plain<false, 0>), grid, block, 0, 0, dU, dr, df, dc, nr, nf, nc, dout);
-    }
-    hipEventRecord(e1, 0);
-    if (hipGetLastError() != hipSuccess) rc = CCFJ_EDEVICE;
-    if (hipMemcpy(out, dout, sizeof(float) * (size_t)nf * nc, hipMemcpyDeviceToHost) != hipSuccess)
+    hipError_t e = hipMemcpy(dU, U, sizeof(float) * nu, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dr, r, sizeof(float) * nr, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(df, f, sizeof(float) * nf, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dc, c, sizeof(float) * nc, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
       rc = CCFJ_EDEVICE;
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    if (kernel_ms) *kernel_ms = ms;
+    } else {
+      dim3 block(32, 16), grid((nf + 31) / 32, (nc + 15) / 16);
+      (void)hipEventRecord(e0, 0);
+      if (ybasis) {
+        if (itype) hipLaunchKernelGGL((k_plain<true, 1>), grid, block, 0, 0, dU, dr, df, dc, nr, nf, nc, dout);
+        else hipLaunchKernelGGL((k_plain<true, 0>), grid, block, 0, 0, dU, dr, df, dc, nr, nf, nc, dout);
+      } else {
+        if (itype) hipLaunchKernelGGL((k_plain<false, 1>), grid, block, 0, 0, dU, dr, df, dc, nr, nf, nc, dout);
+        else hipLaunchKernelGGL((k_plain<false, 0>), grid, block, 0, 0, dU, dr, df, dc, nr, nf, nc, dout);
+      }
+      (void)hipEventRecord(e1, 0);
+      if (hipGetLastError() != hipSuccess) rc = CCFJ_EDEVICE;
+      if (hipMemcpy(out, dout, sizeof(float) * (size_t)nf * nc, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = CCFJ_EDEVICE;
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      if (kernel_ms) *kernel_ms = ms;
+    }
   }
-  if (e0) hipEventDestroy(e0);
-  if (e1) hipEventDestroy(e1);
-  hipFree(dU);
-  hipFree(dr);
-  hipFree(df);
-  hipFree(dc);
-  hipFree(dout);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  (void)hipFree(dU);
+  (void)hipFree(dr);
+  (void)hipFree(df);
+  (void)hipFree(dc);
+  (void)hipFree(dout);
   return rc;
 }

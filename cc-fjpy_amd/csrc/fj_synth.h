@@ -15,6 +15,7 @@ hipError_t launch_synth_labels(const float* cgrid, const float* f, const float* 
                                hipStream_t st);
 hipError_t launch_fj_items(int mode, int itype, const double* rr, const float* hh,
                            const float* coef, const float* f, const float* c, const int* items,
-                           int nitems, int nrp, int nf, float* out, hipStream_t st);
+                           int nitems, int nr, int nrp, int nf, int nc, float* out,
+                           hipStream_t st);
 
 }  // namespace fjk

@@ -13,6 +13,9 @@ Large argument, x >= 8, variable t = (8/x)^2 in (0,1], evaluated in float:
   fit_P0(t) = P_0(x), fit_q0(t) = Q_0(x) x/8, fit_P1, fit_q1 likewise.
   Struve minus Neumann (A&S 12.1.29-12.1.31), smooth and non-oscillating:
     fit_R0(t) = (pi/2) x (H_0(x) - Y_0(x)),  fit_R1(t) = (pi/2) (H_1(x) - Y_1(x))
+Far argument, x >= 20 (t <= 0.16): the same six functions at lower degree,
+fitf_P0 ... fitf_R1 (degree 2 for P and Q, 3 for R; relative error below
+1e-8 on that range).
 Small argument, x < 8, variable y = (x/8)^2 in [0,1), evaluated in double
 (the monomial coefficients alternate in sign up to ~1e2, too much
 cancellation for float):
@@ -67,33 +70,36 @@ def small_truth(x):
 
 LARGE = [("fit_P0", 4), ("fit_q0", 4), ("fit_P1", 4), ("fit_q1", 4),
          ("fit_R0", 7), ("fit_R1", 6)]
+FAR_X = 20.0
+FAR = [("fitf_P0", 2), ("fitf_q0", 2), ("fitf_P1", 2), ("fitf_q1", 2),
+       ("fitf_R0", 3), ("fitf_R1", 3)]
 SMALL = [("fit_J0s", 11), ("fit_J1s", 10), ("fit_Y0s", 11), ("fit_Y1s", 10),
          ("fit_GJs", 10), ("fit_GYs", 11)]
 
 
-def fit_all(names, var, truth_fn, npts=160):
-    v = cheb_nodes(npts, 0.0, 1.0)
-    v = v[v > 1e-7]
+def fit_all(names, var, truth_fn, npts=160, top=1.0):
+    v = cheb_nodes(npts, 0.0, top)
+    v = v[v > 1e-7 * top]
     xs = 8 / np.sqrt(v) if var == "t" else 8 * np.sqrt(v)
     vals = np.array([[float(q) for q in truth_fn(x)] for x in xs])
     # independent check points
     rng = np.random.default_rng(1)
-    vc = np.sort(rng.uniform(1e-6, 1.0, 300))
+    vc = np.sort(rng.uniform(1e-6 * top, top, 300))
     xc = 8 / np.sqrt(vc) if var == "t" else 8 * np.sqrt(vc)
     valc = np.array([[float(q) for q in truth_fn(x)] for x in xc])
     res = []
     for j, (name, deg) in enumerate(names):
-        cheb = P.Chebyshev.fit(v, vals[:, j], deg, domain=[0, 1])
-        coef = cheb.convert(kind=P.Polynomial, domain=[0, 1], window=[0, 1]).coef
+        cheb = P.Chebyshev.fit(v, vals[:, j], deg, domain=[0, top])
+        coef = cheb.convert(kind=P.Polynomial, domain=[0, top], window=[0, top]).coef
         err = float(np.max(np.abs(P.polynomial.polyval(vc, coef) - valc[:, j])))
         res.append((name, coef, err))
     return res
 
 
-def emit(fname, coefs, ctype, err, var):
+def emit(fname, coefs, ctype, err, var, top=1.0):
     fma = "__builtin_fmaf" if ctype == "float" else "__builtin_fma"
     suf = "f" if ctype == "float" else ""
-    lines = [f"/* max abs error of the fit on (0,1]: {err:.2e} */",
+    lines = [f"/* max abs error of the fit on (0,{top:g}]: {err:.2e} */",
              f"FJ_HD inline {ctype} {fname}({ctype} {var}) {{",
              f"  {ctype} p = {float(coefs[-1])!r}{suf};"]
     for c in coefs[-2::-1]:
@@ -105,6 +111,8 @@ def emit(fname, coefs, ctype, err, var):
 
 def main():
     large = fit_all(LARGE, "t", large_truth)
+    far_top = (8.0 / FAR_X) ** 2
+    far = fit_all(FAR, "t", large_truth, npts=80, top=far_top)
     small = fit_all(SMALL, "y", small_truth)
     out = ["/* Generated by gen_fits.py -- do not edit by hand. */",
            "#pragma once",
@@ -116,13 +124,18 @@ def main():
     for name, coef, err in large:
         out.append(emit(name, coef, "float", err, "t"))
     out.append("")
+    out.append(f"/* far argument x >= {FAR_X:g}, t = (8/x)^2 <= {far_top:g}, float */")
+    out.append(f"constexpr float kFarX = {FAR_X!r}f;")
+    for name, coef, err in far:
+        out.append(emit(name, coef, "float", err, "t", far_top))
+    out.append("")
     out.append("/* small argument x < 8, y = (x/8)^2, double */")
     for name, coef, err in small:
         out.append(emit(name, coef, "double", err, "y"))
     path = os.path.join(HERE, "fj_fits.h")
     with open(path, "w") as fh:
         fh.write("\n".join(out) + "\n")
-    for name, coef, err in large + small:
+    for name, coef, err in large + far + small:
         print(f"{name:10s} deg {len(coef) - 1:2d} err {err:.2e}")
 
 

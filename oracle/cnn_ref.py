@@ -5,8 +5,11 @@ conv3x3(1->32)+ReLU, conv3x3(32->32)+ReLU, conv3x3(32->32)+ReLU,
 conv3x3(32->1), mean binary cross-entropy with logits -- the network of
 cc-fjpy_amd/csrc/fj_cnn.hip, with the same flat parameter layout
 (w1 b1 w2 b2 w3 b3 w4 b4).  Weights are rounded to bfloat16 (round to
-nearest even) before use, as the GPU kernels read them; biases are not.
-The tests also check this restatement against central finite differences.
+nearest even) before use, as the GPU kernels read them; biases are not
+(round_weights=False keeps them exact, which the finite-difference check of
+this restatement needs: with rounding the loss is piecewise constant in the
+weights).  The gradient is taken w.r.t. the weights as used, and applied to
+the fp32 master copy unchanged.
 """
 import numpy as np
 
@@ -56,23 +59,34 @@ def conv_back(x, w, dz):
     return dx, dw, dz.sum(axis=(0, 2, 3))
 
 
-def forward(params, x):
+def _w(w, round_weights):
+    return bf16(w) if round_weights else w
+
+
+def forward(params, x, round_weights=True):
     """x [B, H, W] -> logits [B, H, W] and the activations."""
     a = np.asarray(x, np.float64)[:, None]
     acts = [a]
     for l, (w, b) in enumerate(split(np.asarray(params, np.float64))):
-        z = conv(a, bf16(w), b)
+        z = conv(a, _w(w, round_weights), b)
         a = np.maximum(z, 0.0) if l < len(LAYERS) - 1 else z
         acts.append(a)
     return a[:, 0], acts
 
 
-def loss_and_grads(params, x, t):
+def loss(params, x, t, round_weights=True):
+    z, _ = forward(params, x, round_weights)
+    t = np.asarray(t, np.float64)
+    return np.sum(np.maximum(z, 0) - z * t + np.log1p(np.exp(-np.abs(z)))) / z.size
+
+
+def loss_and_grads(params, x, t, round_weights=True):
+    """Mean BCE-with-logits loss and its gradient (flat, parameter layout)."""
     params = np.asarray(params, np.float64)
-    z, acts = forward(params, x)
+    z, acts = forward(params, x, round_weights)
     t = np.asarray(t, np.float64)
     n = z.size
-    loss = np.sum(np.maximum(z, 0) - z * t + np.log1p(np.exp(-np.abs(z)))) / n
+    value = np.sum(np.maximum(z, 0) - z * t + np.log1p(np.exp(-np.abs(z)))) / n
     g = ((1.0 / (1.0 + np.exp(-z)) - t) / n)[:, None]
     grads = []
     layers = split(params)
@@ -80,10 +94,10 @@ def loss_and_grads(params, x, t):
         w, b = layers[l]
         if l < len(LAYERS) - 1:
             g = g * (acts[l + 1] > 0)
-        dx, dw, db = conv_back(acts[l], bf16(w), g)
+        dx, dw, db = conv_back(acts[l], _w(w, round_weights), g)
         grads.append((dw, db))
         g = dx
     flat = []
     for dw, db in reversed(grads):
         flat += [dw.ravel(), db.ravel()]
-    return loss, np.concatenate(flat)
+    return value, np.concatenate(flat)

@@ -32,3 +32,16 @@ def golden():
 
 def golden_case(g, name):
     return tuple(g[f"{name}/{k}"] for k in "Urfc")
+
+
+@pytest.fixture(scope="session")
+def ccfj():
+    """The package with its HIP library loaded and a GPU visible."""
+    import ccfj as m
+    assert m.device_count() >= 1, "no GPU visible"
+    return m
+
+
+@pytest.fixture(scope="session")
+def ctx(ccfj):
+    return ccfj._context(-1)

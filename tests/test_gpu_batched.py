@@ -12,13 +12,6 @@ TOL = 1e-5
 
 
 @pytest.fixture(scope="module")
-def ccfj():
-    import ccfj as m
-    assert m.device_count() >= 1, "no GPU visible"
-    return m
-
-
-@pytest.fixture(scope="module")
 def batch():
     from ccfj import synth
     d = synth.subarrays(seed=11, nb=37, nmin=2, nmax=90, nf=40, nc=150)

@@ -24,18 +24,6 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-5
 
 
-@pytest.fixture(scope="module")
-def ccfj():
-    import ccfj as m
-    assert m.device_count() >= 1, "no GPU visible"
-    return m
-
-
-@pytest.fixture(scope="module")
-def ctx(ccfj):
-    return ccfj._context(-1)
-
-
 def native(ctx, mode, itype, U0, r, f, c, U1=None, normalize=False):
     f32 = lambda a: np.ascontiguousarray(a, np.float32)
     return ctx.transform(mode, itype, normalize, f32(U0), None if U1 is None else f32(U1),
