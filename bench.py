@@ -14,7 +14,15 @@ frequency (double-precision rFFT on the device), as fj_noise does.
 Term-evals: nf * nc * nr per reference-equivalent native call; func = 0 is
 one call, func = 1 two (FJ and FH), so a step is 6 calls = 1.467e12.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--workload dense]
+
+Other workloads (--workload), each printing its own JSON line:
+  batched   4096 sub-arrays of 20-400 pairs (nf 512, nc 1024, itype 1,
+            func 0) in one persistent launch replayed from a HIP graph
+  mwfj      multi-window F-J of 480 receivers x 16384 samples, 64 windows,
+            nc 2000, f < 5 Hz, itype 0, func 0 (windows-as-RHS MFMA)
+  learned   the band-proposal CNN: training on device-generated spectra,
+            then banded dense F-J (nc 4000) against the full grid
 
 Prints one JSON line.  Besides the contract fields it carries
   roofline      the dominant kernel's algorithmic FP32 rate (34 flop per
@@ -59,6 +67,9 @@ def parse():
     p.add_argument("--kernel-reps", type=int, default=2)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--workload", default="dense", choices=["dense", "batched", "mwfj", "learned"])
+    p.add_argument("--train", type=int, default=50000, help="learned: training spectra")
+    p.add_argument("--epochs", type=int, default=3, help="learned: training epochs")
     p.add_argument("--copy-calib", action="store_true",
                    help="only run a device copy of known size (PMC byte-counter calibration)")
     return p.parse_args()
@@ -77,6 +88,14 @@ def measured_traffic(key):
     except OSError:
         pass
     return None
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            return next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
+    except Exception:
+        return ""
 
 
 def copy_calibration():
@@ -99,6 +118,9 @@ def main():
         sys.exit(2)
     if args.copy_calib:
         copy_calibration()
+        return
+    if args.workload != "dense":
+        {"batched": bench_batched, "mwfj": bench_mwfj, "learned": bench_learned}[args.workload](args)
         return
     import torch
     import ccfj
@@ -222,29 +244,207 @@ def main():
     if not args.no_cpu_baseline:
         from oracle import oracle as O
         cores = O.nthreads_default()
-        cols = np.array([2048], np.int32)
+        # the restatement parallelises over frequency columns (OpenMP over f,
+        # as FJcpu): one column per thread, every 4th velocity
+        cols = np.linspace(128, nf - 1, cores).astype(np.int32)
+        cs = np.ascontiguousarray(c[::4])
         print(f"bench: CPU baseline on {cores} threads", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
         terms = 0.0
         for it, fn in variants:
-            O.port_columns(U, r, f, c, it, 0, cols, nthreads=cores)
-            terms += synth.term_evals(nr, len(cols), nc)
+            O.port_columns(U, r, f, cs, it, 0, cols, nthreads=cores)
+            terms += synth.term_evals(nr, len(cols), len(cs))
             if fn == 1:
-                O.port_columns(H, r, f, c, it, 1, cols, nthreads=cores)
-                terms += synth.term_evals(nr, len(cols), nc)
+                O.port_columns(H, r, f, cs, it, 1, cols, nthreads=cores)
+                terms += synth.term_evals(nr, len(cols), len(cs))
         dt = time.perf_counter() - t0
-        cpu_model = ""
-        try:
-            with open("/proc/cpuinfo") as fh:
-                cpu_model = next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
-        except Exception:
-            pass
         out["cpu_baseline"] = {"value": terms / dt, "unit": "term-evals/s", "cores": cores,
                                "kind": "port",
-                               "sample": f"frequency columns {cols.tolist()} of {nf}, all {nc} "
-                                         f"velocities and {nr} pairs, every variant "
-                                         f"({terms:.3e} term-evals in {dt:.1f} s)",
-                               "cpu": cpu_model}
+                               "sample": f"{len(cols)} frequency columns of {nf} (one per thread), "
+                                         f"every 4th of {nc} velocities, all {nr} pairs, every "
+                                         f"variant ({terms:.3e} term-evals in {dt:.1f} s)",
+                               "cpu": cpu_model()}
+    print(json.dumps(out), flush=True)
+
+
+def contract(metric, value, unit, args, ms_per_step, config, dtype="f32"):
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+            "config": config}
+
+
+def bench_batched(args):
+    """4096 independent sub-arrays in one persistent launch (HIP graph)."""
+    import torch
+    import ccfj
+    from ccfj import _ccfj, synth
+    print("bench: generating 4096 sub-arrays", file=sys.stderr, flush=True)
+    d = synth.subarrays(seed=3)
+    ns = np.array([len(r) for r in d["rs"]])
+    soff = np.zeros(len(ns) + 1, np.int64)
+    soff[1:] = np.cumsum(ns)
+    U = np.concatenate(d["ufs"], axis=0)
+    r = np.concatenate(d["rs"]).astype(np.float32)
+    f, c = d["f"].astype(np.float32), d["c"].astype(np.float32)
+    nb, nf, nc = len(ns), len(f), len(c)
+    dev = torch.device("cuda:0")
+    tU, tr, tf, tc = (torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in (U, r, f, c))
+    tout = torch.empty((nb, nc, nf), dtype=torch.float32, device=dev)
+    ctx = ccfj._context(0)
+    tstream = torch.cuda.Stream(device=dev)
+    stream = tstream.cuda_stream
+    plan = ctx.batch_plan(ccfj.MODE_J, 1, True, soff, nf, nc)
+    torch.cuda.synchronize()
+
+    def run():
+        plan.run(tU.data_ptr(), tU.data_ptr(), tr.data_ptr(), tf.data_ptr(), tc.data_ptr(),
+                 tout.data_ptr(), stream, True)
+
+    print("bench: tuning + graph capture + warmup", file=sys.stderr, flush=True)
+    run()
+    for _ in range(args.warmup):
+        run()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(tstream)
+    for k in range(args.steps):
+        run()
+    e1.record(tstream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    terms = float(ns.sum()) * nf * nc
+    value = terms / (ms / 1e3)
+    achieved = terms * FLOP_PER_TERM[1] / (ms / 1e3) / 1e12
+    out = contract(METRIC, value, "term-evals/s", args, ms,
+                   {"workload": "batched sub-arrays fj_noise itype 1 func 0, one persistent "
+                                "launch replayed from a HIP graph, device-resident spectra",
+                    "sub_arrays": nb, "pairs_total": int(ns.sum()), "nf": nf, "nc": nc,
+                    "tile": plan.tile(), "term_evals_per_step": terms})
+    out["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK,
+                       "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK, 4), "traffic": None,
+                       "kernel": "whole graph (stations, prep, persistent k_fj_b, finish)"}
+    if not args.no_plain:
+        print("bench: plain FP64 GPU baseline (every 64th sub-array)", file=sys.stderr, flush=True)
+        pt, pms = 0.0, 0.0
+        for b in range(0, nb, 64):
+            Ub = np.ascontiguousarray(U[soff[b]:soff[b + 1]])
+            _, kms = _ccfj.plain_fp64(0, 0, 1, Ub, r[soff[b]:soff[b + 1]], f, c)
+            pms += kms
+            pt += synth.term_evals(ns[b], nf, nc)
+        prate = pt / (pms / 1e3)
+        out["plain_fp64_gpu"] = {"value": prate, "unit": "term-evals/s",
+                                 "sample": f"{len(range(0, nb, 64))} of {nb} sub-arrays "
+                                           "(every 64th), one call each, kernel time"}
+        out["speedup_vs_plain_fp64_gpu"] = round(value / prate, 2)
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        cores = O.nthreads_default()
+        t0 = time.perf_counter()
+        ct = 0.0
+        picks = list(range(0, nb, nb // 4))[:4]
+        for b in picks:
+            Ub = np.ascontiguousarray(U[soff[b]:soff[b + 1]])
+            O.port_native(Ub, r[soff[b]:soff[b + 1]], f, c, 1, 0, nthreads=cores)
+            ct += synth.term_evals(ns[b], nf, nc)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": ct / dt, "unit": "term-evals/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"sub-arrays {picks} in full ({ct:.3e} term-evals in "
+                                         f"{dt:.1f} s)", "cpu": cpu_model()}
+    print(json.dumps(out), flush=True)
+
+
+def bench_mwfj(args):
+    """MWFJ end to end (numpy in, numpy out) and its MFMA contraction."""
+    import torch  # noqa: F401  (brings up the HIP runtime first)
+    import ccfj
+    from ccfj import _ccfj, synth
+    d = synth.active_source(seed=2)
+    u, r, f, c = d["u"], d["r"], d["f"], d["c"]
+    nwin, nr, nf, nc = d["nwin"], len(r), len(f), len(c)
+    ctx = ccfj._context(0)
+
+    def run():
+        return ccfj.MWFJ(u, r, c, f, d["Fs"], nwin, d["winl"], d["winr"], 0.9, 1, 0, 0, 0)
+
+    print("bench: MWFJ warmup", file=sys.stderr, flush=True)
+    for _ in range(max(1, args.warmup)):
+        out0 = run()
+    ctx.set_timing(True)
+    ctx.get_timing()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run()
+    wall = (time.perf_counter() - t0) / args.steps
+    kms, nk = ctx.get_timing()
+    ctx.set_timing(False)
+    kms /= max(args.steps, 1)
+    terms = synth.term_evals(nr, nf, nc, 2 * nwin)   # fj_earthquake func 0: Re and Im calls
+    # contraction flops: 2 per (f, c, r, right-hand side), 2 nwin right-hand sides
+    mfma_tflops = 2.0 * nf * nc * nr * 2 * nwin / (kms / 1e3) / 1e12
+    out = contract(METRIC, terms / wall, "term-evals/s", args, wall * 1e3,
+                   {"workload": "MWFJ end to end (host u in, float64 [nwin, nc, nf] out), "
+                                "itype 0 func 0, windows as right-hand sides of one MFMA "
+                                "contraction per frequency",
+                    "receivers": nr, "npts": u.shape[1], "nwin": nwin, "nf": nf, "nc": nc,
+                    "term_evals_per_step": terms})
+    out["contraction"] = {"kernel_ms": round(kms, 3), "launches_per_step": nk / max(args.steps, 1),
+                          "term_evals_per_s": terms / (kms / 1e3),
+                          "mfma_fp32_tflops": round(mfma_tflops, 2)}
+    out["roofline"] = {"bound": "mfma", "achieved": round(mfma_tflops, 2), "peak": FP32_PEAK,
+                       "unit": "TFLOP/s", "frac": round(mfma_tflops / FP32_PEAK, 4),
+                       "traffic": None, "kernel": "k_mwfj_mfma"}
+    if not args.no_plain:
+        print("bench: plain FP64 GPU baseline (4 windows)", file=sys.stderr, flush=True)
+        idx = np.argsort(r)
+        pt, pms = 0.0, 0.0
+        for w in range(0, nwin, nwin // 4):
+            tap = np.stack([ccfj.win(u.shape[1], d["Fs"], d["winl"][w, j], d["winr"][w, j], 0.9)
+                            for j in range(nr)])
+            spec = np.fft.rfft(u.astype(np.float64) * tap, axis=1)[idx][:, :nf]
+            for part in (spec.real, spec.imag):
+                _, kms2 = _ccfj.plain_fp64(0, 0, 0, np.ascontiguousarray(part, np.float32),
+                                           r[idx].astype(np.float32), f.astype(np.float32),
+                                           c.astype(np.float32))
+                pms += kms2
+                pt += synth.term_evals(nr, nf, nc)
+        prate = pt / (pms / 1e3)
+        out["plain_fp64_gpu"] = {"value": prate, "unit": "term-evals/s",
+                                 "sample": "4 of 64 windows, Re and Im calls, kernel time"}
+        out["speedup_vs_plain_fp64_gpu"] = round(out["value"] / prate, 2)
+    print(json.dumps(out), flush=True)
+
+
+def bench_learned(args):
+    """Band-proposal CNN: train on device-generated coarse spectra, then the
+    banded dense F-J against the full grid."""
+    import torch  # noqa: F401
+    from ccfj import learn
+    m = learn.Coarse2Fine(num=0, seed=0)
+    t0 = time.perf_counter()
+    X, Y, _ = m.make_set(args.train, seed=1)
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    losses = m.train(X, Y, args.train, epochs=args.epochs, lr=2e-3,
+                     log=lambda s: print("bench:", s, file=sys.stderr, flush=True))
+    m.ctx.synchronize()
+    t_train = time.perf_counter() - t0
+    res = m.evaluate(64, seed=7, levels=(0.5, 0.3))
+    cnn = res["cnn"]
+    full_ms, band_ms = res["full_kernel_ms"], cnn["kernel_ms"]
+    dense_terms = 64 * m.nr * m.W * m.nc_dense
+    value = dense_terms / (band_ms / 1e3)   # full-grid-equivalent term-evals/s of the banded path
+    out = contract("banded dense F-J, full-grid-equivalent term-evals/s at the ridge agreement "
+                   "below", value, "term-evals/s", args, band_ms / 64,
+                   {"workload": "learned coarse-to-fine: coarse 64c x 256f F-J of 300-pair fields, "
+                                "CNN bands, dense nc 4000 (0.01-1 Hz) only inside them",
+                    "training_spectra": args.train, "epochs": args.epochs, "eval_fields": 64})
+    out["ridge"] = res
+    out["training"] = {"data_s": round(t_data, 2), "train_s": round(t_train, 2),
+                       "steps": len(losses), "first_loss": float(losses[0]),
+                       "last_loss": float(np.mean(losses[-20:]))}
+    out["speedup_vs_full_grid"] = round(full_ms / band_ms, 2)
     print(json.dumps(out), flush=True)
 
 

@@ -14,6 +14,14 @@ descriptions leave open are fixed here.
                          c_m(f) = cinf_m + d_m exp(-f/phi_m), plus noise.
                          Only the real part of uf is used by the transform,
                          so `real_only=True` skips the imaginary noise.
+  subarrays(seed=3)      4096 sub-arrays of 20-400 pairs, nf=512 (0.02-1 Hz),
+                         nc=1024 (1000-5000 m/s), two modes each.
+  active_source(seed=2)  480 receivers at 1-120 km, 16384 samples at 100 Hz:
+                         three surface-wave modes from a 1 Hz Ricker source
+                         with 1/sqrt(r) spreading plus 5% white noise, and 64
+                         group-velocity windows (equal width, 50% overlap)
+                         tiling 1.5-4.5 km/s; nc=2000 (1500-4500 m/s),
+                         f = k Fs/npts for k < 820 (0-5 Hz).
 """
 import numpy as np
 from scipy.special import j0
@@ -101,6 +109,37 @@ def subarrays(seed=3, nb=4096, nmin=20, nmax=400, nf=512, nc=1024, real_only=Tru
         ufs.append(acc.astype(np.float32 if real_only else np.complex64))
         rs.append(r)
     return dict(ufs=ufs, rs=rs, c=c, f=f, ns=ns)
+
+
+ACTIVE_MODES = dict(cinf=(2200.0, 3000.0, 3600.0), delta=(1400.0, 1300.0, 800.0),
+                    phi=(0.5, 1.0, 1.5), amp=(1.0, 0.6, 0.3))
+
+
+def active_source(seed=2, nr=480, npts=16384, Fs=100.0, nwin=64, nf=820, nc=2000):
+    """Multi-window workload: u float32 [nr, npts], r [nr] (m), winl/winr
+    [nwin, nr] (s), and the grids f, c, Fs."""
+    rng = np.random.default_rng(seed)
+    r = np.linspace(1e3, 120e3, nr)
+    fa = np.fft.rfftfreq(npts, 1.0 / Fs)
+    fp = 1.0
+    ricker = (2.0 / np.sqrt(np.pi)) * fa ** 2 / fp ** 3 * np.exp(-(fa / fp) ** 2)
+    spec = np.zeros((nr, len(fa)), np.complex128)
+    m = ACTIVE_MODES
+    for ci, d, ph, a in zip(m["cinf"], m["delta"], m["phi"], m["amp"]):
+        cm = ci + d * np.exp(-fa / ph)
+        spec += a * ricker[None, :] * np.exp(-2j * np.pi * fa[None, :] * r[:, None] / cm[None, :])
+    u = np.fft.irfft(spec, n=npts, axis=1) / np.sqrt(r)[:, None]
+    rms = np.sqrt(np.mean(u ** 2))
+    u += rng.normal(0.0, 0.05 * rms, u.shape)
+    w = 3.0 / (nwin + 1) * 2.0          # window width, km/s; steps of w/2 tile [1.5, 4.5]
+    vlo = 1.5 + np.arange(nwin) * w / 2
+    vhi = vlo + w
+    winl = r[None, :] / (vhi[:, None] * 1e3)
+    winr = r[None, :] / (vlo[:, None] * 1e3)
+    f = np.arange(nf) * Fs / npts
+    c = np.linspace(1500.0, 4500.0, nc)
+    return dict(u=u.astype(np.float32), r=r, winl=winl, winr=winr, f=f, c=c, Fs=Fs,
+                nwin=nwin)
 
 
 def term_evals(nr, nf, nc, calls=1):

@@ -48,6 +48,8 @@ namespace {
 using ccfj_internal::fail;
 using ccfj_internal::g_last_error;
 using ccfj_internal::hip_fail;
+using ccfj_internal::timing_begin;
+using ccfj_internal::timing_end;
 
 // Fault injection for tests: CCFJ_FAIL_ALLOC_AT=N makes the N-th device
 // allocation of the process (1-based) fail with CCFJ_ENOMEM.
@@ -61,7 +63,7 @@ bool injected_alloc_failure() {
   return fail_at > 0 && n == fail_at;
 }
 
-enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_COEF, S_T, S_M, S_X0, S_X1, S_COUNT };
+enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_IR, S_ISR, S_COEF, S_T, S_M, S_X0, S_X1, S_COUNT };
 
 }  // namespace
 
@@ -117,28 +119,6 @@ int set_device(ccfj_ctx* ctx) {
   return e == hipSuccess ? CCFJ_OK : hip_fail(e, "hipSetDevice");
 }
 
-// With timing on, events around the main kernel of each call (read back by
-// ccfj_get_timing).
-int timing_begin(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t>* ev) {
-  if (!ctx->timing) return CCFJ_OK;
-  hipError_t e;
-  if (!ctx->spare.empty()) {
-    *ev = ctx->spare.back();
-    ctx->spare.pop_back();
-  } else {
-    if ((e = hipEventCreate(&ev->first)) != hipSuccess) return hip_fail(e, "hipEventCreate");
-    if ((e = hipEventCreate(&ev->second)) != hipSuccess) return hip_fail(e, "hipEventCreate");
-  }
-  (void)hipEventRecord(ev->first, st);
-  return CCFJ_OK;
-}
-
-void timing_end(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t> ev) {
-  if (!ctx->timing) return;
-  (void)hipEventRecord(ev.second, st);
-  ctx->events.push_back(ev);
-}
-
 // Enqueue prep + main kernel (+ normalisation) on st.  Inputs are device
 // pointers; workspace comes from ctx.
 int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U0,
@@ -160,6 +140,8 @@ int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U
     const int nrp = fjk::padded(nr);
     if ((rc = ensure(ctx, S_RR, sizeof(double) * (nrp + 8)))) return rc;
     if ((rc = ensure(ctx, S_HH, sizeof(float) * (nrp + 8)))) return rc;
+    if ((rc = ensure(ctx, S_IR, sizeof(float) * (nrp + 8)))) return rc;
+    if ((rc = ensure(ctx, S_ISR, sizeof(float) * (nrp + 8)))) return rc;
     if ((rc = ensure(ctx, S_COEF, sizeof(float) * (size_t)nf * nrp * NC))) return rc;
     fjk::Problem p;
     p.U0 = U0;
@@ -173,6 +155,8 @@ int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U
     p.nc = nc;
     p.rr = (double*)ctx->buf[S_RR];
     p.hh = (float*)ctx->buf[S_HH];
+    p.ir = (float*)ctx->buf[S_IR];
+    p.isr = (float*)ctx->buf[S_ISR];
     p.coef = (float*)ctx->buf[S_COEF];
     p.T = T;
     e = fjk::launch_prep(mode, itype, p, st);
@@ -237,6 +221,26 @@ int ctx_scratch(ccfj_ctx* ctx, int which, size_t bytes, void** p) {
   return CCFJ_OK;
 }
 hipStream_t ctx_stream(const ccfj_ctx* ctx) { return ctx->stream; }
+
+int timing_begin(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t>* ev) {
+  if (!ctx->timing) return CCFJ_OK;
+  hipError_t e;
+  if (!ctx->spare.empty()) {
+    *ev = ctx->spare.back();
+    ctx->spare.pop_back();
+  } else {
+    if ((e = hipEventCreate(&ev->first)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+    if ((e = hipEventCreate(&ev->second)) != hipSuccess) return hip_fail(e, "hipEventCreate");
+  }
+  (void)hipEventRecord(ev->first, st);
+  return CCFJ_OK;
+}
+
+void timing_end(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t> ev) {
+  if (!ctx->timing) return;
+  (void)hipEventRecord(ev.second, st);
+  ctx->events.push_back(ev);
+}
 
 }  // namespace ccfj_internal
 
@@ -363,7 +367,7 @@ int ccfj_transform_items(ccfj_ctx* ctx, int mode, int itype, const float* U0, co
   if ((rc = set_device(ctx))) return rc;
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
   if (nr < 2) {   // no segment: listed entries are 0, as in the full grid
-    hipError_t e = fjk::launch_fj_items(mode, itype, nullptr, nullptr, nullptr, f, c, items,
+    hipError_t e = fjk::launch_fj_items(mode, itype, fjl::Stations{}, nullptr, f, c, items,
                                         nitems, nr, 0, nf, nc, out, st);
     return e == hipSuccess ? CCFJ_OK : hip_fail(e, "item kernel launch");
   }
@@ -371,6 +375,8 @@ int ccfj_transform_items(ccfj_ctx* ctx, int mode, int itype, const float* U0, co
   const int nrp = fjk::padded(nr);
   if ((rc = ensure(ctx, S_RR, sizeof(double) * (nrp + 8)))) return rc;
   if ((rc = ensure(ctx, S_HH, sizeof(float) * (nrp + 8)))) return rc;
+  if ((rc = ensure(ctx, S_IR, sizeof(float) * (nrp + 8)))) return rc;
+  if ((rc = ensure(ctx, S_ISR, sizeof(float) * (nrp + 8)))) return rc;
   if ((rc = ensure(ctx, S_COEF, sizeof(float) * (size_t)nf * nrp * NC))) return rc;
   fjk::Problem p;
   p.U0 = U0;
@@ -384,14 +390,16 @@ int ccfj_transform_items(ccfj_ctx* ctx, int mode, int itype, const float* U0, co
   p.nc = nc;
   p.rr = (double*)ctx->buf[S_RR];
   p.hh = (float*)ctx->buf[S_HH];
+  p.ir = (float*)ctx->buf[S_IR];
+  p.isr = (float*)ctx->buf[S_ISR];
   p.coef = (float*)ctx->buf[S_COEF];
   p.T = nullptr;   // the item kernel writes out[c][f] directly
   hipError_t e = fjk::launch_prep(mode, itype, p, st);
   if (e != hipSuccess) return hip_fail(e, "prep kernel launch");
   std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
   if ((rc = timing_begin(ctx, st, &ev))) return rc;
-  e = fjk::launch_fj_items(mode, itype, p.rr, p.hh, p.coef, f, c, items, nitems, nr, nrp, nf, nc,
-                           out, st);
+  e = fjk::launch_fj_items(mode, itype, fjk::stations(p), p.coef, f, c, items, nitems, nr, nrp, nf,
+                           nc, out, st);
   if (e != hipSuccess) return hip_fail(e, "item kernel launch");
   timing_end(ctx, st, ev);
   return CCFJ_OK;

@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "../../include/ccfj.h"
 
 namespace ccfj_internal {
@@ -15,5 +17,9 @@ int ctx_device(const ccfj_ctx* ctx);
 // Grow-only scratch buffer `which` (0 or 1) of the context, at least bytes.
 int ctx_scratch(ccfj_ctx* ctx, int which, size_t bytes, void** p);
 hipStream_t ctx_stream(const ccfj_ctx* ctx);
+// With the context's timing on (ccfj_set_timing), events around a main
+// kernel, summed by ccfj_get_timing; no-ops otherwise.
+int timing_begin(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t>* ev);
+void timing_end(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t> ev);
 
 }  // namespace ccfj_internal

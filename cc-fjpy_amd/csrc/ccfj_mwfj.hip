@@ -170,7 +170,10 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
     }
   }
   if (itype == 0) {
+    std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+    if ((rc = ccfj_internal::timing_begin(ctx, st, &ev))) return rc;
     if ((e = fjk::launch_mwfj_mfma(d, st)) != hipSuccess) return hip_fail(e, "MFMA kernel");
+    ccfj_internal::timing_end(ctx, st, ev);
     if ((e = fjk::launch_mwfj_finish(d, st)) != hipSuccess) return hip_fail(e, "finish kernel");
   }
   e = hipMemcpyAsync(out, dout, sizeof(double) * (size_t)nwin * nc * nf, hipMemcpyDeviceToHost, st);

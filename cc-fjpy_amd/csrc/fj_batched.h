@@ -23,6 +23,8 @@ struct BatchDev {
   const float* c;
   double* rr;
   float* hh;
+  float* ir;
+  float* isr;
   float* coef;
   float* T;              // [nb][nf][nc] persistent-kernel results
   float* m;              // [nb][nf] column scales (normalisation)

@@ -12,7 +12,7 @@
 // Layout (device):
 //   U0/U1  f32 [sum nr_b][nf]        rows of all sub-arrays, each sorted by r
 //   r      f32 [sum nr_b]
-//   rr, hh per sub-array b at poff[b], nrp_b + 8 entries each (padded)
+//   rr, hh, ir, isr  per sub-array b at poff[b], nrp_b + 8 entries each (padded)
 //   coef   per sub-array b at coff[b]: [nf][nrp_b][NC]
 //   T      f32 [nb][nf][nc]     persistent-kernel results (coalesced along c)
 //   out    f32 [nb][nc][nf]     formed from T by k_finish (fj_kernels.hip)
@@ -33,16 +33,8 @@ __global__ void k_stations_b(BatchDev d) {
   const int nr = (int)(d.soff[b + 1] - s0);
   const int ntot = d.nrp[b] + kUnroll;
   if (s >= ntot || nr <= 0) return;
-  double* rr = d.rr + d.poff[b];
-  float* hh = d.hh + d.poff[b];
-  const float* r = d.r + s0;
-  if (s < nr) {
-    rr[s] = (double)r[s];
-    hh[s] = s > 0 ? r[s] - r[s - 1] : 0.0f;
-  } else {
-    rr[s] = (double)r[nr - 1];
-    hh[s] = 0.0f;
-  }
+  const int64_t o = d.poff[b];
+  station_entry(d.r + s0, nr, s, d.rr + o, d.hh + o, d.ir + o, d.isr + o);
 }
 
 template <int ITYPE, int NSPEC>
@@ -123,17 +115,18 @@ __global__ __launch_bounds__(TILE) void k_fj_b(BatchDev d, unsigned int limit) {
     const int j = (int)cb * TILE + (int)threadIdx.x;
     const int64_t s0 = d.soff[b];
     const int nr = (int)(d.soff[b + 1] - s0);
-    if (j < nc && nr > 1) {
+    if (nr > 1) {   // uniform over the workgroup: every lane runs the lane code
       const int nrp = d.nrp[b];
-      const double q = (double)d.f[fi] / (double)d.c[j];
+      const double q = (double)d.f[fi] / (double)d.c[j < nc ? j : nc - 1];
       const float* __restrict__ cf = d.coef + d.coff[b] + (size_t)fi * nrp * NC;
-      const double* __restrict__ rr = d.rr + d.poff[b];
+      const int64_t o = d.poff[b];
+      const Stations S{d.rr + o, d.hh + o, d.ir + o, d.isr + o};
       float v;
       if (ITYPE == 0)
-        v = trapz_lane<MODE>(rr, cf, nrp, q);
+        v = trapz_lane<MODE>(S, cf, nrp, q);
       else
-        v = linseg_lane<MODE>(rr, d.hh + d.poff[b], cf, nrp, q);
-      d.T[((size_t)b * nf + fi) * nc + j] = v;
+        v = linseg_lane<MODE>(S, cf, nrp, q);
+      if (j < nc) d.T[((size_t)b * nf + fi) * nc + j] = v;
     } else if (j < nc) {   // fewer than two stations: empty sum, as the reference
       d.T[((size_t)b * nf + fi) * nc + j] = 0.0f;
     }

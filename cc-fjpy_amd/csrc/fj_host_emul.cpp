@@ -24,12 +24,11 @@ void run_mode(int itype, const float* U0, const float* U1, const float* r, const
   const int NC = coef_count(MODE, itype);
   const int nrp = padded_stations(nr);
   // station arrays padded as on the device
-  std::vector<double> rr(nrp + kUnroll, (double)r[nr - 1]);
-  std::vector<float> hh(nrp + kUnroll, 0.0f);
-  for (int s = 0; s < nr; s++) {
-    rr[s] = r[s];
-    hh[s] = s > 0 ? r[s] - r[s - 1] : 0.0f;
-  }
+  const int ntot = nrp + kUnroll;
+  std::vector<double> rr(ntot);
+  std::vector<float> hh(ntot), ir(ntot), isr(ntot);
+  for (int s = 0; s < ntot; s++) station_entry(r, nr, s, rr.data(), hh.data(), ir.data(), isr.data());
+  const Stations S{rr.data(), hh.data(), ir.data(), isr.data()};
   std::vector<float> cf((size_t)nrp * NC, 0.0f);
 #pragma omp parallel for schedule(dynamic, 1) firstprivate(cf)
   for (int i = 0; i < nf; i++) {
@@ -53,8 +52,8 @@ void run_mode(int itype, const float* U0, const float* U1, const float* r, const
     }
     for (int j = 0; j < nc; j++) {
       const double q = (double)f[i] / (double)c[j];
-      out[(size_t)j * nf + i] = itype == 0 ? trapz_lane<MODE>(rr.data(), cf.data(), nrp, q)
-                                           : linseg_lane<MODE>(rr.data(), hh.data(), cf.data(), nrp, q);
+      out[(size_t)j * nf + i] = itype == 0 ? trapz_lane<MODE>(S, cf.data(), nrp, q)
+                                           : linseg_lane<MODE>(S, cf.data(), nrp, q);
     }
   }
 }
@@ -65,7 +64,7 @@ extern "C" int fjemul_transform(int mode, int itype, const float* U0, const floa
                                 const float* r, const float* f, const float* c, int nr, int nf,
                                 int nc, float* out) {
   if (itype != 0 && itype != 1) return CCFJ_EINVAL;
-  if (nr <= 0) {
+  if (nr < 2) {   // no segment: zeros, as the device path
     for (size_t k = 0; k < (size_t)nf * nc; k++) out[k] = 0.f;
     return 0;
   }
@@ -80,28 +79,30 @@ extern "C" int fjemul_transform(int mode, int itype, const float* U0, const floa
   return 0;
 }
 
-// Scalar access to the special-function layer for unit tests.
+// Scalar access to the special-function layer for unit tests, through the
+// same code as the kernels (a lane with k = x at a station at r = 1).
 // which: 0 J0, 1 J1, 2 Y0, 3 Y1, 4 G_J, 5 G_Y (x > 0).
 extern "C" double fjemul_special(int which, double x) {
-  const double q = x / fjm::kTwoPi;
-  const float xf = (float)x;
-  Bess b;
-  float gj = 0.f, gy = 0.f;
+  const LaneK L = lane_consts(x / fjm::kTwoPi);
+  const float xf = (float)(L.q2pi * 1.0);
+  float j0, j1, y0, y1, gj = 0.f, gy = 0.f;
   if (xf >= fjm::kSplit) {
-    float t;
-    b = fjm::bessel_large<true, true>(xf, fjm::reduce_phase(q, 1.0), &t);
-    fjm::g_large(xf, t, b, &gj, &gy, true, true);
+    large_station<true, true>(L, 1.0, 1.0f, 1.0f, xf, true, &j0, &j1, &y0, &y1, &gj, &gy);
   } else {
-    double y, L;
-    b = fjm::bessel_small<true, true>(x, &y, &L);
-    fjm::g_small(x, y, L, true, true, &gj, &gy);
+    double y, Lg;
+    const Bess b = fjm::bessel_small<true, true>(x, &y, &Lg);
+    j0 = b.j0;
+    j1 = b.j1;
+    y0 = b.y0;
+    y1 = b.y1;
+    fjm::g_small(x, y, Lg, true, true, &gj, &gy);
     gy = (float)(gy - fjm::kTwoOverPiD * x);  // unshift
   }
   switch (which) {
-    case 0: return b.j0;
-    case 1: return b.j1;
-    case 2: return b.y0;
-    case 3: return b.y1;
+    case 0: return j0;
+    case 1: return j1;
+    case 2: return y0;
+    case 3: return y1;
     case 4: return gj;
     case 5: return gy;
   }

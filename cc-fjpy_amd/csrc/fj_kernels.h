@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/ccfj.h"
+#include "fj_lane.h"
 
 namespace fjk {
 
@@ -18,10 +19,13 @@ struct Problem {
   int nrp;       // station count padded to the kernels' block (fjl::padded_stations)
   double* rr;    // [nrp + 8] workspace
   float* hh;     // [nrp + 8] workspace
+  float* ir;     // [nrp + 8] workspace, 1 / r
+  float* isr;    // [nrp + 8] workspace, 1 / sqrt(r)
   float* coef;   // [nf][nrp][NC] workspace
   float* T;      // [nf][nc] main-kernel results (k_finish forms [nc][nf])
 };
 
+fjl::Stations stations(const Problem& p);
 int coef_per_station(int mode, int itype);
 int padded(int nr);  // station count padded to the kernels' block
 hipError_t launch_prep(int mode, int itype, const Problem& p, hipStream_t st);

@@ -8,9 +8,10 @@
 // with zero coefficients and zero gaps):
 //   rr[nrp + 8]       double   station distances, ascending
 //   hh[nrp + 8]       float    r_s - r_{s-1} (hh[0] = 0)
+//   ir, isr[nrp + 8]  float    1 / r_s, 1 / sqrt(r_s)
 //   coef[nf][nrp][NC] float    per (frequency, station) coefficients:
-//       itype 0: A_s of each spectrum          (NC = nspec)
-//       itype 1: alpha_s, beta_s per spectrum  (NC = 2 nspec)
+//       itype 0: A_s / sqrt(r_s) of each spectrum    (NC = nspec)
+//       itype 1: alpha_s, beta_s / 2 per spectrum    (NC = 2 nspec)
 //   T[nf][nc]         float    main-kernel results, one row per frequency
 //   out[nc][nf]       float    (the reference's layout, src/FJcpu.cpp:247),
 //                              formed from T by k_finish (optionally
@@ -38,6 +39,24 @@ using namespace fjl;
 
 constexpr int kBlock = 256;
 
+// Occupancy floor of the main kernels (waves per SIMD; the compiler then
+// keeps VGPRs within 512 / waves).  The linear-segment kernels are
+// latency-bound at their natural register count (145-220 VGPRs, 2-3 waves):
+// at the floors below, measured on the MI355X, they run faster even with a
+// few spills on their cold (small-argument) path.  -DFJ_OCC_I1_ONE=N and
+// -DFJ_OCC_I1_TWO=N override them for tuning runs.
+#ifndef FJ_OCC_I1_ONE
+#define FJ_OCC_I1_ONE 4
+#endif
+#ifndef FJ_OCC_I1_TWO
+#define FJ_OCC_I1_TWO 3
+#endif
+template <int MODE, int ITYPE>
+struct FjOcc {
+  static constexpr int waves =
+      ITYPE == 0 ? 1 : (ModeT<MODE>::nspec == 1 ? FJ_OCC_I1_ONE : FJ_OCC_I1_TWO);
+};
+
 // Hardware places workgroup w on XCD (w mod 8); tile T = (w mod 8) * per +
 // w / 8 keeps consecutive tiles (same frequency) on one XCD.  Returns false
 // for the padding workgroups of the last round.
@@ -49,19 +68,13 @@ __device__ inline bool tile_of(int ntiles, int* tile) {
   return T < ntiles;
 }
 
-// station arrays: rr (double), hh (float gap), padded to ntot entries with
-// the last distance and zero gaps
-__global__ void k_stations(const float* __restrict__ r, int nr, int ntot,
-                           double* __restrict__ rr, float* __restrict__ hh) {
+// station arrays (fjl::station_entry), padded to ntot entries with the last
+// distance and zero gaps
+__global__ void k_stations(const float* __restrict__ r, int nr, int ntot, double* __restrict__ rr,
+                           float* __restrict__ hh, float* __restrict__ ir,
+                           float* __restrict__ isr) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= ntot) return;
-  if (s < nr) {
-    rr[s] = (double)r[s];
-    hh[s] = s > 0 ? r[s] - r[s - 1] : 0.0f;
-  } else {
-    rr[s] = (double)r[nr - 1];
-    hh[s] = 0.0f;
-  }
+  if (s < ntot) station_entry(r, nr, s, rr, hh, ir, isr);
 }
 
 // Coefficients + [nr][nf] -> [nf][nr][NC] transpose through LDS.
@@ -113,9 +126,8 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ U0,
 }
 
 template <int MODE, int ITYPE>
-__global__ __launch_bounds__(kBlock) void k_fj(const double* __restrict__ rr,
-                                              const float* __restrict__ hh,
-                                              const float* __restrict__ coef,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj(Stations S, const float* __restrict__ coef,
                                               const float* __restrict__ fv,
                                               const float* __restrict__ cv, int nrp, int nf,
                                               int nc, float* __restrict__ T_out) {
@@ -125,15 +137,16 @@ __global__ __launch_bounds__(kBlock) void k_fj(const double* __restrict__ rr,
   if (!tile_of(nf * ncb, &T)) return;
   const int fi = T / ncb;
   const int j = (T - fi * ncb) * kBlock + threadIdx.x;
-  if (j >= nc) return;
-  const double q = (double)fv[fi] / (double)cv[j];
+  // lanes past the last velocity compute a copy of it (the lane code needs
+  // every lane of the wave active) and store nothing
+  const double q = (double)fv[fi] / (double)cv[j < nc ? j : nc - 1];
   const float* __restrict__ cf = coef + (size_t)fi * nrp * NC;
   float v;
   if (ITYPE == 0)
-    v = trapz_lane<MODE>(rr, cf, nrp, q);
+    v = trapz_lane<MODE>(S, cf, nrp, q);
   else
-    v = linseg_lane<MODE>(rr, hh, cf, nrp, q);
-  T_out[(size_t)fi * nc + j] = v;
+    v = linseg_lane<MODE>(S, cf, nrp, q);
+  if (j < nc) T_out[(size_t)fi * nc + j] = v;
 }
 
 // Per-frequency scale for the normalisation (src/ccfj.pyx:225-226,
@@ -194,16 +207,18 @@ __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
 }
 
 // ---------------------------------------------------------------------------
+Stations stations(const Problem& p) { return Stations{p.rr, p.hh, p.ir, p.isr}; }
+
 template <int MODE>
 static hipError_t launch_main(int itype, const Problem& p, hipStream_t st) {
   const int ncb = (p.nc + kBlock - 1) / kBlock;
   const long ntiles = (long)p.nf * ncb;
   const long nblk = ((ntiles + 7) / 8) * 8;
   if (itype == 0)
-    hipLaunchKernelGGL((k_fj<MODE, 0>), dim3((unsigned)nblk), dim3(kBlock), 0, st, p.rr, p.hh,
+    hipLaunchKernelGGL((k_fj<MODE, 0>), dim3((unsigned)nblk), dim3(kBlock), 0, st, stations(p),
                        p.coef, p.f, p.c, p.nrp, p.nf, p.nc, p.T);
   else
-    hipLaunchKernelGGL((k_fj<MODE, 1>), dim3((unsigned)nblk), dim3(kBlock), 0, st, p.rr, p.hh,
+    hipLaunchKernelGGL((k_fj<MODE, 1>), dim3((unsigned)nblk), dim3(kBlock), 0, st, stations(p),
                        p.coef, p.f, p.c, p.nrp, p.nf, p.nc, p.T);
   return hipGetLastError();
 }
@@ -223,7 +238,7 @@ int padded(int nr) { return padded_stations(nr); }
 hipError_t launch_prep(int mode, int itype, const Problem& p, hipStream_t st) {
   const int ntot = p.nrp + kUnroll;
   hipLaunchKernelGGL(k_stations, dim3((ntot + 255) / 256), dim3(256), 0, st, p.r, p.nr, ntot,
-                     p.rr, p.hh);
+                     p.rr, p.hh, p.ir, p.isr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const bool two = !(mode == MODE_J || mode == MODE_Y);

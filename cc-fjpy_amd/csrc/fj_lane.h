@@ -23,10 +23,22 @@
 // corrections (g = x Z1, g' = x Z0, g''' = -Z1 - x Z0): no Struve function
 // and no difference of nearly equal G values; its truncation error is about
 // dx^6 / 30240 relative (5e-7 at the switch).
+//
+// Operation count.  Everything wave-uniform (per station) is precomputed
+// once into the station arrays (r, gap, 1/r, 1/sqrt r) and the coefficient
+// records, and arrives by scalar loads; everything per lane but
+// station-independent (8/k, sqrt(2/(pi k))) is computed once per lane.
+// Per station and lane that leaves, for x >= 8, z = 8/x and the amplitude
+// as one multiply each, no reciprocal square root, one v_fract_f64 for the
+// phase, and degree-2/3 fits beyond x = 20 (95-99% of the terms of dense
+// arrays).  The trapezoid sum also moves the amplitude out of the loop:
+// its coefficients carry 1/sqrt(r_s) and the lane multiplies by
+// sqrt(2/(pi k)) once.
 #pragma once
 
 #include <math.h>
 #include <stdint.h>
+#include <string.h>
 
 #include "../../include/ccfj.h"
 #include "fj_math.h"
@@ -49,6 +61,97 @@ constexpr int kPartial = 32;  // stations per float partial sum
 constexpr int kUnroll = 8;
 
 FJ_HD inline int padded_stations(int nr) { return (nr + kUnroll - 1) / kUnroll * kUnroll; }
+
+// Per-station arrays of one (sub-)array, nrp + kUnroll entries each; entries
+// past the last station repeat its distance with a zero gap.
+struct Stations {
+  const double* rr;  // r_s
+  const float* hh;   // r_s - r_{s-1} (hh[0] = 0)
+  const float* ir;   // 1 / r_s      (inf at r = 0, where only x < 8 code runs)
+  const float* isr;  // 1 / sqrt(r_s)
+};
+
+// Entry s of the station arrays from the sorted float distances r[0..nr).
+FJ_HD inline void station_entry(const float* r, int nr, int s, double* rr, float* hh, float* ir,
+                                float* isr) {
+  const bool in = s < nr;
+  const double rs = (double)r[in ? s : nr - 1];
+  rr[s] = rs;
+  hh[s] = (in && s > 0) ? r[s] - r[s - 1] : 0.0f;
+  ir[s] = (float)(1.0 / rs);
+  isr[s] = (float)(1.0 / sqrt(rs));
+}
+
+// Per-lane constants of one (f, c).
+struct LaneK {
+  double q;     // f / c, cycles per metre
+  double q2pi;  // k = 2 pi f / c
+  float kf;     // k in float
+  float zk;     // 8 / k           (z = 8/x = zk / r)
+  float ak;     // sqrt(2/(pi k))   (Hankel amplitude = ak / sqrt r)
+};
+
+FJ_HD inline LaneK lane_consts(double q) {
+  LaneK L;
+  L.q = q;
+  L.q2pi = q * kTwoPi;
+  L.kf = (float)L.q2pi;
+  L.zk = (float)(8.0 / L.q2pi);
+  L.ak = (float)sqrt(2.0 / (kPiD * L.q2pi));
+  return L;
+}
+
+// ---- wave-uniform far blocks -----------------------------------------------
+// A block of kUnroll stations is "far" when x >= kFarX for every lane of the
+// wave, i.e. when every station of the block has 1/r <= kmin / kFarX with
+// kmin the smallest k of the wave.  The test is one scalar comparison per
+// block (1/r are non-negative floats, so their bit patterns order as
+// unsigned integers; r = 0 gives +inf, r < 0 a sign bit: never far), and a
+// far block runs the far-argument code with no per-lane branch on x.  It
+// needs every lane of the wave active (kernels clamp surplus lanes to a
+// valid velocity and skip only their store).  The host build treats each
+// lane as its own wave.
+FJ_HD inline uint32_t f32_bits(float v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __float_as_uint(v);
+#else
+  uint32_t u;
+  memcpy(&u, &v, 4);
+  return u;
+#endif
+}
+
+FJ_HD inline uint32_t far_threshold(double q2pi) {
+  double kmin = q2pi;
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) kmin = fmin(kmin, __shfl_xor(kmin, o));
+  return __builtin_amdgcn_readfirstlane(f32_bits((float)(kmin / (double)kFarX)));
+#else
+  return f32_bits((float)(kmin / (double)kFarX));
+#endif
+}
+
+template <int N>
+FJ_HD inline bool block_far(const float* iv, uint32_t irfar) {
+  uint32_t m = f32_bits(iv[0]);
+#pragma unroll
+  for (int k = 1; k < N; k++) {
+    const uint32_t b = f32_bits(iv[k]);
+    m = b > m ? b : m;
+  }
+  return m <= irfar;
+}
+
+// true when the predicate holds on any active lane of the wave (uniform;
+// one scalar compare of the lane mask)
+FJ_HD inline bool wave_any(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_ballot_w64(p) != 0;
+#else
+  return p;
+#endif
+}
 // A segment [x_a, x_b] uses the G difference when |dx| >= min(kTau,
 // kKappa min(x_a, x_b)), else the Euler-Maclaurin rule.  kTau bounds the
 // rule's truncation (dx^6/30240) for oscillating g; kKappa bounds it near
@@ -60,6 +163,10 @@ constexpr float kKappa = 0.25f;
 FJ_HD inline bool use_gdiff(float xa, float xb, float dx) {
   return dx != 0.0f && fabsf(dx) >= fminf(kTau, kKappa * fminf(xa, xb));
 }
+
+// The decision for the segment to the right of a station at x with width
+// dx, taken once there and reused as the left segment of the next station.
+FJ_HD inline bool use_gdiff_right(float x, float dx) { return use_gdiff(x, x + dx, dx); }
 
 template <int MODE>
 struct ModeT;
@@ -83,11 +190,12 @@ FJ_HD inline void station_coef(int s, int nr, float rp, float rs, float rn,
                                const float* g_prev, const float* g_cur, float* rec) {
   if (ITYPE == 0) {
     // trapezoid weight (src/FJcpu.cpp:354-356): each segment gives d/2 to
-    // both of its end stations, d = max(h, 0.1) in float as the reference
+    // both of its end stations, d = max(h, 0.1) in float as the reference.
+    // A_s = W_s r_s U_s, stored divided by sqrt(r_s) (see trapz_lane).
     double w = 0.0;
     if (s > 0) w += (double)fmaxf(rs - rp, 0.1f);
     if (s + 1 < nr) w += (double)fmaxf(rn - rs, 0.1f);
-    const double wr = 0.5 * w * (double)rs;
+    const double wr = 0.5 * w * sqrt((double)rs);
     for (int k = 0; k < NSPEC; k++) rec[k] = (float)(wr * (double)g_cur[k]);
   } else {
     // slope over the clamped gap (src/FJcpu.cpp:259-261): b = (g2-g1)/dr0,
@@ -105,71 +213,127 @@ FJ_HD inline void station_coef(int s, int nr, float rp, float rs, float rn,
       const double el = s > 0 ? g1 + beta : 0.0;  // left limit at r_s
       const double er = s + 1 < nr ? g2 : 0.0;    // right limit at r_s
       rec[2 * k] = (float)(el - er);              // alpha_s
-      rec[2 * k + 1] = (float)beta;               // beta_s
+      rec[2 * k + 1] = (float)(0.5 * beta);       // beta_s / 2 (the lane forms 2 D_s)
     }
   }
 }
 
 // ---- itype 0 ---------------------------------------------------------------
-// q = f/c in cycles per metre (double).  cf: coefficient records of one
-// frequency, NC = nspec floats per station; nrp = padded station count.
+// Coefficients A'_s = A_s / sqrt(r_s) (NC = nspec floats per station); the
+// station adds A'_s Z0(x_s) / amp_s with amp_s = sqrt(2/(pi x_s)) the
+// Hankel amplitude, i.e. P0 cos - Q0 sin for x >= 8, and the lane
+// multiplies the sum by sqrt(2/(pi k)) = amp_s sqrt(r_s).  At k = 0 (f = 0)
+// there is no amplitude: the station adds A'_s Z0 sqrt(r_s), the lane
+// multiplies by 1.
 template <int MODE>
-FJ_HD inline void trapz_station(double rs, double q, double q2pi, const float* c,
-                                float& pR, float& pI) {
+FJ_HD inline void trapz_station(double rs, float ir, const LaneK& L, const float* c, float& pR,
+                                float& pI) {
   using M = ModeT<MODE>;
-  const double xd = q2pi * rs;
-  const float x = (float)xd;
-  float J0 = 0.0f, Y0 = 0.0f;
-  if (x >= kSplit) {
-    bessel0_large<M::J, M::Y>(x, reduce_phase(q, rs), &J0, &Y0);
+  float vj = 0.0f, vy = 0.0f;
+  const float z = L.zk * ir;  // 8 / x; inf when k = 0 or r = 0
+  if (z <= 1.0f) {            // x >= 8
+    const float t = z * z;
+    float p0, q0;
+    if (z <= 8.0f / kFarX) {
+      p0 = fitf_P0(t);
+      q0 = z * fitf_q0(t);
+    } else {
+      p0 = fit_P0(t);
+      q0 = z * fit_q0(t);
+    }
+    float s, co;
+    sincos_rev(phase_rev(L.q, rs), &s, &co);
+    if (M::J) vj = __builtin_fmaf(p0, co, -q0 * s);
+    if (M::Y) vy = __builtin_fmaf(p0, s, q0 * co);
   } else {
-    double y, L;
-    Bess b = bessel_small<false, M::Y>(xd, &y, &L);
-    J0 = b.j0;
-    Y0 = b.y0;
+    const double xd = L.q2pi * rs;
+    double y, Lg;
+    const Bess b = bessel_small<false, M::Y>(xd, &y, &Lg);
+    const float ia = L.kf > 0.0f ? sqrtf(kPiOver2 * (float)xd) : sqrtf((float)rs);
+    vj = b.j0 * ia;
+    vy = b.y0 * ia;
   }
   if (MODE == MODE_J) {
-    pR = __builtin_fmaf(c[0], J0, pR);
+    pR = __builtin_fmaf(c[0], vj, pR);
   } else if (MODE == MODE_Y) {
-    pR = __builtin_fmaf(c[0], Y0, pR);
+    pR = __builtin_fmaf(c[0], vy, pR);
   } else if (MODE == MODE_J_MINUS_Y) {
-    pR = __builtin_fmaf(c[0], J0, pR);
-    pR = __builtin_fmaf(-c[1], Y0, pR);
+    pR = __builtin_fmaf(c[0], vj, pR);
+    pR = __builtin_fmaf(-c[1], vy, pR);
   } else if (MODE == MODE_ABS_J) {
-    pR = __builtin_fmaf(c[0], J0, pR);
-    pI = __builtin_fmaf(c[1], J0, pI);
+    pR = __builtin_fmaf(c[0], vj, pR);
+    pI = __builtin_fmaf(c[1], vj, pI);
   } else {  // MODE_ABS_H: (FJ[re]-FY[im]) + i (FJ[im]+FY[re])
-    pR = __builtin_fmaf(c[0], J0, pR);
-    pR = __builtin_fmaf(-c[1], Y0, pR);
-    pI = __builtin_fmaf(c[1], J0, pI);
-    pI = __builtin_fmaf(c[0], Y0, pI);
+    pR = __builtin_fmaf(c[0], vj, pR);
+    pR = __builtin_fmaf(-c[1], vy, pR);
+    pI = __builtin_fmaf(c[1], vj, pI);
+    pI = __builtin_fmaf(c[0], vy, pI);
+  }
+}
+
+// x >= kFarX on every lane (a far block): no branch on the argument.
+template <int MODE>
+FJ_HD inline void trapz_far(double rs, float ir, const LaneK& L, const float* c, float& pR,
+                            float& pI) {
+  using M = ModeT<MODE>;
+  const float z = L.zk * ir, t = z * z;
+  const float p0 = fitf_P0(t), q0 = z * fitf_q0(t);
+  float s, co;
+  sincos_rev(phase_rev(L.q, rs), &s, &co);
+  const float vj = M::J ? __builtin_fmaf(p0, co, -q0 * s) : 0.0f;
+  const float vy = M::Y ? __builtin_fmaf(p0, s, q0 * co) : 0.0f;
+  if (MODE == MODE_J) {
+    pR = __builtin_fmaf(c[0], vj, pR);
+  } else if (MODE == MODE_Y) {
+    pR = __builtin_fmaf(c[0], vy, pR);
+  } else if (MODE == MODE_J_MINUS_Y) {
+    pR = __builtin_fmaf(c[0], vj, pR);
+    pR = __builtin_fmaf(-c[1], vy, pR);
+  } else if (MODE == MODE_ABS_J) {
+    pR = __builtin_fmaf(c[0], vj, pR);
+    pI = __builtin_fmaf(c[1], vj, pI);
+  } else {
+    pR = __builtin_fmaf(c[0], vj, pR);
+    pR = __builtin_fmaf(-c[1], vy, pR);
+    pI = __builtin_fmaf(c[1], vj, pI);
+    pI = __builtin_fmaf(c[0], vy, pI);
   }
 }
 
 template <int MODE>
-FJ_HD inline float trapz_lane(const double* __restrict__ rr, const float* __restrict__ cf,
-                              int nrp, double q) {
+FJ_HD inline float trapz_lane(const Stations& S, const float* __restrict__ cf, int nrp, double q) {
   constexpr int NC = ModeT<MODE>::nspec;
-  const double q2pi = q * kTwoPi;
+  const LaneK L = lane_consts(q);
+  const uint32_t irfar = far_threshold(L.q2pi);
   double accR = 0.0, accI = 0.0;
   for (int s0 = 0; s0 < nrp; s0 += kPartial) {
     const int s1 = nrp < s0 + kPartial ? nrp : s0 + kPartial;
     float pR = 0.0f, pI = 0.0f;
     for (int sb = s0; sb < s1; sb += kUnroll) {
       double rv[kUnroll];
+      float iv[kUnroll];
       float cv[kUnroll * NC];
 #pragma unroll
-      for (int k = 0; k < kUnroll; k++) rv[k] = rr[sb + k];
+      for (int k = 0; k < kUnroll; k++) rv[k] = S.rr[sb + k];
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) iv[k] = S.ir[sb + k];
 #pragma unroll
       for (int k = 0; k < kUnroll * NC; k++) cv[k] = cf[sb * NC + k];
+      if (block_far<kUnroll>(iv, irfar)) {
 #pragma unroll
-      for (int k = 0; k < kUnroll; k++) trapz_station<MODE>(rv[k], q, q2pi, cv + k * NC, pR, pI);
+        for (int k = 0; k < kUnroll; k++) trapz_far<MODE>(rv[k], iv[k], L, cv + k * NC, pR, pI);
+      } else {
+#pragma unroll
+        for (int k = 0; k < kUnroll; k++)
+          trapz_station<MODE>(rv[k], iv[k], L, cv + k * NC, pR, pI);
+      }
     }
     accR += (double)pR;
     accI += (double)pI;
   }
-  if (MODE == MODE_ABS_J || MODE == MODE_ABS_H) return (float)sqrt(accR * accR + accI * accI);
-  return (float)accR;
+  const double fin = L.kf > 0.0f ? sqrt(2.0 / (kPiD * L.q2pi)) : 1.0;
+  if (MODE == MODE_ABS_J || MODE == MODE_ABS_H) return (float)(sqrt(accR * accR + accI * accI) * fin);
+  return (float)(accR * fin);
 }
 
 // ---- itype 1 ---------------------------------------------------------------
@@ -179,95 +343,144 @@ FJ_HD inline float trapz_lane(const double* __restrict__ rr, const float* __rest
 // g~ = x Y1 + 2/pi and D~ = D + 2/pi, which removes a cancellation of up to
 // ~1/(x^2 ln x) at small x.  G is stored shifted (G + 2x/pi) where it comes
 // from the small-argument branch (sh = 1) and unshifted otherwise.
+// The lane forms E = 2 D (and 2 D~) and the coefficients carry beta / 2.
 struct Ends {  // per-basis values at a station
   float g, gp, g3, G, sh;
 };
 
-// (1/dx) int_{x_a}^{x_b} g by Euler-Maclaurin with two end corrections
-FJ_HD inline float em_divided(const Ends& a, const Ends& b, float dx) {
-  const float dx2 = dx * dx;
-  float d = 0.5f * (a.g + b.g);
-  d = __builtin_fmaf(-dx * (1.0f / 12.0f), b.gp - a.gp, d);
-  d = __builtin_fmaf(dx * dx2 * (1.0f / 720.0f), b.g3 - a.g3, d);
+// 2/dx int_{x_a}^{x_b} g by Euler-Maclaurin with two end corrections;
+// e1 = -dx/6, e3 = dx^3/360 are shared by both bases.
+FJ_HD inline float em_divided2(const Ends& a, const Ends& b, float e1, float e3) {
+  float d = a.g + b.g;
+  d = __builtin_fmaf(e1, b.gp - a.gp, d);
+  d = __builtin_fmaf(e3, b.g3 - a.g3, d);
   return d;
 }
 
-// D~ for the Y basis from stored (possibly shifted) G values.
-FJ_HD inline float gdiff_y(const Ends& a, float xa, const Ends& b, float xb, float inv) {
-  if (a.sh == b.sh) return __builtin_fmaf(b.G - a.G, inv, a.sh != 0.0f ? 0.0f : kTwoOverPi);
+// 2 D~ for the Y basis from stored (possibly shifted) G values; inv2 = 2/dx.
+FJ_HD inline float gdiff_y2(const Ends& a, float xa, const Ends& b, float xb, float inv2) {
+  if (a.sh == b.sh)
+    return __builtin_fmaf(b.G - a.G, inv2, a.sh != 0.0f ? 0.0f : 2.0f * kTwoOverPi);
   const float Ga = a.sh != 0.0f ? a.G : __builtin_fmaf(kTwoOverPi, xa, a.G);
   const float Gb = b.sh != 0.0f ? b.G : __builtin_fmaf(kTwoOverPi, xb, b.G);
-  return (Gb - Ga) * inv;
+  return (Gb - Ga) * inv2;
 }
 
-// cf: NC = 2 nspec floats per station: (alpha, beta) per spectrum.
+// x >= 8 at station (rs, ir = 1/r, isr = 1/sqrt r) for lane L: J0, J1,
+// (NY) Y0, Y1, and (need_g) G_J = 1 + R0 J1 - x R1 J0, G_Y = R0 Y1 - x R1 Y0.
+// Degree-2/3 fits from x = 20 on, degree 4-7 below; FAR_ONLY: the caller
+// knows x >= 20 on every lane.
+template <bool NJ, bool NY, bool FAR_ONLY = false>
+FJ_HD inline void large_station(const LaneK& L, double rs, float ir, float isr, float x,
+                                bool need_g, float* j0, float* j1, float* y0, float* y1,
+                                float* GJ, float* GY) {
+  const float z = L.zk * ir, t = z * z;
+  const float amp = L.ak * isr, az = amp * z;
+  const bool far = FAR_ONLY || x >= kFarX;
+  float P0, Q0, P1, Q1;
+  if (far) {
+    P0 = fitf_P0(t);
+    Q0 = fitf_q0(t);
+    P1 = fitf_P1(t);
+    Q1 = fitf_q1(t);
+  } else {
+    P0 = fit_P0(t);
+    Q0 = fit_q0(t);
+    P1 = fit_P1(t);
+    Q1 = fit_q1(t);
+  }
+  float sn, co;
+  sincos_rev(phase_rev(L.q, rs), &sn, &co);
+  const float A = amp * P0, B = az * Q0, C = amp * P1, D = az * Q1;
+  *j0 = __builtin_fmaf(A, co, -B * sn);
+  *j1 = __builtin_fmaf(C, sn, D * co);
+  if (NY) {
+    *y0 = __builtin_fmaf(A, sn, B * co);
+    *y1 = __builtin_fmaf(D, sn, -C * co);
+  }
+  if (need_g) {
+    float r0, r1;
+    if (far) {
+      r0 = fitf_R0(t);
+      r1 = fitf_R1(t);
+    } else {
+      r0 = fit_R0(t);
+      r1 = fit_R1(t);
+    }
+    const float xr1 = x * r1;
+    if (NJ) *GJ = __builtin_fmaf(r0, *j1, __builtin_fmaf(-xr1, *j0, 1.0f));
+    if (NY) *GY = __builtin_fmaf(r0, *y1, -xr1 * *y0);
+  }
+}
+
 template <int MODE>
 struct LinState {
   Ends pj = {0.f, 0.f, 0.f, 0.f, 0.f}, py = {0.f, 0.f, 0.f, 0.f, 0.f};
   float xprev = 0.0f;
+  bool gnext = false;  // the segment right of the previous station uses the G difference
   float pR = 0.0f, pI = 0.0f;
 };
 
-// One station of the linear-segment sum.  dxL = k (r_s - r_{s-1}),
-// dxR = k (r_{s+1} - r_s) (0 past the last station), c = alpha/beta record.
+// End of a station: its g, g' and g''' per basis, the divided difference
+// of the segment to its left (2 D_s: Euler-Maclaurin, or the G difference
+// on the lanes whose segment needs it -- evaluated only when some lane of
+// the wave does), and the station's contribution alpha*g - (beta/2)*2D.
 template <int MODE>
-FJ_HD inline void linseg_station(int s, double rs, float dxL, float dxR, double q, double q2pi,
-                                 const float* c, LinState<MODE>& st) {
+FJ_HD inline void linseg_tail(int s, float x, float dxL, bool useG_L, float j0, float j1, float y0,
+                              float y1, float gy, Ends& cj, Ends& cy, const float* c,
+                              LinState<MODE>& st) {
   using M = ModeT<MODE>;
   constexpr bool NJ = M::J, NY = M::Y;
-  const double xd = q2pi * rs;
-  const float x = (float)xd;
-  const bool useG_L = s > 0 && use_gdiff(st.xprev, x, dxL);
-  const bool needG = useG_L || use_gdiff(x, x + dxR, dxR);
-  Ends cj = {0.f, 0.f, 0.f, 0.f, 0.f}, cy = {0.f, 0.f, 0.f, 0.f, 0.f};
-  Bess b;
-  if (x >= kSplit) {
-    float t;
-    b = bessel_large<NJ, NY>(x, reduce_phase(q, rs), &t);
-    if (needG) g_large(x, t, b, &cj.G, &cy.G, NJ, NY);
-  } else {
-    double y, L;
-    b = bessel_small<true, NY>(xd, &y, &L);
-    if (needG) g_small(xd, y, L, NJ, NY, &cj.G, &cy.G);
-    cy.sh = 1.0f;
-  }
   if (NJ) {
-    cj.g = x * b.j1;
-    cj.gp = x * b.j0;
-    cj.g3 = -b.j1 - cj.gp;
+    cj.g = x * j1;
+    cj.gp = x * j0;
+    cj.g3 = -(j1 + cj.gp);
   }
   if (NY) {
-    cy.g = b.gy;
-    cy.gp = x * b.y0;
-    cy.g3 = -b.y1 - cy.gp;
+    cy.g = gy;
+    cy.gp = x * y0;
+    cy.g3 = -(y1 + cy.gp);
   }
-  float Dj = 0.0f, Dy = 0.0f;
+  float Ej = 0.0f, Ey = 0.0f;  // 2 D_s per basis
   if (s > 0) {
-    if (useG_L) {
-      const float inv = rcp_f(dxL);
-      if (NJ) Dj = (cj.G - st.pj.G) * inv;
-      if (NY) Dy = gdiff_y(st.py, st.xprev, cy, x, inv);
-    } else {
-      if (NJ) Dj = em_divided(st.pj, cj, dxL);
-      if (NY) Dy = em_divided(st.py, cy, dxL);
+    const float e1 = dxL * (-1.0f / 6.0f);
+    const float e3 = dxL * dxL * dxL * (1.0f / 360.0f);
+    if (NJ) Ej = em_divided2(st.pj, cj, e1, e3);
+    if (NY) Ey = em_divided2(st.py, cy, e1, e3);
+    if (wave_any(useG_L)) {
+      const float inv2 = rcp_f(0.5f * dxL);
+      if (NJ) {
+        const float g = (cj.G - st.pj.G) * inv2;
+        Ej = useG_L ? g : Ej;
+      }
+      if (NY) {
+        const float g = gdiff_y2(st.py, st.xprev, cy, x, inv2);
+        Ey = useG_L ? g : Ey;
+      }
     }
   }
   st.xprev = x;
-  // station s contributes alpha*g - beta*D per spectrum
   if (MODE == MODE_J) {
-    st.pR += c[0] * cj.g - c[1] * Dj;
+    st.pR = __builtin_fmaf(c[0], cj.g, st.pR);
+    st.pR = __builtin_fmaf(-c[1], Ej, st.pR);
   } else if (MODE == MODE_Y) {
-    st.pR += c[0] * cy.g - c[1] * Dy;
+    st.pR = __builtin_fmaf(c[0], cy.g, st.pR);
+    st.pR = __builtin_fmaf(-c[1], Ey, st.pR);
   } else if (MODE == MODE_J_MINUS_Y) {
-    st.pR += (c[0] * cj.g - c[1] * Dj) - (c[2] * cy.g - c[3] * Dy);
+    st.pR = __builtin_fmaf(c[0], cj.g, st.pR);
+    st.pR = __builtin_fmaf(-c[1], Ej, st.pR);
+    st.pR = __builtin_fmaf(-c[2], cy.g, st.pR);
+    st.pR = __builtin_fmaf(c[3], Ey, st.pR);
   } else if (MODE == MODE_ABS_J) {
-    st.pR += c[0] * cj.g - c[1] * Dj;
-    st.pI += c[2] * cj.g - c[3] * Dj;
+    st.pR = __builtin_fmaf(c[0], cj.g, st.pR);
+    st.pR = __builtin_fmaf(-c[1], Ej, st.pR);
+    st.pI = __builtin_fmaf(c[2], cj.g, st.pI);
+    st.pI = __builtin_fmaf(-c[3], Ej, st.pI);
   } else {  // MODE_ABS_H
-    const float jre = c[0] * cj.g - c[1] * Dj;
-    const float jim = c[2] * cj.g - c[3] * Dj;
-    const float yre = c[0] * cy.g - c[1] * Dy;
-    const float yim = c[2] * cy.g - c[3] * Dy;
+    const float jre = __builtin_fmaf(c[0], cj.g, -c[1] * Ej);
+    const float jim = __builtin_fmaf(c[2], cj.g, -c[3] * Ej);
+    const float yre = __builtin_fmaf(c[0], cy.g, -c[1] * Ey);
+    const float yim = __builtin_fmaf(c[2], cy.g, -c[3] * Ey);
     st.pR += jre - yim;
     st.pI += jim + yre;
   }
@@ -275,33 +488,82 @@ FJ_HD inline void linseg_station(int s, double rs, float dxL, float dxR, double 
   st.py = cy;
 }
 
-// cf: NC = 2 nspec floats per station: (alpha, beta) per spectrum.
-// hh[s] = r_s - r_{s-1}, hh[0] = 0, zero past the last station; it holds
-// nrp + kUnroll entries.
+// One station of the linear-segment sum.  dxL = k (r_s - r_{s-1}),
+// dxR = k (r_{s+1} - r_s) (0 past the last station), c = alpha, beta/2
+// record per spectrum.  FAR: x >= kFarX on every lane of the wave.
+template <int MODE, bool FAR>
+FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dxL, float dxR,
+                                 const LaneK& L, const float* c, LinState<MODE>& st) {
+  using M = ModeT<MODE>;
+  constexpr bool NJ = M::J, NY = M::Y;
+  const double xd = L.q2pi * rs;
+  const float x = (float)xd;
+  const bool useG_L = s > 0 && st.gnext;
+  // in a far block both ends of every segment have x >= kFarX, where the
+  // rule's threshold min(kTau, kKappa min(x_a, x_b)) is kTau
+  const bool gR = FAR ? fabsf(dxR) >= kTau : use_gdiff_right(x, dxR);
+  st.gnext = gR;
+  const bool needG = useG_L || gR;
+  Ends cj = {0.f, 0.f, 0.f, 0.f, 0.f}, cy = {0.f, 0.f, 0.f, 0.f, 0.f};
+  float j0, j1, y0 = 0.0f, y1 = 0.0f, gy = 0.0f;
+  if (FAR) {
+    large_station<NJ, NY, true>(L, rs, ir, isr, x, wave_any(needG), &j0, &j1, &y0, &y1, &cj.G,
+                                &cy.G);
+    if (NY) gy = __builtin_fmaf(x, y1, kTwoOverPi);
+  } else if (x >= kSplit) {
+    large_station<NJ, NY>(L, rs, ir, isr, x, needG, &j0, &j1, &y0, &y1, &cj.G, &cy.G);
+    if (NY) gy = __builtin_fmaf(x, y1, kTwoOverPi);
+  } else {
+    double y, Lg;
+    const Bess b = bessel_small<true, NY>(xd, &y, &Lg);
+    j0 = b.j0;
+    j1 = b.j1;
+    y0 = b.y0;
+    y1 = b.y1;
+    gy = b.gy;
+    if (needG) g_small(xd, y, Lg, NJ, NY, &cj.G, &cy.G);
+    cy.sh = 1.0f;
+  }
+  linseg_tail<MODE>(s, x, dxL, useG_L, j0, j1, y0, y1, gy, cj, cy, c, st);
+}
+
+// cf: NC = 2 nspec floats per station: (alpha, beta/2) per spectrum.
 template <int MODE>
-FJ_HD inline float linseg_lane(const double* __restrict__ rr, const float* __restrict__ hh,
-                               const float* __restrict__ cf, int nrp, double q) {
+FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, int nrp, double q) {
   constexpr int NC = 2 * ModeT<MODE>::nspec;
-  const double q2pi = q * kTwoPi;
-  const float kf = (float)q2pi;
+  const LaneK L = lane_consts(q);
+  const uint32_t irfar = far_threshold(L.q2pi);
   LinState<MODE> st;
   double accR = 0.0, accI = 0.0;
   float dxR = 0.0f;
   for (int sb = 0; sb < nrp; sb += kUnroll) {
     double rv[kUnroll];
-    float hv[kUnroll];
+    float hv[kUnroll], iv[kUnroll + 1], sv[kUnroll];
     float cv[kUnroll * NC];
 #pragma unroll
-    for (int k = 0; k < kUnroll; k++) rv[k] = rr[sb + k];
+    for (int k = 0; k < kUnroll; k++) rv[k] = S.rr[sb + k];
 #pragma unroll
-    for (int k = 0; k < kUnroll; k++) hv[k] = hh[sb + k + 1];
+    for (int k = 0; k < kUnroll; k++) hv[k] = S.hh[sb + k + 1];
+#pragma unroll
+    for (int k = 0; k <= kUnroll; k++) iv[k] = S.ir[sb + k];   // + the next block's first
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) sv[k] = S.isr[sb + k];
 #pragma unroll
     for (int k = 0; k < kUnroll * NC; k++) cv[k] = cf[sb * NC + k];
+    if (block_far<kUnroll + 1>(iv, irfar)) {
 #pragma unroll
-    for (int k = 0; k < kUnroll; k++) {
-      const float dxL = dxR;
-      dxR = kf * hv[k];
-      linseg_station<MODE>(sb + k, rv[k], dxL, dxR, q, q2pi, cv + k * NC, st);
+      for (int k = 0; k < kUnroll; k++) {
+        const float dxL = dxR;
+        dxR = L.kf * hv[k];
+        linseg_station<MODE, true>(sb + k, rv[k], iv[k], sv[k], dxL, dxR, L, cv + k * NC, st);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kUnroll; k++) {
+        const float dxL = dxR;
+        dxR = L.kf * hv[k];
+        linseg_station<MODE, false>(sb + k, rv[k], iv[k], sv[k], dxL, dxR, L, cv + k * NC, st);
+      }
     }
     if (((sb + kUnroll) & (kPartial - 1)) == 0) {
       accR += (double)st.pR;
@@ -311,7 +573,7 @@ FJ_HD inline float linseg_lane(const double* __restrict__ rr, const float* __res
   }
   accR += (double)st.pR;
   accI += (double)st.pI;
-  const double inv_k2 = 1.0 / (q2pi * q2pi);
+  const double inv_k2 = 1.0 / (L.q2pi * L.q2pi);
   if (MODE == MODE_ABS_J || MODE == MODE_ABS_H)
     return (float)(sqrt(accR * accR + accI * accI) * inv_k2);
   return (float)(accR * inv_k2);

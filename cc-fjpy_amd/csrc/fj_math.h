@@ -45,6 +45,8 @@ constexpr double kTwoPi = 6.283185307179586476925;
 constexpr float kTwoOverPi = 0.6366197723675814f;
 constexpr double kTwoOverPiD = 0.63661977236758134308;
 constexpr float kLn2 = 0.6931471805599453f;
+constexpr float kPiOver2 = 1.5707963267948966f;
+constexpr double kPiD = 3.14159265358979323846;
 
 // ---- elementary pieces -----------------------------------------------------
 
@@ -210,6 +212,18 @@ FJ_HD inline void g_small(double x, double y, double L, bool need_j, bool need_y
 FJ_HD inline float reduce_phase(double q, double r) {
   double t = __builtin_fma(q, r, -0.125);
   return (float)(t - rint(t));
+}
+
+// The same phase reduced to [0, 1) (one v_fract_f64 on the device).  The
+// float rounding of the result is at most 2^-25 revolutions, as for
+// reduce_phase, and sincos_rev accepts either range.
+FJ_HD inline float phase_rev(double q, double r) {
+  const double t = __builtin_fma(q, r, -0.125);
+#if FJ_USE_HW
+  return (float)__builtin_amdgcn_fract(t);
+#else
+  return (float)(t - floor(t));
+#endif
 }
 
 }  // namespace fjm

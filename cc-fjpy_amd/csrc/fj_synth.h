@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fj_lane.h"
+
 namespace fjk {
 
 constexpr int kMaxModes = 4;
@@ -13,9 +15,8 @@ hipError_t launch_synth_fields(const float* r, const float* f, const float* mode
 hipError_t launch_synth_labels(const float* cgrid, const float* f, const float* modes, int B,
                                int H, int nf, float halfwidth, float amin, float* Y,
                                hipStream_t st);
-hipError_t launch_fj_items(int mode, int itype, const double* rr, const float* hh,
-                           const float* coef, const float* f, const float* c, const int* items,
-                           int nitems, int nr, int nrp, int nf, int nc, float* out,
-                           hipStream_t st);
+hipError_t launch_fj_items(int mode, int itype, const fjl::Stations& S, const float* coef,
+                           const float* f, const float* c, const int* items, int nitems, int nr,
+                           int nrp, int nf, int nc, float* out, hipStream_t st);
 
 }  // namespace fjk

@@ -87,28 +87,28 @@ __global__ void k_synth_labels(const float* __restrict__ cgrid, const float* __r
 // outside the grid is skipped (items live in device memory, so the host
 // cannot check them).  Fewer than two stations: zeros, as the full grid.
 template <int MODE, int ITYPE>
-__global__ __launch_bounds__(256) void k_fj_items(const double* __restrict__ rr,
-                                                 const float* __restrict__ hh,
-                                                 const float* __restrict__ coef,
+__global__ __launch_bounds__(256) void k_fj_items(Stations S, const float* __restrict__ coef,
                                                  const float* __restrict__ fv,
                                                  const float* __restrict__ cv,
                                                  const int* __restrict__ items, int nr, int nrp,
                                                  int nf, int nc, float* __restrict__ out) {
   constexpr int NC = ITYPE == 0 ? ModeT<MODE>::nspec : 2 * ModeT<MODE>::nspec;
   const int* it = items + 3 * (size_t)blockIdx.x;
-  const int fi = it[0], c0 = it[1], cnt = it[2];
-  if ((int)threadIdx.x >= cnt || fi < 0 || fi >= nf || c0 < 0) return;
-  const int j = c0 + threadIdx.x;
-  if (j >= nc) return;
+  const int fi = it[0], c0 = it[1];
+  const int cnt = min(it[2], nc - c0);   // item clipped to the grid
+  // uniform over the workgroup: a skipped item exits as a whole
+  if (fi < 0 || fi >= nf || c0 < 0 || cnt <= 0) return;
+  if ((int)(threadIdx.x & ~63u) >= cnt) return;   // whole waves past the item
+  const int j = c0 + min((int)threadIdx.x, cnt - 1);   // surplus lanes repeat the last
+  const bool store = (int)threadIdx.x < cnt;
   if (nr < 2) {
-    out[(size_t)j * nf + fi] = 0.0f;
+    if (store) out[(size_t)j * nf + fi] = 0.0f;
     return;
   }
   const double q = (double)fv[fi] / (double)cv[j];
   const float* __restrict__ cf = coef + (size_t)fi * nrp * NC;
-  const float v = ITYPE == 0 ? trapz_lane<MODE>(rr, cf, nrp, q)
-                             : linseg_lane<MODE>(rr, hh, cf, nrp, q);
-  out[(size_t)j * nf + fi] = v;
+  const float v = ITYPE == 0 ? trapz_lane<MODE>(S, cf, nrp, q) : linseg_lane<MODE>(S, cf, nrp, q);
+  if (store) out[(size_t)j * nf + fi] = v;
 }
 
 hipError_t launch_synth_fields(const float* r, const float* f, const float* modes,
@@ -127,13 +127,12 @@ hipError_t launch_synth_labels(const float* cgrid, const float* f, const float* 
   return hipGetLastError();
 }
 
-hipError_t launch_fj_items(int mode, int itype, const double* rr, const float* hh,
-                           const float* coef, const float* f, const float* c, const int* items,
-                           int nitems, int nr, int nrp, int nf, int nc, float* out,
-                           hipStream_t st) {
+hipError_t launch_fj_items(int mode, int itype, const Stations& S, const float* coef,
+                           const float* f, const float* c, const int* items, int nitems, int nr,
+                           int nrp, int nf, int nc, float* out, hipStream_t st) {
   if (nitems <= 0) return hipSuccess;
 #define FJ_ITEMS(M, I)                                                                         \
-  hipLaunchKernelGGL((k_fj_items<M, I>), dim3(nitems), dim3(256), 0, st, rr, hh, coef, f, c,  \
+  hipLaunchKernelGGL((k_fj_items<M, I>), dim3(nitems), dim3(256), 0, st, S, coef, f, c,        \
                      items, nr, nrp, nf, nc, out)
   if (mode == MODE_J && itype == 0) FJ_ITEMS(MODE_J, 0);
   else if (mode == MODE_J && itype == 1) FJ_ITEMS(MODE_J, 1);

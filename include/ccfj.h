@@ -181,7 +181,10 @@ int ccfj_fj_noise(ccfj_ctx *ctx, int itype, int func, int fstride, const float *
 
 /* Dense F-J on a list of items (frequency index, first velocity index,
  * count <= 256), items[n][3] int32 on the device.  out[nc][nf] is written
- * only at listed entries (not normalised).  Modes J and J_MINUS_Y. */
+ * only at listed entries (not normalised).  Modes J and J_MINUS_Y.  The
+ * kernel skips an item whose frequency or first velocity lies outside the
+ * grid and clips its count to the grid (the list is device data, which
+ * the host does not inspect). */
 int ccfj_transform_items(ccfj_ctx *ctx, int mode, int itype, const float *U0, const float *U1,
                          const float *r, const float *f, const float *c, int nr, int nf, int nc,
                          const int *items, int nitems, float *out, void *stream);
