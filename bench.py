@@ -320,7 +320,7 @@ def bench_batched(args):
                    {"workload": "batched sub-arrays fj_noise itype 1 func 0, one persistent "
                                 "launch replayed from a HIP graph, device-resident spectra",
                     "sub_arrays": nb, "pairs_total": int(ns.sum()), "nf": nf, "nc": nc,
-                    "tile": plan.tile(), "term_evals_per_step": terms})
+                    "tile": plan.tile, "term_evals_per_step": terms})
     out["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK,
                        "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK, 4), "traffic": None,
                        "kernel": "whole graph (stations, prep, persistent k_fj_b, finish)"}

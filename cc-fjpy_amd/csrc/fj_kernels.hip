@@ -40,21 +40,21 @@ using namespace fjl;
 constexpr int kBlock = 256;
 
 // Occupancy floor of the main kernels (waves per SIMD; the compiler then
-// keeps VGPRs within 512 / waves).  The linear-segment kernels are
-// latency-bound at their natural register count (145-220 VGPRs, 2-3 waves):
-// at the floors below, measured on the MI355X, they run faster even with a
-// few spills on their cold (small-argument) path.  -DFJ_OCC_I1_ONE=N and
-// -DFJ_OCC_I1_TWO=N override them for tuning runs.
-#ifndef FJ_OCC_I1_ONE
-#define FJ_OCC_I1_ONE 4
+// keeps VGPRs within 512 / waves).  Measured on the MI355X: the fused J - Y
+// linear-segment kernel (natural 217 VGPRs, 2 waves) runs 16% faster held
+// to 3 waves, a few spills on its cold small-argument path included; the
+// other modes are fastest at their natural count (forcing them spills on the
+// hot path).  -DFJ_OCC_I1_JMY=N / -DFJ_OCC_I1_J=N override for tuning runs.
+#ifndef FJ_OCC_I1_JMY
+#define FJ_OCC_I1_JMY 3
 #endif
-#ifndef FJ_OCC_I1_TWO
-#define FJ_OCC_I1_TWO 3
+#ifndef FJ_OCC_I1_J
+#define FJ_OCC_I1_J 1
 #endif
 template <int MODE, int ITYPE>
 struct FjOcc {
   static constexpr int waves =
-      ITYPE == 0 ? 1 : (ModeT<MODE>::nspec == 1 ? FJ_OCC_I1_ONE : FJ_OCC_I1_TWO);
+      ITYPE == 0 ? 1 : (MODE == MODE_J_MINUS_Y ? FJ_OCC_I1_JMY : (MODE == MODE_J ? FJ_OCC_I1_J : 1));
 };
 
 // Hardware places workgroup w on XCD (w mod 8); tile T = (w mod 8) * per +

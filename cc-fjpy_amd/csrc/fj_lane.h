@@ -527,6 +527,16 @@ FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dx
   linseg_tail<MODE>(s, x, dxL, useG_L, j0, j1, y0, y1, gy, cj, cy, c, st);
 }
 
+// Far blocks in the linear-segment loop; -DFJ_FAR_LINSEG_ONE=0 turns them
+// off for the one-spectrum modes (a tuning switch).
+#ifndef FJ_FAR_LINSEG_ONE
+#define FJ_FAR_LINSEG_ONE 1
+#endif
+template <int MODE>
+struct LinFar {
+  static constexpr bool on = ModeT<MODE>::nspec == 2 || FJ_FAR_LINSEG_ONE != 0;
+};
+
 // cf: NC = 2 nspec floats per station: (alpha, beta/2) per spectrum.
 template <int MODE>
 FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, int nrp, double q) {
@@ -550,7 +560,7 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
     for (int k = 0; k < kUnroll; k++) sv[k] = S.isr[sb + k];
 #pragma unroll
     for (int k = 0; k < kUnroll * NC; k++) cv[k] = cf[sb * NC + k];
-    if (block_far<kUnroll + 1>(iv, irfar)) {
+    if (LinFar<MODE>::on && block_far<kUnroll + 1>(iv, irfar)) {
 #pragma unroll
       for (int k = 0; k < kUnroll; k++) {
         const float dxL = dxR;
