@@ -23,6 +23,8 @@ Other workloads (--workload), each printing its own JSON line:
             nc 2000, f < 5 Hz, itype 0, func 0 (windows-as-RHS MFMA)
   learned   the band-proposal CNN: training on device-generated spectra,
             then banded dense F-J (nc 4000) against the full grid
+  cc        the cross-correlation producer: 200 stations, one day at 4 Hz,
+            one-hour windows at 90% overlap, 1000 bins of stride 3, whitened
 
 Prints one JSON line.  Besides the contract fields it carries
   roofline      the dominant kernel's algorithmic FP32 rate (34 flop per
@@ -67,7 +69,8 @@ def parse():
     p.add_argument("--kernel-reps", type=int, default=2)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-plain", action="store_true")
-    p.add_argument("--workload", default="dense", choices=["dense", "batched", "mwfj", "learned"])
+    p.add_argument("--workload", default="dense",
+                   choices=["dense", "batched", "mwfj", "learned", "cc"])
     p.add_argument("--train", type=int, default=50000, help="learned: training spectra")
     p.add_argument("--epochs", type=int, default=3, help="learned: training epochs")
     p.add_argument("--copy-calib", action="store_true",
@@ -120,7 +123,8 @@ def main():
         copy_calibration()
         return
     if args.workload != "dense":
-        {"batched": bench_batched, "mwfj": bench_mwfj, "learned": bench_learned}[args.workload](args)
+        {"batched": bench_batched, "mwfj": bench_mwfj, "learned": bench_learned,
+         "cc": bench_cc}[args.workload](args)
         return
     import torch
     import ccfj
@@ -432,7 +436,9 @@ def bench_learned(args):
     t_train = time.perf_counter() - t0
     res = m.evaluate(64, seed=7, levels=(0.5, 0.3))
     cnn = res["cnn"]
-    full_ms, band_ms = res["full_kernel_ms"], cnn["kernel_ms"]
+    full_ms = res["full_kernel_ms"]
+    # the banded path's cost: coarse F-J + network + banded dense kernel
+    band_ms = res["coarse_ms"] + res["cnn_ms"] + cnn["kernel_ms"]
     dense_terms = 64 * m.nr * m.W * m.nc_dense
     value = dense_terms / (band_ms / 1e3)   # full-grid-equivalent term-evals/s of the banded path
     out = contract("banded dense F-J, full-grid-equivalent term-evals/s at the ridge agreement "
@@ -446,6 +452,63 @@ def bench_learned(args):
                        "last_loss": float(np.mean(losses[-20:]))}
     out["speedup_vs_full_grid"] = round(full_ms / band_ms, 2)
     print(json.dumps(out), flush=True)
+
+
+def bench_cc(args):
+    """Cross-correlation of 200 stations over one day (the reference's
+    example settings: 4 Hz, one-hour windows, 90% overlap, nf 1000,
+    fstride 3, whitening), synthetic records with a shared random wavefield
+    plus station noise; station 7 starts late and station 11 ends early."""
+    import torch  # noqa: F401
+    import ccfj
+    rng = np.random.default_rng(5)
+    Fs, nsta, nf, fstride = 4, 200, 1000, 3
+    npts = 24 * 3600 * Fs
+    fftlen = 3600 * Fs
+    common = rng.normal(size=npts + 4000).astype(np.float32)
+    lags = rng.integers(0, 4000, nsta)
+    data = np.stack([common[l:l + npts] for l in lags]) + rng.normal(size=(nsta, npts)).astype(np.float32)
+    data = np.ascontiguousarray(data.astype(np.float32).reshape(-1))
+    se = np.tile(np.array([0, npts], np.int32), nsta)
+    se[2 * 7] = npts // 4
+    se[2 * 11 + 1] = 3 * npts // 4
+    pairs = ccfj.GetStationPairs(nsta)
+    npairs = len(pairs) // 2
+    steplen = int(fftlen * (1 - 0.9))
+    nshifts = 1 + (npts - fftlen) // steplen
+    print("bench: CC warmup", file=sys.stderr, flush=True)
+    for _ in range(max(1, args.warmup)):
+        ccfj.CC(npts, nsta, nf, fftlen, pairs, se, data, overlaprate=0.9, fstride=fstride)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = ccfj.CC(npts, nsta, nf, fftlen, pairs, se, data, overlaprate=0.9, fstride=fstride)
+    wall = (time.perf_counter() - t0) / args.steps
+    # pair-window products actually formed (both stations hold the window)
+    from oracle import cc_ref as C
+    use = np.stack([C.used(se, k, steplen, fftlen) for k in range(nshifts)])
+    pw = int(np.sum(use[:, pairs[0::2]] & use[:, pairs[1::2]]))
+    res = contract("cross-correlation pair-window spectra/s (nf bins each)", pw / wall,
+                   "pair-windows/s", args, wall * 1e3,
+                   {"workload": "ambient-noise cross-correlation, host records in, complex64 "
+                                "[npairs, nf] out", "stations": nsta, "npts": npts,
+                    "fftlen": fftlen, "windows": nshifts, "pairs": npairs, "nf": nf,
+                    "fstride": fstride, "pair_windows": pw})
+    res["output_shape"] = list(out.shape)
+    if not args.no_cpu_baseline:
+        sub = 12   # the restatement on 12 stations (66 pairs), 24 windows
+        sp = ccfj.GetStationPairs(sub)
+        npts_s = fftlen + 23 * steplen
+        ds = np.ascontiguousarray(data.reshape(nsta, npts)[:sub, :npts_s]).reshape(-1)
+        ses = np.tile(np.array([0, npts_s + 1], np.int32), sub)
+        t0 = time.perf_counter()
+        _, cnt = C.cross_correlate(ds, sub, npts_s, nf, fftlen, steplen, sp, ses, fstride=fstride)
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": float(cnt.sum()) / dt, "unit": "pair-windows/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"numpy restatement (float64 FFT) on {sub} stations, "
+                                         f"{len(sp) // 2} pairs, 24 windows ({dt:.1f} s)",
+                               "cpu": cpu_model()}
+    print(json.dumps(res), flush=True)
 
 
 if __name__ == "__main__":

@@ -10,6 +10,8 @@ GPU build; src/ccfj.pyx:148-268, the CPU build):
          itype=1, func=0, num=-1)                                -> float64 [nwin, nc, nf]
     fj, fh, fhr, fhi (un-normalised single-basis transforms), win,
     GetStationPairs
+    CC(npts, nsta, nf, fftlen, Pairs, startend, data, overlaprate=0.0,
+       nThreads=8, fstride=1, ifonebit=0, ifspecwhittenning=1)  -> complex64 [npairs, nf]
 
 `num` selects the GPU (-1 = the current device); the reference's GPU build
 did this through CUDA_VISIBLE_DEVICES (src/ccfj-gpu.pyx:247-248).
@@ -32,7 +34,7 @@ except ImportError as exc:  # the native extension is the product; no fallback
     ) from exc
 
 __all__ = ["fj_noise", "fj_earthquake", "MWFJ", "fj", "fh", "fhr", "fhi", "win",
-           "GetStationPairs", "hilbert_rows", "device_count", "fj_noise_batch"]
+           "GetStationPairs", "CC", "hilbert_rows", "device_count", "fj_noise_batch"]
 
 MODE_J = _ccfj.MODE_J
 MODE_Y = _ccfj.MODE_Y
@@ -317,6 +319,31 @@ def fj_noise_batch(ufs, rs, c, f, fstride=1, itype=1, func=0, num=-1):
     R = np.ascontiguousarray(np.concatenate(Rs))
     mode = MODE_J if func == 0 else MODE_J_MINUS_Y
     return _context(num).transform_batched(mode, int(itype), True, soff, U, H, R, ff, cc)
+
+
+def CC(npts, nsta, nf, fftlen, Pairs, startend, data, overlaprate=0.0, nThreads=8, fstride=1,
+       ifonebit=0, ifspecwhittenning=1, num=-1):
+    """Ambient-noise cross-correlation (src/ccfj.pyx:125-146 with
+    src/CrossCorr.cpp:33-159), on the GPU: windows of fftlen samples every
+    int(fftlen (1 - overlaprate)) samples, one-bit and spectral whitening
+    as flagged, the mean over shared windows of conj(X_a) X_b at bins
+    0, fstride, ... (nf of them) for every pair of Pairs (flattened, as
+    GetStationPairs returns it).  data: the nsta records of npts samples,
+    flattened; startend[2i:2i+2]: the samples of station i that hold data.
+    As the reference, a float32 `data` is replaced by its signs in place when
+    ifonebit is set.  nThreads is accepted and ignored.  Returns complex64
+    [npairs, nf]."""
+    steplen = int(fftlen * (1 - overlaprate))
+    npairs = int(len(Pairs) / 2)
+    if data.dtype != np.float32:
+        data = np.array(data, dtype=np.float32)   # the reference's copy
+    re, im = _context(num).cross_correlate(int(nsta), int(npts), int(nf), int(fstride), int(fftlen),
+                                           steplen, bool(ifonebit), bool(ifspecwhittenning),
+                                           np.asarray(Pairs, np.int32)[:2 * npairs],
+                                           np.asarray(startend, np.int32), data)
+    if ifonebit:
+        np.copyto(data, np.sign(data))
+    return re + im * 1j
 
 
 def GetStationPairs(nsta):

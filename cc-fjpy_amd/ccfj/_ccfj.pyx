@@ -67,6 +67,14 @@ cdef extern from "ccfj.h" nogil:
                           void *stream)
     int ccfj_synth_labels(ccfj_ctx *ctx, int B, int H, int nf, const float *cgrid, const float *f,
                           const float *modes, float halfwidth, float amin, float *Y, void *stream)
+    int ccfj_pool_abs_max(ccfj_ctx *ctx, const float *inp, int B, int H, int S, int W, float *out,
+                          void *stream)
+    int ccfj_label_argmax(ccfj_ctx *ctx, const float *X, int B, int H, int W, float *Y,
+                          void *stream)
+    int ccfj_cross_correlate(ccfj_ctx *ctx, int nsta, int npts, int nf, int fstride, int fftlen,
+                             int steplen, int onebit, int whiten, int npairs, const int *pairs,
+                             const int *startend, const float *data, float *ncfsr,
+                             float *ncfsi) nogil
     ctypedef struct ccfj_cnn:
         pass
     int ccfj_cnn_param_count()
@@ -281,6 +289,31 @@ cdef class Context:
         _check(rc, "ccfj_mwfj")
         return out
 
+    def cross_correlate(self, int nsta, int npts, int nf, int fstride, int fftlen, int steplen,
+                        bint onebit, bint whiten, pairs, startend, data):
+        """Device cross-correlation (host arrays): pairs int32 [2 npairs],
+        startend int32 [2 nsta], data float32 [nsta * npts] ->
+        (ncfsr, ncfsi) float32 [npairs, nf]."""
+        cdef int[::1] pv = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+        cdef int[::1] se = np.ascontiguousarray(startend, dtype=np.int32).reshape(-1)
+        cdef float[::1] dv = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
+        cdef int npairs = pv.shape[0] // 2
+        if se.shape[0] < 2 * nsta or dv.shape[0] < <size_t>nsta * npts:
+            raise ValueError("startend or data shorter than nsta / npts require")
+        re = np.zeros((npairs, nf), np.float32)
+        im = np.zeros((npairs, nf), np.float32)
+        cdef float[:, ::1] rv = re
+        cdef float[:, ::1] iv = im
+        cdef const int *pp = &pv[0] if npairs > 0 else NULL
+        cdef float *rp = &rv[0, 0] if npairs > 0 else NULL
+        cdef float *ip = &iv[0, 0] if npairs > 0 else NULL
+        cdef int rc
+        with nogil:
+            rc = ccfj_cross_correlate(self.ctx, nsta, npts, nf, fstride, fftlen, steplen, onebit,
+                                      whiten, npairs, pp, &se[0], &dv[0], rp, ip)
+        _check(rc, "ccfj_cross_correlate")
+        return re, im
+
     def batch_plan(self, int mode, int itype, bint normalize, soff, int nf, int nc):
         return BatchPlan(self, mode, itype, normalize, soff, nf, nc)
 
@@ -409,6 +442,16 @@ def synth_labels(Context ctx, int B, int H, int nf, size_t cgrid, size_t f, size
     _check(ccfj_synth_labels(ctx.ctx, B, H, nf, <const float *>cgrid, <const float *>f,
                              <const float *>modes, halfwidth, amin, <float *>Y, NULL),
            "ccfj_synth_labels")
+
+
+def pool_abs_max(Context ctx, size_t x, int B, int H, int S, int W, size_t out):
+    _check(ccfj_pool_abs_max(ctx.ctx, <const float *>x, B, H, S, W, <float *>out, NULL),
+           "ccfj_pool_abs_max")
+
+
+def label_argmax(Context ctx, size_t x, int B, int H, int W, size_t y):
+    _check(ccfj_label_argmax(ctx.ctx, <const float *>x, B, H, W, <float *>y, NULL),
+           "ccfj_label_argmax")
 
 
 def cnn_param_count():

@@ -3,8 +3,8 @@
 //
 // A plan fixes the sub-array sizes, the grids, the mode and itype.  Its first
 // run prepares the coefficients, then times the persistent kernel with each
-// candidate velocity tile (64/128/256 lanes per workgroup) on the first
-// sixteenth of the work items and keeps the fastest.  With use_graph != 0 the
+// candidate velocity tile (64/128/256 lanes per workgroup) on the middle
+// eighth of the work items and keeps the fastest.  With use_graph != 0 the
 // whole launch sequence (station arrays, coefficients, counter reset,
 // persistent kernel, transpose with optional normalisation) is captured
 // once into a HIP graph and replayed while the device pointers stay the
@@ -78,7 +78,7 @@ int enqueue(ccfj_batch* p, hipStream_t st, unsigned int limit, int tile) {
   hipError_t e = fjk::launch_prep_batched(p->mode, p->itype, p->d, p->max_nrp, st);
   if (e != hipSuccess) return hip_fail(e, "batched prep launch");
   const void* k = fjk::batched_kernel(p->mode, p->itype, tile);
-  e = fjk::launch_batched(k, tile, p->grid[tile_index(tile)], p->d, limit, st);
+  e = fjk::launch_batched(k, tile, p->grid[tile_index(tile)], p->d, 0, limit, st);
   if (e != hipSuccess) return hip_fail(e, "batched kernel launch");
   e = fjk::launch_finish(p->d.T, p->normalize ? p->d.m : nullptr, p->nb, p->nf, p->nc, p->d.out,
                          st);
@@ -86,7 +86,8 @@ int enqueue(ccfj_batch* p, hipStream_t st, unsigned int limit, int tile) {
   return CCFJ_OK;
 }
 
-// Time each candidate tile on the first sixteenth of the items.
+// Time each candidate tile on the middle eighth of the work queue (the
+// median-sized sub-arrays: the queue runs from the largest down).
 int autotune(ccfj_batch* p, hipStream_t st) {
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
@@ -98,12 +99,13 @@ int autotune(ccfj_batch* p, hipStream_t st) {
   for (int t : kTiles) {
     const int ncb = (p->nc + t - 1) / t;
     const unsigned int total = (unsigned int)((int64_t)p->nb * p->nf * ncb);
-    const unsigned int limit = std::max(1u, total / 16);
+    const unsigned int first = (unsigned int)((uint64_t)total * 7 / 16);
+    const unsigned int limit = std::max(first + 1, (unsigned int)((uint64_t)total * 9 / 16));
     const void* k = fjk::batched_kernel(p->mode, p->itype, t);
     float ms = 0.f;
     for (int rep = 0; rep < 2; rep++) {  // first pass warms caches and code
       (void)hipEventRecord(e0, st);
-      e = fjk::launch_batched(k, t, p->grid[tile_index(t)], p->d, limit, st);
+      e = fjk::launch_batched(k, t, p->grid[tile_index(t)], p->d, first, limit, st);
       if (e != hipSuccess) return hip_fail(e, "batched kernel launch (tuning)");
       (void)hipEventRecord(e1, st);
       e = hipEventSynchronize(e1);
@@ -111,7 +113,7 @@ int autotune(ccfj_batch* p, hipStream_t st) {
       (void)hipEventElapsedTime(&ms, e0, e1);
     }
     // normalise by the fraction of the work the sample covered
-    const float per_total = ms * (float)total / (float)limit;
+    const float per_total = ms * (float)total / (float)(limit - first);
     if (per_total < best) {
       best = per_total;
       best_tile = t;

@@ -63,7 +63,8 @@ bool injected_alloc_failure() {
   return fail_at > 0 && n == fail_at;
 }
 
-enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_IR, S_ISR, S_COEF, S_T, S_M, S_X0, S_X1, S_COUNT };
+enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_IR, S_ISR, S_COEF, S_T, S_M, S_X0,
+            S_COUNT = S_X0 + ccfj_internal::kScratchSlots };
 
 }  // namespace
 
@@ -214,13 +215,16 @@ int device_alloc(void** p, size_t bytes) {
 int ctx_device(const ccfj_ctx* ctx) { return ctx->device; }
 
 int ctx_scratch(ccfj_ctx* ctx, int which, size_t bytes, void** p) {
-  const Slot s = which == 0 ? S_X0 : S_X1;
+  if (which < 0 || which >= kScratchSlots) return fail(CCFJ_EINVAL, "scratch slot %d", which);
+  const Slot s = (Slot)(S_X0 + which);
   int rc = ensure(ctx, s, bytes);
   if (rc) return rc;
   *p = ctx->buf[s];
   return CCFJ_OK;
 }
 hipStream_t ctx_stream(const ccfj_ctx* ctx) { return ctx->stream; }
+
+int default_context(ccfj_ctx** out) { return default_ctx(out); }
 
 int timing_begin(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t>* ev) {
   if (!ctx->timing) return CCFJ_OK;
@@ -426,6 +430,28 @@ int ccfj_synth_labels(ccfj_ctx* ctx, int B, int H, int nf, const float* cgrid, c
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
   hipError_t e = fjk::launch_synth_labels(cgrid, f, modes, B, H, nf, halfwidth, amin, Y, st);
   return e == hipSuccess ? CCFJ_OK : hip_fail(e, "synth labels launch");
+}
+
+int ccfj_pool_abs_max(ccfj_ctx* ctx, const float* in, int B, int H, int S, int W, float* out,
+                      void* stream) {
+  if (!ctx || !in || !out) return fail(CCFJ_EINVAL, "null argument");
+  if (B < 1 || H < 1 || S < 1 || W < 1) return fail(CCFJ_EINVAL, "bad sizes");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  hipError_t e = fjk::launch_pool_abs_max(in, B, H, S, W, out, st);
+  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "pool kernel launch");
+}
+
+int ccfj_label_argmax(ccfj_ctx* ctx, const float* X, int B, int H, int W, float* Y,
+                      void* stream) {
+  if (!ctx || !X || !Y) return fail(CCFJ_EINVAL, "null argument");
+  if (B < 1 || H < 1 || W < 1) return fail(CCFJ_EINVAL, "bad sizes");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  hipError_t e = fjk::launch_label_argmax(X, B, H, W, Y, st);
+  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "label kernel launch");
 }
 
 int ccfj_malloc(ccfj_ctx* ctx, size_t bytes, void** ptr) {

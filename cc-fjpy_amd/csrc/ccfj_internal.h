@@ -1,6 +1,7 @@
 // ccfj_internal.h -- helpers shared by the host-side sources of libccfj.so.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 
 #include <utility>
 
@@ -14,9 +15,17 @@ int hip_fail(hipError_t e, const char* what);
 // Device allocation honouring the CCFJ_FAIL_ALLOC_AT fault injection.
 int device_alloc(void** p, size_t bytes);
 int ctx_device(const ccfj_ctx* ctx);
-// Grow-only scratch buffer `which` (0 or 1) of the context, at least bytes.
+// Grow-only scratch buffer `which` (0 .. kScratchSlots-1) of the context,
+// at least bytes; 0 and 1 belong to the Hilbert transform.
+constexpr int kScratchSlots = 8;
 int ctx_scratch(ccfj_ctx* ctx, int which, size_t bytes, void** p);
 hipStream_t ctx_stream(const ccfj_ctx* ctx);
+// The process-wide context of the current device (the reference-compatible
+// entry points use it).
+int default_context(ccfj_ctx** out);
+// A hipFFT plan for batch packed rows of length n (R2C/D2Z forward,
+// C2R/Z2D backward), cached for the process, bound to stream st.
+int cached_fft_plan(int n, int batch, hipfftType type, hipStream_t st, hipfftHandle* h);
 // With the context's timing on (ccfj_set_timing), events around a main
 // kernel, summed by ccfj_get_timing; no-ops otherwise.
 int timing_begin(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t>* ev);

@@ -29,27 +29,6 @@ struct Bufs {
   }
 };
 
-struct Plan {
-  hipfftHandle h = 0;
-  int batch = 0;
-  ~Plan() {
-    if (h) hipfftDestroy(h);
-  }
-};
-
-int fft_plan(Plan& pl, int npts, int batch, hipStream_t st) {
-  if (pl.h && pl.batch == batch) return CCFJ_OK;
-  if (pl.h) hipfftDestroy(pl.h);
-  pl.h = 0;
-  int n[1] = {npts};
-  if (hipfftPlanMany(&pl.h, 1, n, nullptr, 1, npts, nullptr, 1, npts / 2 + 1, HIPFFT_D2Z,
-                     batch) != HIPFFT_SUCCESS)
-    return fail(CCFJ_EDEVICE, "hipfftPlanMany(D2Z, n=%d, batch=%d) failed", npts, batch);
-  if (hipfftSetStream(pl.h, st) != HIPFFT_SUCCESS) return fail(CCFJ_EDEVICE, "hipfftSetStream failed");
-  pl.batch = batch;
-  return CCFJ_OK;
-}
-
 }  // namespace
 
 extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin,
@@ -84,13 +63,15 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
   d.half = (nr + 31) / 32 * 32;
   d.nrhs = (2 * nwin + 31) / 32 * 32;
   const int ndesc = nwin * nr;
+  // windows per rFFT batch: all of them when the tapered traces and their
+  // spectra fit in 8 GiB (64 windows of 480 x 16384 take 8 GB)
   const size_t per_win = (size_t)nr * npts * 8 + (size_t)nr * nbins * 16;
-  const int cw = (int)std::max<size_t>(1, std::min<size_t>(nwin, ((size_t)2 << 30) / per_win));
+  const int cw = (int)std::max<size_t>(1, std::min<size_t>(nwin, ((size_t)8 << 30) / per_win));
 
   Bufs bufs;
   void *du, *dr, *drr, *df, *dc, *dn1, *dn2, *ddl, *dxi, *dxr = nullptr, *dxw, *dspec, *dout;
   int rc = CCFJ_OK;
-  if (!rc) rc = bufs.get(sizeof(double) * (size_t)nr * npts, &du);
+  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 7, sizeof(double) * (size_t)nr * npts, &du);
   if (!rc) rc = bufs.get(sizeof(float) * nr, &dr);
   if (!rc) rc = bufs.get(sizeof(double) * nr, &drr);
   if (!rc) rc = bufs.get(sizeof(float) * nf, &df);
@@ -100,9 +81,11 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
   if (!rc) rc = bufs.get(sizeof(int) * ndesc, &ddl);
   if (!rc) rc = bufs.get(sizeof(int) * ndesc, &dxi);
   if (!rc && nx > 0) rc = bufs.get(sizeof(double) * (size_t)nx * npts, &dxr);
-  if (!rc) rc = bufs.get(sizeof(double) * (size_t)cw * nr * npts, &dxw);
-  if (!rc) rc = bufs.get(16 * (size_t)cw * nr * nbins, &dspec);
-  if (!rc) rc = bufs.get(sizeof(double) * (size_t)nwin * nc * nf, &dout);
+  // the large work arrays live in the context's grow-only scratch slots
+  // 2-7 (kept between calls; slots 0-1 belong to the Hilbert transform)
+  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 2, sizeof(double) * (size_t)cw * nr * npts, &dxw);
+  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 3, 16 * (size_t)cw * nr * nbins, &dspec);
+  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 4, sizeof(double) * (size_t)nwin * nc * nf, &dout);
   if (rc) return rc;
   std::vector<double> hrr(r, r + nr);
   e = hipMemcpyAsync(du, u, sizeof(double) * (size_t)nr * npts, hipMemcpyHostToDevice, st);
@@ -134,8 +117,9 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
   void *dB = nullptr, *dT = nullptr, *dre = nullptr, *dim = nullptr, *dtmp = nullptr;
   if (itype == 0) {
     const size_t Kp = (size_t)(func ? 2 : 1) * d.half;
-    if ((rc = bufs.get(sizeof(float) * nf * Kp * d.nrhs, &dB))) return rc;
-    if ((rc = bufs.get(sizeof(float) * (size_t)nwin * nf * nc, &dT))) return rc;
+    if ((rc = ccfj_internal::ctx_scratch(ctx, 5, sizeof(float) * nf * Kp * d.nrhs, &dB))) return rc;
+    if ((rc = ccfj_internal::ctx_scratch(ctx, 6, sizeof(float) * (size_t)nwin * nf * nc, &dT)))
+      return rc;
     e = fjk::launch_fill((float*)dB, nf * Kp * d.nrhs, 0.0f, st);
     if (e != hipSuccess) return hip_fail(e, "fill kernel launch");
     d.B = (float*)dB;
@@ -146,12 +130,12 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
     if ((rc = bufs.get(sizeof(float) * (size_t)nc * nf, &dtmp))) return rc;
   }
 
-  Plan plan;
   for (int w0 = 0; w0 < nwin; w0 += cw) {
     const int cwn = std::min(cw, nwin - w0);
     if ((e = fjk::launch_window(d, w0, cwn, st)) != hipSuccess) return hip_fail(e, "window kernel");
-    if ((rc = fft_plan(plan, npts, cwn * nr, st))) return rc;
-    if (hipfftExecD2Z(plan.h, (hipfftDoubleReal*)dxw, (hipfftDoubleComplex*)dspec) != HIPFFT_SUCCESS)
+    hipfftHandle plan = 0;
+    if ((rc = ccfj_internal::cached_fft_plan(npts, cwn * nr, HIPFFT_D2Z, st, &plan))) return rc;
+    if (hipfftExecD2Z(plan, (hipfftDoubleReal*)dxw, (hipfftDoubleComplex*)dspec) != HIPFFT_SUCCESS)
       return fail(CCFJ_EDEVICE, "hipfftExecD2Z failed");
     if (itype == 0) {
       if ((e = fjk::launch_mwfj_coef(d, w0, cwn, st)) != hipSuccess) return hip_fail(e, "coef kernel");

@@ -35,7 +35,8 @@ struct BatchDev {
 hipError_t launch_prep_batched(int mode, int itype, const BatchDev& d, int max_nrp,
                                hipStream_t st);
 const void* batched_kernel(int mode, int itype, int tile);
+// items [start, limit) of the work queue
 hipError_t launch_batched(const void* kern, int tile, int grid, const BatchDev& d,
-                          unsigned int limit, hipStream_t st);
+                          unsigned int start, unsigned int limit, hipStream_t st);
 
 }  // namespace fjk

@@ -89,46 +89,56 @@ __global__ __launch_bounds__(256) void k_prep_b(BatchDev d) {
   }
 }
 
-__global__ void k_reset_counter(unsigned int* counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = 0u;
+__global__ void k_reset_counter(unsigned int* counter, unsigned int start) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *counter = start;
 }
 
-// Persistent F-J over work items (p, fi, cb): p indexes d.order (sub-arrays
-// by descending size), cb a tile of TILE velocities.  Items [0, limit) are
-// processed; every workgroup leaves when the counter passes limit.
+// Persistent F-J over work items (p, fi, cb): p indexes order (sub-arrays
+// by descending size), cb a tile of TILE velocities.  Items [start, limit)
+// are processed (the counter starts at start); every workgroup leaves when
+// the counter passes limit.  The
+// arrays come as separate __restrict__ parameters (not through BatchDev):
+// with no store that may alias them, their uniform loads -- the station
+// data of the loop -- are scalar loads, as in k_fj.
 template <int MODE, int ITYPE, int TILE>
-__global__ __launch_bounds__(TILE) void k_fj_b(BatchDev d, unsigned int limit) {
+__global__ __launch_bounds__(TILE)
+__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj_b(
+    int nf, int nc, unsigned int limit, const int64_t* __restrict__ soff,
+    const int64_t* __restrict__ poff, const int64_t* __restrict__ coff,
+    const int* __restrict__ nrpa, const int* __restrict__ order, const float* __restrict__ fv,
+    const float* __restrict__ cv, const double* __restrict__ rr, const float* __restrict__ hh,
+    const float* __restrict__ ir, const float* __restrict__ isr, const float* __restrict__ coef,
+    float* __restrict__ T, unsigned int* __restrict__ counter) {
   constexpr int NC = ITYPE == 0 ? ModeT<MODE>::nspec : 2 * ModeT<MODE>::nspec;
   __shared__ unsigned int item_s;
-  const int nf = d.nf, nc = d.nc;
   const unsigned int ncb = (unsigned int)((nc + TILE - 1) / TILE);
   while (true) {
-    if (threadIdx.x == 0) item_s = atomicAdd(d.counter, 1u);
+    if (threadIdx.x == 0) item_s = atomicAdd(counter, 1u);
     __syncthreads();
-    const unsigned int item = item_s;
+    const unsigned int item = __builtin_amdgcn_readfirstlane(item_s);   // uniform
     __syncthreads();   // every lane has read item_s before it is overwritten
     if (item >= limit) break;
     const unsigned int cb = item % ncb;
     const unsigned int pf = item / ncb;
     const int fi = (int)(pf % (unsigned int)nf);
-    const int b = d.order[pf / (unsigned int)nf];
+    const int b = order[pf / (unsigned int)nf];
     const int j = (int)cb * TILE + (int)threadIdx.x;
-    const int64_t s0 = d.soff[b];
-    const int nr = (int)(d.soff[b + 1] - s0);
+    const int64_t s0 = soff[b];
+    const int nr = (int)(soff[b + 1] - s0);
     if (nr > 1) {   // uniform over the workgroup: every lane runs the lane code
-      const int nrp = d.nrp[b];
-      const double q = (double)d.f[fi] / (double)d.c[j < nc ? j : nc - 1];
-      const float* __restrict__ cf = d.coef + d.coff[b] + (size_t)fi * nrp * NC;
-      const int64_t o = d.poff[b];
-      const Stations S{d.rr + o, d.hh + o, d.ir + o, d.isr + o};
+      const int nrp = nrpa[b];
+      const double q = (double)fv[fi] / (double)cv[j < nc ? j : nc - 1];
+      const float* __restrict__ cf = coef + coff[b] + (size_t)fi * nrp * NC;
+      const int64_t o = poff[b];
+      const Stations S{rr + o, hh + o, ir + o, isr + o};
       float v;
       if (ITYPE == 0)
         v = trapz_lane<MODE>(S, cf, nrp, q);
       else
         v = linseg_lane<MODE>(S, cf, nrp, q);
-      if (j < nc) d.T[((size_t)b * nf + fi) * nc + j] = v;
+      if (j < nc) T[((size_t)b * nf + fi) * nc + j] = v;
     } else if (j < nc) {   // fewer than two stations: empty sum, as the reference
-      d.T[((size_t)b * nf + fi) * nc + j] = 0.0f;
+      T[((size_t)b * nf + fi) * nc + j] = 0.0f;
     }
   }
 }
@@ -183,12 +193,19 @@ const void* batched_kernel(int mode, int itype, int tile) {
 }
 
 hipError_t launch_batched(const void* kern, int tile, int grid, const BatchDev& d,
-                          unsigned int limit, hipStream_t st) {
-  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, d.counter);
+                          unsigned int start, unsigned int limit, hipStream_t st) {
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, d.counter, start);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  BatchDev dd = d;
-  void* args[] = {&dd, &limit};
+  int nf = d.nf, nc = d.nc;
+  const int64_t *soff = d.soff, *poff = d.poff, *coff = d.coff;
+  const int *nrp = d.nrp, *order = d.order;
+  const float *f = d.f, *c = d.c, *hh = d.hh, *ir = d.ir, *isr = d.isr, *coef = d.coef;
+  const double* rr = d.rr;
+  float* T = d.T;
+  unsigned int* counter = d.counter;
+  void* args[] = {&nf, &nc, &limit, &soff, &poff, &coff, &nrp, &order, &f, &c,
+                  &rr, &hh, &ir, &isr, &coef, &T, &counter};
   return hipLaunchKernel(kern, dim3(grid), dim3(tile), args, 0, st);
 }
 

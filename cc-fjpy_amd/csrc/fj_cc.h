@@ -1,0 +1,20 @@
+// fj_cc.h -- cross-correlation kernels (fj_cc.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace fjk {
+
+hipError_t launch_cc_onebit(float* x, size_t n, hipStream_t st);
+hipError_t launch_cc_gather(const float* data, int nsta, int npts, int fftlen, int steplen, int w0,
+                            int cw, float* seg, hipStream_t st);
+hipError_t launch_cc_spec(const void* spec, int rows, int nbins, int nf, int fstride, int whiten,
+                          void* S, hipStream_t st);
+hipError_t launch_cc_pairs(const void* S, const unsigned char* use, const int* pairs, int npairs,
+                           int nsta, int nf, int w0, int cw, void* acc, hipStream_t st);
+hipError_t launch_cc_interleave(const float* re, const float* im, size_t n, void* acc,
+                                hipStream_t st);
+hipError_t launch_cc_finish(const void* acc, const float* count, int npairs, int nf, float* re,
+                            float* im, hipStream_t st);
+
+}  // namespace fjk

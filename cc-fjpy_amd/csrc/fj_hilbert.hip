@@ -76,30 +76,45 @@ struct Bufs {
   }
 };
 
-// hipFFT plans, cached per (device, length, batch, direction) for the life
-// of the process (a plan holds only its small work area).
-std::mutex g_plan_mu;
-std::map<std::tuple<int, int, int, int>, hipfftHandle> g_plans;
+}  // namespace
 
-int get_plan(int n, int batch, bool forward, hipStream_t st, hipfftHandle* h) {
+namespace ccfj_internal {
+
+// hipFFT plans, cached per (device, length, batch, type) for the life of the
+// process (a plan holds only its small work area); packed rows: real rows of
+// n, complex rows of n/2 + 1.
+static std::mutex g_plan_mu;
+static std::map<std::tuple<int, int, int, int>, hipfftHandle> g_plans;
+
+int cached_fft_plan(int n, int batch, hipfftType type, hipStream_t st, hipfftHandle* h) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   std::lock_guard<std::mutex> lk(g_plan_mu);
-  auto key = std::make_tuple(dev, n, batch, forward ? 1 : 0);
+  auto key = std::make_tuple(dev, n, batch, (int)type);
   auto it = g_plans.find(key);
   if (it == g_plans.end()) {
     hipfftHandle p = 0;
     int nn[1] = {n};
     const int nb = n / 2 + 1;
-    hipfftResult r = forward
-        ? hipfftPlanMany(&p, 1, nn, nullptr, 1, n, nullptr, 1, nb, HIPFFT_D2Z, batch)
-        : hipfftPlanMany(&p, 1, nn, nullptr, 1, nb, nullptr, 1, n, HIPFFT_Z2D, batch);
-    if (r != HIPFFT_SUCCESS) return fail(CCFJ_EDEVICE, "hipfftPlanMany (n=%d, batch=%d) failed", n, batch);
+    const bool fwd = type == HIPFFT_D2Z || type == HIPFFT_R2C;
+    const hipfftResult r = hipfftPlanMany(&p, 1, nn, nullptr, 1, fwd ? n : nb, nullptr, 1,
+                                          fwd ? nb : n, type, batch);
+    if (r != HIPFFT_SUCCESS)
+      return fail(CCFJ_EDEVICE, "hipfftPlanMany (n=%d, batch=%d, type=%d) failed", n, batch,
+                  (int)type);
     it = g_plans.emplace(key, p).first;
   }
   if (hipfftSetStream(it->second, st) != HIPFFT_SUCCESS) return fail(CCFJ_EDEVICE, "hipfftSetStream failed");
   *h = it->second;
   return CCFJ_OK;
+}
+
+}  // namespace ccfj_internal
+
+namespace {
+
+int get_plan(int n, int batch, bool forward, hipStream_t st, hipfftHandle* h) {
+  return ccfj_internal::cached_fft_plan(n, batch, forward ? HIPFFT_D2Z : HIPFFT_Z2D, st, h);
 }
 
 // Hilbert transform of rows x -> y (device, float32), double inside; the

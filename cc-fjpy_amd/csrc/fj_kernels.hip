@@ -39,24 +39,6 @@ using namespace fjl;
 
 constexpr int kBlock = 256;
 
-// Occupancy floor of the main kernels (waves per SIMD; the compiler then
-// keeps VGPRs within 512 / waves).  Measured on the MI355X: the fused J - Y
-// linear-segment kernel (natural 217 VGPRs, 2 waves) runs 16% faster held
-// to 3 waves, a few spills on its cold small-argument path included; the
-// other modes are fastest at their natural count (forcing them spills on the
-// hot path).  -DFJ_OCC_I1_JMY=N / -DFJ_OCC_I1_J=N override for tuning runs.
-#ifndef FJ_OCC_I1_JMY
-#define FJ_OCC_I1_JMY 3
-#endif
-#ifndef FJ_OCC_I1_J
-#define FJ_OCC_I1_J 1
-#endif
-template <int MODE, int ITYPE>
-struct FjOcc {
-  static constexpr int waves =
-      ITYPE == 0 ? 1 : (MODE == MODE_J_MINUS_Y ? FJ_OCC_I1_JMY : (MODE == MODE_J ? FJ_OCC_I1_J : 1));
-};
-
 // Hardware places workgroup w on XCD (w mod 8); tile T = (w mod 8) * per +
 // w / 8 keeps consecutive tiles (same frequency) on one XCD.  Returns false
 // for the padding workgroups of the last round.

@@ -181,6 +181,27 @@ FJ_HD inline int coef_count(int mode, int itype) {
   return itype == 0 ? nspec : 2 * nspec;
 }
 
+// Occupancy floor of the F-J kernels (waves per SIMD; the compiler then
+// keeps VGPRs within 512 / waves), used by the dense, batched and item-list
+// kernels alike.  Measured on the MI355X: the fused J - Y
+// linear-segment kernel (natural 217 VGPRs, 2 waves) runs 16% faster held
+// to 3 waves, a few spills on its cold small-argument path included; the
+// other modes are fastest at their natural count (forcing them spills on the
+// hot path).  The linear-segment J kernel holds 4 waves (121 VGPRs in the
+// dense kernel; the persistent batched kernel's loop would take 170).
+// -DFJ_OCC_I1_JMY=N / -DFJ_OCC_I1_J=N override for tuning runs.
+#ifndef FJ_OCC_I1_JMY
+#define FJ_OCC_I1_JMY 3
+#endif
+#ifndef FJ_OCC_I1_J
+#define FJ_OCC_I1_J 4
+#endif
+template <int MODE, int ITYPE>
+struct FjOcc {
+  static constexpr int waves =
+      ITYPE == 0 ? 1 : (MODE == MODE_J_MINUS_Y ? FJ_OCC_I1_JMY : (MODE == MODE_J ? FJ_OCC_I1_J : 1));
+};
+
 // ---- coefficients of station s (one frequency) ----------------------------
 // g_prev/g_cur: spectrum values at stations s-1 and s (g_prev unused for
 // s = 0).  rp, rs, rn: r_{s-1}, r_s, r_{s+1} (only those that exist are read).
@@ -425,7 +446,7 @@ struct LinState {
 // of the segment to its left (2 D_s: Euler-Maclaurin, or the G difference
 // on the lanes whose segment needs it -- evaluated only when some lane of
 // the wave does), and the station's contribution alpha*g - (beta/2)*2D.
-template <int MODE>
+template <int MODE, bool FAR>
 FJ_HD inline void linseg_tail(int s, float x, float dxL, bool useG_L, float j0, float j1, float y0,
                               float y1, float gy, Ends& cj, Ends& cy, const float* c,
                               LinState<MODE>& st) {
@@ -442,7 +463,13 @@ FJ_HD inline void linseg_tail(int s, float x, float dxL, bool useG_L, float j0, 
     cy.g3 = -(y1 + cy.gp);
   }
   float Ej = 0.0f, Ey = 0.0f;  // 2 D_s per basis
-  if (s > 0) {
+  // Two forms, chosen by measurement on the MI355X: straight-line (far
+  // blocks, and the two-spectrum modes everywhere) or a per-lane branch (the
+  // one-spectrum modes' general loop, which it keeps lighter in registers).
+  constexpr bool SELECT = FAR || ModeT<MODE>::nspec == 2;
+  if (SELECT && s > 0) {
+    // the rule everywhere, the G difference selected on the lanes that need
+    // it when any does (a wave-uniform branch)
     const float e1 = dxL * (-1.0f / 6.0f);
     const float e3 = dxL * dxL * dxL * (1.0f / 360.0f);
     if (NJ) Ej = em_divided2(st.pj, cj, e1, e3);
@@ -457,6 +484,18 @@ FJ_HD inline void linseg_tail(int s, float x, float dxL, bool useG_L, float j0, 
         const float g = gdiff_y2(st.py, st.xprev, cy, x, inv2);
         Ey = useG_L ? g : Ey;
       }
+    }
+  } else if (s > 0) {
+    // per lane: one of the two (fewer live registers)
+    if (useG_L) {
+      const float inv2 = rcp_f(0.5f * dxL);
+      if (NJ) Ej = (cj.G - st.pj.G) * inv2;
+      if (NY) Ey = gdiff_y2(st.py, st.xprev, cy, x, inv2);
+    } else {
+      const float e1 = dxL * (-1.0f / 6.0f);
+      const float e3 = dxL * dxL * dxL * (1.0f / 360.0f);
+      if (NJ) Ej = em_divided2(st.pj, cj, e1, e3);
+      if (NY) Ey = em_divided2(st.py, cy, e1, e3);
     }
   }
   st.xprev = x;
@@ -524,13 +563,15 @@ FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dx
     if (needG) g_small(xd, y, Lg, NJ, NY, &cj.G, &cy.G);
     cy.sh = 1.0f;
   }
-  linseg_tail<MODE>(s, x, dxL, useG_L, j0, j1, y0, y1, gy, cj, cy, c, st);
+  linseg_tail<MODE, FAR>(s, x, dxL, useG_L, j0, j1, y0, y1, gy, cj, cy, c, st);
 }
 
-// Far blocks in the linear-segment loop; -DFJ_FAR_LINSEG_ONE=0 turns them
-// off for the one-spectrum modes (a tuning switch).
+// Far blocks in the linear-segment loop.  Measured on the MI355X they pay
+// for the two-spectrum modes (J - Y: 69 -> 58 ms per 512 dense columns) and
+// not for the one-spectrum modes, whose general loop is lighter in
+// registers; -DFJ_FAR_LINSEG_ONE=1 turns them on there (a tuning switch).
 #ifndef FJ_FAR_LINSEG_ONE
-#define FJ_FAR_LINSEG_ONE 1
+#define FJ_FAR_LINSEG_ONE 0
 #endif
 template <int MODE>
 struct LinFar {

@@ -15,6 +15,9 @@ hipError_t launch_synth_fields(const float* r, const float* f, const float* mode
 hipError_t launch_synth_labels(const float* cgrid, const float* f, const float* modes, int B,
                                int H, int nf, float halfwidth, float amin, float* Y,
                                hipStream_t st);
+hipError_t launch_pool_abs_max(const float* in, int B, int H, int S, int W, float* out,
+                               hipStream_t st);
+hipError_t launch_label_argmax(const float* X, int B, int H, int W, float* Y, hipStream_t st);
 hipError_t launch_fj_items(int mode, int itype, const fjl::Stations& S, const float* coef,
                            const float* f, const float* c, const int* items, int nitems, int nr,
                            int nrp, int nf, int nc, float* out, hipStream_t st);

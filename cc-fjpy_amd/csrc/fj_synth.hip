@@ -82,12 +82,49 @@ __global__ void k_synth_labels(const float* __restrict__ cgrid, const float* __r
   Y[((size_t)b * H + h) * nf + i] = lab;
 }
 
+// out[b][h][w] = max over s < S of |in[b][h*S + s][w]|: the coarse spectrum
+// as the envelope of S sub-velocities per cell (a ridge narrower than a
+// cell still shows).
+__global__ void k_pool_abs_max(const float* __restrict__ in, int H, int S, int W,
+                               float* __restrict__ out) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  const int h = blockIdx.y;
+  const int b = blockIdx.z;
+  if (w >= W) return;
+  const float* src = in + ((size_t)b * H * S + (size_t)h * S) * W + w;
+  float m = fabsf(src[0]);
+  for (int s = 1; s < S; s++) m = fmaxf(m, fabsf(src[(size_t)s * W]));
+  out[((size_t)b * H + h) * W + w] = m;
+}
+
+// Y[b][h*][w] = 1 at the velocity cell h* holding the column maximum of X[b]
+// (first on ties); other entries unchanged.
+__global__ void k_label_argmax(const float* __restrict__ X, int B, int H, int W,
+                               float* __restrict__ Y) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (w >= W || b >= B) return;
+  const float* x = X + (size_t)b * H * W + w;
+  int best = 0;
+  float m = x[0];
+  for (int h = 1; h < H; h++) {
+    const float v = x[(size_t)h * W];
+    if (v > m) {
+      m = v;
+      best = h;
+    }
+  }
+  Y[((size_t)b * H + best) * W + w] = 1.0f;
+}
+
 // F-J on a list of (frequency, first velocity, count <= 256) items; lanes
 // beyond count idle.  out[c][f] is written only for listed entries; an item
 // outside the grid is skipped (items live in device memory, so the host
 // cannot check them).  Fewer than two stations: zeros, as the full grid.
 template <int MODE, int ITYPE>
-__global__ __launch_bounds__(256) void k_fj_items(Stations S, const float* __restrict__ coef,
+__global__ __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj_items(
+    Stations S, const float* __restrict__ coef,
                                                  const float* __restrict__ fv,
                                                  const float* __restrict__ cv,
                                                  const int* __restrict__ items, int nr, int nrp,
@@ -124,6 +161,18 @@ hipError_t launch_synth_labels(const float* cgrid, const float* f, const float* 
                                hipStream_t st) {
   hipLaunchKernelGGL(k_synth_labels, dim3((nf + 255) / 256, H, B), dim3(256), 0, st, cgrid, f,
                      modes, H, nf, halfwidth, amin, Y);
+  return hipGetLastError();
+}
+
+hipError_t launch_pool_abs_max(const float* in, int B, int H, int S, int W, float* out,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_pool_abs_max, dim3((W + 255) / 256, H, B), dim3(256), 0, st, in, H, S, W,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t launch_label_argmax(const float* X, int B, int H, int W, float* Y, hipStream_t st) {
+  hipLaunchKernelGGL(k_label_argmax, dim3((W + 255) / 256, B), dim3(256), 0, st, X, B, H, W, Y);
   return hipGetLastError();
 }
 

@@ -172,6 +172,43 @@ int ccfj_fj_noise(ccfj_ctx *ctx, int itype, int func, int fstride, const float *
                   const float *r, const float *f, const float *c, int nr, int ncol,
                   int nf, int nc, float *out);
 
+/* ---- 5b. cross-correlation (the producer of the noise spectra) ----------
+ * Reference: src/CrossCorr.cpp:33-159, include/CrossCorr.h:14-35.  Windows
+ * of fftlen samples every steplen samples (fftlen >= npts: one window of
+ * npts); station i uses window k when k*steplen >= startend[2i] and
+ * k*steplen + fftlen < startend[2i+1]; optional one-bit normalisation and
+ * spectral whitening (X / max(|X|, 1e-8)); for pair p = (Pairs[2p],
+ * Pairs[2p+1]) the mean over shared windows of conj(X_a) X_b at bins
+ * j*fstride, j < nf, into ncfsr/ncfsi[npairs][nf] (zero without a shared
+ * window; the arrays are overwritten).  Host pointers; synchronous.
+ * Single-precision FFT on the GPU. */
+int ccfj_cross_correlate(ccfj_ctx *ctx, int nsta, int npts, int nf, int fstride, int fftlen,
+                         int steplen, int onebit, int whiten, int npairs, const int *pairs,
+                         const int *startend, const float *data, float *ncfsr, float *ncfsi);
+/* The reference's entry point and argument block (include/CrossCorr.h:
+ * 14-35), on the current device's default context.  As the reference, the
+ * sums start from the values already in ncfsr/ncfsi (its wrapper passes
+ * zeros), and data is replaced by its signs when ifonebit is set; nThreads
+ * is ignored. */
+typedef struct CC_data {
+  int nsta;
+  int npts;
+  int nf;
+  int npairs;
+  int fstride;
+  int fftlen;
+  int steplen;
+  int *Pairs;
+  int *startend;
+  float *data;
+  float *ncfsr;
+  float *ncfsi;
+  int ifonebit;
+  int ifspecwhitenning;
+  int nThreads;
+} CC_data;
+int CrossCorrelation(CC_data d);
+
 /* ---- 6. learned coarse-to-fine ----------------------------------------------
  * No counterpart in the reference.  A small CNN proposes, from a coarse F-J
  * spectrum, the velocity bands that hold modal energy per frequency; the
@@ -200,6 +237,14 @@ int ccfj_synth_fields(ccfj_ctx *ctx, int B, int nr, int nf, const float *r, cons
  * <= halfwidth for a mode with A >= amin. */
 int ccfj_synth_labels(ccfj_ctx *ctx, int B, int H, int nf, const float *cgrid, const float *f,
                       const float *modes, float halfwidth, float amin, float *Y, void *stream);
+
+/* out[B][H][W] = max over s < S of |in[B][H*S + s][W]|: the coarse spectrum
+ * as the envelope of S sub-velocities per cell. */
+int ccfj_pool_abs_max(ccfj_ctx *ctx, const float *in, int B, int H, int S, int W, float *out,
+                      void *stream);
+/* Y[b][h*][w] = 1 where h* is the cell of the column maximum of X[b][:][w]
+ * (X >= 0; first on ties); other entries of Y unchanged. */
+int ccfj_label_argmax(ccfj_ctx *ctx, const float *X, int B, int H, int W, float *Y, void *stream);
 
 /* Band-proposal CNN: conv3x3 1->32->32->32->1 (ReLU between), bf16 weights,
  * fp32 accumulation, input and output [B][1][H][W]. */

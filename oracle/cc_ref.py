@@ -1,0 +1,59 @@
+"""CPU restatement of the reference's ambient-noise cross-correlation.
+TEST INFRASTRUCTURE ONLY.
+
+Follows src/CrossCorr.cpp:33-159 (CrossCorrelation) and src/ccfj.pyx:125-146
+(the CC wrapper):
+  * fftlen >= npts: fftlen = npts and one window; otherwise
+    nshifts = 1 + (npts - fftlen) // steplen              (CrossCorr.cpp:50-56)
+  * one-bit: data = sign(data), in place                  (:76-85)
+  * window k of station i is used when k*steplen >= start_i and
+    k*steplen + fftlen < end_i                             (:91-98)
+  * spectrum: real FFT of the window, bins j*fstride for j < nf; whitening
+    divides by max(|X|, 1e-8) in float                     (:100-119)
+  * pair (a, b): sum over windows where both stations are used of
+    conj(X_a) X_b, divided by that count                   (:124-150)
+The reference's FFT is FFTW in single precision (its third-party dependency,
+absent here); this restatement uses numpy's float64 FFT of the float32 data,
+so the two agree to single-precision rounding, not bit for bit.
+"""
+import numpy as np
+
+
+def windows(npts, fftlen, steplen):
+    """(fftlen, nshifts) after the reference's clamp."""
+    if fftlen >= npts:
+        return npts, 1
+    return fftlen, 1 + (npts - fftlen) // steplen
+
+
+def used(startend, k, steplen, fftlen):
+    """Stations whose window k lies inside [start, end)."""
+    se = np.asarray(startend).reshape(-1, 2)
+    t0 = k * steplen
+    return (t0 >= se[:, 0]) & (t0 + fftlen < se[:, 1])
+
+
+def cross_correlate(data, nsta, npts, nf, fftlen, steplen, pairs, startend, fstride=1,
+                    onebit=False, whiten=True):
+    """-> (ncfs complex128 [npairs, nf], count [npairs])."""
+    x = np.asarray(data, np.float32).reshape(nsta, npts)
+    if onebit:
+        x = np.sign(x).astype(np.float32)
+    fftlen, nshifts = windows(npts, fftlen, steplen)
+    p = np.asarray(pairs).reshape(-1, 2)
+    acc = np.zeros((len(p), nf), np.complex128)
+    count = np.zeros(len(p), np.int64)
+    for k in range(nshifts):
+        ok = used(startend, k, steplen, fftlen)
+        seg = x[:, k * steplen:k * steplen + fftlen].astype(np.float64)
+        spec = np.fft.rfft(seg, n=fftlen, axis=1)[:, ::fstride][:, :nf]
+        if whiten:
+            mag = np.maximum(np.abs(spec).astype(np.float32), np.float32(1e-8))
+            spec = spec / mag
+        both = ok[p[:, 0]] & ok[p[:, 1]]
+        acc[both] += np.conj(spec[p[both, 0]]) * spec[p[both, 1]]
+        count[both] += 1
+    out = np.zeros_like(acc)
+    has = count > 0
+    out[has] = acc[has] / count[has, None]
+    return out, count

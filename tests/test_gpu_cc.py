@@ -1,0 +1,93 @@
+"""Device cross-correlation (ccfj.CC, CrossCorrelation) against the CPU
+restatement (oracle/cc_ref.py).  Bar: |GPU - oracle| <= 1e-5 of the largest
+|oracle| entry (single-precision FFTs on both sides of the reference's
+definition; the oracle's FFT is float64)."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from oracle import cc_ref as C
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _case(seed, nsta, npts, gaps=True):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(nsta, npts)).astype(np.float32)
+    x += 0.3 * np.roll(x[:1], 37, axis=1)   # a correlated component
+    se = np.tile(np.array([0, npts], np.int32), nsta)
+    if gaps:
+        se[2] = npts // 3          # station 1 starts late
+        se[5] = 2 * npts // 3      # station 2 ends early
+    return x, se
+
+
+@pytest.mark.parametrize("whiten", [0, 1])
+@pytest.mark.parametrize("onebit", [0, 1])
+def test_cc_matches_oracle(ccfj, whiten, onebit):
+    nsta, npts, fftlen, nf, fstride = 6, 6000, 1024, 200, 2
+    x, se = _case(3, nsta, npts)
+    pairs = ccfj.GetStationPairs(nsta)
+    data = x.reshape(-1).copy()
+    got = ccfj.CC(npts, nsta, nf, fftlen, pairs, se, data, overlaprate=0.5, fstride=fstride,
+                  ifonebit=onebit, ifspecwhittenning=whiten)
+    want, count = C.cross_correlate(x.reshape(-1), nsta, npts, nf, fftlen, fftlen // 2, pairs, se,
+                                    fstride=fstride, onebit=bool(onebit), whiten=bool(whiten))
+    assert got.dtype == np.complex64 and got.shape == (len(pairs) // 2, nf)
+    assert np.all(count > 0)
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    print(f"whiten={whiten} onebit={onebit}: max rel err {err:.2e}")
+    assert err <= TOL
+    if onebit:   # the reference's in-place side effect on float32 data
+        assert np.array_equal(data, np.sign(x.reshape(-1)))
+    else:
+        assert np.array_equal(data, x.reshape(-1))
+
+
+def test_cc_single_window_and_no_shared_window(ccfj):
+    nsta, npts = 3, 512
+    x, _ = _case(4, nsta, npts, gaps=False)
+    se = np.array([0, 600, 0, 600, 0, 100], np.int32)   # station 2 never has the window
+    pairs = ccfj.GetStationPairs(nsta)
+    got = ccfj.CC(npts, nsta, 100, 4096, pairs, se, x.reshape(-1).copy(), ifspecwhittenning=0)
+    want, count = C.cross_correlate(x.reshape(-1), nsta, npts, 100, 4096, 4096, pairs, se,
+                                    whiten=False)
+    assert list(count) == [1, 0, 0]
+    assert np.all(got[1:] == 0)
+    assert np.max(np.abs(got - want)) <= TOL * np.max(np.abs(want))
+
+
+def test_reference_entry_point_accumulates(ccfj):
+    """CrossCorrelation(CC_data) starts from the caller's arrays, as the
+    reference's += does, and replaces data by its signs when one-bit."""
+    class CCData(ctypes.Structure):
+        _fields_ = [(n, ctypes.c_int) for n in ("nsta", "npts", "nf", "npairs", "fstride",
+                                                 "fftlen", "steplen")] + \
+                   [(n, ctypes.c_void_p) for n in ("Pairs", "startend", "data", "ncfsr",
+                                                  "ncfsi")] + \
+                   [(n, ctypes.c_int) for n in ("ifonebit", "ifspecwhitenning", "nThreads")]
+    lib = ctypes.CDLL(os.path.join(ROOT, "cc-fjpy_amd", "ccfj", "libccfj.so"))
+    lib.CrossCorrelation.argtypes = [CCData]
+    nsta, npts, nf, fftlen = 4, 3000, 64, 512
+    x, se = _case(5, nsta, npts)
+    pairs = ccfj.GetStationPairs(nsta)
+    npairs = len(pairs) // 2
+    re0 = np.full((npairs, nf), 0.25, np.float32)
+    im0 = np.full((npairs, nf), -0.5, np.float32)
+    re, im = re0.copy(), im0.copy()
+    data = x.reshape(-1).copy()
+    d = CCData(nsta, npts, nf, npairs, 1, fftlen, fftlen // 4, pairs.ctypes.data, se.ctypes.data,
+               data.ctypes.data, re.ctypes.data, im.ctypes.data, 1, 1, 8)
+    assert lib.CrossCorrelation(d) == 0
+    sums, count = C.cross_correlate(x.reshape(-1), nsta, npts, nf, fftlen, fftlen // 4, pairs, se,
+                                    onebit=True, whiten=True)
+    want = (sums * count[:, None] + (re0 + 1j * im0)) / np.maximum(count, 1)[:, None]
+    want[count == 0] = re0[count == 0] + 1j * im0[count == 0]
+    got = re + 1j * im
+    assert np.max(np.abs(got - want)) <= TOL * np.max(np.abs(want))
+    assert np.array_equal(data, np.sign(x.reshape(-1)))

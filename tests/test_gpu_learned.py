@@ -179,3 +179,21 @@ def test_items_match_full_grid(ccfj, lib, ctx):
                 mask[c0:min(c0 + n, nc), fi] = True
         assert np.array_equal(got[mask], want[mask]), itype
         assert np.all(got[~mask] == 0)
+
+
+def test_pool_and_argmax_labels(lib, ctx):
+    rng = np.random.default_rng(60)
+    B, H, S, W = 3, 8, 4, 40
+    x = rng.normal(size=(B, H * S, W)).astype(np.float32)
+    dx = lib.device_array(ctx, x)
+    dp = lib.DeviceArray(ctx, (B, H, W))
+    lib.pool_abs_max(ctx, dx.addr, B, H, S, W, dp.addr)
+    pooled = dp.download()
+    assert np.array_equal(pooled, np.abs(x).reshape(B, H, S, W).max(axis=2))
+    y0 = (rng.random((B, H, W)) > 0.9).astype(np.float32)
+    dy = lib.device_array(ctx, y0)
+    lib.label_argmax(ctx, dp.addr, B, H, W, dy.addr)
+    want = y0.copy()
+    for b in range(B):
+        want[b, np.argmax(pooled[b], axis=0), np.arange(W)] = 1.0
+    assert np.array_equal(dy.download(), want)

@@ -1,0 +1,70 @@
+"""The CPU restatement of the cross-correlation (oracle/cc_ref.py), pinned
+by the cross-correlation theorem against a brute-force time-domain
+correlation, by the reference's window rules, and by whitening's unit
+magnitude."""
+import numpy as np
+
+from oracle import cc_ref as C
+
+
+def test_window_count_rule():
+    assert C.windows(1000, 1000, 100) == (1000, 1)
+    assert C.windows(1000, 5000, 100) == (1000, 1)
+    assert C.windows(1000, 256, 128) == (256, 1 + (1000 - 256) // 128)
+    assert C.windows(1000, 256, 77) == (256, 1 + 744 // 77)
+
+
+def test_window_use_rule():
+    se = np.array([0, 1000, 100, 1000, 0, 700], np.int32)
+    # window k covers [k*step, k*step + fftlen): needs start <= k*step and
+    # k*step + fftlen < end
+    assert list(C.used(se, 0, 128, 256)) == [True, False, True]
+    assert list(C.used(se, 1, 128, 256)) == [True, True, True]
+    assert list(C.used(se, 3, 128, 256)) == [True, True, True]    # 384 + 256 = 640 < 700
+    assert list(C.used(se, 4, 128, 256)) == [True, True, False]   # 512 + 256 = 768 >= 700
+
+
+def test_single_window_is_circular_cross_correlation():
+    rng = np.random.default_rng(0)
+    n = 64
+    x = rng.normal(size=(2, n)).astype(np.float32)
+    nf = n // 2 + 1
+    out, count = C.cross_correlate(x.reshape(-1), 2, n, nf, n, n, [0, 1], [0, n + 1, 0, n + 1],
+                                   whiten=False)
+    assert list(count) == [1]
+    cc = np.fft.irfft(out[0], n=n)
+    a, b = x[0].astype(np.float64), x[1].astype(np.float64)
+    brute = np.array([np.sum(a * np.roll(b, -tau)) for tau in range(n)])   # sum_t a[t] b[t+tau]
+    assert np.allclose(cc, brute, atol=1e-9)
+
+
+def test_whitened_products_are_phases_and_average():
+    rng = np.random.default_rng(1)
+    nsta, npts, fftlen, step = 3, 900, 128, 64
+    x = rng.normal(size=(nsta, npts)).astype(np.float32)
+    se = np.array([0, npts, 0, npts, 300, npts], np.int32)
+    pairs = [0, 1, 0, 2, 1, 2]
+    out, count = C.cross_correlate(x.reshape(-1), nsta, npts, 20, fftlen, step, pairs, se,
+                                   fstride=2, whiten=True)
+    nshifts = 1 + (npts - fftlen) // step
+    ok2 = [k for k in range(nshifts) if k * step >= 300 and k * step + fftlen < npts]
+    ok0 = [k for k in range(nshifts) if k * step + fftlen < npts]
+    assert list(count) == [len(ok0), len(ok2), len(ok2)]
+    # pair (0, 1): mean of unit phasors over the windows
+    acc = 0
+    for k in ok0:
+        s = np.fft.rfft(x[:, k * step:k * step + fftlen].astype(np.float64), axis=1)[:, ::2][:, :20]
+        s = s / np.abs(s)
+        acc = acc + np.conj(s[0]) * s[1]
+    assert np.allclose(out[0], acc / len(ok0), atol=1e-6)
+    assert np.all(np.abs(out) <= 1 + 1e-6)
+
+
+def test_onebit_uses_signs():
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=(2, 300)).astype(np.float32)
+    a, _ = C.cross_correlate(x.reshape(-1), 2, 300, 10, 128, 64, [0, 1], [0, 400, 0, 400],
+                             onebit=True, whiten=False)
+    b, _ = C.cross_correlate(np.sign(x).reshape(-1), 2, 300, 10, 128, 64, [0, 1],
+                             [0, 400, 0, 400], whiten=False)
+    assert np.array_equal(a, b)
