@@ -3,6 +3,19 @@
 #ifndef FJ_HD
 #define FJ_HD
 #endif
+/* double tables: __constant__ on the device, plain on the host */
+enum { FIT_LITERAL = 0, FIT_TABLE = 1 };
+#if defined(__HIPCC__)
+#define FJ_TABLE(name, n, ...) static __constant__ double name##_d[n] = __VA_ARGS__; static const double name##_h[n] = __VA_ARGS__;
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FJ_TABLE_PTR(name) name##_d
+#else
+#define FJ_TABLE_PTR(name) name##_h
+#endif
+#else
+#define FJ_TABLE(name, n, ...) static const double name##_h[n] = __VA_ARGS__;
+#define FJ_TABLE_PTR(name) name##_h
+#endif
 
 /* large argument x >= 8, t = (8/x)^2, float */
 /* max abs error of the fit on (0,1]: 8.30e-11 */
@@ -114,7 +127,25 @@ FJ_HD inline float fitf_R1(float t) {
 
 /* small argument x < 8, y = (x/8)^2, double */
 /* max abs error of the fit on (0,1]: 8.04e-11 */
+FJ_TABLE(k_fit_J0s, 12, {-0.005618801482900782, 0.07079209695405728, -0.5037809434408522, 2.625220559644334, -10.557357884793742, 32.35918804582165, -72.81660855664654, 113.77757421102329, -113.77775682787487, 63.99999888465921, -15.999999976722172, 0.9999999999195268})
+template <int TAB = FIT_LITERAL>
 FJ_HD inline double fit_J0s(double y) {
+  if (TAB == FIT_TABLE) {
+    const double* c = FJ_TABLE_PTR(k_fit_J0s);
+    double p = c[0];
+    p = __builtin_fma(p, y, c[1]);
+    p = __builtin_fma(p, y, c[2]);
+    p = __builtin_fma(p, y, c[3]);
+    p = __builtin_fma(p, y, c[4]);
+    p = __builtin_fma(p, y, c[5]);
+    p = __builtin_fma(p, y, c[6]);
+    p = __builtin_fma(p, y, c[7]);
+    p = __builtin_fma(p, y, c[8]);
+    p = __builtin_fma(p, y, c[9]);
+    p = __builtin_fma(p, y, c[10]);
+    p = __builtin_fma(p, y, c[11]);
+    return p;
+  }
   double p = -0.005618801482900782;
   p = __builtin_fma(p, y, 0.07079209695405728);
   p = __builtin_fma(p, y, -0.5037809434408522);
@@ -130,7 +161,24 @@ FJ_HD inline double fit_J0s(double y) {
   return p;
 }
 /* max abs error of the fit on (0,1]: 1.21e-10 */
+FJ_TABLE(k_fit_J1s, 11, {0.001933119992191311, -0.022146255813530545, 0.1417755820614683, -0.6564575115820559, 2.309572547363542, -6.06743715646006, 11.377627299279547, -14.222203592242003, 10.66666548155003, -3.9999999705566402, 0.4999999998789634})
+template <int TAB = FIT_LITERAL>
 FJ_HD inline double fit_J1s(double y) {
+  if (TAB == FIT_TABLE) {
+    const double* c = FJ_TABLE_PTR(k_fit_J1s);
+    double p = c[0];
+    p = __builtin_fma(p, y, c[1]);
+    p = __builtin_fma(p, y, c[2]);
+    p = __builtin_fma(p, y, c[3]);
+    p = __builtin_fma(p, y, c[4]);
+    p = __builtin_fma(p, y, c[5]);
+    p = __builtin_fma(p, y, c[6]);
+    p = __builtin_fma(p, y, c[7]);
+    p = __builtin_fma(p, y, c[8]);
+    p = __builtin_fma(p, y, c[9]);
+    p = __builtin_fma(p, y, c[10]);
+    return p;
+  }
   double p = 0.001933119992191311;
   p = __builtin_fma(p, y, -0.022146255813530545);
   p = __builtin_fma(p, y, 0.1417755820614683);
@@ -145,7 +193,25 @@ FJ_HD inline double fit_J1s(double y) {
   return p;
 }
 /* max abs error of the fit on (0,1]: 1.62e-10 */
+FJ_TABLE(k_fit_Y0s, 12, {0.011007520614961696, -0.13607675855273377, 0.9417493466708342, -4.732348709939299, 18.20275922795894, -52.85781250766169, 111.22092650701482, -159.2993419475141, 141.1914151817568, -65.8389707815695, 11.366785032612231, -0.07380429494619678})
+template <int TAB = FIT_LITERAL>
 FJ_HD inline double fit_Y0s(double y) {
+  if (TAB == FIT_TABLE) {
+    const double* c = FJ_TABLE_PTR(k_fit_Y0s);
+    double p = c[0];
+    p = __builtin_fma(p, y, c[1]);
+    p = __builtin_fma(p, y, c[2]);
+    p = __builtin_fma(p, y, c[3]);
+    p = __builtin_fma(p, y, c[4]);
+    p = __builtin_fma(p, y, c[5]);
+    p = __builtin_fma(p, y, c[6]);
+    p = __builtin_fma(p, y, c[7]);
+    p = __builtin_fma(p, y, c[8]);
+    p = __builtin_fma(p, y, c[9]);
+    p = __builtin_fma(p, y, c[10]);
+    p = __builtin_fma(p, y, c[11]);
+    return p;
+  }
   double p = 0.011007520614961696;
   p = __builtin_fma(p, y, -0.13607675855273377);
   p = __builtin_fma(p, y, 0.9417493466708342);
@@ -161,7 +227,24 @@ FJ_HD inline double fit_Y0s(double y) {
   return p;
 }
 /* max abs error of the fit on (0,1]: 2.41e-10 */
+FJ_TABLE(k_fit_Y1s, 11, {-0.0037279760680778515, 0.04184985392300427, -0.2599904328974209, 1.1572271941470782, -3.8770978479644107, 9.589115060130059, -16.654007220047983, 18.780663548716326, -12.104928295564273, 3.478315983625302, -0.19605709040518532})
+template <int TAB = FIT_LITERAL>
 FJ_HD inline double fit_Y1s(double y) {
+  if (TAB == FIT_TABLE) {
+    const double* c = FJ_TABLE_PTR(k_fit_Y1s);
+    double p = c[0];
+    p = __builtin_fma(p, y, c[1]);
+    p = __builtin_fma(p, y, c[2]);
+    p = __builtin_fma(p, y, c[3]);
+    p = __builtin_fma(p, y, c[4]);
+    p = __builtin_fma(p, y, c[5]);
+    p = __builtin_fma(p, y, c[6]);
+    p = __builtin_fma(p, y, c[7]);
+    p = __builtin_fma(p, y, c[8]);
+    p = __builtin_fma(p, y, c[9]);
+    p = __builtin_fma(p, y, c[10]);
+    return p;
+  }
   double p = -0.0037279760680778515;
   p = __builtin_fma(p, y, 0.04184985392300427);
   p = __builtin_fma(p, y, -0.2599904328974209);
@@ -176,7 +259,24 @@ FJ_HD inline double fit_Y1s(double y) {
   return p;
 }
 /* max abs error of the fit on (0,1]: 5.06e-12 */
+FJ_TABLE(k_fit_GJs, 11, {8.887397533409992e-05, -0.0010796865524819253, 0.007517728519888622, -0.03868451112092575, 0.1540241037174691, -0.4667509709541983, 1.034337151432889, -1.5802461351138604, 1.5238094742576516, -0.7999999987682918, 0.16666666666160115})
+template <int TAB = FIT_LITERAL>
 FJ_HD inline double fit_GJs(double y) {
+  if (TAB == FIT_TABLE) {
+    const double* c = FJ_TABLE_PTR(k_fit_GJs);
+    double p = c[0];
+    p = __builtin_fma(p, y, c[1]);
+    p = __builtin_fma(p, y, c[2]);
+    p = __builtin_fma(p, y, c[3]);
+    p = __builtin_fma(p, y, c[4]);
+    p = __builtin_fma(p, y, c[5]);
+    p = __builtin_fma(p, y, c[6]);
+    p = __builtin_fma(p, y, c[7]);
+    p = __builtin_fma(p, y, c[8]);
+    p = __builtin_fma(p, y, c[9]);
+    p = __builtin_fma(p, y, c[10]);
+    return p;
+  }
   double p = 8.887397533409992e-05;
   p = __builtin_fma(p, y, -0.0010796865524819253);
   p = __builtin_fma(p, y, 0.007517728519888622);
@@ -191,7 +291,25 @@ FJ_HD inline double fit_GJs(double y) {
   return p;
 }
 /* max abs error of the fit on (0,1]: 1.55e-10 */
+FJ_TABLE(k_fit_GYs, 12, {-0.010493016794276316, 0.12932686955425834, -0.8908749111050289, 4.447592307093985, -16.958832589184595, 48.66961558750545, -100.72670076329833, 140.7051499670384, -119.54298053751586, 51.0414296530347, -6.446088190525072, -0.6366197725231723})
+template <int TAB = FIT_LITERAL>
 FJ_HD inline double fit_GYs(double y) {
+  if (TAB == FIT_TABLE) {
+    const double* c = FJ_TABLE_PTR(k_fit_GYs);
+    double p = c[0];
+    p = __builtin_fma(p, y, c[1]);
+    p = __builtin_fma(p, y, c[2]);
+    p = __builtin_fma(p, y, c[3]);
+    p = __builtin_fma(p, y, c[4]);
+    p = __builtin_fma(p, y, c[5]);
+    p = __builtin_fma(p, y, c[6]);
+    p = __builtin_fma(p, y, c[7]);
+    p = __builtin_fma(p, y, c[8]);
+    p = __builtin_fma(p, y, c[9]);
+    p = __builtin_fma(p, y, c[10]);
+    p = __builtin_fma(p, y, c[11]);
+    return p;
+  }
   double p = -0.010493016794276316;
   p = __builtin_fma(p, y, 0.12932686955425834);
   p = __builtin_fma(p, y, -0.8908749111050289);

@@ -196,6 +196,18 @@ FJ_HD inline int coef_count(int mode, int itype) {
 #ifndef FJ_OCC_I1_J
 #define FJ_OCC_I1_J 4
 #endif
+// How the linear-segment kernels hold the small-argument (x < 8) double
+// polynomials.  Measured on the MI355X (J kernel, 19,900 pairs x 512 f x
+// 3000 c): literals 39.6 ms; a __constant__ table 73 ms, because the
+// compiler hoists its loads into scalar registers that spill to vector
+// lanes and are read back in the station loop; the table re-read inside
+// the small-argument branch (address hidden from hoisting) 58 ms, because
+// the station stream through the scalar cache evicts it between uses.  The
+// trapezoid kernels have the registers to spare and take the table (5-13%
+// faster than literals).  -DFJ_FIT_I1=1 selects the table, for tuning runs.
+#ifndef FJ_FIT_I1
+#define FJ_FIT_I1 FIT_LITERAL
+#endif
 template <int MODE, int ITYPE>
 struct FjOcc {
   static constexpr int waves =
@@ -269,7 +281,7 @@ FJ_HD inline void trapz_station(double rs, float ir, const LaneK& L, const float
   } else {
     const double xd = L.q2pi * rs;
     double y, Lg;
-    const Bess b = bessel_small<false, M::Y>(xd, &y, &Lg);
+    const Bess b = bessel_small<false, M::Y, FIT_TABLE>(xd, &y, &Lg);
     const float ia = L.kf > 0.0f ? sqrtf(kPiOver2 * (float)xd) : sqrtf((float)rs);
     vj = b.j0 * ia;
     vy = b.y0 * ia;
@@ -554,13 +566,13 @@ FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dx
     if (NY) gy = __builtin_fmaf(x, y1, kTwoOverPi);
   } else {
     double y, Lg;
-    const Bess b = bessel_small<true, NY>(xd, &y, &Lg);
+    const Bess b = bessel_small<true, NY, FJ_FIT_I1>(xd, &y, &Lg);
     j0 = b.j0;
     j1 = b.j1;
     y0 = b.y0;
     y1 = b.y1;
     gy = b.gy;
-    if (needG) g_small(xd, y, Lg, NJ, NY, &cj.G, &cy.G);
+    if (needG) g_small<FJ_FIT_I1>(xd, y, Lg, NJ, NY, &cj.G, &cy.G);
     cy.sh = 1.0f;
   }
   linseg_tail<MODE, FAR>(s, x, dxL, useG_L, j0, j1, y0, y1, gy, cj, cy, c, st);

@@ -159,22 +159,24 @@ FJ_HD inline void bessel0_large(float x, float u, float* j0, float* y0) {
 }
 
 // x < 8 in double.  lnx2pi = (2/pi) ln x (float accuracy suffices: it only
-// multiplies J, whose size is at most 1).
-template <bool NEED_J1, bool NEED_Y>
+// multiplies J, whose size is at most 1).  TAB picks how the polynomial
+// coefficients are held (FIT_LITERAL / FIT_TABLE); the best choice depends
+// on the calling kernel's register pressure.
+template <bool NEED_J1, bool NEED_Y, int TAB = FIT_LITERAL>
 FJ_HD inline Bess bessel_small(double x, double* y_out, double* lnterm_out) {
   double y = (x * 0.125) * (x * 0.125);
   Bess b;
-  double j0 = fit_J0s(y);
-  double j1 = NEED_J1 ? x * fit_J1s(y) : 0.0;
+  double j0 = fit_J0s<TAB>(y);
+  double j1 = NEED_J1 ? x * fit_J1s<TAB>(y) : 0.0;
   b.j0 = (float)j0;
   b.j1 = (float)j1;
   b.y0 = b.y1 = b.gy = 0.0f;
   double L = 0.0;
   if (NEED_Y) {
     L = (double)(kTwoOverPi * ln_f((float)x));
-    b.y0 = (float)(L * j0 + fit_Y0s(y));
+    b.y0 = (float)(L * j0 + fit_Y0s<TAB>(y));
     if (NEED_J1) {
-      const double xf1 = x * fit_Y1s(y);
+      const double xf1 = x * fit_Y1s<TAB>(y);
       const double inv = (double)(kTwoOverPi * rcp_f((float)x));
       b.y1 = (float)(L * j1 - inv + xf1);
       b.gy = (float)(x * (L * j1 + xf1));  // x Y1 + 2/pi
@@ -198,13 +200,14 @@ FJ_HD inline void g_large(float x, float t, const Bess& b, float* gj, float* gy,
 // G(x), small x (x < 8), from the double polynomials.  For the Y basis it
 // returns the shifted integral G_Y(x) + (2/pi) x = int_0^x (t Y1 + 2/pi) dt,
 // which vanishes like x^3 ln x instead of approaching -(2/pi) x.
+template <int TAB = FIT_LITERAL>
 FJ_HD inline void g_small(double x, double y, double L, bool need_j, bool need_y,
                           float* gj, float* gy_shifted) {
-  double g = x * x * x * fit_GJs(y);
+  double g = x * x * x * fit_GJs<TAB>(y);
   if (need_j) *gj = (float)g;
   // x ln x -> 0 at x = 0; avoid ln(0) * 0
   if (need_y)
-    *gy_shifted = x > 0.0 ? (float)(L * g + x * (fit_GYs(y) + kTwoOverPiD)) : 0.0f;
+    *gy_shifted = x > 0.0 ? (float)(L * g + x * (fit_GYs<TAB>(y) + kTwoOverPiD)) : 0.0f;
 }
 
 // Reduced phase of x - pi/4 in revolutions: q = f/c (cycles per metre),

@@ -56,7 +56,7 @@ __global__ void k_synth_fields(const float* __restrict__ r, const float* __restr
       bessel0_large<true, false>(x, reduce_phase(q, rs), &J0, &Y0);
     } else {
       double y, L;
-      J0 = bessel_small<false, false>(xd, &y, &L).j0;
+      J0 = bessel_small<false, false, FIT_TABLE>(xd, &y, &L).j0;
     }
     acc += p[3] * J0;
   }

@@ -99,7 +99,30 @@ def fit_all(names, var, truth_fn, npts=160, top=1.0):
 def emit(fname, coefs, ctype, err, var, top=1.0):
     fma = "__builtin_fmaf" if ctype == "float" else "__builtin_fma"
     suf = "f" if ctype == "float" else ""
-    lines = [f"/* max abs error of the fit on (0,{top:g}]: {err:.2e} */",
+    head = f"/* max abs error of the fit on (0,{top:g}]: {err:.2e} */"
+    if ctype == "double":
+        # two forms of one polynomial, chosen per kernel by TAB: literal
+        # coefficients (FIT_LITERAL) or a table (FIT_TABLE; on the device a
+        # __constant__ array whose loads the compiler hoists into scalar
+        # registers at kernel entry).  Which is faster depends on the
+        # kernel's register pressure.
+        vals = ", ".join(f"{float(c)!r}" for c in coefs[::-1])
+        lines = [head,
+                 f"FJ_TABLE(k_{fname}, {len(coefs)}, {{{vals}}})",
+                 "template <int TAB = FIT_LITERAL>",
+                 f"FJ_HD inline double {fname}(double {var}) {{",
+                 "  if (TAB == FIT_TABLE) {",
+                 f"    const double* c = FJ_TABLE_PTR(k_{fname});",
+                 "    double p = c[0];"]
+        for k in range(1, len(coefs)):
+            lines.append(f"    p = {fma}(p, {var}, c[{k}]);")
+        lines += ["    return p;", "  }", f"  double p = {float(coefs[-1])!r};"]
+        for c in coefs[-2::-1]:
+            lines.append(f"  p = {fma}(p, {var}, {float(c)!r});")
+        lines.append("  return p;")
+        lines.append("}")
+        return "\n".join(lines)
+    lines = [head,
              f"FJ_HD inline {ctype} {fname}({ctype} {var}) {{",
              f"  {ctype} p = {float(coefs[-1])!r}{suf};"]
     for c in coefs[-2::-1]:
@@ -118,6 +141,20 @@ def main():
            "#pragma once",
            "#ifndef FJ_HD",
            "#define FJ_HD",
+           "#endif",
+           "/* double tables: __constant__ on the device, plain on the host */",
+           "enum { FIT_LITERAL = 0, FIT_TABLE = 1 };",
+           "#if defined(__HIPCC__)",
+           "#define FJ_TABLE(name, n, ...) static __constant__ double name##_d[n] = __VA_ARGS__; "
+           "static const double name##_h[n] = __VA_ARGS__;",
+           "#if defined(__HIP_DEVICE_COMPILE__)",
+           "#define FJ_TABLE_PTR(name) name##_d",
+           "#else",
+           "#define FJ_TABLE_PTR(name) name##_h",
+           "#endif",
+           "#else",
+           "#define FJ_TABLE(name, n, ...) static const double name##_h[n] = __VA_ARGS__;",
+           "#define FJ_TABLE_PTR(name) name##_h",
            "#endif",
            "",
            "/* large argument x >= 8, t = (8/x)^2, float */"]
