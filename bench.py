@@ -360,39 +360,70 @@ def bench_batched(args):
 
 
 def bench_mwfj(args):
-    """MWFJ end to end (numpy in, numpy out) and its MFMA contraction."""
-    import torch  # noqa: F401  (brings up the HIP runtime first)
+    """MWFJ on device-resident traces (ccfj_mwfj_dev: windowing, rFFT, MFMA
+    contraction, normalisation), its MFMA contraction alone, and the host
+    entry point end to end (numpy in, numpy out)."""
+    import torch
     import ccfj
     from ccfj import _ccfj, synth
     d = synth.active_source(seed=2)
     u, r, f, c = d["u"], d["r"], d["f"], d["c"]
     nwin, nr, nf, nc = d["nwin"], len(r), len(f), len(c)
+    npts = u.shape[1]
     ctx = ccfj._context(0)
+    dev = torch.device("cuda:0")
+    indx = np.argsort(r)
+    n1, n2, dl, xidx, xrows = ccfj._window_descriptions(
+        np.asarray(d["winl"])[:nwin], np.asarray(d["winr"])[:nwin], indx, npts, d["Fs"], 0.9)
+    host = (np.ascontiguousarray(u[indx], np.float64), r[indx].astype(np.float32),
+            f.astype(np.float32), c.astype(np.float32), n1, n2, dl, xidx,
+            xrows if len(xrows) else np.zeros((1, npts)))
+    dv = [torch.from_numpy(np.ascontiguousarray(a)).to(dev) for a in host]
+    dout = torch.empty((nwin, nc, nf), dtype=torch.float64, device=dev)
+    st = torch.cuda.Stream(device=dev)
 
-    def run():
+    def run_dev():
+        ctx.mwfj_dev(0, 0, 1, nr, npts, nwin, nf, nc, *[a.data_ptr() for a in dv], len(xrows),
+                     dout.data_ptr(), st.cuda_stream)
+
+    def run_host():
         return ccfj.MWFJ(u, r, c, f, d["Fs"], nwin, d["winl"], d["winr"], 0.9, 1, 0, 0, 0)
 
     print("bench: MWFJ warmup", file=sys.stderr, flush=True)
     for _ in range(max(1, args.warmup)):
-        out0 = run()
+        run_dev()
+        out0 = run_host()
+    st.synchronize()
+    assert np.array_equal(dout.cpu().numpy(), out0)
     ctx.set_timing(True)
     ctx.get_timing()
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        run()
+        run_dev()
+    st.synchronize()
+    torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
     kms, nk = ctx.get_timing()
     ctx.set_timing(False)
     kms /= max(args.steps, 1)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_host()
+    wall_host = (time.perf_counter() - t0) / args.steps
     terms = synth.term_evals(nr, nf, nc, 2 * nwin)   # fj_earthquake func 0: Re and Im calls
     # contraction flops: 2 per (f, c, r, right-hand side), 2 nwin right-hand sides
     mfma_tflops = 2.0 * nf * nc * nr * 2 * nwin / (kms / 1e3) / 1e12
     out = contract(METRIC, terms / wall, "term-evals/s", args, wall * 1e3,
-                   {"workload": "MWFJ end to end (host u in, float64 [nwin, nc, nf] out), "
-                                "itype 0 func 0, windows as right-hand sides of one MFMA "
+                   {"workload": "MWFJ on device-resident traces (float64 [nwin, nc, nf] out in "
+                                "HBM), itype 0 func 0, windows as right-hand sides of one MFMA "
                                 "contraction per frequency",
-                    "receivers": nr, "npts": u.shape[1], "nwin": nwin, "nf": nf, "nc": nc,
+                    "receivers": nr, "npts": npts, "nwin": nwin, "nf": nf, "nc": nc,
                     "term_evals_per_step": terms})
+    out["end_to_end_host"] = {"value": terms / wall_host, "unit": "term-evals/s",
+                              "ms_per_step": round(wall_host * 1e3, 3),
+                              "what": "MWFJ() with numpy in and out: H2D of the traces, the "
+                                      "same device work, D2H of the float64 result"}
     out["contraction"] = {"kernel_ms": round(kms, 3), "launches_per_step": nk / max(args.steps, 1),
                           "term_evals_per_s": terms / (kms / 1e3),
                           "mfma_fp32_tflops": round(mfma_tflops, 2)}
@@ -417,6 +448,31 @@ def bench_mwfj(args):
         out["plain_fp64_gpu"] = {"value": prate, "unit": "term-evals/s",
                                  "sample": "4 of 64 windows, Re and Im calls, kernel time"}
         out["speedup_vs_plain_fp64_gpu"] = round(out["value"] / prate, 2)
+    if not args.no_cpu_baseline:
+        # the FJcpu restatement (itype 0, J basis) on the real parts of the
+        # first 4 windows: 4 of the 2 nwin calls per step (OpenMP over f, as
+        # FJcpu)
+        from oracle import oracle as O
+        cores = O.nthreads_default()
+        idx = np.argsort(r)
+        nw = 4
+        dt, ct = 0.0, 0.0
+        for w in range(nw):
+            tap = np.stack([ccfj.win(u.shape[1], d["Fs"], d["winl"][w, j], d["winr"][w, j], 0.9)
+                            for j in range(nr)])
+            spec = np.fft.rfft(u.astype(np.float64) * tap, axis=1)[idx][:, :nf]
+            t0 = time.perf_counter()
+            O.port_native(np.ascontiguousarray(spec.real, np.float32), r[idx], f, c, 0, 0,
+                          nthreads=cores)
+            dt += time.perf_counter() - t0
+            ct += synth.term_evals(nr, nf, nc)
+        out["cpu_baseline"] = {"value": ct / dt, "unit": "term-evals/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"{nw} FJ calls (real parts of windows 0-{nw - 1}), all "
+                                         f"{nf} frequencies and {nc} velocities ({ct:.3e} "
+                                         f"term-evals in {dt:.1f} s; F-J only, no windowing or "
+                                         f"FFT)",
+                               "cpu": cpu_model()}
     print(json.dumps(out), flush=True)
 
 
@@ -459,7 +515,7 @@ def bench_cc(args):
     example settings: 4 Hz, one-hour windows, 90% overlap, nf 1000,
     fstride 3, whitening), synthetic records with a shared random wavefield
     plus station noise; station 7 starts late and station 11 ends early."""
-    import torch  # noqa: F401
+    import torch
     import ccfj
     rng = np.random.default_rng(5)
     Fs, nsta, nf, fstride = 4, 200, 1000, 3
@@ -476,26 +532,54 @@ def bench_cc(args):
     npairs = len(pairs) // 2
     steplen = int(fftlen * (1 - 0.9))
     nshifts = 1 + (npts - fftlen) // steplen
+    dev = torch.device("cuda:0")
+    ctx = ccfj._context(0)
+    tp = torch.from_numpy(np.ascontiguousarray(pairs, np.int32)).to(dev)
+    ts = torch.from_numpy(np.ascontiguousarray(se, np.int32)).to(dev)
+    tx = torch.from_numpy(data).to(dev)
+    tre = torch.empty((npairs, nf), dtype=torch.float32, device=dev)
+    tim = torch.empty((npairs, nf), dtype=torch.float32, device=dev)
+    st = torch.cuda.Stream(device=dev)
+
+    def run_dev():
+        ctx.cross_correlate_dev(nsta, npts, nf, fstride, fftlen, steplen, False, True, npairs,
+                                tp.data_ptr(), ts.data_ptr(), tx.data_ptr(), tre.data_ptr(),
+                                tim.data_ptr(), st.cuda_stream)
+
     print("bench: CC warmup", file=sys.stderr, flush=True)
     for _ in range(max(1, args.warmup)):
-        ccfj.CC(npts, nsta, nf, fftlen, pairs, se, data, overlaprate=0.9, fstride=fstride)
+        run_dev()
+        out = ccfj.CC(npts, nsta, nf, fftlen, pairs, se, data, overlaprate=0.9, fstride=fstride)
+    st.synchronize()
+    assert np.array_equal(tre.cpu().numpy(), out.real) and np.array_equal(tim.cpu().numpy(), out.imag)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        run_dev()
+    st.synchronize()
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / args.steps
     t0 = time.perf_counter()
     for _ in range(args.steps):
         out = ccfj.CC(npts, nsta, nf, fftlen, pairs, se, data, overlaprate=0.9, fstride=fstride)
-    wall = (time.perf_counter() - t0) / args.steps
+    wall_host = (time.perf_counter() - t0) / args.steps
     # pair-window products actually formed (both stations hold the window)
     from oracle import cc_ref as C
     use = np.stack([C.used(se, k, steplen, fftlen) for k in range(nshifts)])
     pw = int(np.sum(use[:, pairs[0::2]] & use[:, pairs[1::2]]))
     res = contract("cross-correlation pair-window spectra/s (nf bins each)", pw / wall,
                    "pair-windows/s", args, wall * 1e3,
-                   {"workload": "ambient-noise cross-correlation, host records in, complex64 "
-                                "[npairs, nf] out", "stations": nsta, "npts": npts,
-                    "fftlen": fftlen, "windows": nshifts, "pairs": npairs, "nf": nf,
-                    "fstride": fstride, "pair_windows": pw})
+                   {"workload": "ambient-noise cross-correlation, device-resident records, "
+                                "float32 re/im [npairs, nf] out in HBM", "stations": nsta,
+                    "npts": npts, "fftlen": fftlen, "windows": nshifts, "pairs": npairs,
+                    "nf": nf, "fstride": fstride, "pair_windows": pw})
+    res["end_to_end_host"] = {"value": pw / wall_host, "unit": "pair-windows/s",
+                              "ms_per_step": round(wall_host * 1e3, 3),
+                              "what": "CC() with numpy in and out: H2D of the records, the same "
+                                      "device work, D2H of the sums, complex64 assembly"}
     res["output_shape"] = list(out.shape)
     if not args.no_cpu_baseline:
-        nw = 16   # the restatement on every station and pair, first 16 windows
+        nw = 64   # the restatement on every station and pair, first 64 windows
         npts_s = fftlen + (nw - 1) * steplen
         ds = np.ascontiguousarray(data.reshape(nsta, npts)[:, :npts_s]).reshape(-1)
         ses = np.tile(np.array([0, npts_s + 1], np.int32), nsta)

@@ -75,6 +75,10 @@ cdef extern from "ccfj.h" nogil:
                              int steplen, int onebit, int whiten, int npairs, const int *pairs,
                              const int *startend, const float *data, float *ncfsr,
                              float *ncfsi) nogil
+    int ccfj_cross_correlate_dev(ccfj_ctx *ctx, int nsta, int npts, int nf, int fstride,
+                                 int fftlen, int steplen, int onebit, int whiten, int npairs,
+                                 const int *pairs, const int *startend, float *data, float *ncfsr,
+                                 float *ncfsi, void *stream)
     ctypedef struct ccfj_cnn:
         pass
     int ccfj_cnn_param_count()
@@ -92,6 +96,11 @@ cdef extern from "ccfj.h" nogil:
                   int nc, int fstride, const double *u, const float *r, const float *f,
                   const float *c, const int *n1, const int *n2, const int *delta,
                   const int *xidx, const double *xrows, int nx, double *out)
+    int ccfj_mwfj_dev(ccfj_ctx *ctx, int itype, int func, int nr, int npts, int nwin, int nf,
+                      int nc, int fstride, const double *u, const float *r, const float *f,
+                      const float *c, const int *n1, const int *n2, const int *delta,
+                      const int *xidx, const double *xrows, int nx, double *out,
+                      void *stream)
 
 MODE_J = CCFJ_MODE_J
 MODE_Y = CCFJ_MODE_Y
@@ -289,6 +298,20 @@ cdef class Context:
         _check(rc, "ccfj_mwfj")
         return out
 
+    def mwfj_dev(self, int itype, int func, int fstride, int nr, int npts, int nwin, int nf,
+                 int nc, size_t u, size_t r, size_t f, size_t c, size_t n1, size_t n2,
+                 size_t delta, size_t xidx, size_t xrows, int nx, size_t out, size_t stream=0):
+        """Device multi-window F-J on device pointers (layouts as mwfj),
+        enqueued on `stream` (0 = the context's stream)."""
+        cdef int rc
+        with nogil:
+            rc = ccfj_mwfj_dev(self.ctx, itype, func, nr, npts, nwin, nf, nc, fstride,
+                               <const double *>u, <const float *>r, <const float *>f,
+                               <const float *>c, <const int *>n1, <const int *>n2,
+                               <const int *>delta, <const int *>xidx, <const double *>xrows, nx,
+                               <double *>out, <void *>stream)
+        _check(rc, "ccfj_mwfj_dev")
+
     def cross_correlate(self, int nsta, int npts, int nf, int fstride, int fftlen, int steplen,
                         bint onebit, bint whiten, pairs, startend, data):
         """Device cross-correlation (host arrays): pairs int32 [2 npairs],
@@ -313,6 +336,20 @@ cdef class Context:
                                       whiten, npairs, pp, &se[0], &dv[0], rp, ip)
         _check(rc, "ccfj_cross_correlate")
         return re, im
+
+    def cross_correlate_dev(self, int nsta, int npts, int nf, int fstride, int fftlen,
+                            int steplen, bint onebit, bint whiten, int npairs, size_t pairs,
+                            size_t startend, size_t data, size_t ncfsr, size_t ncfsi,
+                            size_t stream=0):
+        """Device cross-correlation on device pointers (layouts as
+        cross_correlate), enqueued on `stream` (0 = the context's stream)."""
+        cdef int rc
+        with nogil:
+            rc = ccfj_cross_correlate_dev(self.ctx, nsta, npts, nf, fstride, fftlen, steplen,
+                                          onebit, whiten, npairs, <const int *>pairs,
+                                          <const int *>startend, <float *>data, <float *>ncfsr,
+                                          <float *>ncfsi, <void *>stream)
+        _check(rc, "ccfj_cross_correlate_dev")
 
     def batch_plan(self, int mode, int itype, bint normalize, soff, int nf, int nc):
         return BatchPlan(self, mode, itype, normalize, soff, nf, nc)

@@ -1,16 +1,15 @@
 // ccfj_cc.hip -- host side of the device cross-correlation (fj_cc.hip).
 //
 // Reference: src/CrossCorr.cpp:33-159 (CrossCorrelation) and its Cython
-// shim src/ccfj.pyx:44-69 (CC_full).  The window count, the "station has
-// this window" rule and the per-pair counts are decided on the host exactly
-// as the reference decides them; the device does the FFTs, the whitening,
-// the pair products and the averaging.
+// shim src/ccfj.pyx:44-69 (CC_full).  The window count is decided on the
+// host as the reference decides it; the device builds the "station has
+// this window" mask and the per-pair window counts by the reference's rule,
+// and does the FFTs, the whitening, the pair products and the averaging.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
 #include <cmath>
-#include <vector>
 
 #include "../../include/ccfj.h"
 #include "ccfj_internal.h"
@@ -22,31 +21,29 @@ using ccfj_internal::hip_fail;
 
 namespace {
 
-struct Bufs {
-  std::vector<void*> p;
-  ~Bufs() {
-    (void)hipDeviceSynchronize();
-    for (void* q : p) (void)hipFree(q);
-  }
-  int get(size_t bytes, void** out) {
-    int rc = ccfj_internal::device_alloc(out, bytes);
-    if (!rc) p.push_back(*out);
-    return rc;
-  }
-};
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-// accumulate: the sums start from the caller's ncfsr/ncfsi (as the
-// reference's do; its wrapper passes zeros), else from zero
-int cross_correlate(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen,
-                    int steplen, int onebit, int whiten, int npairs, const int* pairs,
-                    const int* startend, const float* data, float* ncfsr, float* ncfsi,
-                    bool accumulate) {
-  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+int check_sizes(int nsta, int npts, int nf, int fstride, int fftlen, int steplen, int npairs) {
   if (nsta < 1 || npts < 1 || nf < 1 || fstride < 1 || fftlen < 1 || npairs < 0)
     return fail(CCFJ_EINVAL, "bad sizes (nsta=%d npts=%d nf=%d fstride=%d fftlen=%d npairs=%d)",
                 nsta, npts, nf, fstride, fftlen, npairs);
-  if (!startend || !data || (npairs > 0 && (!pairs || !ncfsr || !ncfsi)))
-    return fail(CCFJ_EINVAL, "null pointer");
+  if (fftlen < npts && steplen < 1) return fail(CCFJ_EINVAL, "step length %d", steplen);
+  const int len = fftlen >= npts ? npts : fftlen;
+  const int nbins = len / 2 + 1;
+  if ((long)(nf - 1) * fstride >= nbins)
+    return fail(CCFJ_EINVAL, "bin %d x stride %d beyond the %d rFFT bins", nf - 1, fstride, nbins);
+  return CCFJ_OK;
+}
+
+// The cross-correlation on device pointers, enqueued on st.  data is
+// replaced by its signs when onebit is set (as the reference does to its
+// caller's array).  re/im hold the caller's running sums when accumulate
+// is set (the reference's +=), and receive the averages.  Work arrays live
+// in the context's grow-only scratch slots 3-6 and 8 (2, 7 and 9 belong to
+// the host-pointer entry point), so a repeated call allocates nothing.
+int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, int steplen,
+           int onebit, int whiten, int npairs, const int* pairs, const int* startend, float* data,
+           float* re, float* im, bool accumulate, hipStream_t st) {
   // window count (src/CrossCorr.cpp:50-56): a window longer than the record
   // is cut to it and used once
   long nshifts;
@@ -54,78 +51,42 @@ int cross_correlate(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int 
     fftlen = npts;
     nshifts = 1;
   } else {
-    if (steplen < 1) return fail(CCFJ_EINVAL, "step length %d", steplen);
     nshifts = 1 + (long)(npts - fftlen) / steplen;
   }
   const int nbins = fftlen / 2 + 1;
-  if ((long)(nf - 1) * fstride >= nbins)
-    return fail(CCFJ_EINVAL, "bin %d x stride %d beyond the %d rFFT bins", nf - 1, fstride, nbins);
-  for (int p = 0; p < 2 * npairs; p++)
-    if (pairs[p] < 0 || pairs[p] >= nsta) return fail(CCFJ_EINVAL, "pair station %d of %d", pairs[p], nsta);
-  if (npairs == 0) return CCFJ_OK;
-
-  // station i has window k when k*steplen >= start_i and
-  // k*steplen + fftlen < end_i (src/CrossCorr.cpp:91-98)
-  std::vector<unsigned char> use((size_t)nshifts * nsta);
-  for (long k = 0; k < nshifts; k++)
-    for (int i = 0; i < nsta; i++) {
-      const long t0 = k * (long)steplen;
-      use[(size_t)k * nsta + i] = (t0 >= startend[2 * i] && t0 + fftlen < startend[2 * i + 1]);
-    }
-  std::vector<float> count(npairs, 0.0f);
-  for (int p = 0; p < npairs; p++) {
-    long n = 0;
-    for (long k = 0; k < nshifts; k++)
-      n += use[(size_t)k * nsta + pairs[2 * p]] & use[(size_t)k * nsta + pairs[2 * p + 1]];
-    count[p] = (float)n;   // the reference divides the float sums by it
-  }
-
-  hipError_t e = hipSetDevice(ccfj_internal::ctx_device(ctx));
-  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-  hipStream_t st = ccfj_internal::ctx_stream(ctx);
   // windows per batch: gathered segments, their spectra and the extracted
-  // bins within 4 GiB
+  // bins within 4 GiB (and within one grid dimension)
   const size_t per_win = (size_t)nsta * ((size_t)fftlen * 4 + (size_t)nbins * 8 + (size_t)nf * 8);
-  const int cw = (int)std::max<size_t>(1, std::min<size_t>(nshifts, ((size_t)4 << 30) / per_win));
+  const int cw = (int)std::max<size_t>(
+      1, std::min<size_t>(std::min<long>(nshifts, 65535), ((size_t)4 << 30) / per_win));
 
-  Bufs bufs;
-  void *ddata, *dseg, *dspec, *dS, *dacc, *duse, *dpairs, *dcount, *dre, *dim;
-  int rc = ccfj_internal::ctx_scratch(ctx, 2, sizeof(float) * (size_t)nsta * npts, &ddata);
-  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 3, sizeof(float) * (size_t)cw * nsta * fftlen, &dseg);
+  void *dseg, *dspec, *dS, *dacc, *dmask;
+  const size_t use_bytes = align256((size_t)nshifts * nsta);
+  int rc = ccfj_internal::ctx_scratch(ctx, 3, sizeof(float) * (size_t)cw * nsta * fftlen, &dseg);
   if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 4, 8 * (size_t)cw * nsta * nbins, &dspec);
   if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 5, 8 * (size_t)cw * nsta * nf, &dS);
   if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 6, 8 * (size_t)npairs * nf, &dacc);
-  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 7, 8 * (size_t)npairs * nf, &dre);
-  if (!rc) rc = bufs.get(use.size(), &duse);
-  if (!rc) rc = bufs.get(sizeof(int) * 2 * (size_t)npairs, &dpairs);
-  if (!rc) rc = bufs.get(sizeof(float) * npairs, &dcount);
+  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 8, use_bytes + sizeof(float) * npairs, &dmask);
   if (rc) return rc;
-  dim = (float*)dre + (size_t)npairs * nf;
-  e = hipMemcpyAsync(ddata, data, sizeof(float) * (size_t)nsta * npts, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(duse, use.data(), use.size(), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(dpairs, pairs, sizeof(int) * 2 * (size_t)npairs, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(dcount, count.data(), sizeof(float) * npairs, hipMemcpyHostToDevice, st);
-  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
-  if (onebit && (e = fjk::launch_cc_onebit((float*)ddata, (size_t)nsta * npts, st)) != hipSuccess)
+  unsigned char* duse = (unsigned char*)dmask;
+  float* dcount = (float*)((char*)dmask + use_bytes);
+  hipError_t e;
+  if ((e = fjk::launch_cc_use(startend, nsta, (int)nshifts, steplen, fftlen, duse, st)) != hipSuccess)
+    return hip_fail(e, "use kernel");
+  if ((e = fjk::launch_cc_count(duse, pairs, npairs, nsta, (int)nshifts, dcount, st)) != hipSuccess)
+    return hip_fail(e, "count kernel");
+  if (onebit && (e = fjk::launch_cc_onebit(data, (size_t)nsta * npts, st)) != hipSuccess)
     return hip_fail(e, "one-bit kernel");
   if (accumulate) {
-    // interleave the caller's running sums into acc (complex pairs)
-    e = hipMemcpyAsync(dre, ncfsr, sizeof(float) * (size_t)npairs * nf, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(dim, ncfsi, sizeof(float) * (size_t)npairs * nf, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-      e = fjk::launch_cc_interleave((const float*)dre, (const float*)dim, (size_t)npairs * nf, dacc,
-                                    st);
-    if (e != hipSuccess) return hip_fail(e, "initial sums");
+    if ((e = fjk::launch_cc_interleave(re, im, (size_t)npairs * nf, dacc, st)) != hipSuccess)
+      return hip_fail(e, "initial sums");
   } else if ((e = fjk::launch_fill((float*)dacc, 2 * (size_t)npairs * nf, 0.0f, st)) != hipSuccess) {
     return hip_fail(e, "fill kernel");
   }
   for (long w0 = 0; w0 < nshifts; w0 += cw) {
     const int n = (int)std::min<long>(cw, nshifts - w0);
-    if ((e = fjk::launch_cc_gather((const float*)ddata, nsta, npts, fftlen, steplen, (int)w0, n,
-                                   (float*)dseg, st)) != hipSuccess)
+    if ((e = fjk::launch_cc_gather(data, nsta, npts, fftlen, steplen, (int)w0, n, (float*)dseg,
+                                   st)) != hipSuccess)
       return hip_fail(e, "gather kernel");
     hipfftHandle plan = 0;
     if ((rc = ccfj_internal::cached_fft_plan(fftlen, n * nsta, HIPFFT_R2C, st, &plan))) return rc;
@@ -133,16 +94,62 @@ int cross_correlate(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int 
       return fail(CCFJ_EDEVICE, "hipfftExecR2C failed");
     if ((e = fjk::launch_cc_spec(dspec, n * nsta, nbins, nf, fstride, whiten, dS, st)) != hipSuccess)
       return hip_fail(e, "spectrum kernel");
-    if ((e = fjk::launch_cc_pairs(dS, (const unsigned char*)duse, (const int*)dpairs, npairs, nsta,
-                                  nf, (int)w0, n, dacc, st)) != hipSuccess)
+    if ((e = fjk::launch_cc_pairs(dS, duse, pairs, npairs, nsta, nf, (int)w0, n, dacc, st)) !=
+        hipSuccess)
       return hip_fail(e, "pair kernel");
   }
-  if ((e = fjk::launch_cc_finish(dacc, (const float*)dcount, npairs, nf, (float*)dre, (float*)dim,
-                                 st)) != hipSuccess)
+  if ((e = fjk::launch_cc_finish(dacc, dcount, npairs, nf, re, im, st)) != hipSuccess)
     return hip_fail(e, "finish kernel");
-  e = hipMemcpyAsync(ncfsr, dre, sizeof(float) * (size_t)npairs * nf, hipMemcpyDeviceToHost, st);
+  return CCFJ_OK;
+}
+
+int cross_correlate(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen,
+                    int steplen, int onebit, int whiten, int npairs, const int* pairs,
+                    const int* startend, const float* data, float* ncfsr, float* ncfsi,
+                    bool accumulate) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  int rc = check_sizes(nsta, npts, nf, fstride, fftlen, steplen, npairs);
+  if (rc) return rc;
+  if (!startend || !data || (npairs > 0 && (!pairs || !ncfsr || !ncfsi)))
+    return fail(CCFJ_EINVAL, "null pointer");
+  for (int p = 0; p < 2 * npairs; p++)
+    if (pairs[p] < 0 || pairs[p] >= nsta) return fail(CCFJ_EINVAL, "pair station %d of %d", pairs[p], nsta);
+  if (npairs == 0) return CCFJ_OK;
+
+  hipError_t e = hipSetDevice(ccfj_internal::ctx_device(ctx));
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  hipStream_t st = ccfj_internal::ctx_stream(ctx);
+  // records in slot 2, running sums / results in slot 7, pairs and
+  // start/end samples in slot 9
+  void *ddata, *dres, *dlist;
+  const size_t nout = (size_t)npairs * nf;
+  const size_t pbytes = align256(sizeof(int) * 2 * (size_t)npairs);
+  rc = ccfj_internal::ctx_scratch(ctx, 2, sizeof(float) * (size_t)nsta * npts, &ddata);
+  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 7, 2 * align256(sizeof(float) * nout), &dres);
+  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 9, pbytes + sizeof(int) * 2 * (size_t)nsta, &dlist);
+  if (rc) return rc;
+  float* dre = (float*)dres;
+  float* dim = (float*)((char*)dres + align256(sizeof(float) * nout));
+  int* dpairs = (int*)dlist;
+  int* dse = (int*)((char*)dlist + pbytes);
+  e = hipMemcpyAsync(ddata, data, sizeof(float) * (size_t)nsta * npts, hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(ncfsi, dim, sizeof(float) * (size_t)npairs * nf, hipMemcpyDeviceToHost, st);
+    e = hipMemcpyAsync(dpairs, pairs, sizeof(int) * 2 * (size_t)npairs, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(dse, startend, sizeof(int) * 2 * (size_t)nsta, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && accumulate) {
+    e = hipMemcpyAsync(dre, ncfsr, sizeof(float) * nout, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dim, ncfsi, sizeof(float) * nout, hipMemcpyHostToDevice, st);
+  }
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
+  rc = cc_run(ctx, nsta, npts, nf, fstride, fftlen, steplen, onebit, whiten, npairs, dpairs, dse,
+              (float*)ddata, dre, dim, accumulate, st);
+  if (rc) {
+    (void)hipStreamSynchronize(st);
+    return rc;
+  }
+  e = hipMemcpyAsync(ncfsr, dre, sizeof(float) * nout, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(ncfsi, dim, sizeof(float) * nout, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   return e == hipSuccess ? CCFJ_OK : hip_fail(e, "hipMemcpyAsync D2H");
 }
@@ -156,6 +163,23 @@ int ccfj_cross_correlate(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride,
                          const int* startend, const float* data, float* ncfsr, float* ncfsi) {
   return cross_correlate(ctx, nsta, npts, nf, fstride, fftlen, steplen, onebit, whiten, npairs,
                          pairs, startend, data, ncfsr, ncfsi, false);
+}
+
+int ccfj_cross_correlate_dev(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen,
+                             int steplen, int onebit, int whiten, int npairs, const int* pairs,
+                             const int* startend, float* data, float* ncfsr, float* ncfsi,
+                             void* stream) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  int rc = check_sizes(nsta, npts, nf, fstride, fftlen, steplen, npairs);
+  if (rc) return rc;
+  if (!startend || !data || (npairs > 0 && (!pairs || !ncfsr || !ncfsi)))
+    return fail(CCFJ_EINVAL, "null pointer");
+  if (npairs == 0) return CCFJ_OK;
+  hipError_t e = hipSetDevice(ccfj_internal::ctx_device(ctx));
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  hipStream_t st = stream ? (hipStream_t)stream : ccfj_internal::ctx_stream(ctx);
+  return cc_run(ctx, nsta, npts, nf, fstride, fftlen, steplen, onebit, whiten, npairs, pairs,
+                startend, data, ncfsr, ncfsi, false, st);
 }
 
 int CrossCorrelation(CC_data d) {

@@ -17,7 +17,7 @@ int device_alloc(void** p, size_t bytes);
 int ctx_device(const ccfj_ctx* ctx);
 // Grow-only scratch buffer `which` (0 .. kScratchSlots-1) of the context,
 // at least bytes; 0 and 1 belong to the Hilbert transform.
-constexpr int kScratchSlots = 8;
+constexpr int kScratchSlots = 10;
 int ctx_scratch(ccfj_ctx* ctx, int which, size_t bytes, void** p);
 hipStream_t ctx_stream(const ccfj_ctx* ctx);
 // The process-wide context of the current device (the reference-compatible

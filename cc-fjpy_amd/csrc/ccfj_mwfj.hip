@@ -3,7 +3,8 @@
 #include <hipfft/hipfft.h>
 
 #include <algorithm>
-#include <vector>
+#include <cstdint>
+#include <utility>
 
 #include "../../include/ccfj.h"
 #include "ccfj_internal.h"
@@ -15,42 +16,29 @@ using ccfj_internal::hip_fail;
 
 namespace {
 
-struct Bufs {
-  std::vector<void*> p;
-  ~Bufs() {
-    // error paths may leave work in flight that reads these buffers
-    (void)hipDeviceSynchronize();
-    for (void* q : p) (void)hipFree(q);
-  }
-  int get(size_t bytes, void** out) {
-    int rc = ccfj_internal::device_alloc(out, bytes);
-    if (!rc) p.push_back(*out);
-    return rc;
-  }
-};
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
-}  // namespace
-
-extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin,
-                         int nf, int nc, int fstride, const double* u, const float* r,
-                         const float* f, const float* c, const int* n1, const int* n2,
-                         const int* delta, const int* xidx, const double* xrows, int nx,
-                         double* out) {
-  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+int check_sizes(int itype, int func, int nr, int npts, int nwin, int nf, int nc, int fstride,
+                int nx) {
   if (itype != 0 && itype != 1) return fail(CCFJ_EINVAL, "itype must be 0 or 1, got %d", itype);
   if (func != 0 && func != 1) return fail(CCFJ_EINVAL, "func must be 0 or 1, got %d", func);
-  if (nr < 1 || npts < 1 || nwin < 1 || nf < 1 || nc < 1 || fstride < 1)
-    return fail(CCFJ_EINVAL, "bad sizes (nr=%d npts=%d nwin=%d nf=%d nc=%d fstride=%d)", nr, npts,
-                nwin, nf, nc, fstride);
-  if (!u || !r || !f || !c || !n1 || !n2 || !delta || !xidx || !out || (nx > 0 && !xrows))
-    return fail(CCFJ_EINVAL, "null pointer");
+  if (nr < 1 || npts < 1 || nwin < 1 || nf < 1 || nc < 1 || fstride < 1 || nx < 0)
+    return fail(CCFJ_EINVAL, "bad sizes (nr=%d npts=%d nwin=%d nf=%d nc=%d fstride=%d nx=%d)", nr,
+                npts, nwin, nf, nc, fstride, nx);
   const int nbins = npts / 2 + 1;
   if ((int64_t)(nf - 1) * fstride >= nbins)
     return fail(CCFJ_EINVAL, "frequency %d x stride %d beyond the %d rFFT bins", nf, fstride, nbins);
-  hipError_t e = hipSetDevice(ccfj_internal::ctx_device(ctx));
-  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
-  hipStream_t st = ccfj_internal::ctx_stream(ctx);
+  return CCFJ_OK;
+}
 
+// The whole multi-window transform on device pointers, enqueued on st.
+// Work arrays live in the context's grow-only scratch slots 2-7 (slots 0-1
+// belong to the Hilbert transform, 8-9 to the host-pointer entry point), so
+// a call allocates nothing once the slots have grown.
+int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int nf, int nc,
+             int fstride, const double* u, const float* r, const float* f, const float* c,
+             const int* n1, const int* n2, const int* delta, const int* xidx,
+             const double* xrows, int nx, double* out, hipStream_t st) {
   fjk::MwfjDev d{};
   d.nr = nr;
   d.npts = npts;
@@ -58,65 +46,41 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
   d.nc = nc;
   d.nwin = nwin;
   d.fstride = fstride;
-  d.nbins = nbins;
+  d.nbins = npts / 2 + 1;
   d.func = func;
   d.half = (nr + 31) / 32 * 32;
   d.nrhs = (2 * nwin + 31) / 32 * 32;
-  const int ndesc = nwin * nr;
   // windows per rFFT batch: all of them when the tapered traces and their
   // spectra fit in 8 GiB (64 windows of 480 x 16384 take 8 GB)
-  const size_t per_win = (size_t)nr * npts * 8 + (size_t)nr * nbins * 16;
+  const size_t per_win = (size_t)nr * npts * 8 + (size_t)nr * d.nbins * 16;
   const int cw = (int)std::max<size_t>(1, std::min<size_t>(nwin, ((size_t)8 << 30) / per_win));
 
-  Bufs bufs;
-  void *du, *dr, *drr, *df, *dc, *dn1, *dn2, *ddl, *dxi, *dxr = nullptr, *dxw, *dspec, *dout;
-  int rc = CCFJ_OK;
-  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 7, sizeof(double) * (size_t)nr * npts, &du);
-  if (!rc) rc = bufs.get(sizeof(float) * nr, &dr);
-  if (!rc) rc = bufs.get(sizeof(double) * nr, &drr);
-  if (!rc) rc = bufs.get(sizeof(float) * nf, &df);
-  if (!rc) rc = bufs.get(sizeof(float) * nc, &dc);
-  if (!rc) rc = bufs.get(sizeof(int) * ndesc, &dn1);
-  if (!rc) rc = bufs.get(sizeof(int) * ndesc, &dn2);
-  if (!rc) rc = bufs.get(sizeof(int) * ndesc, &ddl);
-  if (!rc) rc = bufs.get(sizeof(int) * ndesc, &dxi);
-  if (!rc && nx > 0) rc = bufs.get(sizeof(double) * (size_t)nx * npts, &dxr);
-  // the large work arrays live in the context's grow-only scratch slots
-  // 2-7 (kept between calls; slots 0-1 belong to the Hilbert transform)
-  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 2, sizeof(double) * (size_t)cw * nr * npts, &dxw);
-  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 3, 16 * (size_t)cw * nr * nbins, &dspec);
-  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 4, sizeof(double) * (size_t)nwin * nc * nf, &dout);
+  void *dxw, *dspec, *drr;
+  int rc = ccfj_internal::ctx_scratch(ctx, 2, sizeof(double) * (size_t)cw * nr * npts, &dxw);
+  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 3, 16 * (size_t)cw * nr * d.nbins, &dspec);
+  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 7, sizeof(double) * nr, &drr);
   if (rc) return rc;
-  std::vector<double> hrr(r, r + nr);
-  e = hipMemcpyAsync(du, u, sizeof(double) * (size_t)nr * npts, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(dr, r, sizeof(float) * nr, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(drr, hrr.data(), sizeof(double) * nr, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(df, f, sizeof(float) * nf, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(dc, c, sizeof(float) * nc, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(dn1, n1, sizeof(int) * ndesc, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(dn2, n2, sizeof(int) * ndesc, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(ddl, delta, sizeof(int) * ndesc, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(dxi, xidx, sizeof(int) * ndesc, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess && nx > 0)
-    e = hipMemcpyAsync(dxr, xrows, sizeof(double) * (size_t)nx * npts, hipMemcpyHostToDevice, st);
-  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
-  d.u = (const double*)du;
-  d.r = (const float*)dr;
+  hipError_t e = fjk::launch_to_double(r, nr, (double*)drr, st);
+  if (e != hipSuccess) return hip_fail(e, "convert kernel");
+  d.u = u;
+  d.r = r;
   d.rr = (const double*)drr;
-  d.f = (const float*)df;
-  d.c = (const float*)dc;
-  d.n1 = (const int*)dn1;
-  d.n2 = (const int*)dn2;
-  d.dl = (const int*)ddl;
-  d.xidx = (const int*)dxi;
-  d.xrows = (const double*)dxr;
+  d.f = f;
+  d.c = c;
+  d.n1 = n1;
+  d.n2 = n2;
+  d.dl = delta;
+  d.xidx = xidx;
+  d.xrows = xrows;
+  d.nx = xrows ? nx : 0;
   d.xw = (double*)dxw;
   d.spec = dspec;
-  d.out = (double*)dout;
+  d.out = out;
 
-  void *dB = nullptr, *dT = nullptr, *dre = nullptr, *dim = nullptr, *dtmp = nullptr;
+  float *dre = nullptr, *dim = nullptr, *dtmp = nullptr;
   if (itype == 0) {
     const size_t Kp = (size_t)(func ? 2 : 1) * d.half;
+    void *dB, *dT;
     if ((rc = ccfj_internal::ctx_scratch(ctx, 5, sizeof(float) * nf * Kp * d.nrhs, &dB))) return rc;
     if ((rc = ccfj_internal::ctx_scratch(ctx, 6, sizeof(float) * (size_t)nwin * nf * nc, &dT)))
       return rc;
@@ -125,9 +89,14 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
     d.B = (float*)dB;
     d.T = (float*)dT;
   } else {
-    if ((rc = bufs.get(sizeof(float) * (size_t)nr * nf, &dre))) return rc;
-    if ((rc = bufs.get(sizeof(float) * (size_t)nr * nf, &dim))) return rc;
-    if ((rc = bufs.get(sizeof(float) * (size_t)nc * nf, &dtmp))) return rc;
+    // per-window staging: Re and Im spectra [nr][nf], one [nc][nf] result
+    const size_t a = align256(sizeof(float) * (size_t)nr * nf);
+    void* dst;
+    if ((rc = ccfj_internal::ctx_scratch(ctx, 4, 2 * a + sizeof(float) * (size_t)nc * nf, &dst)))
+      return rc;
+    dre = (float*)dst;
+    dim = (float*)((char*)dst + a);
+    dtmp = (float*)((char*)dst + 2 * a);
   }
 
   for (int w0 = 0; w0 < nwin; w0 += cw) {
@@ -141,14 +110,12 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
       if ((e = fjk::launch_mwfj_coef(d, w0, cwn, st)) != hipSuccess) return hip_fail(e, "coef kernel");
     } else {
       for (int wl = 0; wl < cwn; wl++) {
-        if ((e = fjk::launch_mwfj_stage(d, wl, (float*)dre, (float*)dim, st)) != hipSuccess)
+        if ((e = fjk::launch_mwfj_stage(d, wl, dre, dim, st)) != hipSuccess)
           return hip_fail(e, "stage kernel");
-        rc = ccfj_transform_dev(ctx, func ? CCFJ_MODE_ABS_H : CCFJ_MODE_ABS_J, 1, 1,
-                                (const float*)dre, (const float*)dim, d.r, d.f, d.c, nr, nf, nc,
-                                (float*)dtmp, st);
+        rc = ccfj_transform_dev(ctx, func ? CCFJ_MODE_ABS_H : CCFJ_MODE_ABS_J, 1, 1, dre, dim, d.r,
+                                d.f, d.c, nr, nf, nc, dtmp, st);
         if (rc) return rc;
-        e = fjk::launch_to_double((const float*)dtmp, (size_t)nc * nf,
-                                  d.out + (size_t)(w0 + wl) * nc * nf, st);
+        e = fjk::launch_to_double(dtmp, (size_t)nc * nf, d.out + (size_t)(w0 + wl) * nc * nf, st);
         if (e != hipSuccess) return hip_fail(e, "convert kernel");
       }
     }
@@ -159,6 +126,87 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
     if ((e = fjk::launch_mwfj_mfma(d, st)) != hipSuccess) return hip_fail(e, "MFMA kernel");
     ccfj_internal::timing_end(ctx, st, ev);
     if ((e = fjk::launch_mwfj_finish(d, st)) != hipSuccess) return hip_fail(e, "finish kernel");
+  }
+  return CCFJ_OK;
+}
+
+}  // namespace
+
+extern "C" int ccfj_mwfj_dev(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin,
+                             int nf, int nc, int fstride, const double* u, const float* r,
+                             const float* f, const float* c, const int* n1, const int* n2,
+                             const int* delta, const int* xidx, const double* xrows, int nx,
+                             double* out, void* stream) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  int rc = check_sizes(itype, func, nr, npts, nwin, nf, nc, fstride, nx);
+  if (rc) return rc;
+  if (!u || !r || !f || !c || !n1 || !n2 || !delta || !xidx || !out || (nx > 0 && !xrows))
+    return fail(CCFJ_EINVAL, "null pointer");
+  hipError_t e = hipSetDevice(ccfj_internal::ctx_device(ctx));
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  hipStream_t st = stream ? (hipStream_t)stream : ccfj_internal::ctx_stream(ctx);
+  return mwfj_run(ctx, itype, func, nr, npts, nwin, nf, nc, fstride, u, r, f, c, n1, n2, delta,
+                  xidx, xrows, nx, out, st);
+}
+
+extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin,
+                         int nf, int nc, int fstride, const double* u, const float* r,
+                         const float* f, const float* c, const int* n1, const int* n2,
+                         const int* delta, const int* xidx, const double* xrows, int nx,
+                         double* out) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  int rc = check_sizes(itype, func, nr, npts, nwin, nf, nc, fstride, nx);
+  if (rc) return rc;
+  if (!u || !r || !f || !c || !n1 || !n2 || !delta || !xidx || !out || (nx > 0 && !xrows))
+    return fail(CCFJ_EINVAL, "null pointer");
+  const int ndesc = nwin * nr;
+  for (int k = 0; k < ndesc; k++)
+    if (xidx[k] < -1 || xidx[k] >= nx)
+      return fail(CCFJ_EINVAL, "xidx[%d] = %d outside [-1, %d)", k, xidx[k], nx);
+  hipError_t e = hipSetDevice(ccfj_internal::ctx_device(ctx));
+  if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+  hipStream_t st = ccfj_internal::ctx_stream(ctx);
+
+  // inputs staged in scratch slot 8 (one region, 256-byte aligned parts),
+  // the result in slot 9
+  const size_t o_u = 0;
+  const size_t o_r = o_u + align256(sizeof(double) * (size_t)nr * npts);
+  const size_t o_f = o_r + align256(sizeof(float) * nr);
+  const size_t o_c = o_f + align256(sizeof(float) * nf);
+  const size_t o_n1 = o_c + align256(sizeof(float) * nc);
+  const size_t o_n2 = o_n1 + align256(sizeof(int) * ndesc);
+  const size_t o_dl = o_n2 + align256(sizeof(int) * ndesc);
+  const size_t o_xi = o_dl + align256(sizeof(int) * ndesc);
+  const size_t o_xr = o_xi + align256(sizeof(int) * ndesc);
+  const size_t total = o_xr + sizeof(double) * (size_t)nx * npts;
+  void *din, *dout;
+  if ((rc = ccfj_internal::ctx_scratch(ctx, 8, total, &din))) return rc;
+  if ((rc = ccfj_internal::ctx_scratch(ctx, 9, sizeof(double) * (size_t)nwin * nc * nf, &dout)))
+    return rc;
+  char* b = (char*)din;
+  struct Part { size_t off; const void* src; size_t bytes; };
+  const Part parts[] = {{o_u, u, sizeof(double) * (size_t)nr * npts},
+                        {o_r, r, sizeof(float) * nr},
+                        {o_f, f, sizeof(float) * nf},
+                        {o_c, c, sizeof(float) * nc},
+                        {o_n1, n1, sizeof(int) * ndesc},
+                        {o_n2, n2, sizeof(int) * ndesc},
+                        {o_dl, delta, sizeof(int) * ndesc},
+                        {o_xi, xidx, sizeof(int) * ndesc},
+                        {o_xr, xrows, sizeof(double) * (size_t)nx * npts}};
+  for (const Part& q : parts) {
+    if (!q.bytes) continue;
+    e = hipMemcpyAsync(b + q.off, q.src, q.bytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
+  }
+  rc = mwfj_run(ctx, itype, func, nr, npts, nwin, nf, nc, fstride, (const double*)(b + o_u),
+                (const float*)(b + o_r), (const float*)(b + o_f), (const float*)(b + o_c),
+                (const int*)(b + o_n1), (const int*)(b + o_n2), (const int*)(b + o_dl),
+                (const int*)(b + o_xi), nx > 0 ? (const double*)(b + o_xr) : nullptr, nx,
+                (double*)dout, st);
+  if (rc) {
+    (void)hipStreamSynchronize(st);
+    return rc;
   }
   e = hipMemcpyAsync(out, dout, sizeof(double) * (size_t)nwin * nc * nf, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);

@@ -10,6 +10,10 @@ hipError_t launch_cc_gather(const float* data, int nsta, int npts, int fftlen, i
                             int cw, float* seg, hipStream_t st);
 hipError_t launch_cc_spec(const void* spec, int rows, int nbins, int nf, int fstride, int whiten,
                           void* S, hipStream_t st);
+hipError_t launch_cc_use(const int* startend, int nsta, int nshifts, int steplen, int fftlen,
+                         unsigned char* use, hipStream_t st);
+hipError_t launch_cc_count(const unsigned char* use, const int* pairs, int npairs, int nsta,
+                           int nshifts, float* count, hipStream_t st);
 hipError_t launch_cc_pairs(const void* S, const unsigned char* use, const int* pairs, int npairs,
                            int nsta, int nf, int w0, int cw, void* acc, hipStream_t st);
 hipError_t launch_cc_interleave(const float* re, const float* im, size_t n, void* acc,

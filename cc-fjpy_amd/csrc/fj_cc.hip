@@ -14,7 +14,8 @@
 //   spec  c64 [cw][nsta][fftlen/2+1]  their rFFT (hipFFT, single precision)
 //   S     c64 [cw][nsta][nf]          bins j*fstride, whitened
 //   acc   c64 [npairs][nf]            running sums over windows
-//   use   u8  [nshifts][nsta]         station has the whole window (host)
+//   use   u8  [nshifts][nsta]         station has the whole window
+//   count f32 [npairs]                windows both stations of a pair hold
 //
 // The pair kernel reads S rows of two stations per pair; a chunk's S
 // (cw * nsta * nf * 8 bytes) stays in L2/MALL while all pairs sweep it, and
@@ -60,8 +61,33 @@ __global__ void k_cc_spec(const float2* __restrict__ spec, int nbins, int nf, in
   S[row * nf + j] = v;
 }
 
+// use[k][i] = station i holds window k: k*steplen >= start_i and
+// k*steplen + fftlen < end_i (src/CrossCorr.cpp:91-98)
+__global__ void k_cc_use(const int* __restrict__ startend, int nsta, size_t n, int steplen,
+                         int fftlen, unsigned char* __restrict__ use) {
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const int i = (int)(e % nsta);
+  const long long t0 = (long long)(e / nsta) * steplen;
+  use[e] = t0 >= startend[2 * i] && t0 + fftlen < startend[2 * i + 1];
+}
+
+// count[p] = windows that both stations of pair p hold (0 for a pair that
+// names a station outside [0, nsta))
+__global__ void k_cc_count(const unsigned char* __restrict__ use, const int* __restrict__ pairs,
+                           int npairs, int nsta, int nshifts, float* __restrict__ count) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int a = pairs[2 * p], b = pairs[2 * p + 1];
+  int n = 0;
+  if (a >= 0 && a < nsta && b >= 0 && b < nsta)
+    for (int k = 0; k < nshifts; k++) n += use[(size_t)k * nsta + a] & use[(size_t)k * nsta + b];
+  count[p] = (float)n;   // the reference divides the float sums by it
+}
+
 // acc[p][j] += sum over wl < cw with use[w0+wl][a] && use[w0+wl][b] of
 // conj(S[wl][a][j]) S[wl][b][j].  Block: one pair (grid x), 256 bins (y).
+// A pair naming a station outside [0, nsta) adds nothing.
 __global__ __launch_bounds__(256) void k_cc_pairs(const float2* __restrict__ S,
                                                   const unsigned char* __restrict__ use,
                                                   const int* __restrict__ pairs, int nsta,
@@ -70,7 +96,7 @@ __global__ __launch_bounds__(256) void k_cc_pairs(const float2* __restrict__ S,
   const int p = blockIdx.x;
   const int j = blockIdx.y * blockDim.x + threadIdx.x;
   const int a = pairs[2 * p], b = pairs[2 * p + 1];
-  if (j >= nf) return;
+  if (j >= nf || a < 0 || a >= nsta || b < 0 || b >= nsta) return;
   float re = 0.0f, im = 0.0f;
   for (int wl = 0; wl < cw; wl++) {
     const unsigned char* u = use + (size_t)(w0 + wl) * nsta;
@@ -124,6 +150,21 @@ hipError_t launch_cc_spec(const void* spec, int rows, int nbins, int nf, int fst
                           void* S, hipStream_t st) {
   hipLaunchKernelGGL(k_cc_spec, dim3(rows, (nf + 255) / 256), dim3(256), 0, st,
                      (const float2*)spec, nbins, nf, fstride, whiten, (float2*)S);
+  return hipGetLastError();
+}
+
+hipError_t launch_cc_use(const int* startend, int nsta, int nshifts, int steplen, int fftlen,
+                         unsigned char* use, hipStream_t st) {
+  const size_t n = (size_t)nshifts * nsta;
+  hipLaunchKernelGGL(k_cc_use, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, startend, nsta,
+                     n, steplen, fftlen, use);
+  return hipGetLastError();
+}
+
+hipError_t launch_cc_count(const unsigned char* use, const int* pairs, int npairs, int nsta,
+                           int nshifts, float* count, hipStream_t st) {
+  hipLaunchKernelGGL(k_cc_count, dim3((npairs + 255) / 256), dim3(256), 0, st, use, pairs, npairs,
+                     nsta, nshifts, count);
   return hipGetLastError();
 }
 

@@ -15,7 +15,8 @@ struct MwfjDev {
   const int* n2;
   const int* dl;
   const int* xidx;     // [nwin][nr] row of xrows for degenerate windows, else -1
-  const double* xrows; // explicit window rows
+  const double* xrows; // [nx][npts] explicit window rows
+  int nx;
   const float* r;      // [nr] sorted, float32 (the reference's cast)
   const double* rr;    // [nr] the same in double
   const float* f;

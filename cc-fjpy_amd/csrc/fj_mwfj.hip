@@ -36,10 +36,12 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // ---------------------------------------------------------------------------
 // tapered traces: xw[(wl*nr + s)*npts + t], windows w0 .. w0+cw-1
+// A description whose xidx is outside [-1, nx) gives a zero row (the
+// device entry point cannot check descriptions on the host).
 __global__ void k_window(const double* __restrict__ u, const int* __restrict__ n1,
                          const int* __restrict__ n2, const int* __restrict__ dl,
                          const int* __restrict__ xidx, const double* __restrict__ xrows,
-                         int w0, int nr, int npts, double* __restrict__ xw) {
+                         int nx, int w0, int nr, int npts, double* __restrict__ xw) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = blockIdx.y;
   const int wl = blockIdx.z;
@@ -47,7 +49,9 @@ __global__ void k_window(const double* __restrict__ u, const int* __restrict__ n
   const int desc = (w0 + wl) * nr + s;
   double v;
   const int xi = xidx[desc];
-  if (xi >= 0) {
+  if (xi >= nx || xi < -1) {
+    v = 0.0;
+  } else if (xi >= 0) {
     v = xrows[(size_t)xi * npts + t];
   } else {
     const int a = n1[desc], b = n2[desc], d = dl[desc];
@@ -226,7 +230,7 @@ __global__ void k_to_double(const float* __restrict__ a, size_t n, double* __res
 hipError_t launch_window(const MwfjDev& d, int w0, int cw, hipStream_t st) {
   dim3 grid((d.npts + 255) / 256, d.nr, cw);
   hipLaunchKernelGGL(k_window, grid, dim3(256), 0, st, d.u, d.n1, d.n2, d.dl, d.xidx, d.xrows,
-                     w0, d.nr, d.npts, d.xw);
+                     d.nx, w0, d.nr, d.npts, d.xw);
   return hipGetLastError();
 }
 

@@ -157,6 +157,16 @@ int ccfj_mwfj(ccfj_ctx *ctx, int itype, int func, int nr, int npts, int nwin, in
               int nc, int fstride, const double *u, const float *r, const float *f,
               const float *c, const int *n1, const int *n2, const int *delta,
               const int *xidx, const double *xrows, int nx, double *out);
+/* The same on device pointers (every array above, out included, in device
+ * memory), enqueued on `stream` (NULL = the context's stream); returns once
+ * enqueued.  Descriptions are not checked on the host: a (window, trace)
+ * whose xidx lies outside [-1, nx) gives a zero trace.  Work arrays live in
+ * the context and are reused, so a repeated call of the same or smaller
+ * size allocates nothing; calls on one context must not overlap. */
+int ccfj_mwfj_dev(ccfj_ctx *ctx, int itype, int func, int nr, int npts, int nwin, int nf,
+                  int nc, int fstride, const double *u, const float *r, const float *f,
+                  const float *c, const int *n1, const int *n2, const int *delta,
+                  const int *xidx, const double *xrows, int nx, double *out, void *stream);
 
 /* ---- 5. noise F-J end to end on the device -------------------------------
  * Row-wise Hilbert transform along frequency with scipy.fftpack's
@@ -185,6 +195,17 @@ int ccfj_fj_noise(ccfj_ctx *ctx, int itype, int func, int fstride, const float *
 int ccfj_cross_correlate(ccfj_ctx *ctx, int nsta, int npts, int nf, int fstride, int fftlen,
                          int steplen, int onebit, int whiten, int npairs, const int *pairs,
                          const int *startend, const float *data, float *ncfsr, float *ncfsi);
+/* The same on device pointers (pairs, startend, data, ncfsr, ncfsi all in
+ * device memory), enqueued on `stream` (NULL = the context's stream);
+ * returns once enqueued.  data is replaced by its signs when onebit is set
+ * (the reference does this to its caller's array).  Pairs are not checked
+ * on the host: a pair naming a station outside [0, nsta) gives zeros.  Work
+ * arrays live in the context and are reused; calls on one context must not
+ * overlap. */
+int ccfj_cross_correlate_dev(ccfj_ctx *ctx, int nsta, int npts, int nf, int fstride, int fftlen,
+                             int steplen, int onebit, int whiten, int npairs, const int *pairs,
+                             const int *startend, float *data, float *ncfsr, float *ncfsi,
+                             void *stream);
 /* The reference's entry point and argument block (include/CrossCorr.h:
  * 14-35), on the current device's default context.  As the reference, the
  * sums start from the values already in ncfsr/ncfsi (its wrapper passes

@@ -91,3 +91,35 @@ def test_reference_entry_point_accumulates(ccfj):
     got = re + 1j * im
     assert np.max(np.abs(got - want)) <= TOL * np.max(np.abs(want))
     assert np.array_equal(data, np.sign(x.reshape(-1)))
+
+
+def test_cc_device_pointers_match_host_entry(ccfj, ctx):
+    """ccfj_cross_correlate_dev on device arrays gives the host entry
+    point's sums bit for bit, on two calls in a row (the second smaller);
+    one-bit replaces the device records by their signs; a pair naming a
+    station outside the array gives zeros."""
+    import torch
+    dev = torch.device("cuda:0")
+    for nsta, npts, fftlen, nf, onebit in ((6, 6000, 1024, 200, 1), (4, 3000, 512, 100, 0)):
+        x, se = _case(5, nsta, npts)
+        pairs = ccfj.GetStationPairs(nsta)
+        steplen = fftlen // 2
+        re, im = ctx.cross_correlate(nsta, npts, nf, 2, fftlen, steplen, bool(onebit), True,
+                                     pairs, se, x.reshape(-1))
+        bad = np.concatenate([pairs, [0, nsta]]).astype(np.int32)   # an extra, invalid pair
+        npairs = len(bad) // 2
+        tp = torch.from_numpy(bad).to(dev)
+        ts = torch.from_numpy(se).to(dev)
+        tx = torch.from_numpy(x.reshape(-1).copy()).to(dev)
+        tre = torch.full((npairs, nf), float("nan"), device=dev)
+        tim = torch.full((npairs, nf), float("nan"), device=dev)
+        st = torch.cuda.Stream(device=dev)
+        ctx.cross_correlate_dev(nsta, npts, nf, 2, fftlen, steplen, bool(onebit), True, npairs,
+                                tp.data_ptr(), ts.data_ptr(), tx.data_ptr(), tre.data_ptr(),
+                                tim.data_ptr(), st.cuda_stream)
+        st.synchronize()
+        gre, gim = tre.cpu().numpy(), tim.cpu().numpy()
+        assert np.array_equal(gre[:-1], re) and np.array_equal(gim[:-1], im)
+        assert np.all(gre[-1] == 0) and np.all(gim[-1] == 0)
+        want = np.sign(x.reshape(-1)) if onebit else x.reshape(-1)
+        assert np.array_equal(tx.cpu().numpy(), want)

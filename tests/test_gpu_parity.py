@@ -208,6 +208,40 @@ def test_mwfj_many_windows_mfma_tiles(ccfj):
         assert mism == 0 and emax <= TOL, (w, emax)
 
 
+@pytest.mark.parametrize("itype", [0, 1])
+def test_mwfj_device_pointers_match_host_entry(ccfj, ctx, itype):
+    """ccfj_mwfj_dev on device arrays gives the host entry point's result
+    bit for bit, on two calls in a row (the second smaller, reusing the
+    context's work arrays); a bad xidx is an error on the host entry."""
+    import torch
+    dev = torch.device("cuda:0")
+    for nr, npts, nwin in ((24, 512, 5), (16, 256, 3)):
+        u, r, c, f, Fs = _quake(nr=nr, npts=npts)
+        v = np.array([2500.0, 3000.0, 3500.0, 4500.0, 5500.0, 6500.0])
+        winl = np.stack([r / v[w + 1] for w in range(nwin)])
+        winr = np.stack([r / v[w] + 8.0 for w in range(nwin)])
+        winl[1, 3], winr[1, 3] = 20.0, 10.0       # reversed: an explicit row
+        indx = np.argsort(r)
+        us = np.ascontiguousarray(u[indx], np.float64)
+        rs = r[indx].astype(np.float32)
+        f32, c32 = f.astype(np.float32), c.astype(np.float32)
+        n1, n2, dl, xidx, xrows = ccfj._window_descriptions(winl, winr, indx, npts, Fs, 0.9)
+        assert len(xrows) >= 1
+        want = ctx.mwfj(itype, 0, 1, us, rs, f32, c32, n1, n2, dl, xidx, xrows)
+        t = [torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+             for a in (us, rs, f32, c32, n1, n2, dl, xidx, xrows)]
+        out = torch.full((nwin, len(c), len(f)), float("nan"), dtype=torch.float64, device=dev)
+        st = torch.cuda.Stream(device=dev)
+        ctx.mwfj_dev(itype, 0, 1, nr, npts, nwin, len(f), len(c), *[a.data_ptr() for a in t],
+                     len(xrows), out.data_ptr(), st.cuda_stream)
+        st.synchronize()
+        assert np.array_equal(out.cpu().numpy(), want)
+    bad = xidx.copy()
+    bad[0, 0] = len(xrows)
+    with pytest.raises(RuntimeError, match="xidx"):
+        ctx.mwfj(itype, 0, 1, us, rs, f32, c32, n1, n2, dl, bad, xrows)
+
+
 # ------------------------------------------------------------- edge cases
 def test_edges_and_errors(ccfj, ctx):
     r = np.array([1000.0, 1000.0, 1000.05, 3000.0, 9000.0], np.float32)
