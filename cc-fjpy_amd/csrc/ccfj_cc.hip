@@ -60,21 +60,45 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
   const int cw = (int)std::max<size_t>(
       1, std::min<size_t>(std::min<long>(nshifts, 65535), ((size_t)4 << 30) / per_win));
 
+  // Gram form for dense pair lists (at least 1/8 of all station pairs),
+  // the per-pair form otherwise
+  const bool gram = nsta <= 4096 && (nf + 3) / 4 <= 65535 &&
+                    (long)npairs * 8 >= (long)nsta * (nsta - 1);
+  const int ntile = (nsta + 31) / 32;
   void *dseg, *dspec, *dS, *dacc, *dmask;
   const size_t use_bytes = align256((size_t)nshifts * nsta);
+  const size_t cnt_bytes = align256(sizeof(float) * npairs);
+  const size_t dup_bytes = gram ? align256((size_t)npairs) : 0;
+  const size_t tile_bytes = gram ? align256((size_t)ntile * ntile) : 0;
+  const size_t pidx_bytes = gram ? align256(sizeof(int) * (size_t)nsta * nsta) : 0;
+  const size_t flag_bytes = gram ? sizeof(int) : 0;
   int rc = ccfj_internal::ctx_scratch(ctx, 3, sizeof(float) * (size_t)cw * nsta * fftlen, &dseg);
   if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 4, 8 * (size_t)cw * nsta * nbins, &dspec);
   if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 5, 8 * (size_t)cw * nsta * nf, &dS);
   if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 6, 8 * (size_t)npairs * nf, &dacc);
-  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 8, use_bytes + sizeof(float) * npairs, &dmask);
+  if (!rc)
+    rc = ccfj_internal::ctx_scratch(
+        ctx, 8, use_bytes + cnt_bytes + dup_bytes + tile_bytes + pidx_bytes + flag_bytes, &dmask);
   if (rc) return rc;
-  unsigned char* duse = (unsigned char*)dmask;
-  float* dcount = (float*)((char*)dmask + use_bytes);
+  char* m = (char*)dmask;
+  unsigned char* duse = (unsigned char*)m;
+  float* dcount = (float*)(m + use_bytes);
+  unsigned char* ddup = (unsigned char*)(m + use_bytes + cnt_bytes);
+  unsigned char* dtiles = (unsigned char*)(m + use_bytes + cnt_bytes + dup_bytes);
+  int* dpidx = (int*)(m + use_bytes + cnt_bytes + dup_bytes + tile_bytes);
+  int* dflag = (int*)(m + use_bytes + cnt_bytes + dup_bytes + tile_bytes + pidx_bytes);
   hipError_t e;
   if ((e = fjk::launch_cc_use(startend, nsta, (int)nshifts, steplen, fftlen, duse, st)) != hipSuccess)
     return hip_fail(e, "use kernel");
   if ((e = fjk::launch_cc_count(duse, pairs, npairs, nsta, (int)nshifts, dcount, st)) != hipSuccess)
     return hip_fail(e, "count kernel");
+  if (gram) {
+    e = fjk::launch_cc_fill_i32(dpidx, (size_t)nsta * nsta, -1, st);
+    if (e == hipSuccess) e = fjk::launch_fill((float*)dtiles, tile_bytes / 4, 0.0f, st);
+    if (e == hipSuccess)
+      e = fjk::launch_cc_pair_map(pairs, npairs, nsta, ntile, dpidx, ddup, dtiles, st);
+    if (e != hipSuccess) return hip_fail(e, "pair map");
+  }
   if (onebit && (e = fjk::launch_cc_onebit(data, (size_t)nsta * npts, st)) != hipSuccess)
     return hip_fail(e, "one-bit kernel");
   if (accumulate) {
@@ -92,11 +116,25 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
     if ((rc = ccfj_internal::cached_fft_plan(fftlen, n * nsta, HIPFFT_R2C, st, &plan))) return rc;
     if (hipfftExecR2C(plan, (hipfftReal*)dseg, (hipfftComplex*)dspec) != HIPFFT_SUCCESS)
       return fail(CCFJ_EDEVICE, "hipfftExecR2C failed");
-    if ((e = fjk::launch_cc_spec(dspec, n * nsta, nbins, nf, fstride, whiten, dS, st)) != hipSuccess)
-      return hip_fail(e, "spectrum kernel");
-    if ((e = fjk::launch_cc_pairs(dS, duse, pairs, npairs, nsta, nf, (int)w0, n, dacc, st)) !=
-        hipSuccess)
-      return hip_fail(e, "pair kernel");
+    if (gram) {
+      e = fjk::launch_cc_fill_i32(dflag, 1, 0, st);
+      if (e == hipSuccess)
+        e = fjk::launch_cc_spec_t(dspec, n * nsta, nsta, nbins, nf, fstride, whiten, duse,
+                                  (int)w0, dS, dflag, st);
+      if (e == hipSuccess)
+        e = fjk::launch_cc_gram(dS, nsta, nf, n, ntile, dtiles, dpidx, dflag, dacc, st);
+      if (e == hipSuccess)
+        e = fjk::launch_cc_pairs_t(dS, duse, pairs, npairs, nsta, nf, (int)w0, n, ddup, dflag,
+                                   dacc, st);
+      if (e != hipSuccess) return hip_fail(e, "Gram pair stage");
+    } else {
+      if ((e = fjk::launch_cc_spec(dspec, n * nsta, nbins, nf, fstride, whiten, dS, st)) !=
+          hipSuccess)
+        return hip_fail(e, "spectrum kernel");
+      if ((e = fjk::launch_cc_pairs(dS, duse, pairs, npairs, nsta, nf, (int)w0, n, dacc, st)) !=
+          hipSuccess)
+        return hip_fail(e, "pair kernel");
+    }
   }
   if ((e = fjk::launch_cc_finish(dacc, dcount, npairs, nf, re, im, st)) != hipSuccess)
     return hip_fail(e, "finish kernel");

@@ -16,6 +16,19 @@ hipError_t launch_cc_count(const unsigned char* use, const int* pairs, int npair
                            int nshifts, float* count, hipStream_t st);
 hipError_t launch_cc_pairs(const void* S, const unsigned char* use, const int* pairs, int npairs,
                            int nsta, int nf, int w0, int cw, void* acc, hipStream_t st);
+hipError_t launch_cc_spec_t(const void* spec, int rows, int nsta, int nbins, int nf, int fstride,
+                            int whiten, const unsigned char* use, int w0, void* S2, int* nonfinite,
+                            hipStream_t st);
+hipError_t launch_cc_pair_map(const int* pairs, int npairs, int nsta, int ntile, int* pidx,
+                              unsigned char* dup, unsigned char* tiles, hipStream_t st);
+hipError_t launch_cc_gram(const void* S2, int nsta, int nf, int cw, int ntile,
+                          const unsigned char* tiles, const int* pidx, const int* nonfinite,
+                          void* acc, hipStream_t st);
+hipError_t launch_cc_pairs_t(const void* S2, const unsigned char* use, const int* pairs,
+                             int npairs, int nsta, int nf, int w0, int cw,
+                             const unsigned char* dup, const int* nonfinite, void* acc,
+                             hipStream_t st);
+hipError_t launch_cc_fill_i32(int* x, size_t n, int v, hipStream_t st);
 hipError_t launch_cc_interleave(const float* re, const float* im, size_t n, void* acc,
                                 hipStream_t st);
 hipError_t launch_cc_finish(const void* acc, const float* count, int npairs, int nf, float* re,

@@ -113,6 +113,166 @@ __global__ __launch_bounds__(256) void k_cc_pairs(const float2* __restrict__ S,
   *o = v;
 }
 
+// ---- Gram form of the pair stage -------------------------------------------
+// For a dense pair list the pair sums are a Gram product per bin: with the
+// spectra of the stations lacking window w set to zero, for every bin j
+//   G_j[a][b] = sum_w conj(S[w][a][j]) S[w][b][j],
+// which v_mfma_f32_32x32x2_f32 evaluates for a 32 x 32 station tile per
+// wave: the real part with K = (re, im) of one window, the imaginary part
+// with (re, -im) against (im, re).  Each (station pair, bin) is read once
+// per tile instead of once per pair.  Exactness against the per-pair form:
+// a pair listed twice, and a chunk holding a non-finite used spectrum value
+// (whose product with a masked zero would give NaN where the reference
+// skips the window), fall back to the per-pair kernel.
+
+// S2[j][wl][s] = spectrum bin j*fstride of window w0+wl, station s,
+// whitened, zero where the station lacks the window; flags a non-finite
+// used value.  32 x 32 (rows x bins) tiles through LDS.
+__global__ __launch_bounds__(256) void k_cc_spec_t(const float2* __restrict__ spec, int rows,
+                                                   int nsta, int nbins, int nf, int fstride,
+                                                   int whiten, const unsigned char* __restrict__ use,
+                                                   int w0, float2* __restrict__ S2,
+                                                   int* __restrict__ nonfinite) {
+  __shared__ float2 t[32][33];
+  const int r0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
+  bool bad = false;
+  for (int k = ty; k < 32; k += 8) {
+    const int row = r0 + k, j = j0 + tx;
+    float2 v = make_float2(0.0f, 0.0f);
+    if (row < rows && j < nf) {
+      const int wl = row / nsta, st = row - wl * nsta;
+      if (use[(size_t)(w0 + wl) * nsta + st]) {
+        v = spec[(size_t)row * nbins + (size_t)j * fstride];
+        if (whiten) {
+          const float a = fmaxf(sqrtf(v.x * v.x + v.y * v.y), 1e-8f);
+          v.x = v.x / a;
+          v.y = v.y / a;
+        }
+        bad |= !(isfinite(v.x) && isfinite(v.y));
+      }
+    }
+    t[k][tx] = v;
+  }
+  if (__builtin_amdgcn_ballot_w64(bad) != 0 && (threadIdx.x & 63) == 0) atomicOr(nonfinite, 1);
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int j = j0 + k, row = r0 + tx;
+    if (j < nf && row < rows) S2[(size_t)j * rows + row] = t[tx][k];
+  }
+}
+
+// pidx[a][b] = p for the pairs of the list (first come; a later duplicate
+// is marked in dup and left to the per-pair kernel); tiles[(a/32)][(b/32)]
+// marks the 32 x 32 tiles holding a listed pair.  pidx starts at -1.
+__global__ void k_cc_pair_map(const int* __restrict__ pairs, int npairs, int nsta, int ntile,
+                              int* __restrict__ pidx, unsigned char* __restrict__ dup,
+                              unsigned char* __restrict__ tiles) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npairs) return;
+  const int a = pairs[2 * p], b = pairs[2 * p + 1];
+  unsigned char d = 0;
+  if (a >= 0 && a < nsta && b >= 0 && b < nsta) {
+    const int old = atomicCAS(&pidx[(size_t)a * nsta + b], -1, p);
+    if (old != -1) {
+      d = 1;
+    } else {
+      tiles[(a >> 5) * ntile + (b >> 5)] = 1;
+    }
+  }
+  dup[p] = d;
+}
+
+typedef float ccx16 __attribute__((ext_vector_type(16)));
+
+// One wave per (32 x 32 station tile, bin); 4 waves (4 bins) per block.
+// acc[p][j] += G_j[a][b] for the listed pairs of the tile.
+// v_mfma_f32_32x32x2_f32 operand layout (lane l): A[i = l%32][k = l/32],
+// B[k = l/32][n = l%32]; result register v holds row 8(v/4) + 4(l/32) + v%4,
+// column l%32.
+__global__ __launch_bounds__(256) void k_cc_gram(const float2* __restrict__ S2, int nsta, int nf,
+                                                 int cw, int ntile,
+                                                 const unsigned char* __restrict__ tiles,
+                                                 const int* __restrict__ pidx,
+                                                 const int* __restrict__ nonfinite,
+                                                 float2* __restrict__ acc) {
+  const int ta = blockIdx.x / ntile, tb = blockIdx.x - ta * ntile;
+  if (!tiles[blockIdx.x] || *nonfinite) return;   // uniform over the block
+  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  if (j >= nf) return;                            // uniform over the wave
+  const int lane = threadIdx.x & 63, i = lane & 31, kq = lane >> 5;
+  const int a = ta * 32 + i, b = tb * 32 + i;
+  const bool av = a < nsta, bv = b < nsta;
+  const size_t rows = (size_t)cw * nsta;
+  const float2* Sj = S2 + (size_t)j * rows;
+  ccx16 gre, gim;
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    gre[v] = 0.0f;
+    gim[v] = 0.0f;
+  }
+#pragma unroll 4
+  for (int wl = 0; wl < cw; wl++) {
+    const float2* Sw = Sj + (size_t)wl * nsta;
+    const float2 xa = av ? Sw[a] : make_float2(0.0f, 0.0f);
+    const float2 xb = bv ? Sw[b] : make_float2(0.0f, 0.0f);
+    // real: ar br + ai bi;  imaginary: ar bi + (-ai) br
+    const float a1 = kq ? xa.y : xa.x, b1 = kq ? xb.y : xb.x;
+    const float a2 = kq ? -xa.y : xa.x, b2 = kq ? xb.x : xb.y;
+    gre = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, gre, 0, 0, 0);
+    gim = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b2, gim, 0, 0, 0);
+  }
+  if (!bv) return;
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    const int ar = ta * 32 + 8 * (v >> 2) + 4 * kq + (v & 3);
+    if (ar >= nsta) continue;
+    const int p = pidx[(size_t)ar * nsta + b];
+    if (p < 0) continue;
+    float2* o = acc + (size_t)p * nf + j;
+    float2 s = *o;
+    s.x += gre[v];
+    s.y += gim[v];
+    *o = s;
+  }
+}
+
+// The per-pair form on the S2 layout, for what the Gram form leaves: every
+// pair when the chunk holds a non-finite used value, else the duplicates.
+__global__ __launch_bounds__(256) void k_cc_pairs_t(const float2* __restrict__ S2,
+                                                    const unsigned char* __restrict__ use,
+                                                    const int* __restrict__ pairs, int nsta,
+                                                    int nf, int w0, int cw,
+                                                    const unsigned char* __restrict__ dup,
+                                                    const int* __restrict__ nonfinite,
+                                                    float2* __restrict__ acc) {
+  const int p = blockIdx.x;
+  if (!(*nonfinite || dup[p])) return;   // uniform over the block
+  const int j = blockIdx.y * blockDim.x + threadIdx.x;
+  const int a = pairs[2 * p], b = pairs[2 * p + 1];
+  if (j >= nf || a < 0 || a >= nsta || b < 0 || b >= nsta) return;
+  const size_t rows = (size_t)cw * nsta;
+  float re = 0.0f, im = 0.0f;
+  for (int wl = 0; wl < cw; wl++) {
+    const unsigned char* u = use + (size_t)(w0 + wl) * nsta;
+    if (!(u[a] && u[b])) continue;   // uniform over the block
+    const float2 xa = S2[(size_t)j * rows + (size_t)wl * nsta + a];
+    const float2 xb = S2[(size_t)j * rows + (size_t)wl * nsta + b];
+    re = __builtin_fmaf(xa.x, xb.x, __builtin_fmaf(xa.y, xb.y, re));
+    im = __builtin_fmaf(xa.x, xb.y, __builtin_fmaf(-xa.y, xb.x, im));
+  }
+  float2* o = acc + (size_t)p * nf + j;
+  float2 v = *o;
+  v.x += re;
+  v.y += im;
+  *o = v;
+}
+
+__global__ void k_cc_fill_i32(int* __restrict__ x, size_t n, int v) {
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) x[k] = v;
+}
+
 // acc[k] = (re[k], im[k]): running sums from separate arrays
 __global__ void k_cc_interleave(const float* __restrict__ re, const float* __restrict__ im,
                                 size_t n, float2* __restrict__ acc) {
@@ -172,6 +332,47 @@ hipError_t launch_cc_pairs(const void* S, const unsigned char* use, const int* p
                            int nsta, int nf, int w0, int cw, void* acc, hipStream_t st) {
   hipLaunchKernelGGL(k_cc_pairs, dim3(npairs, (nf + 255) / 256), dim3(256), 0, st,
                      (const float2*)S, use, pairs, nsta, nf, w0, cw, (float2*)acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_cc_spec_t(const void* spec, int rows, int nsta, int nbins, int nf, int fstride,
+                            int whiten, const unsigned char* use, int w0, void* S2, int* nonfinite,
+                            hipStream_t st) {
+  hipLaunchKernelGGL(k_cc_spec_t, dim3((rows + 31) / 32, (nf + 31) / 32), dim3(256), 0, st,
+                     (const float2*)spec, rows, nsta, nbins, nf, fstride, whiten, use, w0,
+                     (float2*)S2, nonfinite);
+  return hipGetLastError();
+}
+
+hipError_t launch_cc_pair_map(const int* pairs, int npairs, int nsta, int ntile, int* pidx,
+                              unsigned char* dup, unsigned char* tiles, hipStream_t st) {
+  hipLaunchKernelGGL(k_cc_pair_map, dim3((npairs + 255) / 256), dim3(256), 0, st, pairs, npairs,
+                     nsta, ntile, pidx, dup, tiles);
+  return hipGetLastError();
+}
+
+hipError_t launch_cc_gram(const void* S2, int nsta, int nf, int cw, int ntile,
+                          const unsigned char* tiles, const int* pidx, const int* nonfinite,
+                          void* acc, hipStream_t st) {
+  hipLaunchKernelGGL(k_cc_gram, dim3(ntile * ntile, (nf + 3) / 4), dim3(256), 0, st,
+                     (const float2*)S2, nsta, nf, cw, ntile, tiles, pidx, nonfinite,
+                     (float2*)acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_cc_pairs_t(const void* S2, const unsigned char* use, const int* pairs,
+                             int npairs, int nsta, int nf, int w0, int cw,
+                             const unsigned char* dup, const int* nonfinite, void* acc,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_cc_pairs_t, dim3(npairs, (nf + 255) / 256), dim3(256), 0, st,
+                     (const float2*)S2, use, pairs, nsta, nf, w0, cw, dup, nonfinite,
+                     (float2*)acc);
+  return hipGetLastError();
+}
+
+hipError_t launch_cc_fill_i32(int* x, size_t n, int v, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cc_fill_i32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, x, n, v);
   return hipGetLastError();
 }
 

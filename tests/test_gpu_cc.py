@@ -123,3 +123,43 @@ def test_cc_device_pointers_match_host_entry(ccfj, ctx):
         assert np.all(gre[-1] == 0) and np.all(gim[-1] == 0)
         want = np.sign(x.reshape(-1)) if onebit else x.reshape(-1)
         assert np.array_equal(tx.cpu().numpy(), want)
+
+
+@pytest.mark.parametrize("kind", ["dense_with_duplicates", "sparse"])
+def test_cc_pair_lists(ccfj, kind):
+    """Dense lists take the Gram form (with a duplicate, a reversed pair and
+    an autocorrelation, which the per-pair fallback and the map handle);
+    sparse lists take the per-pair form.  Both against the oracle."""
+    nsta, npts, fftlen, nf = 10, 4000, 512, 120
+    x, se = _case(6, nsta, npts)
+    if kind == "sparse":
+        pairs = np.array([0, 1, 3, 2, 9, 4], np.int32)
+    else:
+        pairs = np.concatenate([ccfj.GetStationPairs(nsta), [2, 5, 7, 3, 4, 4]]).astype(np.int32)
+    got = ccfj.CC(npts, nsta, nf, fftlen, pairs, se, x.reshape(-1).copy(), overlaprate=0.5)
+    want, count = C.cross_correlate(x.reshape(-1), nsta, npts, nf, fftlen, fftlen // 2, pairs, se)
+    assert np.all(count > 0)
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    assert err <= TOL, err
+    if kind != "sparse":
+        p25 = 2 * nsta - 3 + 2     # (2, 5) in GetStationPairs order
+        dup = len(pairs) // 2 - 3   # the appended (2, 5): per-pair form, so rounding differs
+        assert np.max(np.abs(got[p25] - got[dup])) <= TOL * np.max(np.abs(want))
+
+
+def test_cc_non_finite_samples_follow_the_reference(ccfj):
+    """A NaN sample poisons exactly the pairs that share a window holding
+    it (the reference skips windows a station lacks): the Gram form hands
+    such a batch to the per-pair form."""
+    nsta, npts, fftlen, nf = 5, 3000, 512, 60
+    x, se = _case(7, nsta, npts, gaps=False)
+    se[2 * 3 + 1] = 1000          # station 3 ends early: it lacks the later windows
+    x[1, 2000] = np.nan           # station 1, inside windows that station 3 lacks
+    pairs = ccfj.GetStationPairs(nsta)
+    got = ccfj.CC(npts, nsta, nf, fftlen, pairs, se, x.reshape(-1).copy(), overlaprate=0.5)
+    want, _ = C.cross_correlate(x.reshape(-1), nsta, npts, nf, fftlen, fftlen // 2, pairs, se)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    fin = ~np.isnan(want)
+    assert np.isnan(want).any() and fin.any()
+    err = np.max(np.abs(got[fin] - want[fin])) / np.max(np.abs(want[fin]))
+    assert err <= TOL, err
