@@ -25,7 +25,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 HIP_SOURCES = ["fj_kernels.hip", "ccfj_capi.hip", "fj_plain64.hip", "fj_batched.hip",
                "ccfj_batch.hip", "fj_mwfj.hip", "ccfj_mwfj.hip", "fj_hilbert.hip",
-               "fj_cnn.hip", "ccfj_cnn.hip", "fj_synth.hip", "fj_cc.hip", "ccfj_cc.hip"]
+               "fj_cnn.hip", "ccfj_cnn.hip", "fj_synth.hip", "fj_cc.hip", "ccfj_cc.hip",
+               "ccfj_stage.hip"]
 
 
 def _run(cmd, **kw):

@@ -78,6 +78,8 @@ struct ccfj_ctx {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> spare;
   double timed_ms = 0.0;
   int timed_launches = 0;
+  void* pin[2] = {nullptr, nullptr};   // pinned staging ring (ccfj_stage.hip)
+  hipEvent_t pin_ev[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -226,6 +228,31 @@ hipStream_t ctx_stream(const ccfj_ctx* ctx) { return ctx->stream; }
 
 int default_context(ccfj_ctx** out) { return default_ctx(out); }
 
+constexpr size_t kPinChunk = (size_t)32 << 20;
+
+int ctx_pinned_ring(ccfj_ctx* ctx, void* buf[2], hipEvent_t ev[2], size_t* chunk) {
+  for (int i = 0; i < 2; i++) {
+    if (!ctx->pin[i]) {
+      hipError_t e = hipHostMalloc(&ctx->pin[i], kPinChunk, hipHostMallocDefault);
+      if (e != hipSuccess) {
+        ctx->pin[i] = nullptr;
+        return hip_fail(e, "hipHostMalloc");
+      }
+    }
+    if (!ctx->pin_ev[i]) {
+      hipError_t e = hipEventCreateWithFlags(&ctx->pin_ev[i], hipEventDisableTiming);
+      if (e != hipSuccess) {
+        ctx->pin_ev[i] = nullptr;
+        return hip_fail(e, "hipEventCreate");
+      }
+    }
+    buf[i] = ctx->pin[i];
+    ev[i] = ctx->pin_ev[i];
+  }
+  *chunk = kPinChunk;
+  return CCFJ_OK;
+}
+
 int timing_begin(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t>* ev) {
   if (!ctx->timing) return CCFJ_OK;
   hipError_t e;
@@ -299,6 +326,10 @@ int ccfj_destroy(ccfj_ctx* ctx) {
     (void)hipEventDestroy(ev.first);
     (void)hipEventDestroy(ev.second);
   }
+  for (int i = 0; i < 2; i++) {
+    if (ctx->pin[i]) (void)hipHostFree(ctx->pin[i]);
+    if (ctx->pin_ev[i]) (void)hipEventDestroy(ctx->pin_ev[i]);
+  }
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CCFJ_OK;
@@ -334,11 +365,10 @@ int ccfj_transform(ccfj_ctx* ctx, int mode, int itype, int normalize, const floa
   hipStream_t st = ctx->stream;
   hipError_t e = hipSuccess;
   if (nu) {
-    e = hipMemcpyAsync(ctx->buf[S_U0], U0, sizeof(float) * nu, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && two)
-      e = hipMemcpyAsync(ctx->buf[S_U1], U1, sizeof(float) * nu, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess)
-      e = hipMemcpyAsync(ctx->buf[S_R], r, sizeof(float) * nr, hipMemcpyHostToDevice, st);
+    if ((rc = ccfj_internal::copy_h2d(ctx, ctx->buf[S_U0], U0, sizeof(float) * nu, st))) return rc;
+    if (two && (rc = ccfj_internal::copy_h2d(ctx, ctx->buf[S_U1], U1, sizeof(float) * nu, st)))
+      return rc;
+    e = hipMemcpyAsync(ctx->buf[S_R], r, sizeof(float) * nr, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
       e = hipMemcpyAsync(ctx->buf[S_F], f, sizeof(float) * nf, hipMemcpyHostToDevice, st);
   }
@@ -350,12 +380,7 @@ int ccfj_transform(ccfj_ctx* ctx, int mode, int itype, int normalize, const floa
                   (const float*)ctx->buf[S_F], (const float*)ctx->buf[S_C], nr, nf, nc,
                   (float*)ctx->buf[S_OUT], st);
   if (rc) return rc;
-  e = hipMemcpyAsync(out, ctx->buf[S_OUT], sizeof(float) * (size_t)nf * nc,
-                     hipMemcpyDeviceToHost, st);
-  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync D2H");
-  e = hipStreamSynchronize(st);
-  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-  return CCFJ_OK;
+  return ccfj_internal::copy_d2h(ctx, out, ctx->buf[S_OUT], sizeof(float) * (size_t)nf * nc, st);
 }
 
 int ccfj_transform_items(ccfj_ctx* ctx, int mode, int itype, const float* U0, const float* U1,

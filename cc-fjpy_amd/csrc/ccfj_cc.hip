@@ -170,26 +170,24 @@ int cross_correlate(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int 
   float* dim = (float*)((char*)dres + align256(sizeof(float) * nout));
   int* dpairs = (int*)dlist;
   int* dse = (int*)((char*)dlist + pbytes);
-  e = hipMemcpyAsync(ddata, data, sizeof(float) * (size_t)nsta * npts, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(dpairs, pairs, sizeof(int) * 2 * (size_t)npairs, hipMemcpyHostToDevice, st);
+  if ((rc = ccfj_internal::copy_h2d(ctx, ddata, data, sizeof(float) * (size_t)nsta * npts, st)))
+    return rc;
+  e = hipMemcpyAsync(dpairs, pairs, sizeof(int) * 2 * (size_t)npairs, hipMemcpyHostToDevice, st);
   if (e == hipSuccess)
     e = hipMemcpyAsync(dse, startend, sizeof(int) * 2 * (size_t)nsta, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess && accumulate) {
-    e = hipMemcpyAsync(dre, ncfsr, sizeof(float) * nout, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipMemcpyAsync(dim, ncfsi, sizeof(float) * nout, hipMemcpyHostToDevice, st);
-  }
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
+  if (accumulate) {
+    if ((rc = ccfj_internal::copy_h2d(ctx, dre, ncfsr, sizeof(float) * nout, st))) return rc;
+    if ((rc = ccfj_internal::copy_h2d(ctx, dim, ncfsi, sizeof(float) * nout, st))) return rc;
+  }
   rc = cc_run(ctx, nsta, npts, nf, fstride, fftlen, steplen, onebit, whiten, npairs, dpairs, dse,
               (float*)ddata, dre, dim, accumulate, st);
   if (rc) {
     (void)hipStreamSynchronize(st);
     return rc;
   }
-  e = hipMemcpyAsync(ncfsr, dre, sizeof(float) * nout, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(ncfsi, dim, sizeof(float) * nout, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "hipMemcpyAsync D2H");
+  if ((rc = ccfj_internal::copy_d2h(ctx, ncfsr, dre, sizeof(float) * nout, st))) return rc;
+  return ccfj_internal::copy_d2h(ctx, ncfsi, dim, sizeof(float) * nout, st);
 }
 
 }  // namespace

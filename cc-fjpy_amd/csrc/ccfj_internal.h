@@ -31,4 +31,12 @@ int cached_fft_plan(int n, int batch, hipfftType type, hipStream_t st, hipfftHan
 int timing_begin(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t>* ev);
 void timing_end(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t> ev);
 
+// The context's pinned staging ring: two host buffers of *chunk bytes and
+// an event for each (allocated on first use, freed with the context).
+int ctx_pinned_ring(ccfj_ctx* ctx, void* buf[2], hipEvent_t ev[2], size_t* chunk);
+// Synchronous copies for host entry points: large ones go through the
+// pinned ring with the host side split over worker threads.
+int copy_d2h(ccfj_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st);
+int copy_h2d(ccfj_ctx* ctx, void* dst, const void* src, size_t bytes, hipStream_t st);
+
 }  // namespace ccfj_internal

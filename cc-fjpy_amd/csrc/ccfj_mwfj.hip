@@ -82,12 +82,14 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
     const size_t Kp = (size_t)(func ? 2 : 1) * d.half;
     void *dB, *dT;
     if ((rc = ccfj_internal::ctx_scratch(ctx, 5, sizeof(float) * nf * Kp * d.nrhs, &dB))) return rc;
-    if ((rc = ccfj_internal::ctx_scratch(ctx, 6, sizeof(float) * (size_t)nwin * nf * nc, &dT)))
+    const size_t tb = align256(sizeof(float) * (size_t)nwin * nf * nc);
+    if ((rc = ccfj_internal::ctx_scratch(ctx, 6, tb + sizeof(float) * (size_t)nwin * nf, &dT)))
       return rc;
     e = fjk::launch_fill((float*)dB, nf * Kp * d.nrhs, 0.0f, st);
     if (e != hipSuccess) return hip_fail(e, "fill kernel launch");
     d.B = (float*)dB;
     d.T = (float*)dT;
+    d.M = (float*)((char*)dT + tb);
   } else {
     // per-window staging: Re and Im spectra [nr][nf], one [nc][nf] result
     const size_t a = align256(sizeof(float) * (size_t)nr * nf);
@@ -196,8 +198,7 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
                         {o_xr, xrows, sizeof(double) * (size_t)nx * npts}};
   for (const Part& q : parts) {
     if (!q.bytes) continue;
-    e = hipMemcpyAsync(b + q.off, q.src, q.bytes, hipMemcpyHostToDevice, st);
-    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
+    if ((rc = ccfj_internal::copy_h2d(ctx, b + q.off, q.src, q.bytes, st))) return rc;
   }
   rc = mwfj_run(ctx, itype, func, nr, npts, nwin, nf, nc, fstride, (const double*)(b + o_u),
                 (const float*)(b + o_r), (const float*)(b + o_f), (const float*)(b + o_c),
@@ -208,8 +209,5 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
     (void)hipStreamSynchronize(st);
     return rc;
   }
-  e = hipMemcpyAsync(out, dout, sizeof(double) * (size_t)nwin * nc * nf, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync D2H");
-  return CCFJ_OK;
+  return ccfj_internal::copy_d2h(ctx, out, dout, sizeof(double) * (size_t)nwin * nc * nf, st);
 }

@@ -191,8 +191,8 @@ extern "C" int ccfj_fj_noise(ccfj_ctx* ctx, int itype, int func, int fstride, co
   if (!rc) rc = b.get(sizeof(float) * nc, &dc);
   if (!rc) rc = b.get(sizeof(float) * (size_t)nf * nc, &dout);
   if (rc) return rc;
-  e = hipMemcpyAsync(du, uf, sizeof(float) * nfull, hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) e = hipMemcpyAsync(dr, r, sizeof(float) * nr, hipMemcpyHostToDevice, st);
+  if ((rc = ccfj_internal::copy_h2d(ctx, du, uf, sizeof(float) * nfull, st))) return rc;
+  e = hipMemcpyAsync(dr, r, sizeof(float) * nr, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(df, f, sizeof(float) * nf, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = hipMemcpyAsync(dc, c, sizeof(float) * nc, hipMemcpyHostToDevice, st);
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
@@ -212,7 +212,5 @@ extern "C" int ccfj_fj_noise(ccfj_ctx* ctx, int itype, int func, int fstride, co
                           (const float*)dU, (const float*)dH, (const float*)dr, (const float*)df,
                           (const float*)dc, nr, nf, nc, (float*)dout, st);
   if (rc) return rc;
-  e = hipMemcpyAsync(out, dout, sizeof(float) * (size_t)nf * nc, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "hipMemcpyAsync D2H");
+  return ccfj_internal::copy_d2h(ctx, out, dout, sizeof(float) * (size_t)nf * nc, st);
 }

@@ -157,15 +157,16 @@ __global__ __launch_bounds__(256) void k_rowmax(const float* __restrict__ T, int
 // out[b][c][f] = T[b][f][c] (/ m[b][f] when m is given): 64 x 64 tiles
 // through LDS, loads coalesced along c, stores along f.  The division (not a
 // reciprocal product) rounds as the reference's in-place division.
+template <typename OutT>
 __global__ __launch_bounds__(256) void k_finish(const float* __restrict__ T,
                                                 const float* __restrict__ m, int nf, int nc,
-                                                float* __restrict__ out) {
+                                                OutT* __restrict__ out) {
   __shared__ float tile[64][65];
   const int b = blockIdx.z;
   const int c0 = blockIdx.x * 64, f0 = blockIdx.y * 64;
   const size_t plane = (size_t)nf * nc;
   const float* Tb = T + b * plane;
-  float* ob = out + b * plane;
+  OutT* ob = out + b * plane;
   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
   for (int fl = grp; fl < 64; fl += 4) {
     const int f = f0 + fl, c = c0 + lane;
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(256) void k_finish(const float* __restrict__ T,
     const int c = c0 + cl;
     if (f < nf && c < nc) {
       const float v = tile[lane][cl];
-      ob[(size_t)c * nf + f] = m ? v / scale_src : v;
+      ob[(size_t)c * nf + f] = (OutT)(m ? v / scale_src : v);
     }
   }
 }
@@ -247,8 +248,19 @@ hipError_t launch_finish(const float* T, float* m, int nb, int nf, int nc, float
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL(k_finish, dim3((nc + 63) / 64, (nf + 63) / 64, nb), dim3(256), 0, st, T,
-                     (const float*)m, nf, nc, out);
+  hipLaunchKernelGGL(k_finish<float>, dim3((nc + 63) / 64, (nf + 63) / 64, nb), dim3(256), 0, st,
+                     T, (const float*)m, nf, nc, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_finish_f64(const float* T, float* m, int nb, int nf, int nc, double* out,
+                             hipStream_t st) {
+  if (nb <= 0 || nf <= 0 || nc <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rowmax, dim3(nf, nb), dim3(256), 0, st, T, nf, nc, m);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_finish<double>, dim3((nc + 63) / 64, (nf + 63) / 64, nb), dim3(256), 0,
+                     st, T, (const float*)m, nf, nc, out);
   return hipGetLastError();
 }
 

@@ -25,6 +25,7 @@ struct MwfjDev {
   void* spec;          // rFFT of the chunk, complex double
   float* B;            // [nf][Kp][nrhs] coefficients (itype 0)
   float* T;            // [nwin][nf][nc] |O| before normalisation (itype 0)
+  float* M;            // [nwin][nf] column maxima (itype 0)
   double* out;         // [nwin][nc][nf]
 };
 

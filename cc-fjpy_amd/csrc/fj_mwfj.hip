@@ -19,13 +19,15 @@
 //                   B[k][rhs]; the kernel matrix Z (J0 or Y0) is generated by
 //                   VALU and contracted with v_mfma_f32_32x32x2_f32 (full
 //                   FP32), 32 velocities x 128 right-hand sides per wave
-//   k_mwfj_finish   |Re + i Im| was formed in the MFMA epilogue; per-column
-//                   max, divide, transpose to [nwin][nc][nf], float64
+//   finish          |Re + i Im| was formed in the MFMA epilogue; per-column
+//                   max (k_rowmax), divide and transpose to [nwin][nc][nf]
+//                   in float64 through LDS tiles (k_finish<double>)
 // itype 1 reuses the single-array kernels per window on the device spectra.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "fj_lane.h"
+#include "fj_kernels.h"
 #include "fj_mwfj.h"
 
 namespace fjk {
@@ -191,22 +193,6 @@ __global__ __launch_bounds__(256) void k_mwfj_mfma(const float* __restrict__ B,
   }
 }
 
-// per (window, frequency): builtin-max normalisation over c, transpose to
-// out[w][c][f] in float64 (the reference's MWFJ output dtype).
-__global__ void k_mwfj_finish(const float* __restrict__ T, int nf, int nc, int nwin,
-                              double* __restrict__ out) {
-  const int fi = blockIdx.x * blockDim.x + threadIdx.x;
-  const int w = blockIdx.y;
-  if (fi >= nf) return;
-  const float* col = T + ((size_t)w * nf + fi) * nc;
-  float m = fabsf(col[0]);
-  for (int c = 1; c < nc; c++) {
-    const float v = fabsf(col[c]);
-    if (v > m) m = v;
-  }
-  for (int c = 0; c < nc; c++) out[((size_t)w * nc + c) * nf + fi] = (double)(col[c] / m);
-}
-
 // float32 staging of one window's spectra for the itype-1 path:
 // re/im [nr][nf]
 __global__ void k_mwfj_stage(const double2* __restrict__ spec, int nr, int wl, int nf,
@@ -253,9 +239,7 @@ hipError_t launch_mwfj_mfma(const MwfjDev& d, hipStream_t st) {
 }
 
 hipError_t launch_mwfj_finish(const MwfjDev& d, hipStream_t st) {
-  dim3 grid((d.nf + 255) / 256, d.nwin);
-  hipLaunchKernelGGL(k_mwfj_finish, grid, dim3(256), 0, st, d.T, d.nf, d.nc, d.nwin, d.out);
-  return hipGetLastError();
+  return launch_finish_f64(d.T, d.M, d.nwin, d.nf, d.nc, d.out, st);
 }
 
 hipError_t launch_mwfj_stage(const MwfjDev& d, int wl, float* re, float* im, hipStream_t st) {
