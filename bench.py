@@ -69,6 +69,7 @@ def parse():
     p.add_argument("--kernel-reps", type=int, default=2)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-plain", action="store_true")
+    p.add_argument("--no-host", action="store_true", help="dense: skip the host end-to-end call")
     p.add_argument("--workload", default="dense",
                    choices=["dense", "batched", "mwfj", "learned", "cc"])
     p.add_argument("--train", type=int, default=50000, help="learned: training spectra")
@@ -116,7 +117,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this build measures one GPU. "
-              "The F-J path shards by frequency with no communication (see DESIGN.md).",
+              "The F-J path would shard by frequency columns, with no communication.",
               file=sys.stderr)
         sys.exit(2)
     if args.copy_calib:
@@ -224,6 +225,26 @@ def main():
         "kernels": kernels,
         "wall_s_timed": round(wall, 3),
     }
+
+    if not args.no_host and (1, 0) in variants:
+        # the reference call end to end: complex64 spectra in host memory,
+        # unsorted distances, numpy float32 [nc, nf] out (sort, cast, H2D,
+        # transform, normalise, D2H)
+        import warnings
+        ufc = d["uf"].astype(np.complex64)
+        print("bench: fj_noise end to end from host memory", file=sys.stderr, flush=True)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            ccfj.fj_noise(ufc, d["r"], d["c"], d["f"], itype=1, func=0, num=0)
+            t0 = time.perf_counter()
+            ccfj.fj_noise(ufc, d["r"], d["c"], d["f"], itype=1, func=0, num=0)
+            dt = time.perf_counter() - t0
+        del ufc
+        out["end_to_end_host"] = {
+            "value": synth.term_evals(nr, nf, nc) / dt, "unit": "term-evals/s",
+            "ms_per_call": round(dt * 1e3, 1),
+            "what": "fj_noise(uf, r, c, f, itype=1, func=0): complex64 spectra in host memory, "
+                    "unsorted r, numpy float32 [nc, nf] out"}
 
     if not args.no_plain:
         from ccfj import _ccfj

@@ -50,7 +50,7 @@ def build_cython():
     ext = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
     out = os.path.join(PKG, "_ccfj" + ext)
     pyinc = sysconfig.get_paths()["include"]
-    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17",
+    _run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-fopenmp",
           "-DNPY_NO_DEPRECATED_API=NPY_1_7_API_VERSION",
           "-I", pyinc, "-I", numpy.get_include(), "-I", INCLUDE,
           "-o", out, cpp, "-L", PKG, "-lccfj", "-Wl,-rpath,$ORIGIN"])

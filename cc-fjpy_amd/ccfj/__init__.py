@@ -154,7 +154,10 @@ def fj_noise(uf, r, c, f, fstride=1, itype=1, func=0, num=-1):
     """
     indx = np.argsort(r)
     r = r[indx]
-    uf = uf[indx]
+    fast = (isinstance(uf, np.ndarray) and uf.ndim == 2 and uf.flags.c_contiguous
+            and uf.dtype in (np.complex64, np.float32))
+    if not fast:
+        uf = uf[indx]
     if func not in (0, 1):
         print('set func as 0 for Bessel function 1 for Hankel function')
         return 0
@@ -162,7 +165,15 @@ def fj_noise(uf, r, c, f, fstride=1, itype=1, func=0, num=-1):
     uf[:, ::fstride].reshape(nr * nf)      # the reference's shape check (ValueError)
     if itype not in (0, 1):
         return _native(MODE_J, itype, True, None, None, r, c, f, num)
-    U = _as_f32(np.asarray(uf))
+    if fast:
+        # the row sort and the float32 cast in one threaded pass; numpy's
+        # cast warning for complex input, as the reference's cast gives it
+        if uf.dtype == np.complex64:
+            warnings.warn("Casting complex values to real discards the imaginary part",
+                          np.exceptions.ComplexWarning, stacklevel=2)
+        U = _ccfj.gather_real_rows(uf, indx)
+    else:
+        U = _as_f32(np.asarray(uf))
     return _context(num).fj_noise(int(itype), int(func), int(fstride), U.reshape(nr, -1),
                                   _as_f32(np.asarray(r)), _as_f32(np.asarray(f)),
                                   _as_f32(np.asarray(c)))

@@ -9,6 +9,7 @@ RuntimeError carrying ccfj_last_error().
 """
 import numpy as np
 cimport numpy as cnp
+from cython.parallel cimport prange
 
 cdef extern from "ccfj.h" nogil:
     ctypedef struct ccfj_ctx:
@@ -541,3 +542,36 @@ cdef class CNN:
         _check(ccfj_cnn_train_step(self.net, <const float *>x, <const float *>target, B, lr,
                                    &loss), "ccfj_cnn_train_step")
         return loss
+
+
+def gather_real_rows(src, order):
+    """float32 [nr, ncol] = real part of src[order] (rows permuted), for a
+    C-contiguous complex64 or float32 src [nr, ncol]; OpenMP threads over
+    rows.  The same values as np.array(src[order], dtype=np.float32)."""
+    if src.ndim != 2 or not src.flags.c_contiguous:
+        raise ValueError("src must be a C-contiguous 2-D array")
+    cdef Py_ssize_t nr = src.shape[0], ncol = src.shape[1]
+    cdef cnp.int64_t[::1] idx = np.ascontiguousarray(order, dtype=np.int64)
+    if idx.shape[0] != nr:
+        raise ValueError("order has %d entries for %d rows" % (idx.shape[0], nr))
+    if nr and (np.min(order) < 0 or np.max(order) >= nr):
+        raise ValueError("order out of range")
+    cdef int step
+    if src.dtype == np.complex64:
+        step = 2
+    elif src.dtype == np.float32:
+        step = 1
+    else:
+        raise TypeError("src dtype %s" % src.dtype)
+    out = np.empty((nr, ncol), np.float32)
+    if nr == 0 or ncol == 0:
+        return out
+    cdef float[:, ::1] s = src.view(np.float32)
+    cdef float[:, ::1] o = out
+    cdef Py_ssize_t i, j, k
+    for i in prange(nr, nogil=True, schedule="static"):
+        k = idx[i]
+        for j in range(ncol):
+            o[i, j] = s[k, step * j]
+    return out
+

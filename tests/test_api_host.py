@@ -3,6 +3,7 @@ transform along frequency, station pairs, staging errors and the
 reference's behaviour for bad func / itype."""
 import io
 import contextlib
+import warnings
 
 import numpy as np
 import pytest
@@ -81,3 +82,23 @@ def test_bench_refuses_multi_gpu():
     p = subprocess.run([sys.executable, f"{ROOT}/bench.py", "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert p.returncode == 2 and "one GPU" in p.stderr
+
+
+def test_gather_real_rows_is_sort_then_cast():
+    """The threaded row gather fj_noise uses equals the reference's
+    uf[indx] followed by the float32 cast, for complex64 and float32 input."""
+    from ccfj import _ccfj
+    rng = np.random.default_rng(3)
+    for dt in (np.complex64, np.float32):
+        uf = rng.standard_normal((37, 129)).astype(np.float32)
+        if dt == np.complex64:
+            uf = (uf + 1j * rng.standard_normal((37, 129))).astype(np.complex64)
+        indx = np.argsort(rng.uniform(size=37))
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            want = np.array(uf[indx], dtype=np.float32)
+        assert np.array_equal(_ccfj.gather_real_rows(uf, indx), want)
+    with pytest.raises(ValueError):
+        _ccfj.gather_real_rows(uf, np.arange(36))
+    with pytest.raises(TypeError):
+        _ccfj.gather_real_rows(uf.astype(np.float64), indx)
