@@ -597,7 +597,7 @@ def bench_cc(args):
     res["end_to_end_host"] = {"value": pw / wall_host, "unit": "pair-windows/s",
                               "ms_per_step": round(wall_host * 1e3, 3),
                               "what": "CC() with numpy in and out: H2D of the records, the same "
-                                      "device work, D2H of the sums, complex64 assembly"}
+                                      "device work, D2H of the complex64 result"}
     res["output_shape"] = list(out.shape)
     if not args.no_cpu_baseline:
         nw = 64   # the restatement on every station and pair, first 64 windows

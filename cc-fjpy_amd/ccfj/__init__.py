@@ -348,13 +348,14 @@ def CC(npts, nsta, nf, fftlen, Pairs, startend, data, overlaprate=0.0, nThreads=
     npairs = int(len(Pairs) / 2)
     if data.dtype != np.float32:
         data = np.array(data, dtype=np.float32)   # the reference's copy
-    re, im = _context(num).cross_correlate(int(nsta), int(npts), int(nf), int(fstride), int(fftlen),
-                                           steplen, bool(ifonebit), bool(ifspecwhittenning),
-                                           np.asarray(Pairs, np.int32)[:2 * npairs],
-                                           np.asarray(startend, np.int32), data)
+    out = _context(num).cross_correlate_c64(int(nsta), int(npts), int(nf), int(fstride),
+                                            int(fftlen), steplen, bool(ifonebit),
+                                            bool(ifspecwhittenning),
+                                            np.asarray(Pairs, np.int32)[:2 * npairs],
+                                            np.asarray(startend, np.int32), data)
     if ifonebit:
         np.copyto(data, np.sign(data))
-    return re + im * 1j
+    return out
 
 
 def GetStationPairs(nsta):

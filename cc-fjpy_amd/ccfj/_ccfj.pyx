@@ -76,6 +76,10 @@ cdef extern from "ccfj.h" nogil:
                              int steplen, int onebit, int whiten, int npairs, const int *pairs,
                              const int *startend, const float *data, float *ncfsr,
                              float *ncfsi) nogil
+    int ccfj_cross_correlate_c64(ccfj_ctx *ctx, int nsta, int npts, int nf, int fstride,
+                                 int fftlen, int steplen, int onebit, int whiten, int npairs,
+                                 const int *pairs, const int *startend, const float *data,
+                                 float *ncfs)
     int ccfj_cross_correlate_dev(ccfj_ctx *ctx, int nsta, int npts, int nf, int fstride,
                                  int fftlen, int steplen, int onebit, int whiten, int npairs,
                                  const int *pairs, const int *startend, float *data, float *ncfsr,
@@ -337,6 +341,26 @@ cdef class Context:
                                       whiten, npairs, pp, &se[0], &dv[0], rp, ip)
         _check(rc, "ccfj_cross_correlate")
         return re, im
+
+    def cross_correlate_c64(self, int nsta, int npts, int nf, int fstride, int fftlen,
+                            int steplen, bint onebit, bint whiten, pairs, startend, data):
+        """As cross_correlate, returning complex64 [npairs, nf] directly."""
+        cdef int[::1] pv = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1)
+        cdef int[::1] se = np.ascontiguousarray(startend, dtype=np.int32).reshape(-1)
+        cdef float[::1] dv = np.ascontiguousarray(data, dtype=np.float32).reshape(-1)
+        cdef int npairs = pv.shape[0] // 2
+        if se.shape[0] < 2 * nsta or dv.shape[0] < <size_t>nsta * npts:
+            raise ValueError("startend or data shorter than nsta / npts require")
+        out = np.empty((npairs, nf), np.complex64)
+        cdef float[:, ::1] ov = out.view(np.float32)
+        cdef const int *pp = &pv[0] if npairs > 0 else NULL
+        cdef float *op = &ov[0, 0] if npairs > 0 else NULL
+        cdef int rc
+        with nogil:
+            rc = ccfj_cross_correlate_c64(self.ctx, nsta, npts, nf, fstride, fftlen, steplen,
+                                          onebit, whiten, npairs, pp, &se[0], &dv[0], op)
+        _check(rc, "ccfj_cross_correlate_c64")
+        return out
 
     def cross_correlate_dev(self, int nsta, int npts, int nf, int fstride, int fftlen,
                             int steplen, bint onebit, bint whiten, int npairs, size_t pairs,

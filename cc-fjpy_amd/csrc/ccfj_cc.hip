@@ -53,10 +53,16 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
   } else {
     nshifts = 1 + (long)(npts - fftlen) / steplen;
   }
-  const int nbins = fftlen / 2 + 1;
+  // with a bin stride that divides the window, each window is folded to
+  // L = fftlen / fstride samples and its length-L rFFT gives exactly the
+  // bins that are kept (every fstride-th of the length-fftlen one)
+  const int fold = (fstride > 1 && fftlen % fstride == 0) ? fstride : 1;
+  const int L = fftlen / fold;
+  const int bstride = fold > 1 ? 1 : fstride;
+  const int nbins = L / 2 + 1;
   // windows per batch: gathered segments, their spectra and the extracted
   // bins within 4 GiB (and within one grid dimension)
-  const size_t per_win = (size_t)nsta * ((size_t)fftlen * 4 + (size_t)nbins * 8 + (size_t)nf * 8);
+  const size_t per_win = (size_t)nsta * ((size_t)L * 4 + (size_t)nbins * 8 + (size_t)nf * 8);
   const int cw = (int)std::max<size_t>(
       1, std::min<size_t>(std::min<long>(nshifts, 65535), ((size_t)4 << 30) / per_win));
 
@@ -72,7 +78,7 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
   const size_t tile_bytes = gram ? align256((size_t)ntile * ntile) : 0;
   const size_t pidx_bytes = gram ? align256(sizeof(int) * (size_t)nsta * nsta) : 0;
   const size_t flag_bytes = gram ? sizeof(int) : 0;
-  int rc = ccfj_internal::ctx_scratch(ctx, 3, sizeof(float) * (size_t)cw * nsta * fftlen, &dseg);
+  int rc = ccfj_internal::ctx_scratch(ctx, 3, sizeof(float) * (size_t)cw * nsta * L, &dseg);
   if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 4, 8 * (size_t)cw * nsta * nbins, &dspec);
   if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 5, 8 * (size_t)cw * nsta * nf, &dS);
   if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 6, 8 * (size_t)npairs * nf, &dacc);
@@ -109,17 +115,17 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
   }
   for (long w0 = 0; w0 < nshifts; w0 += cw) {
     const int n = (int)std::min<long>(cw, nshifts - w0);
-    if ((e = fjk::launch_cc_gather(data, nsta, npts, fftlen, steplen, (int)w0, n, (float*)dseg,
+    if ((e = fjk::launch_cc_gather(data, nsta, npts, L, fold, steplen, (int)w0, n, (float*)dseg,
                                    st)) != hipSuccess)
       return hip_fail(e, "gather kernel");
     hipfftHandle plan = 0;
-    if ((rc = ccfj_internal::cached_fft_plan(fftlen, n * nsta, HIPFFT_R2C, st, &plan))) return rc;
+    if ((rc = ccfj_internal::cached_fft_plan(L, n * nsta, HIPFFT_R2C, st, &plan))) return rc;
     if (hipfftExecR2C(plan, (hipfftReal*)dseg, (hipfftComplex*)dspec) != HIPFFT_SUCCESS)
       return fail(CCFJ_EDEVICE, "hipfftExecR2C failed");
     if (gram) {
       e = fjk::launch_cc_fill_i32(dflag, 1, 0, st);
       if (e == hipSuccess)
-        e = fjk::launch_cc_spec_t(dspec, n * nsta, nsta, nbins, nf, fstride, whiten, duse,
+        e = fjk::launch_cc_spec_t(dspec, n * nsta, nsta, nbins, nf, bstride, whiten, duse,
                                   (int)w0, dS, dflag, st);
       if (e == hipSuccess)
         e = fjk::launch_cc_gram(dS, nsta, nf, n, ntile, dtiles, dpidx, dflag, dacc, st);
@@ -128,7 +134,7 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
                                    dacc, st);
       if (e != hipSuccess) return hip_fail(e, "Gram pair stage");
     } else {
-      if ((e = fjk::launch_cc_spec(dspec, n * nsta, nbins, nf, fstride, whiten, dS, st)) !=
+      if ((e = fjk::launch_cc_spec(dspec, n * nsta, nbins, nf, bstride, whiten, dS, st)) !=
           hipSuccess)
         return hip_fail(e, "spectrum kernel");
       if ((e = fjk::launch_cc_pairs(dS, duse, pairs, npairs, nsta, nf, (int)w0, n, dacc, st)) !=
@@ -141,6 +147,8 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
   return CCFJ_OK;
 }
 
+// ncfsi == nullptr: ncfsr receives interleaved complex [npairs][nf][2]
+// (accumulate must be false then)
 int cross_correlate(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen,
                     int steplen, int onebit, int whiten, int npairs, const int* pairs,
                     const int* startend, const float* data, float* ncfsr, float* ncfsi,
@@ -148,7 +156,9 @@ int cross_correlate(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int 
   if (!ctx) return fail(CCFJ_EINVAL, "null context");
   int rc = check_sizes(nsta, npts, nf, fstride, fftlen, steplen, npairs);
   if (rc) return rc;
-  if (!startend || !data || (npairs > 0 && (!pairs || !ncfsr || !ncfsi)))
+  const bool interleaved = ncfsi == nullptr;
+  if (!startend || !data || (npairs > 0 && !pairs) || (npairs > 0 && !ncfsr) ||
+      (interleaved && accumulate))
     return fail(CCFJ_EINVAL, "null pointer");
   for (int p = 0; p < 2 * npairs; p++)
     if (pairs[p] < 0 || pairs[p] >= nsta) return fail(CCFJ_EINVAL, "pair station %d of %d", pairs[p], nsta);
@@ -166,7 +176,7 @@ int cross_correlate(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int 
   if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 7, 2 * align256(sizeof(float) * nout), &dres);
   if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 9, pbytes + sizeof(int) * 2 * (size_t)nsta, &dlist);
   if (rc) return rc;
-  float* dre = (float*)dres;
+  float* dre = (float*)dres;   // interleaved: the whole slot, 2 nout floats
   float* dim = (float*)((char*)dres + align256(sizeof(float) * nout));
   int* dpairs = (int*)dlist;
   int* dse = (int*)((char*)dlist + pbytes);
@@ -181,11 +191,12 @@ int cross_correlate(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int 
     if ((rc = ccfj_internal::copy_h2d(ctx, dim, ncfsi, sizeof(float) * nout, st))) return rc;
   }
   rc = cc_run(ctx, nsta, npts, nf, fstride, fftlen, steplen, onebit, whiten, npairs, dpairs, dse,
-              (float*)ddata, dre, dim, accumulate, st);
+              (float*)ddata, dre, interleaved ? nullptr : dim, accumulate, st);
   if (rc) {
     (void)hipStreamSynchronize(st);
     return rc;
   }
+  if (interleaved) return ccfj_internal::copy_d2h(ctx, ncfsr, dre, 2 * sizeof(float) * nout, st);
   if ((rc = ccfj_internal::copy_d2h(ctx, ncfsr, dre, sizeof(float) * nout, st))) return rc;
   return ccfj_internal::copy_d2h(ctx, ncfsi, dim, sizeof(float) * nout, st);
 }
@@ -199,6 +210,14 @@ int ccfj_cross_correlate(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride,
                          const int* startend, const float* data, float* ncfsr, float* ncfsi) {
   return cross_correlate(ctx, nsta, npts, nf, fstride, fftlen, steplen, onebit, whiten, npairs,
                          pairs, startend, data, ncfsr, ncfsi, false);
+}
+
+int ccfj_cross_correlate_c64(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen,
+                             int steplen, int onebit, int whiten, int npairs, const int* pairs,
+                             const int* startend, const float* data, float* ncfs) {
+  if (npairs > 0 && !ncfs) return fail(CCFJ_EINVAL, "null pointer");
+  return cross_correlate(ctx, nsta, npts, nf, fstride, fftlen, steplen, onebit, whiten, npairs,
+                         pairs, startend, data, ncfs, nullptr, false);
 }
 
 int ccfj_cross_correlate_dev(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen,

@@ -6,8 +6,8 @@
 namespace fjk {
 
 hipError_t launch_cc_onebit(float* x, size_t n, hipStream_t st);
-hipError_t launch_cc_gather(const float* data, int nsta, int npts, int fftlen, int steplen, int w0,
-                            int cw, float* seg, hipStream_t st);
+hipError_t launch_cc_gather(const float* data, int nsta, int npts, int L, int fold, int steplen,
+                            int w0, int cw, float* seg, hipStream_t st);
 hipError_t launch_cc_spec(const void* spec, int rows, int nbins, int nf, int fstride, int whiten,
                           void* S, hipStream_t st);
 hipError_t launch_cc_use(const int* startend, int nsta, int nshifts, int steplen, int fftlen,

@@ -10,8 +10,9 @@
 //
 // Layout (device):
 //   data  f32 [nsta][npts]            (one-bit: replaced by its sign first)
-//   seg   f32 [cw][nsta][fftlen]      windows of a chunk, gathered
-//   spec  c64 [cw][nsta][fftlen/2+1]  their rFFT (hipFFT, single precision)
+//   seg   f32 [cw][nsta][L]           windows of a chunk, gathered (L = fftlen,
+//                                     or fftlen / fstride when folded)
+//   spec  c64 [cw][nsta][L/2+1]       their rFFT (hipFFT, single precision)
 //   S     c64 [cw][nsta][nf]          bins j*fstride, whitened
 //   acc   c64 [npairs][nf]            running sums over windows
 //   use   u8  [nshifts][nsta]         station has the whole window
@@ -34,15 +35,21 @@ __global__ void k_cc_onebit(float* __restrict__ x, size_t n) {
   x[i] = v > 0.0f ? 1.0f : (v < 0.0f ? -1.0f : 0.0f);
 }
 
-// seg[wl][i][t] = data[i][(w0 + wl) * steplen + t]; grid (t blocks, i, wl)
-__global__ void k_cc_gather(const float* __restrict__ data, int npts, int fftlen, int steplen,
-                            int w0, float* __restrict__ seg) {
+// seg[wl][i][t] = sum over q < fold of data[i][(w0 + wl) * steplen + t + q L],
+// t < L = fftlen / fold; grid (t blocks, i, wl).  fold = 1 is the plain
+// window.  Folding by the bin stride s (when s divides fftlen) is exact for
+// the bins that are kept: X[s m] = sum_n (sum_q x[n + q L]) e^{-2 pi i m n / L},
+// so an rFFT of length L gives bins 0, s, 2s, ... of the length-fftlen one.
+__global__ void k_cc_gather(const float* __restrict__ data, int npts, int L, int fold,
+                            int steplen, int w0, float* __restrict__ seg) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y;
   const int wl = blockIdx.z;
-  if (t >= fftlen) return;
-  const size_t src = (size_t)i * npts + (size_t)(w0 + wl) * steplen + t;
-  seg[((size_t)wl * gridDim.y + i) * fftlen + t] = data[src];
+  if (t >= L) return;
+  const float* x = data + (size_t)i * npts + (size_t)(w0 + wl) * steplen + t;
+  float v = x[0];
+  for (int q = 1; q < fold; q++) v += x[(size_t)q * L];
+  seg[((size_t)wl * gridDim.y + i) * L + t] = v;
 }
 
 // S[row][j] = spec[row][j * fstride] (whitened); rows = cw * nsta
@@ -282,6 +289,7 @@ __global__ void k_cc_interleave(const float* __restrict__ re, const float* __res
 
 // ncfsr/ncfsi[p][j] = acc[p][j] / count[p] where count > 0, else acc
 // (the untouched running sums, as the reference leaves them)
+// With im == nullptr, re holds interleaved complex [npairs][nf][2] instead.
 __global__ void k_cc_finish(const float2* __restrict__ acc, const float* __restrict__ count,
                             int nf, float* __restrict__ re, float* __restrict__ im) {
   const int p = blockIdx.x;
@@ -289,8 +297,13 @@ __global__ void k_cc_finish(const float2* __restrict__ acc, const float* __restr
   if (j >= nf) return;
   const float n = count[p];
   const float2 v = acc[(size_t)p * nf + j];
-  re[(size_t)p * nf + j] = n > 0.0f ? v.x / n : v.x;
-  im[(size_t)p * nf + j] = n > 0.0f ? v.y / n : v.y;
+  const float a = n > 0.0f ? v.x / n : v.x, b = n > 0.0f ? v.y / n : v.y;
+  if (im) {
+    re[(size_t)p * nf + j] = a;
+    im[(size_t)p * nf + j] = b;
+  } else {
+    reinterpret_cast<float2*>(re)[(size_t)p * nf + j] = make_float2(a, b);
+  }
 }
 
 hipError_t launch_cc_onebit(float* x, size_t n, hipStream_t st) {
@@ -299,10 +312,10 @@ hipError_t launch_cc_onebit(float* x, size_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_cc_gather(const float* data, int nsta, int npts, int fftlen, int steplen, int w0,
-                            int cw, float* seg, hipStream_t st) {
-  hipLaunchKernelGGL(k_cc_gather, dim3((fftlen + 255) / 256, nsta, cw), dim3(256), 0, st, data,
-                     npts, fftlen, steplen, w0, seg);
+hipError_t launch_cc_gather(const float* data, int nsta, int npts, int L, int fold, int steplen,
+                            int w0, int cw, float* seg, hipStream_t st) {
+  hipLaunchKernelGGL(k_cc_gather, dim3((L + 255) / 256, nsta, cw), dim3(256), 0, st, data, npts, L,
+                     fold, steplen, w0, seg);
   return hipGetLastError();
 }
 

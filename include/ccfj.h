@@ -195,6 +195,13 @@ int ccfj_fj_noise(ccfj_ctx *ctx, int itype, int func, int fstride, const float *
 int ccfj_cross_correlate(ccfj_ctx *ctx, int nsta, int npts, int nf, int fstride, int fftlen,
                          int steplen, int onebit, int whiten, int npairs, const int *pairs,
                          const int *startend, const float *data, float *ncfsr, float *ncfsi);
+/* The same with the result as interleaved complex ncfs[npairs][nf][2]
+ * (numpy's complex64 layout, what the reference's Python CC returns after
+ * combining its two arrays, src/ccfj.pyx:145).  No counterpart in the
+ * reference's C layer. */
+int ccfj_cross_correlate_c64(ccfj_ctx *ctx, int nsta, int npts, int nf, int fstride, int fftlen,
+                             int steplen, int onebit, int whiten, int npairs, const int *pairs,
+                             const int *startend, const float *data, float *ncfs);
 /* The same on device pointers (pairs, startend, data, ncfsr, ncfsi all in
  * device memory), enqueued on `stream` (NULL = the context's stream);
  * returns once enqueued.  data is replaced by its signs when onebit is set

@@ -163,3 +163,21 @@ def test_cc_non_finite_samples_follow_the_reference(ccfj):
     assert np.isnan(want).any() and fin.any()
     err = np.max(np.abs(got[fin] - want[fin])) / np.max(np.abs(want[fin]))
     assert err <= TOL, err
+
+
+@pytest.mark.parametrize("fftlen,fstride", [(1200, 3), (1000, 3), (1024, 4)])
+def test_cc_bin_stride_fold_and_plain(ccfj, fftlen, fstride):
+    """A bin stride that divides the window folds it before a shorter rFFT
+    (1200/3, 1024/4); one that does not (1000/3) keeps the full rFFT and
+    strides its bins.  Both against the oracle."""
+    nsta, npts = 5, 5000
+    nf = (fftlen // 2) // fstride
+    x, se = _case(8, nsta, npts)
+    pairs = ccfj.GetStationPairs(nsta)
+    got = ccfj.CC(npts, nsta, nf, fftlen, pairs, se, x.reshape(-1).copy(), overlaprate=0.75,
+                  fstride=fstride)
+    want, count = C.cross_correlate(x.reshape(-1), nsta, npts, nf, fftlen, int(fftlen * 0.25),
+                                    pairs, se, fstride=fstride)
+    assert np.all(count > 0)
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    assert err <= TOL, err

@@ -68,3 +68,15 @@ def test_onebit_uses_signs():
     b, _ = C.cross_correlate(np.sign(x).reshape(-1), 2, 300, 10, 128, 64, [0, 1],
                              [0, 400, 0, 400], whiten=False)
     assert np.array_equal(a, b)
+
+
+def test_bin_stride_fold_identity():
+    """Bins 0, s, 2s, ... of a length-N rFFT are the rFFT of the window
+    folded to N/s samples (the device path's shortcut when s divides N)."""
+    rng = np.random.default_rng(4)
+    for n, s in ((14400, 3), (1024, 4), (90, 2)):
+        x = rng.normal(size=n)
+        full = np.fft.rfft(x)[::s]
+        folded = np.fft.rfft(x.reshape(s, n // s).sum(axis=0))
+        m = min(len(full), len(folded))
+        assert np.allclose(full[:m], folded[:m], rtol=0, atol=1e-9 * np.abs(full).max())
