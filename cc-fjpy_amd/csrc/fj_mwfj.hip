@@ -37,38 +37,41 @@ using namespace fjl;
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // ---------------------------------------------------------------------------
-// tapered traces: xw[(wl*nr + s)*npts + t], windows w0 .. w0+cw-1
+// tapered traces: xw[(wl*nr + s)*npts + t], windows w0 .. w0+cw-1.  Each
+// thread reads its sample u[s][t] once and writes it, tapered, for every
+// window of the chunk (the trace is read once per chunk, not per window).
 // A description whose xidx is outside [-1, nx) gives a zero row (the
 // device entry point cannot check descriptions on the host).
 __global__ void k_window(const double* __restrict__ u, const int* __restrict__ n1,
                          const int* __restrict__ n2, const int* __restrict__ dl,
                          const int* __restrict__ xidx, const double* __restrict__ xrows,
-                         int nx, int w0, int nr, int npts, double* __restrict__ xw) {
+                         int nx, int w0, int cw, int nr, int npts, double* __restrict__ xw) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = blockIdx.y;
-  const int wl = blockIdx.z;
   if (t >= npts) return;
-  const int desc = (w0 + wl) * nr + s;
-  double v;
-  const int xi = xidx[desc];
-  if (xi >= nx || xi < -1) {
-    v = 0.0;
-  } else if (xi >= 0) {
-    v = xrows[(size_t)xi * npts + t];
-  } else {
-    const int a = n1[desc], b = n2[desc], d = dl[desc];
-    if (t < a || t >= b) {
+  const double us = u[(size_t)s * npts + t];
+  for (int wl = 0; wl < cw; wl++) {
+    const int desc = (w0 + wl) * nr + s;   // uniform over the block
+    double v;
+    const int xi = xidx[desc];
+    if (xi >= nx || xi < -1) {
       v = 0.0;
-    } else if (t < a + d) {
-      v = sin(3.141592653589793 / 2 * (double)(t - a) / (double)d);
-    } else if (t >= b - d) {
-      v = sin(3.141592653589793 / 2 * (double)(b - 1 - t) / (double)d);
+    } else if (xi >= 0) {
+      v = xrows[(size_t)xi * npts + t];
     } else {
-      v = 1.0;
+      const int a = n1[desc], b = n2[desc], d = dl[desc];
+      if (t < a || t >= b) {
+        v = 0.0;
+      } else if (t < a + d) {
+        v = sin(3.141592653589793 / 2 * (double)(t - a) / (double)d);
+      } else if (t >= b - d) {
+        v = sin(3.141592653589793 / 2 * (double)(b - 1 - t) / (double)d);
+      } else {
+        v = 1.0;
+      }
     }
+    xw[((size_t)wl * nr + s) * npts + t] = us * v;
   }
-  const size_t row = (size_t)wl * nr + s;
-  xw[row * npts + t] = u[(size_t)s * npts + t] * v;
 }
 
 // spectra -> float32 staging -> trapezoid coefficients.
@@ -214,9 +217,9 @@ __global__ void k_to_double(const float* __restrict__ a, size_t n, double* __res
 
 // ---------------------------------------------------------------------------
 hipError_t launch_window(const MwfjDev& d, int w0, int cw, hipStream_t st) {
-  dim3 grid((d.npts + 255) / 256, d.nr, cw);
+  dim3 grid((d.npts + 255) / 256, d.nr);
   hipLaunchKernelGGL(k_window, grid, dim3(256), 0, st, d.u, d.n1, d.n2, d.dl, d.xidx, d.xrows,
-                     d.nx, w0, d.nr, d.npts, d.xw);
+                     d.nx, w0, cw, d.nr, d.npts, d.xw);
   return hipGetLastError();
 }
 
