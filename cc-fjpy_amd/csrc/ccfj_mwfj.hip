@@ -58,13 +58,14 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
   void *dxw, *dspec, *drr;
   int rc = ccfj_internal::ctx_scratch(ctx, 2, sizeof(double) * (size_t)cw * nr * npts, &dxw);
   if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 3, 16 * (size_t)cw * nr * d.nbins, &dspec);
-  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 7, sizeof(double) * nr, &drr);
+  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 7, 12 * (size_t)d.half, &drr);
   if (rc) return rc;
-  hipError_t e = fjk::launch_to_double(r, nr, (double*)drr, st);
-  if (e != hipSuccess) return hip_fail(e, "convert kernel");
-  d.u = u;
   d.r = r;
+  hipError_t e = fjk::launch_mwfj_stations(d, (double*)drr, (float*)((double*)drr + d.half), st);
+  if (e != hipSuccess) return hip_fail(e, "station kernel");
+  d.u = u;
   d.rr = (const double*)drr;
+  d.ir = (const float*)((const double*)drr + d.half);
   d.f = f;
   d.c = c;
   d.n1 = n1;

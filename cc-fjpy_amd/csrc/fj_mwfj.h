@@ -18,7 +18,8 @@ struct MwfjDev {
   const double* xrows; // [nx][npts] explicit window rows
   int nx;
   const float* r;      // [nr] sorted, float32 (the reference's cast)
-  const double* rr;    // [nr] the same in double
+  const double* rr;    // [half] the same in double, padded with the last distance
+  const float* ir;     // [half] 1 / r
   const float* f;
   const float* c;
   double* xw;          // tapered traces of one chunk of windows
@@ -30,6 +31,7 @@ struct MwfjDev {
 };
 
 hipError_t launch_window(const MwfjDev& d, int w0, int cw, hipStream_t st);
+hipError_t launch_mwfj_stations(const MwfjDev& d, double* rr, float* ir, hipStream_t st);
 hipError_t launch_mwfj_coef(const MwfjDev& d, int w0, int cw, hipStream_t st);
 hipError_t launch_mwfj_mfma(const MwfjDev& d, hipStream_t st);
 hipError_t launch_mwfj_finish(const MwfjDev& d, hipStream_t st);

@@ -91,7 +91,10 @@ __global__ void k_mwfj_coef(const double2* __restrict__ spec, const float* __res
   double w = 0.0;
   if (s > 0) w += (double)fmaxf(rs - r[s - 1], 0.1f);
   if (s + 1 < nr) w += (double)fmaxf(r[s + 1] - rs, 0.1f);
-  const double wr = 0.5 * w * (double)rs;
+  // trapezoid weight x r, over sqrt(r): the kernel matrix carries
+  // sqrt(r) Z0(k r) (its large-argument amplitude without 1/sqrt r) and the
+  // contraction's epilogue multiplies by sqrt(2/(pi k))
+  const double wr = 0.5 * w * sqrt((double)rs);
   const float are = (float)(wr * (double)re), aim = (float)(wr * (double)im);
   const int w2 = 2 * (w0 + wl);
   const int Kp = func ? 2 * half : half;
@@ -105,6 +108,45 @@ __global__ void k_mwfj_coef(const double2* __restrict__ spec, const float* __res
   }
 }
 
+// padded station arrays: rr (double) and ir = 1/r (float), ntot >= nr
+// entries, padding repeating the last distance (its rows of B are zero)
+__global__ void k_mwfj_stations(const float* __restrict__ r, int nr, int ntot,
+                                double* __restrict__ rr, float* __restrict__ ir) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ntot) return;
+  const double v = (double)r[s < nr ? s : nr - 1];
+  rr[s] = v;
+  ir[s] = (float)(1.0 / v);
+}
+
+// sqrt(2/(pi k)) * (that value) = Z0(k r) sqrt(r): the kernel-matrix entry
+// of one lane (velocity) and station.  FAR: x >= kFarX on every lane of
+// the wave (degree-2 fits, no branch); otherwise the large-argument fits
+// for x >= 8 and the double polynomials below.
+template <bool Y, bool FAR>
+__device__ inline float mwfj_value(double rs, float ir, const LaneK& L) {
+  const float z = L.zk * ir;   // 8 / x
+  if (FAR || z <= 1.0f) {
+    const float t = z * z;
+    float p0, q0;
+    if (FAR || z <= 8.0f / kFarX) {
+      p0 = fitf_P0(t);
+      q0 = z * fitf_q0(t);
+    } else {
+      p0 = fit_P0(t);
+      q0 = z * fit_q0(t);
+    }
+    float sn, co;
+    sincos_rev(phase_rev(L.q, rs), &sn, &co);
+    return Y ? __builtin_fmaf(p0, sn, q0 * co) : __builtin_fmaf(p0, co, -q0 * sn);
+  }
+  const double xd = L.q2pi * rs;
+  double y, Lg;
+  const Bess b = bessel_small<false, Y, FIT_TABLE>(xd, &y, &Lg);
+  const float ia = L.kf > 0.0f ? sqrtf(kPiOver2 * (float)xd) : sqrtf((float)rs);
+  return (Y ? b.y0 : b.j0) * ia;
+}
+
 // O[c][rhs] = sum_k Z(k_c r_k) B[k][rhs] for one frequency, 128 velocities
 // (4 waves x 32) and 128 right-hand sides (4 MFMA tiles) per workgroup.
 // v_mfma_f32_32x32x2_f32 operand layout (lane l): A[i = l%32][k = l/32],
@@ -114,14 +156,24 @@ constexpr int kMwC = 128;   // velocities per workgroup
 constexpr int kMwR = 128;   // right-hand sides per workgroup
 constexpr int kMwK = 32;    // K rows per LDS chunk
 
+// waves per SIMD the contraction kernel is held to at least (a tuning
+// switch: -DFJ_MWFJ_WAVES=N)
+#ifndef FJ_MWFJ_WAVES
+#define FJ_MWFJ_WAVES 3
+#endif
+
 template <int FUNC>
-__global__ __launch_bounds__(256) void k_mwfj_mfma(const float* __restrict__ B,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_MWFJ_WAVES, 8))) void k_mwfj_mfma(const float* __restrict__ B,
                                                    const double* __restrict__ rr,
+                                                   const float* __restrict__ irr,
                                                    const float* __restrict__ fv,
                                                    const float* __restrict__ cv, int nr,
                                                    int half, int nf, int nc, int nrhs,
                                                    int nwin, float* __restrict__ T) {
-  __shared__ float Bs[kMwK][kMwR + 4];
+  // B chunks (kMwK rows x kMwR right-hand sides) double-buffered in LDS:
+  // chunk c+1 is fetched into registers while chunk c is contracted
+  __shared__ float Bs[2][kMwK][kMwR + 4];
+  constexpr int kPer = kMwK * kMwR / 256;   // B elements per thread and chunk
   const int fi = blockIdx.y;
   const int rg = blockIdx.z;   // right-hand-side group
   const int wave = threadIdx.x >> 6;
@@ -129,9 +181,8 @@ __global__ __launch_bounds__(256) void k_mwfj_mfma(const float* __restrict__ B,
   const int i = lane & 31, kq = lane >> 5;
   const int crow = blockIdx.x * kMwC + wave * 32 + i;
   const int cidx = crow < nc ? crow : nc - 1;
-  const double q = (double)fv[fi] / (double)cv[cidx];
-  const double q2pi = q * kTwoPi;
-  const int Kp = FUNC ? 2 * half : half;
+  const LaneK L = lane_consts((double)fv[fi] / (double)cv[cidx]);
+  const int Kp = FUNC ? 2 * half : half;   // a multiple of kMwK
   const float* Bf = B + (size_t)fi * Kp * nrhs + (size_t)rg * kMwR;
   const int rcols = min(kMwR, nrhs - rg * kMwR);   // multiple of 32
   floatx16 acc[4];
@@ -140,45 +191,70 @@ __global__ __launch_bounds__(256) void k_mwfj_mfma(const float* __restrict__ B,
 #pragma unroll
     for (int v = 0; v < 16; v++) acc[t][v] = 0.0f;
 
-  for (int k0 = 0; k0 < Kp; k0 += kMwK) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < kMwK * kMwR; e += 256) {
+  float nb[kPer];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = threadIdx.x + u * 256;
       const int kr = e / kMwR, cc = e - kr * kMwR;
-      Bs[kr][cc] = cc < rcols ? Bf[(size_t)(k0 + kr) * nrhs + cc] : 0.0f;
+      nb[u] = cc < rcols ? Bf[(size_t)(k0 + kr) * nrhs + cc] : 0.0f;
     }
-    __syncthreads();
+  };
+  auto stash = [&](int buf) {
+#pragma unroll
+    for (int u = 0; u < kPer; u++) {
+      const int e = threadIdx.x + u * 256;
+      const int kr = e / kMwR, cc = e - kr * kMwR;
+      Bs[buf][kr][cc] = nb[u];
+    }
+  };
+  fetch(0);
+  stash(0);
+  __syncthreads();
+  for (int k0 = 0, buf = 0; k0 < Kp; k0 += kMwK, buf ^= 1) {
+    const bool more = k0 + kMwK < Kp;
+    if (more) fetch(k0 + kMwK);
     const bool ypart = FUNC && k0 >= half;
     const int sbase = ypart ? k0 - half : k0;
-#pragma unroll 2
-    for (int kk = 0; kk < kMwK; kk += 2) {
-      int s = sbase + kk + kq;
-      s = s < nr ? s : nr - 1;   // padding rows carry zero coefficients
+    // VALU phase: the chunk's 16 kernel-matrix values of this lane, as
+    // independent chains; then the MFMA phase back to back.  Waves of the
+    // block alternate between the two, so one wave's Bessel work overlaps
+    // another's matrix work.
+    float av[kMwK / 2];
+#pragma unroll
+    for (int h = 0; h < kMwK / 2; h++) {
+      const int s = sbase + 2 * h + kq;   // < half: padded station arrays
       const double rs = rr[s];
-      const double xd = q2pi * rs;
-      const float x = (float)xd;
-      float J0 = 0.0f, Y0 = 0.0f;
-      if (x >= kSplit) {
-        if (ypart) bessel0_large<false, true>(x, reduce_phase(q, rs), &J0, &Y0);
-        else bessel0_large<true, false>(x, reduce_phase(q, rs), &J0, &Y0);
-      } else {
-        double y, L;
-        if (ypart) {
-          Bess b = bessel_small<false, true>(xd, &y, &L);
-          Y0 = b.y0;
-        } else {
-          Bess b = bessel_small<false, false>(xd, &y, &L);
-          J0 = b.j0;
-        }
-      }
-      const float a = ypart ? Y0 : J0;
+      const float ir = irr[s];
+      // far when x >= kFarX on every lane (a wave-uniform branch)
+      const bool far = !wave_any(L.zk * ir > 8.0f / kFarX);
+      if (ypart)
+        av[h] = far ? mwfj_value<true, true>(rs, ir, L) : mwfj_value<true, false>(rs, ir, L);
+      else
+        av[h] = far ? mwfj_value<false, true>(rs, ir, L) : mwfj_value<false, false>(rs, ir, L);
+    }
+#pragma unroll
+    for (int h = 0; h < kMwK / 2; h++) {
 #pragma unroll
       for (int t = 0; t < 4; t++) {
-        const float b = Bs[kk + kq][t * 32 + i];
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[t], 0, 0, 0);
+        const float b = Bs[buf][2 * h + kq][t * 32 + i];
+        acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[h], b, acc[t], 0, 0, 0);
       }
     }
+    // the other buffer was last read before the previous barrier
+    if (more) stash(buf ^ 1);
+    __syncthreads();
   }
-  // epilogue: |Re + i Im| per window; Re in even columns, Im in odd
+  // epilogue: |Re + i Im| per window (Re in even columns, Im in odd) times
+  // the amplitude sqrt(2/(pi k)) of the row's velocity (1 at k = 0)
+  float amp[16];
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    const int m = 8 * (v >> 2) + 4 * kq + (v & 3);
+    const int c = blockIdx.x * kMwC + wave * 32 + m;
+    const double k = (double)fv[fi] / (double)cv[c < nc ? c : nc - 1] * kTwoPi;
+    amp[v] = (float)k > 0.0f ? (float)sqrt(2.0 / (kPiD * k)) : 1.0f;
+  }
 #pragma unroll
   for (int t = 0; t < 4; t++) {
 #pragma unroll
@@ -190,7 +266,7 @@ __global__ __launch_bounds__(256) void k_mwfj_mfma(const float* __restrict__ B,
       const int c = blockIdx.x * kMwC + wave * 32 + m;
       if ((i & 1) == 0 && c < nc && col < 2 * nwin) {
         const int w = col >> 1;
-        T[((size_t)w * nf + fi) * nc + c] = sqrtf(mine * mine + other * other);
+        T[((size_t)w * nf + fi) * nc + c] = sqrtf(mine * mine + other * other) * amp[v];
       }
     }
   }
@@ -223,6 +299,12 @@ hipError_t launch_window(const MwfjDev& d, int w0, int cw, hipStream_t st) {
   return hipGetLastError();
 }
 
+hipError_t launch_mwfj_stations(const MwfjDev& d, double* rr, float* ir, hipStream_t st) {
+  hipLaunchKernelGGL(k_mwfj_stations, dim3((d.half + 255) / 256), dim3(256), 0, st, d.r, d.nr,
+                     d.half, rr, ir);
+  return hipGetLastError();
+}
+
 hipError_t launch_mwfj_coef(const MwfjDev& d, int w0, int cw, hipStream_t st) {
   dim3 grid((cw + 63) / 64, d.nr, d.nf);
   hipLaunchKernelGGL(k_mwfj_coef, grid, dim3(64), 0, st, (const double2*)d.spec, d.r, d.nr, w0,
@@ -233,10 +315,10 @@ hipError_t launch_mwfj_coef(const MwfjDev& d, int w0, int cw, hipStream_t st) {
 hipError_t launch_mwfj_mfma(const MwfjDev& d, hipStream_t st) {
   dim3 grid((d.nc + kMwC - 1) / kMwC, d.nf, (d.nrhs + kMwR - 1) / kMwR);
   if (d.func)
-    hipLaunchKernelGGL(k_mwfj_mfma<1>, grid, dim3(256), 0, st, d.B, d.rr, d.f, d.c, d.nr,
+    hipLaunchKernelGGL(k_mwfj_mfma<1>, grid, dim3(256), 0, st, d.B, d.rr, d.ir, d.f, d.c, d.nr,
                        d.half, d.nf, d.nc, d.nrhs, d.nwin, d.T);
   else
-    hipLaunchKernelGGL(k_mwfj_mfma<0>, grid, dim3(256), 0, st, d.B, d.rr, d.f, d.c, d.nr,
+    hipLaunchKernelGGL(k_mwfj_mfma<0>, grid, dim3(256), 0, st, d.B, d.rr, d.ir, d.f, d.c, d.nr,
                        d.half, d.nf, d.nc, d.nrhs, d.nwin, d.T);
   return hipGetLastError();
 }
