@@ -46,7 +46,11 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
   d.nc = nc;
   d.nwin = nwin;
   d.fstride = fstride;
-  d.nbins = npts / 2 + 1;
+  // even trace length: a half-length complex FFT of the packed samples,
+  // the used bins formed from it by the coefficient kernel (no pass over
+  // the npts/2 + 1 bins of a full rFFT)
+  d.zhalf = npts % 2 == 0 && npts >= 2;
+  d.nbins = d.zhalf ? npts / 2 : npts / 2 + 1;
   d.func = func;
   d.half = (nr + 31) / 32 * 32;
   d.nrhs = (2 * nwin + 31) / 32 * 32;
@@ -106,9 +110,18 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
     const int cwn = std::min(cw, nwin - w0);
     if ((e = fjk::launch_window(d, w0, cwn, st)) != hipSuccess) return hip_fail(e, "window kernel");
     hipfftHandle plan = 0;
-    if ((rc = ccfj_internal::cached_fft_plan(npts, cwn * nr, HIPFFT_D2Z, st, &plan))) return rc;
-    if (hipfftExecD2Z(plan, (hipfftDoubleReal*)dxw, (hipfftDoubleComplex*)dspec) != HIPFFT_SUCCESS)
-      return fail(CCFJ_EDEVICE, "hipfftExecD2Z failed");
+    if (d.zhalf) {
+      if ((rc = ccfj_internal::cached_fft_plan(npts / 2, cwn * nr, HIPFFT_Z2Z, st, &plan)))
+        return rc;
+      if (hipfftExecZ2Z(plan, (hipfftDoubleComplex*)dxw, (hipfftDoubleComplex*)dspec,
+                        HIPFFT_FORWARD) != HIPFFT_SUCCESS)
+        return fail(CCFJ_EDEVICE, "hipfftExecZ2Z failed");
+    } else {
+      if ((rc = ccfj_internal::cached_fft_plan(npts, cwn * nr, HIPFFT_D2Z, st, &plan))) return rc;
+      if (hipfftExecD2Z(plan, (hipfftDoubleReal*)dxw, (hipfftDoubleComplex*)dspec) !=
+          HIPFFT_SUCCESS)
+        return fail(CCFJ_EDEVICE, "hipfftExecD2Z failed");
+    }
     if (itype == 0) {
       if ((e = fjk::launch_mwfj_coef(d, w0, cwn, st)) != hipSuccess) return hip_fail(e, "coef kernel");
     } else {

@@ -97,8 +97,9 @@ int cached_fft_plan(int n, int batch, hipfftType type, hipStream_t st, hipfftHan
     int nn[1] = {n};
     const int nb = n / 2 + 1;
     const bool fwd = type == HIPFFT_D2Z || type == HIPFFT_R2C;
-    const hipfftResult r = hipfftPlanMany(&p, 1, nn, nullptr, 1, fwd ? n : nb, nullptr, 1,
-                                          fwd ? nb : n, type, batch);
+    const bool c2c = type == HIPFFT_Z2Z || type == HIPFFT_C2C;
+    const hipfftResult r = hipfftPlanMany(&p, 1, nn, nullptr, 1, (fwd || c2c) ? n : nb, nullptr,
+                                          1, (fwd && !c2c) ? nb : n, type, batch);
     if (r != HIPFFT_SUCCESS)
       return fail(CCFJ_EDEVICE, "hipfftPlanMany (n=%d, batch=%d, type=%d) failed", n, batch,
                   (int)type);

@@ -6,7 +6,8 @@ namespace fjk {
 
 struct MwfjDev {
   int nr, npts, nf, nc, nwin;
-  int fstride, nbins;  // rFFT bins = npts/2 + 1
+  int fstride, nbins;  // spec row length: npts/2 + 1 rFFT bins, or npts/2 (zhalf)
+  int zhalf;           // spec rows hold the half-length complex DFT (even npts)
   int func;
   int half;            // K rows per basis half (nr rounded up to 32)
   int nrhs;            // 2 nwin rounded up to 32

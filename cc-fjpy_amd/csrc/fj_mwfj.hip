@@ -10,7 +10,10 @@
 //   k_window        tapered traces in double (window from its integer
 //                   description n1, n2, delta; degenerate windows are passed
 //                   as explicit rows), a chunk of windows at a time
-//   hipFFT D2Z      the rFFT, double precision like numpy's
+//   hipFFT          the rFFT, double precision like numpy's: for an even
+//                   trace length a length-npts/2 complex FFT (Z2Z) of the
+//                   packed samples, the used bins formed from it on read
+//                   (rfft_bin); else D2Z
 //   k_mwfj_coef     bins -> float32 spectra (the reference's cast) ->
 //                   trapezoid coefficients B[f][k][rhs], rhs = (window,
 //                   Re/Im); for func 1 the K axis holds a J half and a Y half
@@ -74,18 +77,35 @@ __global__ void k_window(const double* __restrict__ u, const int* __restrict__ n
   }
 }
 
+// Bin k of the real DFT of a row of npts = 2M samples.  zhalf = false: row
+// holds the rFFT itself (M + 1 bins).  zhalf = true: row holds the length-M
+// complex DFT Z of z[n] = x[2n] + i x[2n+1], and for k <= M
+//   X[k] = E[k] + e^{-i pi k / M} O[k],
+//   E[k] = (Z[k] + conj Z[M-k]) / 2,  O[k] = (Z[k] - conj Z[M-k]) / (2i),
+// with Z[M] = Z[0]: only the bins that are used are formed.
+__device__ inline double2 rfft_bin(const double2* __restrict__ row, int k, int M, bool zhalf) {
+  if (!zhalf) return row[k];
+  const double2 a = row[k < M ? k : 0];
+  const double2 b = row[k == 0 ? 0 : M - k];   // Z[M-k], Z[M] = Z[0]
+  const double er = 0.5 * (a.x + b.x), ei = 0.5 * (a.y - b.y);
+  const double orr = 0.5 * (a.y + b.y), oi = -0.5 * (a.x - b.x);
+  double sn, cs;
+  sincospi((double)k / (double)M, &sn, &cs);   // e^{-i pi k/M} = cs - i sn
+  return make_double2(er + cs * orr + sn * oi, ei + cs * oi - sn * orr);
+}
+
 // spectra -> float32 staging -> trapezoid coefficients.
 // B layout [nf][Kp][nrhs]; rhs 2w = Re, 2w+1 = Im of window w.
 // func 0: rows k = s.  func 1: rows s (J half: Re, Im) and half + s
 // (Y half: -Im, Re), so O_re = FJ[Re] - FY[Im], O_im = FJ[Im] + FY[Re].
 __global__ void k_mwfj_coef(const double2* __restrict__ spec, const float* __restrict__ r,
-                            int nr, int w0, int cw, int nf, int fstride, int nbins,
+                            int nr, int w0, int cw, int nf, int fstride, int nbins, int zhalf,
                             int func, int half, int nrhs, float* __restrict__ B) {
   const int wl = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = blockIdx.y;
   const int f = blockIdx.z;
   if (wl >= cw) return;
-  const double2 z = spec[((size_t)wl * nr + s) * nbins + (size_t)f * fstride];
+  const double2 z = rfft_bin(spec + ((size_t)wl * nr + s) * nbins, f * fstride, nbins, zhalf);
   const float re = (float)z.x, im = (float)z.y;   // the reference's float32 staging
   const float rs = r[s];
   double w = 0.0;
@@ -275,12 +295,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_MWFJ_WAV
 // float32 staging of one window's spectra for the itype-1 path:
 // re/im [nr][nf]
 __global__ void k_mwfj_stage(const double2* __restrict__ spec, int nr, int wl, int nf,
-                             int fstride, int nbins, float* __restrict__ re,
+                             int fstride, int nbins, int zhalf, float* __restrict__ re,
                              float* __restrict__ im) {
   const int f = blockIdx.x * blockDim.x + threadIdx.x;
   const int s = blockIdx.y;
   if (f >= nf) return;
-  const double2 z = spec[((size_t)wl * nr + s) * nbins + (size_t)f * fstride];
+  const double2 z = rfft_bin(spec + ((size_t)wl * nr + s) * nbins, f * fstride, nbins, zhalf);
   re[(size_t)s * nf + f] = (float)z.x;
   im[(size_t)s * nf + f] = (float)z.y;
 }
@@ -308,7 +328,7 @@ hipError_t launch_mwfj_stations(const MwfjDev& d, double* rr, float* ir, hipStre
 hipError_t launch_mwfj_coef(const MwfjDev& d, int w0, int cw, hipStream_t st) {
   dim3 grid((cw + 63) / 64, d.nr, d.nf);
   hipLaunchKernelGGL(k_mwfj_coef, grid, dim3(64), 0, st, (const double2*)d.spec, d.r, d.nr, w0,
-                     cw, d.nf, d.fstride, d.nbins, d.func, d.half, d.nrhs, d.B);
+                     cw, d.nf, d.fstride, d.nbins, d.zhalf, d.func, d.half, d.nrhs, d.B);
   return hipGetLastError();
 }
 
@@ -330,7 +350,7 @@ hipError_t launch_mwfj_finish(const MwfjDev& d, hipStream_t st) {
 hipError_t launch_mwfj_stage(const MwfjDev& d, int wl, float* re, float* im, hipStream_t st) {
   dim3 grid((d.nf + 255) / 256, d.nr);
   hipLaunchKernelGGL(k_mwfj_stage, grid, dim3(256), 0, st, (const double2*)d.spec, d.nr, wl,
-                     d.nf, d.fstride, d.nbins, re, im);
+                     d.nf, d.fstride, d.nbins, d.zhalf, re, im);
   return hipGetLastError();
 }
 

@@ -194,6 +194,22 @@ def test_mwfj_wrapper(ccfj, itype, func):
         assert mism == 0 and emax <= TOL, (w, emax)
 
 
+@pytest.mark.parametrize("npts", [511, 512])
+def test_mwfj_trace_length_parity(ccfj, npts):
+    """Odd trace lengths take the full rFFT (D2Z), even ones the half-length
+    complex FFT with the used bins formed on read; both against the oracle."""
+    u, r, c, f, Fs = _quake(nr=20, npts=npts, seed=5)
+    f = f[:100]
+    nwin = 3
+    winl = np.stack([r / 4500.0 + w for w in range(nwin)])
+    winr = winl + 9.0
+    got = ccfj.MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=0, func=1)
+    want = O.MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=0, func=1, engine="fp64")
+    for w in range(nwin):
+        emax, _, mism = O.column_error(got[w], want[w])
+        assert mism == 0 and emax <= TOL, (w, emax)
+
+
 def test_mwfj_many_windows_mfma_tiles(ccfj):
     """More than 64 windows: the right-hand sides span two MFMA groups."""
     u, r, c, f, Fs = _quake(nr=16, npts=256, seed=9)
