@@ -59,7 +59,10 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
   const int fold = (fstride > 1 && fftlen % fstride == 0) ? fstride : 1;
   const int L = fftlen / fold;
   const int bstride = fold > 1 ? 1 : fstride;
-  const int nbins = L / 2 + 1;
+  // even L: a length-L/2 complex FFT of the packed samples, the used bins
+  // formed from it on read
+  const int zhalf = L % 2 == 0 && L >= 2;
+  const int nbins = zhalf ? L / 2 : L / 2 + 1;   // spec row length
   // windows per batch: gathered segments, their spectra and the extracted
   // bins within 4 GiB (and within one grid dimension)
   const size_t per_win = (size_t)nsta * ((size_t)L * 4 + (size_t)nbins * 8 + (size_t)nf * 8);
@@ -119,13 +122,20 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
                                    st)) != hipSuccess)
       return hip_fail(e, "gather kernel");
     hipfftHandle plan = 0;
-    if ((rc = ccfj_internal::cached_fft_plan(L, n * nsta, HIPFFT_R2C, st, &plan))) return rc;
-    if (hipfftExecR2C(plan, (hipfftReal*)dseg, (hipfftComplex*)dspec) != HIPFFT_SUCCESS)
-      return fail(CCFJ_EDEVICE, "hipfftExecR2C failed");
+    if (zhalf) {
+      if ((rc = ccfj_internal::cached_fft_plan(L / 2, n * nsta, HIPFFT_C2C, st, &plan))) return rc;
+      if (hipfftExecC2C(plan, (hipfftComplex*)dseg, (hipfftComplex*)dspec, HIPFFT_FORWARD) !=
+          HIPFFT_SUCCESS)
+        return fail(CCFJ_EDEVICE, "hipfftExecC2C failed");
+    } else {
+      if ((rc = ccfj_internal::cached_fft_plan(L, n * nsta, HIPFFT_R2C, st, &plan))) return rc;
+      if (hipfftExecR2C(plan, (hipfftReal*)dseg, (hipfftComplex*)dspec) != HIPFFT_SUCCESS)
+        return fail(CCFJ_EDEVICE, "hipfftExecR2C failed");
+    }
     if (gram) {
       e = fjk::launch_cc_fill_i32(dflag, 1, 0, st);
       if (e == hipSuccess)
-        e = fjk::launch_cc_spec_t(dspec, n * nsta, nsta, nbins, nf, bstride, whiten, duse,
+        e = fjk::launch_cc_spec_t(dspec, n * nsta, nsta, nbins, zhalf, nf, bstride, whiten, duse,
                                   (int)w0, dS, dflag, st);
       if (e == hipSuccess)
         e = fjk::launch_cc_gram(dS, nsta, nf, n, ntile, dtiles, dpidx, dflag, dacc, st);
@@ -134,7 +144,7 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
                                    dacc, st);
       if (e != hipSuccess) return hip_fail(e, "Gram pair stage");
     } else {
-      if ((e = fjk::launch_cc_spec(dspec, n * nsta, nbins, nf, bstride, whiten, dS, st)) !=
+      if ((e = fjk::launch_cc_spec(dspec, n * nsta, nbins, zhalf, nf, bstride, whiten, dS, st)) !=
           hipSuccess)
         return hip_fail(e, "spectrum kernel");
       if ((e = fjk::launch_cc_pairs(dS, duse, pairs, npairs, nsta, nf, (int)w0, n, dacc, st)) !=

@@ -8,17 +8,17 @@ namespace fjk {
 hipError_t launch_cc_onebit(float* x, size_t n, hipStream_t st);
 hipError_t launch_cc_gather(const float* data, int nsta, int npts, int L, int fold, int steplen,
                             int w0, int cw, float* seg, hipStream_t st);
-hipError_t launch_cc_spec(const void* spec, int rows, int nbins, int nf, int fstride, int whiten,
-                          void* S, hipStream_t st);
+hipError_t launch_cc_spec(const void* spec, int rows, int nbins, int zhalf, int nf, int fstride,
+                          int whiten, void* S, hipStream_t st);
 hipError_t launch_cc_use(const int* startend, int nsta, int nshifts, int steplen, int fftlen,
                          unsigned char* use, hipStream_t st);
 hipError_t launch_cc_count(const unsigned char* use, const int* pairs, int npairs, int nsta,
                            int nshifts, float* count, hipStream_t st);
 hipError_t launch_cc_pairs(const void* S, const unsigned char* use, const int* pairs, int npairs,
                            int nsta, int nf, int w0, int cw, void* acc, hipStream_t st);
-hipError_t launch_cc_spec_t(const void* spec, int rows, int nsta, int nbins, int nf, int fstride,
-                            int whiten, const unsigned char* use, int w0, void* S2, int* nonfinite,
-                            hipStream_t st);
+hipError_t launch_cc_spec_t(const void* spec, int rows, int nsta, int nbins, int zhalf, int nf,
+                            int fstride, int whiten, const unsigned char* use, int w0, void* S2,
+                            int* nonfinite, hipStream_t st);
 hipError_t launch_cc_pair_map(const int* pairs, int npairs, int nsta, int ntile, int* pidx,
                               unsigned char* dup, unsigned char* tiles, hipStream_t st);
 hipError_t launch_cc_gram(const void* S2, int nsta, int nf, int cw, int ntile,

@@ -12,7 +12,9 @@
 //   data  f32 [nsta][npts]            (one-bit: replaced by its sign first)
 //   seg   f32 [cw][nsta][L]           windows of a chunk, gathered (L = fftlen,
 //                                     or fftlen / fstride when folded)
-//   spec  c64 [cw][nsta][L/2+1]       their rFFT (hipFFT, single precision)
+//   spec  c64 [cw][nsta][L/2 (+1)]    their rFFT (hipFFT, single precision): for
+//                                     even L the length-L/2 complex DFT of
+//                                     the packed samples, bins formed on read
 //   S     c64 [cw][nsta][nf]          bins j*fstride, whitened
 //   acc   c64 [npairs][nf]            running sums over windows
 //   use   u8  [nshifts][nsta]         station has the whole window
@@ -52,13 +54,30 @@ __global__ void k_cc_gather(const float* __restrict__ data, int npts, int L, int
   seg[((size_t)wl * gridDim.y + i) * L + t] = v;
 }
 
+// Bin k of the real DFT of a row of L = 2M samples.  zhalf = 0: row holds
+// the rFFT itself (M + 1 bins).  zhalf = 1: row holds the length-M complex
+// DFT Z of z[n] = x[2n] + i x[2n+1]; for k <= M
+//   X[k] = E[k] + e^{-i pi k / M} O[k],
+//   E[k] = (Z[k] + conj Z[M-k]) / 2,  O[k] = (Z[k] - conj Z[M-k]) / (2i),
+// with Z[M] = Z[0]: only the bins that are used are formed.
+__device__ inline float2 rfft_bin_f(const float2* __restrict__ row, int k, int M, int zhalf) {
+  if (!zhalf) return row[k];
+  const float2 a = row[k < M ? k : 0];
+  const float2 b = row[k == 0 ? 0 : M - k];
+  const float er = 0.5f * (a.x + b.x), ei = 0.5f * (a.y - b.y);
+  const float orr = 0.5f * (a.y + b.y), oi = -0.5f * (a.x - b.x);
+  float sn, cs;
+  sincospif((float)k / (float)M, &sn, &cs);   // e^{-i pi k/M} = cs - i sn
+  return make_float2(er + cs * orr + sn * oi, ei + cs * oi - sn * orr);
+}
+
 // S[row][j] = spec[row][j * fstride] (whitened); rows = cw * nsta
-__global__ void k_cc_spec(const float2* __restrict__ spec, int nbins, int nf, int fstride,
-                          int whiten, float2* __restrict__ S) {
+__global__ void k_cc_spec(const float2* __restrict__ spec, int nbins, int zhalf, int nf,
+                          int fstride, int whiten, float2* __restrict__ S) {
   const int j = blockIdx.y * blockDim.x + threadIdx.x;
   const size_t row = blockIdx.x;
   if (j >= nf) return;
-  float2 v = spec[row * nbins + (size_t)j * fstride];
+  float2 v = rfft_bin_f(spec + row * nbins, j * fstride, nbins, zhalf);
   if (whiten) {
     // as the reference: |X| in float, floored at 1e-8, then a division
     const float a = fmaxf(sqrtf(v.x * v.x + v.y * v.y), 1e-8f);
@@ -136,7 +155,8 @@ __global__ __launch_bounds__(256) void k_cc_pairs(const float2* __restrict__ S,
 // whitened, zero where the station lacks the window; flags a non-finite
 // used value.  32 x 32 (rows x bins) tiles through LDS.
 __global__ __launch_bounds__(256) void k_cc_spec_t(const float2* __restrict__ spec, int rows,
-                                                   int nsta, int nbins, int nf, int fstride,
+                                                   int nsta, int nbins, int zhalf, int nf,
+                                                   int fstride,
                                                    int whiten, const unsigned char* __restrict__ use,
                                                    int w0, float2* __restrict__ S2,
                                                    int* __restrict__ nonfinite) {
@@ -150,7 +170,7 @@ __global__ __launch_bounds__(256) void k_cc_spec_t(const float2* __restrict__ sp
     if (row < rows && j < nf) {
       const int wl = row / nsta, st = row - wl * nsta;
       if (use[(size_t)(w0 + wl) * nsta + st]) {
-        v = spec[(size_t)row * nbins + (size_t)j * fstride];
+        v = rfft_bin_f(spec + (size_t)row * nbins, j * fstride, nbins, zhalf);
         if (whiten) {
           const float a = fmaxf(sqrtf(v.x * v.x + v.y * v.y), 1e-8f);
           v.x = v.x / a;
@@ -319,10 +339,10 @@ hipError_t launch_cc_gather(const float* data, int nsta, int npts, int L, int fo
   return hipGetLastError();
 }
 
-hipError_t launch_cc_spec(const void* spec, int rows, int nbins, int nf, int fstride, int whiten,
-                          void* S, hipStream_t st) {
+hipError_t launch_cc_spec(const void* spec, int rows, int nbins, int zhalf, int nf, int fstride,
+                          int whiten, void* S, hipStream_t st) {
   hipLaunchKernelGGL(k_cc_spec, dim3(rows, (nf + 255) / 256), dim3(256), 0, st,
-                     (const float2*)spec, nbins, nf, fstride, whiten, (float2*)S);
+                     (const float2*)spec, nbins, zhalf, nf, fstride, whiten, (float2*)S);
   return hipGetLastError();
 }
 
@@ -348,11 +368,11 @@ hipError_t launch_cc_pairs(const void* S, const unsigned char* use, const int* p
   return hipGetLastError();
 }
 
-hipError_t launch_cc_spec_t(const void* spec, int rows, int nsta, int nbins, int nf, int fstride,
-                            int whiten, const unsigned char* use, int w0, void* S2, int* nonfinite,
-                            hipStream_t st) {
+hipError_t launch_cc_spec_t(const void* spec, int rows, int nsta, int nbins, int zhalf, int nf,
+                            int fstride, int whiten, const unsigned char* use, int w0, void* S2,
+                            int* nonfinite, hipStream_t st) {
   hipLaunchKernelGGL(k_cc_spec_t, dim3((rows + 31) / 32, (nf + 31) / 32), dim3(256), 0, st,
-                     (const float2*)spec, rows, nsta, nbins, nf, fstride, whiten, use, w0,
+                     (const float2*)spec, rows, nsta, nbins, zhalf, nf, fstride, whiten, use, w0,
                      (float2*)S2, nonfinite);
   return hipGetLastError();
 }
