@@ -281,6 +281,8 @@ int ccfj_transform_batched(ccfj_ctx* ctx, int mode, int itype, int normalize, in
                            const long long* soff, const float* U0, const float* U1,
                            const float* r, const float* f, const float* c, int nf, int nc,
                            float* out) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  auto lk = ccfj_internal::ctx_lock(ctx);
   ccfj_batch* p = nullptr;
   int rc = ccfj_batch_create(ctx, mode, itype, normalize, nb, soff, nf, nc, &p);
   if (rc) return rc;

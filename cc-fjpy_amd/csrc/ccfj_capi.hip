@@ -78,6 +78,7 @@ struct ccfj_ctx {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> spare;
   double timed_ms = 0.0;
   int timed_launches = 0;
+  std::recursive_mutex mu;              // host-pointer entry points (ctx_lock)
   void* pin[2] = {nullptr, nullptr};   // pinned staging ring (ccfj_stage.hip)
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
 };
@@ -197,7 +198,6 @@ int ref_entry(int mode, float* u_f, float* r, float* f, float* out, float* c, in
   ccfj_ctx* ctx;
   int rc = default_ctx(&ctx);
   if (rc) return rc;
-  std::lock_guard<std::mutex> lk(g_default_mu);
   return ccfj_transform(ctx, mode, type, 0, u_f, nullptr, r, f, c, nr, nf, nc, out);
 }
 
@@ -227,6 +227,10 @@ int ctx_scratch(ccfj_ctx* ctx, int which, size_t bytes, void** p) {
 hipStream_t ctx_stream(const ccfj_ctx* ctx) { return ctx->stream; }
 
 int default_context(ccfj_ctx** out) { return default_ctx(out); }
+
+std::unique_lock<std::recursive_mutex> ctx_lock(ccfj_ctx* ctx) {
+  return std::unique_lock<std::recursive_mutex>(ctx->mu);
+}
 
 constexpr size_t kPinChunk = (size_t)32 << 20;
 
@@ -350,6 +354,7 @@ int ccfj_transform(ccfj_ctx* ctx, int mode, int itype, int normalize, const floa
                    const float* U1, const float* r, const float* f, const float* c, int nr,
                    int nf, int nc, float* out) {
   if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  auto lk = ccfj_internal::ctx_lock(ctx);
   int rc = check_args(mode, itype, U0, U1, r, f, c, nr, nf, nc, out);
   if (rc) return rc;
   if (nf == 0 || nc == 0) return CCFJ_OK;

@@ -164,6 +164,7 @@ int cross_correlate(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int 
                     const int* startend, const float* data, float* ncfsr, float* ncfsi,
                     bool accumulate) {
   if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  auto lk = ccfj_internal::ctx_lock(ctx);
   int rc = check_sizes(nsta, npts, nf, fstride, fftlen, steplen, npairs);
   if (rc) return rc;
   const bool interleaved = ncfsi == nullptr;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
+#include <mutex>
 #include <utility>
 
 #include "../../include/ccfj.h"
@@ -32,6 +33,10 @@ int cached_fft_plan(int n, int batch, hipfftType type, hipStream_t st, hipfftHan
 int timing_begin(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t>* ev);
 void timing_end(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t> ev);
 
+// Held by the host-pointer (synchronous) entry points for the whole call,
+// so threads sharing a context (the Python API's per-device default) run
+// them one at a time, as the reference's GIL-holding calls did.
+std::unique_lock<std::recursive_mutex> ctx_lock(ccfj_ctx* ctx);
 // The context's pinned staging ring: two host buffers of *chunk bytes and
 // an event for each (allocated on first use, freed with the context).
 int ctx_pinned_ring(ccfj_ctx* ctx, void* buf[2], hipEvent_t ev[2], size_t* chunk);

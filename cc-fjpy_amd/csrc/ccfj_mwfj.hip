@@ -171,6 +171,7 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
                          const int* delta, const int* xidx, const double* xrows, int nx,
                          double* out) {
   if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  auto lk = ccfj_internal::ctx_lock(ctx);
   int rc = check_sizes(itype, func, nr, npts, nwin, nf, nc, fstride, nx);
   if (rc) return rc;
   if (!u || !r || !f || !c || !n1 || !n2 || !delta || !xidx || !out || (nx > 0 && !xrows))

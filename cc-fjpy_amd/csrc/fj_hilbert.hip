@@ -169,6 +169,7 @@ extern "C" int ccfj_fj_noise(ccfj_ctx* ctx, int itype, int func, int fstride, co
                              const float* r, const float* f, const float* c, int nr, int ncol,
                              int nf, int nc, float* out) {
   if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  auto lk = ccfj_internal::ctx_lock(ctx);
   if (func != 0 && func != 1) return fail(CCFJ_EINVAL, "func must be 0 or 1, got %d", func);
   if (itype != 0 && itype != 1) return fail(CCFJ_EINVAL, "itype must be 0 or 1, got %d", itype);
   if (fstride < 1 || nr < 0 || ncol < 0 || nf < 0 || nc < 0)

@@ -258,6 +258,29 @@ def test_mwfj_device_pointers_match_host_entry(ccfj, ctx, itype):
         ctx.mwfj(itype, 0, 1, us, rs, f32, c32, n1, n2, dl, bad, xrows)
 
 
+def test_threads_share_the_default_context(ccfj):
+    """Python threads calling the drop-in API at once (the GIL is released
+    in the native call) get the results of serial calls: host-pointer entry
+    points hold the context's lock."""
+    import threading
+    u, r, c, f, _ = _quake(nr=40, npts=512, seed=11)
+    spec = np.fft.rfft(u, axis=1)[:, :100].astype(np.complex64)
+    jobs = [(itype, func) for itype in (0, 1) for func in (0, 1)] * 2
+    serial = {j: ccfj.fj_noise(spec, r, c, f[:100], itype=j[0], func=j[1]) for j in jobs[:4]}
+    got = {}
+
+    def work(k, j):
+        got[k] = ccfj.fj_noise(spec, r, c, f[:100], itype=j[0], func=j[1])
+
+    ts = [threading.Thread(target=work, args=(k, j)) for k, j in enumerate(jobs)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    for k, j in enumerate(jobs):
+        assert np.array_equal(got[k], serial[j]), (k, j)
+
+
 # ------------------------------------------------------------- edge cases
 def test_edges_and_errors(ccfj, ctx):
     r = np.array([1000.0, 1000.0, 1000.05, 3000.0, 9000.0], np.float32)
