@@ -210,6 +210,31 @@ def test_mwfj_trace_length_parity(ccfj, npts):
         assert mism == 0 and emax <= TOL, (w, emax)
 
 
+def test_mwfj_full_size_sampled(ccfj):
+    """MWFJ at the active-source size (480 receivers x 16,384 samples, 64
+    windows, 820 f, 2000 c) on the GPU; three windows and five frequency
+    columns against the FP64 oracle (the reference's window, numpy's rFFT,
+    the FP64 trapezoid sum, |Re + i Im|, per-column normalisation)."""
+    from ccfj import synth
+    d = synth.active_source(seed=2)
+    u, r, f, c, Fs = d["u"], d["r"], d["f"], d["c"], d["Fs"]
+    nr, npts, nwin = len(r), u.shape[1], d["nwin"]
+    got = ccfj.MWFJ(u, r, c, f, Fs, nwin, d["winl"], d["winr"], 0.9, 1, 0, 0)
+    assert got.shape == (nwin, len(c), len(f)) and got.dtype == np.float64
+    cols = np.array([1, 100, 300, 550, len(f) - 1])
+    idx = np.argsort(r)
+    for w in (0, 31, nwin - 1):
+        tap = np.stack([O.win(npts, Fs, d["winl"][w, j], d["winr"][w, j], 0.9) for j in range(nr)])
+        spec = np.fft.rfft(u.astype(np.float64) * tap, axis=1)[idx][:, :len(f)]
+        re = O.oracle64(np.ascontiguousarray(spec.real, np.float32), r[idx], f, c, 0, 0, fidx=cols)
+        im = O.oracle64(np.ascontiguousarray(spec.imag, np.float32), r[idx], f, c, 0, 0, fidx=cols)
+        want = np.sqrt(re.astype(np.float64) ** 2 + im.astype(np.float64) ** 2)
+        want = want / np.max(want, axis=0, keepdims=True)
+        emax, erms, mism = O.column_error(got[w][:, cols], want)
+        print(f"MWFJ window {w}: emax={emax:.2e} erms={erms:.2e}")
+        assert mism == 0 and emax <= TOL, (w, emax)
+
+
 def test_mwfj_many_windows_mfma_tiles(ccfj):
     """More than 64 windows: the right-hand sides span two MFMA groups."""
     u, r, c, f, Fs = _quake(nr=16, npts=256, seed=9)
