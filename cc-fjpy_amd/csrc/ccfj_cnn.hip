@@ -52,7 +52,9 @@ struct ccfj_cnn {
   float* g = nullptr;       // gradients
   float* m = nullptr;       // Adam moments
   float* v = nullptr;
-  uint16_t* pb = nullptr;   // bf16 copy (weights and biases; kernels use the weight part)
+  uint16_t* pb = nullptr;   // bf16 copy (weights and biases)
+  float* pw = nullptr;      // the bf16 values widened to fp32 (the convs read the weights)
+  float* part = nullptr;    // bias-gradient partial sums
   float* act[kLayers] = {};  // layer outputs [B][cout][H][W]
   float* gA = nullptr;      // gradient buffers [B][32][H][W]
   float* gB = nullptr;
@@ -70,7 +72,7 @@ int alloc(ccfj_cnn* n, size_t bytes, void** out) {
 int forward(ccfj_cnn* n, const float* x, int B, hipStream_t st) {
   const float* in = x;
   for (int l = 0; l < kLayers; l++) {
-    hipError_t e = fjk::cnn_conv_forward(kCin[l], kCout[l], in, n->pb + kOff.w[l], n->p + kOff.b[l],
+    hipError_t e = fjk::cnn_conv_forward(kCin[l], kCout[l], in, n->pw + kOff.w[l], n->p + kOff.b[l],
                                          B, n->H, n->W, l < kLayers - 1 ? 1 : 0, n->act[l], st);
     if (e != hipSuccess) return hip_fail(e, "conv forward");
     in = n->act[l];
@@ -95,10 +97,10 @@ int backward(ccfj_cnn* n, const float* x, const float* target, int B, hipStream_
     e = fjk::cnn_conv_backward_weight(kCin[l], kCout[l], in, gcur, mask, B, n->H, n->W,
                                       n->g + kOff.w[l], st);
     if (e != hipSuccess) return hip_fail(e, "conv backward weight");
-    e = fjk::cnn_bias_grad(gcur, mask, B, kCout[l], HW, n->g + kOff.b[l], st);
+    e = fjk::cnn_bias_grad(gcur, mask, B, kCout[l], HW, n->part, n->g + kOff.b[l], st);
     if (e != hipSuccess) return hip_fail(e, "bias grad");
     if (l > 0) {
-      e = fjk::cnn_conv_backward_data(kCin[l], kCout[l], gcur, mask, n->pb + kOff.w[l], B, n->H,
+      e = fjk::cnn_conv_backward_data(kCin[l], kCout[l], gcur, mask, n->pw + kOff.w[l], B, n->H,
                                       n->W, gnext, st);
       if (e != hipSuccess) return hip_fail(e, "conv backward data");
       float* t = gcur;
@@ -147,6 +149,9 @@ int ccfj_cnn_create(ccfj_ctx* ctx, int H, int W, int max_batch, unsigned long lo
   if (!rc && !(rc = alloc(n, sizeof(float) * P, &q))) n->m = (float*)q;
   if (!rc && !(rc = alloc(n, sizeof(float) * P, &q))) n->v = (float*)q;
   if (!rc && !(rc = alloc(n, sizeof(uint16_t) * P, &q))) n->pb = (uint16_t*)q;
+  if (!rc && !(rc = alloc(n, sizeof(float) * P, &q))) n->pw = (float*)q;
+  if (!rc && !(rc = alloc(n, sizeof(float) * fjk::cnn_bias_part_floats(kC), &q)))
+    n->part = (float*)q;
   for (int l = 0; l < kLayers && !rc; l++)
     if (!(rc = alloc(n, sizeof(float) * (size_t)max_batch * kCout[l] * H * W, &q))) n->act[l] = (float*)q;
   if (!rc && !(rc = alloc(n, sizeof(float) * A, &q))) n->gA = (float*)q;
@@ -190,7 +195,7 @@ int ccfj_cnn_set_params(ccfj_cnn* n, const float* params) {
   e = hipMemcpyAsync(n->p, params, sizeof(float) * P, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) e = fjk::launch_fill(n->m, P, 0.0f, st);
   if (e == hipSuccess) e = fjk::launch_fill(n->v, P, 0.0f, st);
-  if (e == hipSuccess) e = fjk::cnn_to_bf16(n->p, (int)P, n->pb, st);
+  if (e == hipSuccess) e = fjk::cnn_to_bf16(n->p, (int)P, n->pb, n->pw, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   n->step = 0;
   return e == hipSuccess ? CCFJ_OK : hip_fail(e, "set params");
@@ -233,7 +238,7 @@ int ccfj_cnn_train_step(ccfj_cnn* n, const float* x, const float* target, int B,
   hipStream_t st = ccfj_internal::ctx_stream(n->ctx);
   if ((rc = forward(n, x, B, st)) || (rc = backward(n, x, target, B, st))) return rc;
   n->step++;
-  hipError_t e = fjk::cnn_adam(n->p, n->g, n->m, n->v, n->pb, (int)kOff.total, lr, 0.9f, 0.999f,
+  hipError_t e = fjk::cnn_adam(n->p, n->g, n->m, n->v, n->pb, n->pw, (int)kOff.total, lr, 0.9f, 0.999f,
                                1e-8f, n->step, st);
   if (e != hipSuccess) return hip_fail(e, "adam");
   if (loss) {

@@ -16,7 +16,7 @@
 //                        BWD = true : dx = conv^T(dy * mask, w) -- the
 //                        gradient w.r.t. the input; mask = (y_prev > 0)
 //   k_conv_dw<CI, CO>    dW = sum over pixels of (dy * mask) x-patch
-//   k_bias_grad          dB = sum over pixels of (dy * mask)
+//   k_bias_grad_part/final  dB = sum over pixels of (dy * mask), two passes
 //   k_bce                logits -> loss sum and dL/dlogit (mean BCE)
 //   k_adam               Adam on the fp32 master, bf16 copy rewritten
 #include <hip/hip_runtime.h>
@@ -38,13 +38,15 @@ __device__ inline uint16_t f32_to_bf16(float f) {
 
 constexpr int TH = 16, TW = 16;
 
-// Weight of the forward layer w[co][ci][kh][kw] (bf16).  In the backward
+// Weight of the forward layer w[co][ci][kh][kw]: the bf16 weights widened
+// to fp32 (exactly their bf16 values; the kernels then take them as
+// wave-uniform fp32 operands with no conversion per use).  In the backward
 // (input-gradient) pass the roles swap: output channel = forward ci, input
 // channel = forward co, taps flipped.
 template <int CI, int CO, bool BWD>
 __global__ __launch_bounds__(256) void k_conv(const float* __restrict__ x,
                                              const float* __restrict__ mask,
-                                             const uint16_t* __restrict__ w,
+                                             const float* __restrict__ w,
                                              const float* __restrict__ bias, int H, int W,
                                              int relu, float* __restrict__ y) {
   __shared__ float xs[CI][TH + 2][TW + 2];
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(256) void k_conv(const float* __restrict__ x,
           // forward: w[co][ci][kh][kw]; backward: w[ci][co][2-kh][2-kw]
           const int widx = BWD ? ((ci * CO + co) * 3 + (2 - kh)) * 3 + (2 - kw)
                                : ((co * CI + ci) * 3 + kh) * 3 + kw;
-          acc[co] = __builtin_fmaf(bf16_to_f32(w[widx]), xv, acc[co]);
+          acc[co] = __builtin_fmaf(w[widx], xv, acc[co]);
         }
       }
     }
@@ -182,26 +184,40 @@ __global__ __launch_bounds__(256) void k_conv_dw(const float* __restrict__ x,
   }
 }
 
-// dB[co] = sum over batch and pixels of dy * (mask > 0); block per channel
-__global__ void k_bias_grad(const float* __restrict__ dy, const float* __restrict__ mask, int B,
-                            int C, int HW, float* __restrict__ db) {
-  __shared__ float red[256];
-  const int co = blockIdx.x;
+// dB[co] = sum over batch and pixels of dy * (mask > 0), in two passes
+// with a fixed summation order: part[co][g] over the g-th slice of the
+// (batch, pixel) range (grid C x G), then db[co] = sum over g.
+constexpr int kBiasSlices = 64;
+
+__global__ __launch_bounds__(256) void k_bias_grad_part(const float* __restrict__ dy,
+                                                        const float* __restrict__ mask, int B,
+                                                        int C, int HW,
+                                                        float* __restrict__ part) {
+  __shared__ float red[4];
+  const int co = blockIdx.x, g = blockIdx.y;
+  const long long n = (long long)B * HW;
+  const long long per = (n + kBiasSlices - 1) / kBiasSlices;
+  const long long e0 = (long long)g * per, e1 = e0 + per < n ? e0 + per : n;
   float s = 0.0f;
-  for (int b = 0; b < B; b++) {
-    const size_t base = ((size_t)b * C + co) * HW;
-    for (int p = threadIdx.x; p < HW; p += 256) {
-      const float g = dy[base + p];
-      if (!mask || mask[base + p] > 0.0f) s += g;
-    }
+  for (long long e = e0 + threadIdx.x; e < e1; e += 256) {
+    const long long b = e / HW, p = e - b * HW;
+    const size_t idx = ((size_t)b * C + co) * HW + (size_t)p;
+    const float v = dy[idx];
+    if (!mask || mask[idx] > 0.0f) s += v;
   }
-  red[threadIdx.x] = s;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) db[co] = red[0];
+  if (threadIdx.x == 0) part[(size_t)co * kBiasSlices + g] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+__global__ void k_bias_grad_final(const float* __restrict__ part, int C, float* __restrict__ db) {
+  const int co = blockIdx.x * blockDim.x + threadIdx.x;
+  if (co >= C) return;
+  float s = 0.0f;
+  for (int g = 0; g < kBiasSlices; g++) s += part[(size_t)co * kBiasSlices + g];
+  db[co] = s;
 }
 
 // mean BCE with logits: loss += sum(max(z,0) - z t + log1p(exp(-|z|))) / n,
@@ -227,8 +243,8 @@ __global__ void k_bce(const float* __restrict__ z, const float* __restrict__ t, 
 }
 
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, uint16_t* __restrict__ pbf, int n, float lr,
-                       float b1, float b2, float eps, float c1, float c2) {
+                       float* __restrict__ v, uint16_t* __restrict__ pbf, float* __restrict__ pw,
+                       int n, float lr, float b1, float b2, float eps, float c1, float c2) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float gi = g[i];
@@ -238,24 +254,31 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   v[i] = vi;
   const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
   p[i] = pi;
-  pbf[i] = f32_to_bf16(pi);
+  const uint16_t q = f32_to_bf16(pi);
+  pbf[i] = q;
+  pw[i] = bf16_to_f32(q);
 }
 
-__global__ void k_to_bf16(const float* __restrict__ p, int n, uint16_t* __restrict__ out) {
+// bf16 copy of the master weights, and the same values widened to fp32
+__global__ void k_to_bf16(const float* __restrict__ p, int n, uint16_t* __restrict__ out,
+                          float* __restrict__ pw) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = f32_to_bf16(p[i]);
+  if (i >= n) return;
+  const uint16_t q = f32_to_bf16(p[i]);
+  out[i] = q;
+  pw[i] = bf16_to_f32(q);
 }
 
 // ---------------------------------------------------------------------------
 template <int CI, int CO, bool BWD>
-static hipError_t conv_t(const float* x, const float* mask, const uint16_t* w, const float* b,
+static hipError_t conv_t(const float* x, const float* mask, const float* w, const float* b,
                          int B, int H, int W, int relu, float* y, hipStream_t st) {
   dim3 grid((W + TW - 1) / TW, (H + TH - 1) / TH, B);
   hipLaunchKernelGGL((k_conv<CI, CO, BWD>), grid, dim3(256), 0, st, x, mask, w, b, H, W, relu, y);
   return hipGetLastError();
 }
 
-hipError_t cnn_conv_forward(int ci, int co, const float* x, const uint16_t* w, const float* b,
+hipError_t cnn_conv_forward(int ci, int co, const float* x, const float* w, const float* b,
                             int B, int H, int W, int relu, float* y, hipStream_t st) {
   if (ci == 1 && co == 32) return conv_t<1, 32, false>(x, nullptr, w, b, B, H, W, relu, y, st);
   if (ci == 32 && co == 32) return conv_t<32, 32, false>(x, nullptr, w, b, B, H, W, relu, y, st);
@@ -266,7 +289,7 @@ hipError_t cnn_conv_forward(int ci, int co, const float* x, const uint16_t* w, c
 // dx (ci_fwd channels) from dy (co_fwd channels); mask: the forward
 // output of this layer when it had a ReLU (else nullptr)
 hipError_t cnn_conv_backward_data(int ci, int co, const float* dy, const float* mask,
-                                  const uint16_t* w, int B, int H, int W, float* dx,
+                                  const float* w, int B, int H, int W, float* dx,
                                   hipStream_t st) {
   if (ci == 32 && co == 32) return conv_t<32, 32, true>(dy, mask, w, nullptr, B, H, W, 0, dx, st);
   if (ci == 32 && co == 1) return conv_t<1, 32, true>(dy, mask, w, nullptr, B, H, W, 0, dx, st);
@@ -290,11 +313,17 @@ hipError_t cnn_conv_backward_weight(int ci, int co, const float* x, const float*
   return hipGetLastError();
 }
 
-hipError_t cnn_bias_grad(const float* dy, const float* mask, int B, int C, int HW, float* db,
-                         hipStream_t st) {
-  hipLaunchKernelGGL(k_bias_grad, dim3(C), dim3(256), 0, st, dy, mask, B, C, HW, db);
+hipError_t cnn_bias_grad(const float* dy, const float* mask, int B, int C, int HW, float* part,
+                         float* db, hipStream_t st) {
+  hipLaunchKernelGGL(k_bias_grad_part, dim3(C, kBiasSlices), dim3(256), 0, st, dy, mask, B, C, HW,
+                     part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_bias_grad_final, dim3((C + 63) / 64), dim3(64), 0, st, part, C, db);
   return hipGetLastError();
 }
+
+int cnn_bias_part_floats(int C) { return C * kBiasSlices; }
 
 hipError_t cnn_bce(const float* z, const float* t, int64_t n, float* grad, float* loss,
                    hipStream_t st) {
@@ -303,16 +332,16 @@ hipError_t cnn_bce(const float* z, const float* t, int64_t n, float* grad, float
   return hipGetLastError();
 }
 
-hipError_t cnn_adam(float* p, const float* g, float* m, float* v, uint16_t* pbf, int n, float lr,
-                    float b1, float b2, float eps, int step, hipStream_t st) {
+hipError_t cnn_adam(float* p, const float* g, float* m, float* v, uint16_t* pbf, float* pw, int n,
+                    float lr, float b1, float b2, float eps, int step, hipStream_t st) {
   const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
-  hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, p, g, m, v, pbf, n, lr, b1,
-                     b2, eps, c1, c2);
+  hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, st, p, g, m, v, pbf, pw, n, lr,
+                     b1, b2, eps, c1, c2);
   return hipGetLastError();
 }
 
-hipError_t cnn_to_bf16(const float* p, int n, uint16_t* out, hipStream_t st) {
-  hipLaunchKernelGGL(k_to_bf16, dim3((n + 255) / 256), dim3(256), 0, st, p, n, out);
+hipError_t cnn_to_bf16(const float* p, int n, uint16_t* out, float* pw, hipStream_t st) {
+  hipLaunchKernelGGL(k_to_bf16, dim3((n + 255) / 256), dim3(256), 0, st, p, n, out, pw);
   return hipGetLastError();
 }
 
