@@ -104,9 +104,12 @@ __global__ __launch_bounds__(256) void k_conv(const float* __restrict__ x,
 
 // dW[co][ci][kh][kw] += sum_pixels g[b][co][p] * x[b][ci][p + (kh-1, kw-1)],
 // g = dy * (mask > 0) when mask is given.  Block: one sample, a band of
-// rows.  Thread t owns the (co, ci) pairs t*PAIRS .. t*PAIRS+PAIRS-1 (all
-// nine taps each), accumulates over the band's pixel tiles in registers and
-// adds its partial sums once.
+// rows.  Thread t owns input channel ci = t mod CI and the PAIRS output
+// channels co = (t / CI) * PAIRS + k (all nine taps each), so a pixel costs
+// it 9 input reads and PAIRS gradient reads; it accumulates over the band's
+// pixel tiles in registers and adds its partial sums once.  Each input
+// channel's tile is padded to an odd length, so the lanes of a wave (32
+// different channels at one pixel) read 32 different LDS banks.
 template <int CI, int CO>
 __global__ __launch_bounds__(256) void k_conv_dw(const float* __restrict__ x,
                                                 const float* __restrict__ dy,
@@ -115,8 +118,9 @@ __global__ __launch_bounds__(256) void k_conv_dw(const float* __restrict__ x,
   constexpr int NP = CO * CI;                 // (co, ci) pairs
   constexpr int PAIRS = (NP + 255) / 256;
   constexpr int RH = 8, RW = 16;
-  static_assert(NP <= 256 || NP % 256 == 0, "each thread owns whole (co, ci) pairs");
-  __shared__ float xs[CI][RH + 2][RW + 2];
+  constexpr int XR = RW + 2, XS = (RH + 2) * XR + 1;   // odd channel stride
+  static_assert(NP <= 256 || (NP % 256 == 0 && 256 % CI == 0), "thread owns one ci, PAIRS co");
+  __shared__ float xs[CI][XS];
   __shared__ float gs[CO][RH][RW];
   const int b = blockIdx.y;
   const int hb = blockIdx.x * rows_per_block;
@@ -124,7 +128,9 @@ __global__ __launch_bounds__(256) void k_conv_dw(const float* __restrict__ x,
   const float* xb = x + (size_t)b * CI * plane;
   const float* gb = dy + (size_t)b * CO * plane;
   const float* mb = mask ? mask + (size_t)b * CO * plane : nullptr;
-  const int p0 = threadIdx.x * PAIRS;
+  const int ci = threadIdx.x % CI;
+  const int co0 = (threadIdx.x / CI) * PAIRS;
+  const bool owner = NP >= 256 || (int)threadIdx.x < NP;
   float acc[PAIRS][9];
 #pragma unroll
   for (int k = 0; k < PAIRS; k++)
@@ -133,54 +139,55 @@ __global__ __launch_bounds__(256) void k_conv_dw(const float* __restrict__ x,
   for (int h0 = hb; h0 < hb + rows_per_block && h0 < H; h0 += RH) {
     for (int w0 = 0; w0 < W; w0 += RW) {
       __syncthreads();
-      for (int e = threadIdx.x; e < CI * (RH + 2) * (RW + 2); e += 256) {
-        const int ci = e / ((RH + 2) * (RW + 2));
-        const int rem = e - ci * (RH + 2) * (RW + 2);
-        const int yy = rem / (RW + 2), xx = rem - yy * (RW + 2);
+      for (int e = threadIdx.x; e < CI * (RH + 2) * XR; e += 256) {
+        const int c = e / ((RH + 2) * XR);
+        const int rem = e - c * (RH + 2) * XR;
+        const int yy = rem / XR, xx = rem - yy * XR;
         const int hh = h0 + yy - 1, ww = w0 + xx - 1;
-        xs[ci][yy][xx] = (hh >= 0 && hh < H && ww >= 0 && ww < W)
-                             ? xb[(size_t)ci * plane + (size_t)hh * W + ww] : 0.0f;
+        xs[c][rem] = (hh >= 0 && hh < H && ww >= 0 && ww < W)
+                         ? xb[(size_t)c * plane + (size_t)hh * W + ww] : 0.0f;
       }
       for (int e = threadIdx.x; e < CO * RH * RW; e += 256) {
-        const int co = e / (RH * RW);
-        const int rem = e - co * RH * RW;
+        const int c = e / (RH * RW);
+        const int rem = e - c * RH * RW;
         const int yy = rem / RW, xx = rem - yy * RW;
         const int hh = h0 + yy, ww = w0 + xx;
         float v = 0.0f;
         if (hh < H && ww < W) {
-          const size_t idx = (size_t)co * plane + (size_t)hh * W + ww;
+          const size_t idx = (size_t)c * plane + (size_t)hh * W + ww;
           v = gb[idx];
           if (mb && !(mb[idx] > 0.0f)) v = 0.0f;
         }
-        gs[co][yy][xx] = v;
+        gs[c][yy][xx] = v;
       }
       __syncthreads();
-      if (p0 < NP) {
+      if (owner) {
 #pragma unroll 1
         for (int yy = 0; yy < RH; yy++) {
-#pragma unroll 1
+#pragma unroll 2
           for (int xx = 0; xx < RW; xx++) {
+            float xv[9];
+#pragma unroll
+            for (int kh = 0; kh < 3; kh++)
+#pragma unroll
+              for (int kw = 0; kw < 3; kw++) xv[kh * 3 + kw] = xs[ci][(yy + kh) * XR + xx + kw];
 #pragma unroll
             for (int k = 0; k < PAIRS; k++) {
-              const int pr = p0 + k;
-              const int co = pr / CI, ci = pr - (pr / CI) * CI;
-              const float g = gs[co][yy][xx];
+              const float g = gs[co0 + k][yy][xx];
 #pragma unroll
-              for (int kh = 0; kh < 3; kh++)
-#pragma unroll
-                for (int kw = 0; kw < 3; kw++)
-                  acc[k][kh * 3 + kw] = __builtin_fmaf(g, xs[ci][yy + kh][xx + kw], acc[k][kh * 3 + kw]);
+              for (int t = 0; t < 9; t++) acc[k][t] = __builtin_fmaf(g, xv[t], acc[k][t]);
             }
           }
         }
       }
     }
   }
-  if (p0 < NP) {
+  if (owner) {
 #pragma unroll
     for (int k = 0; k < PAIRS; k++)
 #pragma unroll
-      for (int t = 0; t < 9; t++) atomicAdd(&dw[(size_t)(p0 + k) * 9 + t], acc[k][t]);
+      for (int t = 0; t < 9; t++)
+        atomicAdd(&dw[((size_t)(co0 + k) * CI + ci) * 9 + t], acc[k][t]);
   }
 }
 
