@@ -125,6 +125,22 @@ FJ_HD inline float fitf_R1(float t) {
   return p;
 }
 
+/* far argument, modulus and phase of the order-0 functions: M0(t), and
+   the phase shift in revolutions over 8/x */
+/* max abs error of the fit on (0,0.16]: 2.52e-10 */
+FJ_HD inline float fitf_M0(float t) {
+  float p = 2.47902815240584e-05f;
+  p = __builtin_fmaf(p, t, -0.0009765338659266368f);
+  p = __builtin_fmaf(p, t, 0.9999999997467515f);
+  return p;
+}
+/* max abs error of the fit on (0,0.16]: 3.17e-09 */
+FJ_HD inline float fitf_ph0(float t) {
+  float p = 2.0077624210811506e-05f;
+  p = __builtin_fmaf(p, t, -0.002486792805295915f);
+  return p;
+}
+
 /* small argument x < 8, y = (x/8)^2, double */
 /* max abs error of the fit on (0,1]: 8.04e-11 */
 FJ_TABLE(k_fit_J0s, 12, {-0.005618801482900782, 0.07079209695405728, -0.5037809434408522, 2.625220559644334, -10.557357884793742, 32.35918804582165, -72.81660855664654, 113.77757421102329, -113.77775682787487, 63.99999888465921, -15.999999976722172, 0.9999999999195268})

@@ -309,12 +309,22 @@ template <int MODE>
 FJ_HD inline void trapz_far(double rs, float ir, const LaneK& L, const float* c, float& pR,
                             float& pI) {
   using M = ModeT<MODE>;
+  // modulus-phase form: J0 = amp M0 cos(th + phi), Y0 = amp M0 sin(th + phi),
+  // one transcendental per basis (the amplitude is applied per lane)
   const float z = L.zk * ir, t = z * z;
-  const float p0 = fitf_P0(t), q0 = z * fitf_q0(t);
-  float s, co;
-  sincos_rev(phase_rev(L.q, rs), &s, &co);
-  const float vj = M::J ? __builtin_fmaf(p0, co, -q0 * s) : 0.0f;
-  const float vy = M::Y ? __builtin_fmaf(p0, s, q0 * co) : 0.0f;
+  const float m0 = fitf_M0(t);
+  const float u = __builtin_fmaf(z, fitf_ph0(t), phase_rev(L.q, rs));
+  float vj = 0.0f, vy = 0.0f;
+  if (M::J && M::Y) {
+    float s, co;
+    sincos_rev(u, &s, &co);
+    vj = m0 * co;
+    vy = m0 * s;
+  } else if (M::J) {
+    vj = m0 * cos_rev(u);
+  } else {
+    vy = m0 * sin_rev(u);
+  }
   if (MODE == MODE_J) {
     pR = __builtin_fmaf(c[0], vj, pR);
   } else if (MODE == MODE_Y) {

@@ -210,6 +210,27 @@ FJ_HD inline void g_small(double x, double y, double L, bool need_j, bool need_y
     *gy_shifted = x > 0.0 ? (float)(L * g + x * (fit_GYs<TAB>(y) + kTwoOverPiD)) : 0.0f;
 }
 
+// cos(2 pi u) and sin(2 pi u) alone (revolutions; u within a few turns)
+FJ_HD inline float cos_rev(float u) {
+#if FJ_USE_HW
+  return __builtin_amdgcn_cosf(u);
+#else
+  float s, c;
+  sincos_rev(u, &s, &c);
+  return c;
+#endif
+}
+
+FJ_HD inline float sin_rev(float u) {
+#if FJ_USE_HW
+  return __builtin_amdgcn_sinf(u);
+#else
+  float s, c;
+  sincos_rev(u, &s, &c);
+  return s;
+#endif
+}
+
 // Reduced phase of x - pi/4 in revolutions: q = f/c (cycles per metre),
 // r in metres, x / 2pi = q r.
 FJ_HD inline float reduce_phase(double q, double r) {

@@ -146,10 +146,15 @@ __global__ void k_mwfj_stations(const float* __restrict__ r, int nr, int ntot,
 template <bool Y, bool FAR>
 __device__ inline float mwfj_value(double rs, float ir, const LaneK& L) {
   const float z = L.zk * ir;   // 8 / x
-  if (FAR || z <= 1.0f) {
+  if (FAR) {   // modulus-phase form, one transcendental
+    const float t = z * z;
+    const float u = __builtin_fmaf(z, fitf_ph0(t), phase_rev(L.q, rs));
+    return fitf_M0(t) * (Y ? sin_rev(u) : cos_rev(u));
+  }
+  if (z <= 1.0f) {
     const float t = z * z;
     float p0, q0;
-    if (FAR || z <= 8.0f / kFarX) {
+    if (z <= 8.0f / kFarX) {
       p0 = fitf_P0(t);
       q0 = z * fitf_q0(t);
     } else {
