@@ -528,6 +528,27 @@ def bench_learned(args):
                        "steps": len(losses), "first_loss": float(losses[0]),
                        "last_loss": float(np.mean(losses[-20:]))}
     out["speedup_vs_full_grid"] = round(full_ms / band_ms, 2)
+    if not args.no_cpu_baseline:
+        # a CPU has no bands: the FJcpu restatement (OpenMP over f) on the
+        # full dense grid of one field of the same kind
+        from ccfj import learn as L
+        from oracle import oracle as O
+        cores = O.nthreads_default()
+        rng = np.random.default_rng(123)
+        r, modes, sigma = L.sample_fields(rng, 1, m.nr)
+        U, d_r, _ = m._fields(r, modes, sigma, 99)
+        m.ctx.synchronize()
+        Uh = U.download().reshape(m.nr, m.W)
+        t0 = time.perf_counter()
+        O.port_native(Uh, np.asarray(r, np.float32).reshape(-1), m.f, m.c_dense, m.itype, 0,
+                      nthreads=cores)
+        dt = time.perf_counter() - t0
+        ct = float(m.nr) * m.W * m.nc_dense
+        out["cpu_baseline"] = {"value": ct / dt, "unit": "term-evals/s", "cores": cores,
+                               "kind": "port",
+                               "sample": f"one field's full dense grid ({m.nr} pairs x {m.W} f x "
+                                         f"{m.nc_dense} c, itype {m.itype}: {ct:.3e} term-evals "
+                                         f"in {dt:.1f} s)", "cpu": cpu_model()}
     print(json.dumps(out), flush=True)
 
 
