@@ -154,7 +154,6 @@ int ccfj_batch_create(ccfj_ctx* ctx, int mode, int itype, int normalize, int nb,
   p->nf = nf;
   p->nc = nc;
   p->rows = soff[nb];
-  const int NC = fjl::coef_count(mode, itype);
   std::vector<int64_t> hsoff(soff, soff + nb + 1), hpoff(nb), hcoff(nb);
   std::vector<int> hnrp(nb), horder(nb);
   int64_t pacc = 0, cacc = 0;
@@ -165,7 +164,7 @@ int ccfj_batch_create(ccfj_ctx* ctx, int mode, int itype, int normalize, int nb,
     hpoff[b] = pacc;
     pacc += hnrp[b] + fjl::kUnroll;
     hcoff[b] = cacc;
-    cacc += (int64_t)nf * hnrp[b] * NC;
+    cacc += (int64_t)nf * (int64_t)fjl::coef_block(hnrp[b], mode, itype);
   }
   std::iota(horder.begin(), horder.end(), 0);
   std::stable_sort(horder.begin(), horder.end(), [&](int a, int b) { return hnrp[a] > hnrp[b]; });

@@ -140,13 +140,12 @@ int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U
     e = fjk::launch_fill(T, (size_t)nf * nc, 0.0f, st);
     if (e != hipSuccess) return hip_fail(e, "fill kernel launch");
   } else {
-    const int NC = fjk::coef_per_station(mode, itype);
     const int nrp = fjk::padded(nr);
     if ((rc = ensure(ctx, S_RR, sizeof(double) * (nrp + 8)))) return rc;
     if ((rc = ensure(ctx, S_HH, sizeof(float) * (nrp + 8)))) return rc;
     if ((rc = ensure(ctx, S_IR, sizeof(float) * (nrp + 8)))) return rc;
     if ((rc = ensure(ctx, S_ISR, sizeof(float) * (nrp + 8)))) return rc;
-    if ((rc = ensure(ctx, S_COEF, sizeof(float) * (size_t)nf * nrp * NC))) return rc;
+    if ((rc = ensure(ctx, S_COEF, sizeof(float) * fjk::coef_floats(mode, itype, nrp, nf)))) return rc;
     fjk::Problem p;
     p.U0 = U0;
     p.U1 = two_spectra(mode) ? U1 : U0;
@@ -405,13 +404,12 @@ int ccfj_transform_items(ccfj_ctx* ctx, int mode, int itype, const float* U0, co
                                         nitems, nr, 0, nf, nc, out, st);
     return e == hipSuccess ? CCFJ_OK : hip_fail(e, "item kernel launch");
   }
-  const int NC = fjk::coef_per_station(mode, itype);
   const int nrp = fjk::padded(nr);
   if ((rc = ensure(ctx, S_RR, sizeof(double) * (nrp + 8)))) return rc;
   if ((rc = ensure(ctx, S_HH, sizeof(float) * (nrp + 8)))) return rc;
   if ((rc = ensure(ctx, S_IR, sizeof(float) * (nrp + 8)))) return rc;
   if ((rc = ensure(ctx, S_ISR, sizeof(float) * (nrp + 8)))) return rc;
-  if ((rc = ensure(ctx, S_COEF, sizeof(float) * (size_t)nf * nrp * NC))) return rc;
+  if ((rc = ensure(ctx, S_COEF, sizeof(float) * fjk::coef_floats(mode, itype, nrp, nf)))) return rc;
   fjk::Problem p;
   p.U0 = U0;
   p.U1 = mode == CCFJ_MODE_J_MINUS_Y ? U1 : U0;

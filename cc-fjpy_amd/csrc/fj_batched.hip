@@ -13,7 +13,7 @@
 //   U0/U1  f32 [sum nr_b][nf]        rows of all sub-arrays, each sorted by r
 //   r      f32 [sum nr_b]
 //   rr, hh, ir, isr  per sub-array b at poff[b], nrp_b + 8 entries each (padded)
-//   coef   per sub-array b at coff[b]: [nf][nrp_b][NC]
+//   coef   per sub-array b at coff[b]: nf frequencies of nrp_b records (fj_kernels.hip)
 //   T      f32 [nb][nf][nc]     persistent-kernel results (coalesced along c)
 //   out    f32 [nb][nc][nf]     formed from T by k_finish (fj_kernels.hip)
 #include <hip/hip_runtime.h>
@@ -21,6 +21,7 @@
 
 #include "fj_batched.h"
 #include "fj_lane.h"
+#include "fj_prep.h"
 
 namespace fjk {
 
@@ -39,54 +40,14 @@ __global__ void k_stations_b(BatchDev d) {
 
 template <int ITYPE, int NSPEC>
 __global__ __launch_bounds__(256) void k_prep_b(BatchDev d) {
-  constexpr int TS = 64, TF = 64, LD = TF + 1;
-  constexpr int NC = ITYPE == 0 ? NSPEC : 2 * NSPEC;
-  __shared__ float tile[NSPEC][TS + 1][LD];
   const int b = blockIdx.z;
   const int64_t s00 = d.soff[b];
   const int nr = (int)(d.soff[b + 1] - s00);
   const int nrp = d.nrp[b];
-  const int nf = d.nf;
-  const int f0 = blockIdx.x * TF;
-  const int s0 = blockIdx.y * TS;
+  const int s0 = blockIdx.y * 64;
   if (s0 >= nrp || nr <= 0) return;   // uniform per block: no barrier skipped partially
-  const float* U0 = d.U0 + s00 * nf;
-  const float* U1 = d.U1 + s00 * nf;
-  const float* r = d.r + s00;
-  float* coef = d.coef + d.coff[b];
-  const int lane = threadIdx.x & 63;
-  const int grp = threadIdx.x >> 6;
-  for (int row = grp; row < TS + 1; row += 4) {
-    const int s = s0 - 1 + row;
-    const int f = f0 + lane;
-    const bool ok = s >= 0 && s < nr && f < nf;
-    const size_t idx = ok ? (size_t)s * nf + f : 0;
-    tile[0][row][lane] = ok ? U0[idx] : 0.0f;
-    if (NSPEC > 1) tile[NSPEC - 1][row][lane] = ok ? U1[idx] : 0.0f;
-  }
-  __syncthreads();
-  for (int it = threadIdx.x; it < TS * TF; it += 256) {
-    const int sl = it & (TS - 1);
-    const int fl = it >> 6;
-    const int s = s0 + sl, f = f0 + fl;
-    if (s >= nrp || f >= nf) continue;
-    float* dst = coef + ((size_t)f * nrp + s) * NC;
-    if (s >= nr) {
-#pragma unroll
-      for (int k = 0; k < NC; k++) dst[k] = 0.0f;
-      continue;
-    }
-    float gp[NSPEC], gc[NSPEC], rec[NC];
-    for (int k = 0; k < NSPEC; k++) {
-      gp[k] = tile[k][sl][fl];
-      gc[k] = tile[k][sl + 1][fl];
-    }
-    const float rp = s > 0 ? r[s - 1] : 0.0f;
-    const float rn = s + 1 < nr ? r[s + 1] : 0.0f;
-    station_coef<ITYPE, NSPEC>(s, nr, rp, r[s], rn, gp, gc, rec);
-#pragma unroll
-    for (int k = 0; k < NC; k++) dst[k] = rec[k];
-  }
+  prep_tile<ITYPE, NSPEC>(d.U0 + s00 * d.nf, d.U1 + s00 * d.nf, d.r + s00, d.f, d.c, nr, nrp,
+                          d.nf, d.nc, d.coef + d.coff[b], blockIdx.x * 64, s0);
 }
 
 __global__ void k_reset_counter(unsigned int* counter, unsigned int start) {
@@ -102,14 +63,13 @@ __global__ void k_reset_counter(unsigned int* counter, unsigned int start) {
 // data of the loop -- are scalar loads, as in k_fj.
 template <int MODE, int ITYPE, int TILE>
 __global__ __launch_bounds__(TILE)
-__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj_b(
+__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE, true>::waves, 8))) void k_fj_b(
     int nf, int nc, unsigned int limit, const int64_t* __restrict__ soff,
     const int64_t* __restrict__ poff, const int64_t* __restrict__ coff,
     const int* __restrict__ nrpa, const int* __restrict__ order, const float* __restrict__ fv,
     const float* __restrict__ cv, const double* __restrict__ rr, const float* __restrict__ hh,
     const float* __restrict__ ir, const float* __restrict__ isr, const float* __restrict__ coef,
     float* __restrict__ T, unsigned int* __restrict__ counter) {
-  constexpr int NC = ITYPE == 0 ? ModeT<MODE>::nspec : 2 * ModeT<MODE>::nspec;
   __shared__ unsigned int item_s;
   const unsigned int ncb = (unsigned int)((nc + TILE - 1) / TILE);
   while (true) {
@@ -128,7 +88,7 @@ __attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj_b(
     if (nr > 1) {   // uniform over the workgroup: every lane runs the lane code
       const int nrp = nrpa[b];
       const double q = (double)fv[fi] / (double)cv[j < nc ? j : nc - 1];
-      const float* __restrict__ cf = coef + coff[b] + (size_t)fi * nrp * NC;
+      const float* __restrict__ cf = coef + coff[b] + (size_t)fi * coef_block(nrp, MODE, ITYPE);
       const int64_t o = poff[b];
       const Stations S{rr + o, hh + o, ir + o, isr + o};
       float v;

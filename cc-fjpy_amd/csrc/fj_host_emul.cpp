@@ -29,25 +29,33 @@ void run_mode(int itype, const float* U0, const float* U1, const float* r, const
   std::vector<float> hh(ntot), ir(ntot), isr(ntot);
   for (int s = 0; s < ntot; s++) station_entry(r, nr, s, rr.data(), hh.data(), ir.data(), isr.data());
   const Stations S{rr.data(), hh.data(), ir.data(), isr.data()};
-  std::vector<float> cf((size_t)nrp * NC, 0.0f);
+  const size_t block = coef_block(nrp, MODE, itype);
+  std::vector<float> cf(block, 0.0f);
+  float cmin = INFINITY;
+  for (int j = 0; j < nc; j++) cmin = fminf(cmin, fabsf(c[j]));
 #pragma omp parallel for schedule(dynamic, 1) firstprivate(cf)
   for (int i = 0; i < nf; i++) {
+    const float hlim = itype == 1 ? long_segment_limit(f[i], cmin) : 0.0f;
+    if (itype == 1) cf[(size_t)nrp * NC] = hlim;
     for (int s = 0; s < nr; s++) {
-      float gp[2] = {0.f, 0.f}, gc[2];
-      gc[0] = U0[(size_t)s * nf + i];
-      gc[1] = NS > 1 ? U1[(size_t)s * nf + i] : 0.f;
-      if (s > 0) {
-        gp[0] = U0[(size_t)(s - 1) * nf + i];
-        gp[1] = NS > 1 ? U1[(size_t)(s - 1) * nf + i] : 0.f;
+      float gp[2] = {0.f, 0.f}, gc[2] = {0.f, 0.f}, gn[2] = {0.f, 0.f}, rec[8];
+      for (int m = 0; m < NS; m++) {
+        const float* U = m == 0 ? U0 : U1;
+        gc[m] = U[(size_t)s * nf + i];
+        if (s > 0) gp[m] = U[(size_t)(s - 1) * nf + i];
+        if (s + 1 < nr) gn[m] = U[(size_t)(s + 1) * nf + i];
       }
       const float rp = s > 0 ? r[s - 1] : 0.f, rn = s + 1 < nr ? r[s + 1] : 0.f;
-      float* rec = &cf[(size_t)s * NC];
       if (itype == 0) {
-        if (NS == 1) station_coef<0, 1>(s, nr, rp, r[s], rn, gp, gc, rec);
-        else station_coef<0, 2>(s, nr, rp, r[s], rn, gp, gc, rec);
+        if (NS == 1) station_coef<0, 1>(s, nr, rp, r[s], rn, gp, gc, gn, hlim, rec);
+        else station_coef<0, 2>(s, nr, rp, r[s], rn, gp, gc, gn, hlim, rec);
+        if (NS == 1) coef_store<0, 1>(cf.data(), nrp, s, rec);
+        else coef_store<0, 2>(cf.data(), nrp, s, rec);
       } else {
-        if (NS == 1) station_coef<1, 1>(s, nr, rp, r[s], rn, gp, gc, rec);
-        else station_coef<1, 2>(s, nr, rp, r[s], rn, gp, gc, rec);
+        if (NS == 1) station_coef<1, 1>(s, nr, rp, r[s], rn, gp, gc, gn, hlim, rec);
+        else station_coef<1, 2>(s, nr, rp, r[s], rn, gp, gc, gn, hlim, rec);
+        if (NS == 1) coef_store<1, 1>(cf.data(), nrp, s, rec);
+        else coef_store<1, 2>(cf.data(), nrp, s, rec);
       }
     }
     for (int j = 0; j < nc; j++) {

@@ -21,12 +21,12 @@ struct Problem {
   float* hh;     // [nrp + 8] workspace
   float* ir;     // [nrp + 8] workspace, 1 / r
   float* isr;    // [nrp + 8] workspace, 1 / sqrt(r)
-  float* coef;   // [nf][nrp][NC] workspace
+  float* coef;   // coef_floats(...) workspace
   float* T;      // [nf][nc] main-kernel results (k_finish forms [nc][nf])
 };
 
 fjl::Stations stations(const Problem& p);
-int coef_per_station(int mode, int itype);
+size_t coef_floats(int mode, int itype, int nrp, int nf);  // coef workspace
 int padded(int nr);  // station count padded to the kernels' block
 hipError_t launch_prep(int mode, int itype, const Problem& p, hipStream_t st);
 hipError_t launch_fj(int mode, int itype, const Problem& p, hipStream_t st);

@@ -9,9 +9,11 @@
 //   rr[nrp + 8]       double   station distances, ascending
 //   hh[nrp + 8]       float    r_s - r_{s-1} (hh[0] = 0)
 //   ir, isr[nrp + 8]  float    1 / r_s, 1 / sqrt(r_s)
-//   coef[nf][nrp][NC] float    per (frequency, station) coefficients:
-//       itype 0: A_s / sqrt(r_s) of each spectrum    (NC = nspec)
-//       itype 1: alpha_s, beta_s / 2 per spectrum    (NC = 2 nspec)
+//   coef              float    per frequency, fjl::coef_block floats:
+//       itype 0: [nrp][nspec] A_s / sqrt(r_s) of each spectrum
+//       itype 1: [nrp][3 nspec] (a_s, p_s, t_s) per spectrum, then
+//                [nrp][nspec] b_s, then the long-segment limit
+//                (fjl::station_coef, fjl::coef_store)
 //   T[nf][nc]         float    main-kernel results, one row per frequency
 //   out[nc][nf]       float    (the reference's layout, src/FJcpu.cpp:247),
 //                              formed from T by k_finish (optionally
@@ -19,7 +21,7 @@
 //
 // Launch: a workgroup is 256 lanes = 256 consecutive velocities of ONE
 // frequency.  The station data of the loop are then wave-uniform and are
-// read with scalar loads (one NC-float record per station per wave); each
+// read with scalar loads (one record per station per wave); each
 // lane owns one (f,c) output and keeps the whole station loop in registers,
 // so the main kernels are VALU-bound by construction.  Workgroups are
 // numbered XCD-major so that the workgroups of one frequency share an XCD
@@ -32,6 +34,7 @@
 
 #include "fj_kernels.h"
 #include "fj_lane.h"
+#include "fj_prep.h"
 
 namespace fjk {
 
@@ -59,52 +62,16 @@ __global__ void k_stations(const float* __restrict__ r, int nr, int ntot, double
   if (s < ntot) station_entry(r, nr, s, rr, hh, ir, isr);
 }
 
-// Coefficients + [nr][nf] -> [nf][nr][NC] transpose through LDS.
-// Tile: 64 stations x 64 frequencies plus one halo station below; the loads
-// are coalesced along f, the stores along s.
+// Coefficients + [nr][nf] -> per-frequency records (fj_prep.h).
 template <int ITYPE, int NSPEC>
 __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ U0,
                                               const float* __restrict__ U1,
-                                              const float* __restrict__ r, int nr, int nrp,
-                                              int nf, float* __restrict__ coef) {
-  constexpr int TS = 64, TF = 64, LD = TF + 1;
-  constexpr int NC = ITYPE == 0 ? NSPEC : 2 * NSPEC;
-  __shared__ float tile[NSPEC][TS + 1][LD];
-  const int f0 = blockIdx.x * TF;
-  const int s0 = blockIdx.y * TS;
-  const int lane = threadIdx.x & 63;
-  const int grp = threadIdx.x >> 6;
-  for (int row = grp; row < TS + 1; row += 4) {
-    const int s = s0 - 1 + row;
-    const int f = f0 + lane;
-    const bool ok = s >= 0 && s < nr && f < nf;
-    const size_t idx = ok ? (size_t)s * nf + f : 0;
-    tile[0][row][lane] = ok ? U0[idx] : 0.0f;
-    if (NSPEC > 1) tile[NSPEC - 1][row][lane] = ok ? U1[idx] : 0.0f;
-  }
-  __syncthreads();
-  for (int it = threadIdx.x; it < TS * TF; it += 256) {
-    const int sl = it & (TS - 1);
-    const int fl = it >> 6;
-    const int s = s0 + sl, f = f0 + fl;
-    if (s >= nrp || f >= nf) continue;
-    float* dst = coef + ((size_t)f * nrp + s) * NC;
-    if (s >= nr) {  // padding stations contribute nothing
-#pragma unroll
-      for (int k = 0; k < NC; k++) dst[k] = 0.0f;
-      continue;
-    }
-    float gp[NSPEC], gc[NSPEC], rec[NC];
-    for (int k = 0; k < NSPEC; k++) {
-      gp[k] = tile[k][sl][fl];
-      gc[k] = tile[k][sl + 1][fl];
-    }
-    const float rp = s > 0 ? r[s - 1] : 0.0f;
-    const float rn = s + 1 < nr ? r[s + 1] : 0.0f;
-    station_coef<ITYPE, NSPEC>(s, nr, rp, r[s], rn, gp, gc, rec);
-#pragma unroll
-    for (int k = 0; k < NC; k++) dst[k] = rec[k];
-  }
+                                              const float* __restrict__ r,
+                                              const float* __restrict__ f,
+                                              const float* __restrict__ c, int nr, int nrp,
+                                              int nf, int nc, float* __restrict__ coef) {
+  prep_tile<ITYPE, NSPEC>(U0, U1, r, f, c, nr, nrp, nf, nc, coef, blockIdx.x * 64,
+                          blockIdx.y * 64);
 }
 
 template <int MODE, int ITYPE>
@@ -113,7 +80,6 @@ __attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj(Sta
                                               const float* __restrict__ fv,
                                               const float* __restrict__ cv, int nrp, int nf,
                                               int nc, float* __restrict__ T_out) {
-  constexpr int NC = ITYPE == 0 ? ModeT<MODE>::nspec : 2 * ModeT<MODE>::nspec;
   const int ncb = (nc + kBlock - 1) / kBlock;
   int T;
   if (!tile_of(nf * ncb, &T)) return;
@@ -122,7 +88,7 @@ __attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj(Sta
   // lanes past the last velocity compute a copy of it (the lane code needs
   // every lane of the wave active) and store nothing
   const double q = (double)fv[fi] / (double)cv[j < nc ? j : nc - 1];
-  const float* __restrict__ cf = coef + (size_t)fi * nrp * NC;
+  const float* __restrict__ cf = coef + (size_t)fi * coef_block(nrp, MODE, ITYPE);
   float v;
   if (ITYPE == 0)
     v = trapz_lane<MODE>(S, cf, nrp, q);
@@ -209,12 +175,14 @@ static hipError_t launch_main(int itype, const Problem& p, hipStream_t st) {
 template <int ITYPE, int NSPEC>
 static hipError_t launch_prep_t(const Problem& p, hipStream_t st) {
   dim3 grid((p.nf + 63) / 64, (p.nrp + 63) / 64);
-  hipLaunchKernelGGL((k_prep<ITYPE, NSPEC>), grid, dim3(256), 0, st, p.U0, p.U1, p.r, p.nr,
-                     p.nrp, p.nf, p.coef);
+  hipLaunchKernelGGL((k_prep<ITYPE, NSPEC>), grid, dim3(256), 0, st, p.U0, p.U1, p.r, p.f, p.c,
+                     p.nr, p.nrp, p.nf, p.nc, p.coef);
   return hipGetLastError();
 }
 
-int coef_per_station(int mode, int itype) { return coef_count(mode, itype); }
+size_t coef_floats(int mode, int itype, int nrp, int nf) {
+  return (size_t)nf * coef_block(nrp, mode, itype);
+}
 
 int padded(int nr) { return padded_stations(nr); }
 

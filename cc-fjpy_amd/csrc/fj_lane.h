@@ -160,6 +160,26 @@ FJ_HD inline bool wave_any(bool p) {
 constexpr float kTau = 0.5f;
 constexpr float kKappa = 0.25f;
 
+// Bits of the largest gap h for which no segment of the wave's lanes is long
+// or takes the G difference in a far block: min(h_limit, 0.99 kTau / k_max)
+// with k_max the largest |k| of the wave (the margin covers the rounding of
+// dx = k h, so every lane's own test agrees).  A block whose gaps are all at
+// or below it compares them with block_far.
+FJ_HD inline uint32_t smooth_threshold(double q2pi, uint32_t hlim) {
+  double kmax = fabs(q2pi);
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) kmax = fmax(kmax, __shfl_xor(kmax, o));
+#endif
+  const uint32_t g = f32_bits(kmax > 0.0 ? (float)(0.99 * (double)kTau / kmax) : INFINITY);
+  const uint32_t t = g < hlim ? g : hlim;
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_readfirstlane(t);
+#else
+  return t;
+#endif
+}
+
 FJ_HD inline bool use_gdiff(float xa, float xb, float dx) {
   return dx != 0.0f && fabsf(dx) >= fminf(kTau, kKappa * fminf(xa, xb));
 }
@@ -176,25 +196,59 @@ template <> struct ModeT<MODE_J_MINUS_Y> { static constexpr int nspec = 2; stati
 template <> struct ModeT<MODE_ABS_J> { static constexpr int nspec = 2; static constexpr bool J = true, Y = false; };
 template <> struct ModeT<MODE_ABS_H> { static constexpr int nspec = 2; static constexpr bool J = true, Y = true; };
 
-FJ_HD inline int coef_count(int mode, int itype) {
-  int nspec = (mode == MODE_J || mode == MODE_Y) ? 1 : 2;
-  return itype == 0 ? nspec : 2 * nspec;
+// Floats of coefficient record per station and frequency.  itype 0: A'_s per
+// spectrum.  itype 1: (a_s, p_s, t_s) per spectrum in the station's record
+// and b_s per spectrum in a second plane after the nrp records (see
+// station_coef and coef_store).
+FJ_HD constexpr int coef_count(int mode, int itype) {
+  return ((mode == MODE_J || mode == MODE_Y) ? 1 : 2) * (itype == 0 ? 1 : 4);
+}
+// Floats of the coefficients of one frequency: nrp records, and for itype 1
+// the b plane and a header of 8 floats whose first is that frequency's
+// long-segment threshold (long_segment_limit).
+FJ_HD constexpr size_t coef_block(int nrp, int mode, int itype) {
+  return (size_t)nrp * coef_count(mode, itype) + (itype == 0 ? 0 : 8);
+}
+
+// itype 1: segments with k_max h > kLongDx at some frequency ("long": k_max
+// = 2 pi |f| / min |c| over the velocity grid) are left out of the (a, p, t)
+// records of that frequency and summed per lane instead.  For the others
+// the rule's terms in the records stay within about 1.3 times g even where
+// a lane replaces them by the G difference, so that replacement cancels
+// nothing large.  (A limit of kTau, which sends every segment on which a
+// lane may take the G difference to the per-lane sum, was slower on small
+// arrays too: 4096 ragged sub-arrays 672 ms against 645 ms.)
+constexpr float kLongDx = 2.0f;
+
+// h_limit of a frequency: a segment is long when its float gap h has
+// f32_bits(h) > f32_bits(h_limit) (gaps are non-negative; +inf at f = 0).
+FJ_HD inline float long_segment_limit(float f, float cmin) {
+  const double k = kTwoPi * fabs((double)f) / (double)cmin;
+  return k > 0.0 ? (float)((double)kLongDx / k) : INFINITY;
 }
 
 // Occupancy floor of the F-J kernels (waves per SIMD; the compiler then
-// keeps VGPRs within 512 / waves), used by the dense, batched and item-list
-// kernels alike.  Measured on the MI355X: the fused J - Y
+// keeps VGPRs within 512 / waves).  Measured on the MI355X: the fused J - Y
 // linear-segment kernel (natural 217 VGPRs, 2 waves) runs 16% faster held
-// to 3 waves, a few spills on its cold small-argument path included; the
-// other modes are fastest at their natural count (forcing them spills on the
-// hot path).  The linear-segment J kernel holds 4 waves (121 VGPRs in the
-// dense kernel; the persistent batched kernel's loop would take 170).
-// -DFJ_OCC_I1_JMY=N / -DFJ_OCC_I1_J=N override for tuning runs.
+// to 3 waves, a few spills on its cold small-argument path included (held
+// to 4 it spills on the hot path: 490 ms against 408 ms for 19,900 pairs x
+// 4096 f x 3000 c); the other modes are fastest at their natural count.
+// The linear-segment J kernel is held to 5 waves (96 VGPRs, a few spills
+// off the smooth far path) in the dense kernel, where smooth far blocks
+// dominate: 247 ms against 266 ms at its natural 4 waves on the same grid.
+// The persistent batched kernel and the item-list kernel, whose small
+// sub-arrays spend more of their time in near and non-smooth blocks, keep
+// 4 (FjOcc's SMALL form).
+// -DFJ_OCC_I1_JMY=N / -DFJ_OCC_I1_J=N / -DFJ_OCC_I1_J_SMALL=N override for
+// tuning runs.
 #ifndef FJ_OCC_I1_JMY
 #define FJ_OCC_I1_JMY 3
 #endif
 #ifndef FJ_OCC_I1_J
-#define FJ_OCC_I1_J 4
+#define FJ_OCC_I1_J 5
+#endif
+#ifndef FJ_OCC_I1_J_SMALL
+#define FJ_OCC_I1_J_SMALL 5
 #endif
 // How the linear-segment kernels hold the small-argument (x < 8) double
 // polynomials.  Measured on the MI355X (J kernel, 19,900 pairs x 512 f x
@@ -208,19 +262,39 @@ FJ_HD inline int coef_count(int mode, int itype) {
 #ifndef FJ_FIT_I1
 #define FJ_FIT_I1 FIT_LITERAL
 #endif
-template <int MODE, int ITYPE>
+template <int MODE, int ITYPE, bool SMALL = false>
 struct FjOcc {
   static constexpr int waves =
-      ITYPE == 0 ? 1 : (MODE == MODE_J_MINUS_Y ? FJ_OCC_I1_JMY : (MODE == MODE_J ? FJ_OCC_I1_J : 1));
+      ITYPE == 0 ? 1
+                 : (MODE == MODE_J_MINUS_Y ? FJ_OCC_I1_JMY
+                                           : (MODE == MODE_J ? (SMALL ? FJ_OCC_I1_J_SMALL : FJ_OCC_I1_J)
+                                                             : 1));
 };
 
 // ---- coefficients of station s (one frequency) ----------------------------
-// g_prev/g_cur: spectrum values at stations s-1 and s (g_prev unused for
-// s = 0).  rp, rs, rn: r_{s-1}, r_s, r_{s+1} (only those that exist are read).
-// itype 0 writes A_s per spectrum; itype 1 writes (alpha_s, beta_s).
+// g_prev/g_cur/g_next: spectrum values at stations s-1, s, s+1 (only those
+// that exist are read).  rp, rs, rn: r_{s-1}, r_s, r_{s+1} (likewise).
+// itype 0 writes A'_s per spectrum.  itype 1 writes, per spectrum m,
+// rec[m] = a_s, rec[NSPEC + m] = p_s, rec[2 NSPEC + m] = t_s and
+// rec[3 NSPEC + m] = b_s:
+//   the station's share of sum_s alpha_s g_s - sum_s b_s E_s with every E_s
+//   (= 2 D_s, segment s between stations s-1 and s) taken by the
+//   Euler-Maclaurin rule,
+//     E_s = g_{s-1} + g_s - (k h_s / 6)(g'_s - g'_{s-1})
+//           + (k^3 h_s^3 / 360)(g'''_s - g'''_{s-1}),
+//   collected per station:  a_s g_s + k p_s g'_s + k^3 t_s g'''_s with
+//     a_s = alpha_s - b_s - b_{s+1},
+//     p_s = (b_s h_s - b_{s+1} h_{s+1}) / 6,
+//     t_s = -(b_s h_s^3 - b_{s+1} h_{s+1}^3) / 360,
+//   b_s = beta_s / 2 (zero for s = 0 and past the last station), h_s the
+//   float gap r_s - r_{s-1} as in the station arrays.  A lane whose segment
+//   s takes the G difference instead adds b_s (E_s^rule - E_s^G).  Long
+//   segments (h_s beyond h_limit, long_segment_limit) are left out of a, p
+//   and t; the lanes add their -b_s E_s themselves.
 template <int ITYPE, int NSPEC>
 FJ_HD inline void station_coef(int s, int nr, float rp, float rs, float rn,
-                               const float* g_prev, const float* g_cur, float* rec) {
+                               const float* g_prev, const float* g_cur, const float* g_next,
+                               float h_limit, float* rec) {
   if (ITYPE == 0) {
     // trapezoid weight (src/FJcpu.cpp:354-356): each segment gives d/2 to
     // both of its end stations, d = max(h, 0.1) in float as the reference.
@@ -232,22 +306,43 @@ FJ_HD inline void station_coef(int s, int nr, float rp, float rs, float rn,
     for (int k = 0; k < NSPEC; k++) rec[k] = (float)(wr * (double)g_cur[k]);
   } else {
     // slope over the clamped gap (src/FJcpu.cpp:259-261): b = (g2-g1)/dr0,
-    // dr0 = max(h, 0.1); the segment's line reaches g1 + b h at r_s.
-    double ratio = 0.0;
-    if (s > 0) {
-      const float h = rs - rp;
-      const float dr0 = fmaxf(h, 0.1f);
-      ratio = (double)h / (double)dr0;
-    }
+    // dr0 = max(h, 0.1); the segment's line reaches g1 + b h at its right end.
+    const bool left = s > 0, right = s + 1 < nr;
+    const uint32_t hb = f32_bits(h_limit);
+    const bool inl = left && f32_bits(rs - rp) <= hb;    // segment s in the records
+    const bool inr = right && f32_bits(rn - rs) <= hb;   // segment s + 1 in the records
+    const double hl = left ? (double)(rs - rp) : 0.0;
+    const double hr = right ? (double)(rn - rs) : 0.0;
+    const double ratl = left ? hl / (double)fmaxf(rs - rp, 0.1f) : 0.0;
+    const double ratr = right ? hr / (double)fmaxf(rn - rs, 0.1f) : 0.0;
     for (int k = 0; k < NSPEC; k++) {
       const double g2 = g_cur[k];
-      const double g1 = s > 0 ? (double)g_prev[k] : 0.0;
-      const double beta = s > 0 ? (g2 - g1) * ratio : 0.0;
-      const double el = s > 0 ? g1 + beta : 0.0;  // left limit at r_s
-      const double er = s + 1 < nr ? g2 : 0.0;    // right limit at r_s
-      rec[2 * k] = (float)(el - er);              // alpha_s
-      rec[2 * k + 1] = (float)(0.5 * beta);       // beta_s / 2 (the lane forms 2 D_s)
+      const double g1 = left ? (double)g_prev[k] : 0.0;
+      const double g3 = right ? (double)g_next[k] : 0.0;
+      const double bl = left ? 0.5 * (g2 - g1) * ratl : 0.0;   // b_s
+      const double br = right ? 0.5 * (g3 - g2) * ratr : 0.0;  // b_{s+1}
+      const double el = left ? g1 + 2.0 * bl : 0.0;  // left limit at r_s
+      const double er = right ? g2 : 0.0;             // right limit at r_s
+      const double alpha = el - er;
+      const double cl = inl ? bl : 0.0, cr = inr ? br : 0.0;
+      rec[k] = (float)(alpha - cl - cr);
+      rec[NSPEC + k] = (float)((cl * hl - cr * hr) / 6.0);
+      rec[2 * NSPEC + k] = (float)(-(cl * hl * hl * hl - cr * hr * hr * hr) / 360.0);
+      rec[3 * NSPEC + k] = (float)bl;
     }
+  }
+}
+
+// Stores station s's record into the coefficients of one frequency (cf,
+// nrp stations): itype 0 at cf[s NSPEC]; itype 1 the hot part at
+// cf[s 3 NSPEC] and b_s at cf[nrp 3 NSPEC + s NSPEC].
+template <int ITYPE, int NSPEC>
+FJ_HD inline void coef_store(float* cf, int nrp, int s, const float* rec) {
+  if (ITYPE == 0) {
+    for (int k = 0; k < NSPEC; k++) cf[(size_t)s * NSPEC + k] = rec[k];
+  } else {
+    for (int k = 0; k < 3 * NSPEC; k++) cf[(size_t)s * 3 * NSPEC + k] = rec[k];
+    for (int k = 0; k < NSPEC; k++) cf[(size_t)nrp * 3 * NSPEC + (size_t)s * NSPEC + k] = rec[3 * NSPEC + k];
   }
 }
 
@@ -258,34 +353,9 @@ FJ_HD inline void station_coef(int s, int nr, float rp, float rs, float rn,
 // multiplies the sum by sqrt(2/(pi k)) = amp_s sqrt(r_s).  At k = 0 (f = 0)
 // there is no amplitude: the station adds A'_s Z0 sqrt(r_s), the lane
 // multiplies by 1.
+// Station term A'_s (vj, vy) added to the lane's sums.
 template <int MODE>
-FJ_HD inline void trapz_station(double rs, float ir, const LaneK& L, const float* c, float& pR,
-                                float& pI) {
-  using M = ModeT<MODE>;
-  float vj = 0.0f, vy = 0.0f;
-  const float z = L.zk * ir;  // 8 / x; inf when k = 0 or r = 0
-  if (z <= 1.0f) {            // x >= 8
-    const float t = z * z;
-    float p0, q0;
-    if (z <= 8.0f / kFarX) {
-      p0 = fitf_P0(t);
-      q0 = z * fitf_q0(t);
-    } else {
-      p0 = fit_P0(t);
-      q0 = z * fit_q0(t);
-    }
-    float s, co;
-    sincos_rev(phase_rev(L.q, rs), &s, &co);
-    if (M::J) vj = __builtin_fmaf(p0, co, -q0 * s);
-    if (M::Y) vy = __builtin_fmaf(p0, s, q0 * co);
-  } else {
-    const double xd = L.q2pi * rs;
-    double y, Lg;
-    const Bess b = bessel_small<false, M::Y, FIT_TABLE>(xd, &y, &Lg);
-    const float ia = L.kf > 0.0f ? sqrtf(kPiOver2 * (float)xd) : sqrtf((float)rs);
-    vj = b.j0 * ia;
-    vy = b.y0 * ia;
-  }
+FJ_HD inline void trapz_add(const float* c, float vj, float vy, float& pR, float& pI) {
   if (MODE == MODE_J) {
     pR = __builtin_fmaf(c[0], vj, pR);
   } else if (MODE == MODE_Y) {
@@ -304,17 +374,16 @@ FJ_HD inline void trapz_station(double rs, float ir, const LaneK& L, const float
   }
 }
 
-// x >= kFarX on every lane (a far block): no branch on the argument.
+// x >= kFarX, z = 8/x: J0 / amp and Y0 / amp in the modulus-phase form
+// J0 = amp M0 cos(th + phi), Y0 = amp M0 sin(th + phi), one transcendental
+// per basis.  Far blocks and the per-lane path both use it, so a term does
+// not depend on which lanes share its wave.
 template <int MODE>
-FJ_HD inline void trapz_far(double rs, float ir, const LaneK& L, const float* c, float& pR,
-                            float& pI) {
+FJ_HD inline void far_unit(double rs, const LaneK& L, float z, float& vj, float& vy) {
   using M = ModeT<MODE>;
-  // modulus-phase form: J0 = amp M0 cos(th + phi), Y0 = amp M0 sin(th + phi),
-  // one transcendental per basis (the amplitude is applied per lane)
-  const float z = L.zk * ir, t = z * z;
+  const float t = z * z;
   const float m0 = fitf_M0(t);
   const float u = __builtin_fmaf(z, fitf_ph0(t), phase_rev(L.q, rs));
-  float vj = 0.0f, vy = 0.0f;
   if (M::J && M::Y) {
     float s, co;
     sincos_rev(u, &s, &co);
@@ -325,22 +394,42 @@ FJ_HD inline void trapz_far(double rs, float ir, const LaneK& L, const float* c,
   } else {
     vy = m0 * sin_rev(u);
   }
-  if (MODE == MODE_J) {
-    pR = __builtin_fmaf(c[0], vj, pR);
-  } else if (MODE == MODE_Y) {
-    pR = __builtin_fmaf(c[0], vy, pR);
-  } else if (MODE == MODE_J_MINUS_Y) {
-    pR = __builtin_fmaf(c[0], vj, pR);
-    pR = __builtin_fmaf(-c[1], vy, pR);
-  } else if (MODE == MODE_ABS_J) {
-    pR = __builtin_fmaf(c[0], vj, pR);
-    pI = __builtin_fmaf(c[1], vj, pI);
+}
+
+template <int MODE>
+FJ_HD inline void trapz_station(double rs, float ir, const LaneK& L, const float* c, float& pR,
+                                float& pI) {
+  using M = ModeT<MODE>;
+  float vj = 0.0f, vy = 0.0f;
+  const float z = L.zk * ir;  // 8 / x; inf when k = 0 or r = 0
+  if (z <= 8.0f / kFarX) {    // x >= kFarX
+    far_unit<MODE>(rs, L, z, vj, vy);
+  } else if (z <= 1.0f) {     // 8 <= x < kFarX
+    const float t = z * z;
+    const float p0 = fit_P0(t);
+    const float q0 = z * fit_q0(t);
+    float s, co;
+    sincos_rev(phase_rev(L.q, rs), &s, &co);
+    if (M::J) vj = __builtin_fmaf(p0, co, -q0 * s);
+    if (M::Y) vy = __builtin_fmaf(p0, s, q0 * co);
   } else {
-    pR = __builtin_fmaf(c[0], vj, pR);
-    pR = __builtin_fmaf(-c[1], vy, pR);
-    pI = __builtin_fmaf(c[1], vj, pI);
-    pI = __builtin_fmaf(c[0], vy, pI);
+    const double xd = L.q2pi * rs;
+    double y, Lg;
+    const Bess b = bessel_small<false, M::Y, FIT_TABLE>(xd, &y, &Lg);
+    const float ia = L.kf > 0.0f ? sqrtf(kPiOver2 * (float)xd) : sqrtf((float)rs);
+    vj = b.j0 * ia;
+    vy = b.y0 * ia;
   }
+  trapz_add<MODE>(c, vj, vy, pR, pI);
+}
+
+// x >= kFarX on every lane (a far block): no branch on the argument.
+template <int MODE>
+FJ_HD inline void trapz_far(double rs, float ir, const LaneK& L, const float* c, float& pR,
+                            float& pI) {
+  float vj = 0.0f, vy = 0.0f;
+  far_unit<MODE>(rs, L, L.zk * ir, vj, vy);
+  trapz_add<MODE>(c, vj, vy, pR, pI);
 }
 
 template <int MODE>
@@ -386,7 +475,10 @@ FJ_HD inline float trapz_lane(const Stations& S, const float* __restrict__ cf, i
 // g~ = x Y1 + 2/pi and D~ = D + 2/pi, which removes a cancellation of up to
 // ~1/(x^2 ln x) at small x.  G is stored shifted (G + 2x/pi) where it comes
 // from the small-argument branch (sh = 1) and unshifted otherwise.
-// The lane forms E = 2 D (and 2 D~) and the coefficients carry beta / 2.
+// E = 2 D (and 2 D~) with b = beta / 2 (station_coef); the rule's form of
+// every E is collected per station into the (a, p, t) records, so the
+// station loop adds three products per spectrum and evaluates E itself
+// only on the segments that take the G difference.
 struct Ends {  // per-basis values at a station
   float g, gp, g3, G, sh;
 };
@@ -461,19 +553,51 @@ struct LinState {
   Ends pj = {0.f, 0.f, 0.f, 0.f, 0.f}, py = {0.f, 0.f, 0.f, 0.f, 0.f};
   float xprev = 0.0f;
   bool gnext = false;  // the segment right of the previous station uses the G difference
-  float pR = 0.0f, pI = 0.0f;
+  bool lnext = false;  // ... is long (not in the records; uniform)
+  // float partial sums of the a g, p g' and t g''' terms (R and, for the
+  // modulus modes, I)
+  float aR = 0.0f, aI = 0.0f, pR = 0.0f, pI = 0.0f, tR = 0.0f, tI = 0.0f;
 };
 
-// End of a station: its g, g' and g''' per basis, the divided difference
-// of the segment to its left (2 D_s: Euler-Maclaurin, or the G difference
-// on the lanes whose segment needs it -- evaluated only when some lane of
-// the wave does), and the station's contribution alpha*g - (beta/2)*2D.
-template <int MODE, bool FAR>
-FJ_HD inline void linseg_tail(int s, float x, float dxL, bool useG_L, float j0, float j1, float y0,
-                              float y1, float gy, Ends& cj, Ends& cy, const float* c,
-                              LinState<MODE>& st) {
+// R (+= w . v) and I for one quantity v per basis (vJ, vY) and one
+// coefficient per spectrum (w[0], w[1]):
+//   J: R = w0 vJ.   Y: R = w0 vY.   J - Y: R = w0 vJ - w1 vY.
+//   |J| (Re, Im spectra): R = w0 vJ, I = w1 vJ.
+//   |H| = |sum U (J + iY)|: R = w0 vJ - w1 vY, I = w1 vJ + w0 vY.
+template <int MODE>
+FJ_HD inline void lin_acc(const float* w, float vJ, float vY, float& R, float& I) {
+  if (MODE == MODE_J) {
+    R = __builtin_fmaf(w[0], vJ, R);
+  } else if (MODE == MODE_Y) {
+    R = __builtin_fmaf(w[0], vY, R);
+  } else if (MODE == MODE_J_MINUS_Y) {
+    R = __builtin_fmaf(w[0], vJ, R);
+    R = __builtin_fmaf(-w[1], vY, R);
+  } else if (MODE == MODE_ABS_J) {
+    R = __builtin_fmaf(w[0], vJ, R);
+    I = __builtin_fmaf(w[1], vJ, I);
+  } else {  // MODE_ABS_H
+    R = __builtin_fmaf(w[0], vJ, R);
+    R = __builtin_fmaf(-w[1], vY, R);
+    I = __builtin_fmaf(w[1], vJ, I);
+    I = __builtin_fmaf(w[0], vY, I);
+  }
+}
+
+// End of a station: its g, g' and g''' per basis go into the three partial
+// sums with the station's (a, p, t) record c.  The Euler-Maclaurin form of
+// every segment but the long ones is already in those records.  Two rare
+// cases need the segment to the left (a wave-uniform branch; the b record
+// cb is used only there): a lane whose segment takes the G difference adds
+// b (E^rule - E^G) per basis, and a long segment (longL, uniform) adds
+// -b E on every lane, E by the rule or the G difference as the lane chose.
+template <int MODE>
+FJ_HD inline void linseg_tail(int s, float x, float dxL, bool useG_L, bool longL, float j0,
+                              float j1, float y0, float y1, float gy, Ends& cj, Ends& cy,
+                              const float* c, const float* cb, LinState<MODE>& st) {
   using M = ModeT<MODE>;
   constexpr bool NJ = M::J, NY = M::Y;
+  constexpr int NS = M::nspec;
   if (NJ) {
     cj.g = x * j1;
     cj.gp = x * j0;
@@ -484,92 +608,62 @@ FJ_HD inline void linseg_tail(int s, float x, float dxL, bool useG_L, float j0, 
     cy.gp = x * y0;
     cy.g3 = -(y1 + cy.gp);
   }
-  float Ej = 0.0f, Ey = 0.0f;  // 2 D_s per basis
-  // Two forms, chosen by measurement on the MI355X: straight-line (far
-  // blocks, and the two-spectrum modes everywhere) or a per-lane branch (the
-  // one-spectrum modes' general loop, which it keeps lighter in registers).
-  constexpr bool SELECT = FAR || ModeT<MODE>::nspec == 2;
-  if (SELECT && s > 0) {
-    // the rule everywhere, the G difference selected on the lanes that need
-    // it when any does (a wave-uniform branch)
-    const float e1 = dxL * (-1.0f / 6.0f);
-    const float e3 = dxL * dxL * dxL * (1.0f / 360.0f);
-    if (NJ) Ej = em_divided2(st.pj, cj, e1, e3);
-    if (NY) Ey = em_divided2(st.py, cy, e1, e3);
-    if (wave_any(useG_L)) {
+  lin_acc<MODE>(c, cj.g, cy.g, st.aR, st.aI);
+  lin_acc<MODE>(c + NS, cj.gp, cy.gp, st.pR, st.pI);
+  lin_acc<MODE>(c + 2 * NS, cj.g3, cy.g3, st.tR, st.tI);
+  // never in a SMOOTH block; G is formed only on the lanes that use it
+  if (s > 0 && (longL || wave_any(useG_L))) {
+    float dJ = 0.0f, dY = 0.0f;
+    if (useG_L) {   // per lane: the G difference (and the rule, for a short segment)
       const float inv2 = rcp_f(0.5f * dxL);
-      if (NJ) {
-        const float g = (cj.G - st.pj.G) * inv2;
-        Ej = useG_L ? g : Ej;
+      if (NJ) dJ = -((cj.G - st.pj.G) * inv2);
+      if (NY) dY = -gdiff_y2(st.py, st.xprev, cy, x, inv2);
+      if (!longL) {   // uniform
+        const float e1 = dxL * (-1.0f / 6.0f);
+        const float e3 = dxL * dxL * dxL * (1.0f / 360.0f);
+        if (NJ) dJ += em_divided2(st.pj, cj, e1, e3);
+        if (NY) dY += em_divided2(st.py, cy, e1, e3);
       }
-      if (NY) {
-        const float g = gdiff_y2(st.py, st.xprev, cy, x, inv2);
-        Ey = useG_L ? g : Ey;
-      }
-    }
-  } else if (s > 0) {
-    // per lane: one of the two (fewer live registers)
-    if (useG_L) {
-      const float inv2 = rcp_f(0.5f * dxL);
-      if (NJ) Ej = (cj.G - st.pj.G) * inv2;
-      if (NY) Ey = gdiff_y2(st.py, st.xprev, cy, x, inv2);
-    } else {
+    } else if (longL) {   // the rule, on the lanes of a long segment without G
       const float e1 = dxL * (-1.0f / 6.0f);
       const float e3 = dxL * dxL * dxL * (1.0f / 360.0f);
-      if (NJ) Ej = em_divided2(st.pj, cj, e1, e3);
-      if (NY) Ey = em_divided2(st.py, cy, e1, e3);
+      if (NJ) dJ = -em_divided2(st.pj, cj, e1, e3);
+      if (NY) dY = -em_divided2(st.py, cy, e1, e3);
     }
+    lin_acc<MODE>(cb, dJ, dY, st.aR, st.aI);
   }
   st.xprev = x;
-  if (MODE == MODE_J) {
-    st.pR = __builtin_fmaf(c[0], cj.g, st.pR);
-    st.pR = __builtin_fmaf(-c[1], Ej, st.pR);
-  } else if (MODE == MODE_Y) {
-    st.pR = __builtin_fmaf(c[0], cy.g, st.pR);
-    st.pR = __builtin_fmaf(-c[1], Ey, st.pR);
-  } else if (MODE == MODE_J_MINUS_Y) {
-    st.pR = __builtin_fmaf(c[0], cj.g, st.pR);
-    st.pR = __builtin_fmaf(-c[1], Ej, st.pR);
-    st.pR = __builtin_fmaf(-c[2], cy.g, st.pR);
-    st.pR = __builtin_fmaf(c[3], Ey, st.pR);
-  } else if (MODE == MODE_ABS_J) {
-    st.pR = __builtin_fmaf(c[0], cj.g, st.pR);
-    st.pR = __builtin_fmaf(-c[1], Ej, st.pR);
-    st.pI = __builtin_fmaf(c[2], cj.g, st.pI);
-    st.pI = __builtin_fmaf(-c[3], Ej, st.pI);
-  } else {  // MODE_ABS_H
-    const float jre = __builtin_fmaf(c[0], cj.g, -c[1] * Ej);
-    const float jim = __builtin_fmaf(c[2], cj.g, -c[3] * Ej);
-    const float yre = __builtin_fmaf(c[0], cy.g, -c[1] * Ey);
-    const float yim = __builtin_fmaf(c[2], cy.g, -c[3] * Ey);
-    st.pR += jre - yim;
-    st.pI += jim + yre;
-  }
   st.pj = cj;
   st.py = cy;
 }
 
 // One station of the linear-segment sum.  dxL = k (r_s - r_{s-1}),
-// dxR = k (r_{s+1} - r_s) (0 past the last station), c = alpha, beta/2
-// record per spectrum.  FAR: x >= kFarX on every lane of the wave.
-template <int MODE, bool FAR>
+// dxR = k (r_{s+1} - r_s) (0 past the last station), hR = r_{s+1} - r_s,
+// hlim = the bits of the frequency's long-segment limit, c = the station's
+// (a, p, t) record, cb = its b record.  FAR: x >= kFarX on every lane of
+// the wave.  SMOOTH (with FAR): no segment at this station is long or takes
+// the G difference on any lane, so no per-station test is made.
+template <int MODE, bool FAR, bool SMOOTH = false>
 FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dxL, float dxR,
-                                 const LaneK& L, const float* c, LinState<MODE>& st) {
+                                 float hR, uint32_t hlim, const LaneK& L, const float* c,
+                                 const float* cb, LinState<MODE>& st) {
   using M = ModeT<MODE>;
   constexpr bool NJ = M::J, NY = M::Y;
   const double xd = L.q2pi * rs;
   const float x = (float)xd;
-  const bool useG_L = s > 0 && st.gnext;
+  const bool useG_L = !SMOOTH && s > 0 && st.gnext;
+  const bool longL = !SMOOTH && st.lnext;
+  st.lnext = !SMOOTH && f32_bits(hR) > hlim;
   // in a far block both ends of every segment have x >= kFarX, where the
   // rule's threshold min(kTau, kKappa min(x_a, x_b)) is kTau
-  const bool gR = FAR ? fabsf(dxR) >= kTau : use_gdiff_right(x, dxR);
+  const bool gR = SMOOTH ? false : FAR ? fabsf(dxR) >= kTau : use_gdiff_right(x, dxR);
   st.gnext = gR;
   const bool needG = useG_L || gR;
   Ends cj = {0.f, 0.f, 0.f, 0.f, 0.f}, cy = {0.f, 0.f, 0.f, 0.f, 0.f};
   float j0, j1, y0 = 0.0f, y1 = 0.0f, gy = 0.0f;
   if (FAR) {
-    large_station<NJ, NY, true>(L, rs, ir, isr, x, wave_any(needG), &j0, &j1, &y0, &y1, &cj.G,
-                                &cy.G);
+    large_station<NJ, NY, true>(L, rs, ir, isr, x, !SMOOTH && wave_any(needG), &j0, &j1, &y0,
+                                &y1, &cj.G, &cy.G);
     if (NY) gy = __builtin_fmaf(x, y1, kTwoOverPi);
   } else if (x >= kSplit) {
     large_station<NJ, NY>(L, rs, ir, isr, x, needG, &j0, &j1, &y0, &y1, &cj.G, &cy.G);
@@ -585,13 +679,15 @@ FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dx
     if (needG) g_small<FJ_FIT_I1>(xd, y, Lg, NJ, NY, &cj.G, &cy.G);
     cy.sh = 1.0f;
   }
-  linseg_tail<MODE, FAR>(s, x, dxL, useG_L, j0, j1, y0, y1, gy, cj, cy, c, st);
+  linseg_tail<MODE>(s, x, dxL, useG_L, longL, j0, j1, y0, y1, gy, cj, cy, c, cb, st);
 }
 
-// Far blocks in the linear-segment loop.  Measured on the MI355X they pay
-// for the two-spectrum modes (J - Y: 69 -> 58 ms per 512 dense columns) and
-// not for the one-spectrum modes, whose general loop is lighter in
-// registers; -DFJ_FAR_LINSEG_ONE=1 turns them on there (a tuning switch).
+// Far blocks in the linear-segment loop that still test every station (a
+// long segment or a G difference somewhere in the block).  Measured on the
+// MI355X they pay for the two-spectrum modes (J - Y: 69 -> 58 ms per 512
+// dense columns) and not for the one-spectrum modes, whose general loop is
+// lighter in registers; -DFJ_FAR_LINSEG_ONE=1 turns them on there (a tuning
+// switch).  Smooth far blocks (no test at all) run in every mode.
 #ifndef FJ_FAR_LINSEG_ONE
 #define FJ_FAR_LINSEG_ONE 0
 #endif
@@ -600,52 +696,99 @@ struct LinFar {
   static constexpr bool on = ModeT<MODE>::nspec == 2 || FJ_FAR_LINSEG_ONE != 0;
 };
 
-// cf: NC = 2 nspec floats per station: (alpha, beta/2) per spectrum.
+// Stations per block of the linear-segment loop.  4, not kUnroll: with 8,
+// the (a, p, t) records (up to 48 floats) and the station data exceed the
+// scalar registers and spill to vector lanes (J - Y: 718 ms against 447 ms
+// per 4096 dense columns; J: 280 ms against 249 ms).  -DFJ_LIN_UNROLL=N
+// overrides it (a divisor of kUnroll).
+#ifndef FJ_LIN_UNROLL
+#define FJ_LIN_UNROLL 4
+#endif
+// cf: the coefficients of one frequency (coef_block floats): NC = 3 nspec
+// floats (a, p, t) per station, the b plane (nspec floats per station),
+// then the long-segment limit.  The lane sums the three kinds of term in
+// float partial sums and, every kPartial stations, adds a + k p + k^3 t to
+// a double accumulator.
 template <int MODE>
 FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, int nrp, double q) {
-  constexpr int NC = 2 * ModeT<MODE>::nspec;
+  constexpr int NS = ModeT<MODE>::nspec;
+  constexpr int NC = 3 * NS;
+  constexpr int LU = FJ_LIN_UNROLL;   // divides kUnroll, so nrp is a multiple of it
+  static_assert(kUnroll % LU == 0 && kPartial % LU == 0, "station block");
+  const float* __restrict__ cfb = cf + (size_t)nrp * NC;
+  const uint32_t hlim = f32_bits(cf[(size_t)nrp * 4 * NS]);
   const LaneK L = lane_consts(q);
   const uint32_t irfar = far_threshold(L.q2pi);
+  const uint32_t hsm = smooth_threshold(L.q2pi, hlim);
   LinState<MODE> st;
   double accR = 0.0, accI = 0.0;
+  const double k1 = L.q2pi, k3 = L.q2pi * L.q2pi * L.q2pi;
+  auto flush = [&](LinState<MODE>& t) {
+    accR += (double)t.aR + k1 * (double)t.pR + k3 * (double)t.tR;
+    t.aR = t.pR = t.tR = 0.0f;
+    if (MODE == MODE_ABS_J || MODE == MODE_ABS_H) {
+      accI += (double)t.aI + k1 * (double)t.pI + k3 * (double)t.tI;
+      t.aI = t.pI = t.tI = 0.0f;
+    }
+  };
   float dxR = 0.0f;
-  for (int sb = 0; sb < nrp; sb += kUnroll) {
-    double rv[kUnroll];
-    float hv[kUnroll], iv[kUnroll + 1], sv[kUnroll];
-    float cv[kUnroll * NC];
+  for (int sb = 0; sb < nrp; sb += LU) {
+    double rv[LU];
+    float hw[LU + 1], iv[LU + 1], sv[LU];   // hw[k + 1]: gap right of station sb + k
+    float cv[LU * NC];
 #pragma unroll
-    for (int k = 0; k < kUnroll; k++) rv[k] = S.rr[sb + k];
+    for (int k = 0; k < LU; k++) rv[k] = S.rr[sb + k];
 #pragma unroll
-    for (int k = 0; k < kUnroll; k++) hv[k] = S.hh[sb + k + 1];
+    for (int k = 0; k <= LU; k++) hw[k] = S.hh[sb + k];
 #pragma unroll
-    for (int k = 0; k <= kUnroll; k++) iv[k] = S.ir[sb + k];   // + the next block's first
+    for (int k = 0; k <= LU; k++) iv[k] = S.ir[sb + k];   // + the next block's first
 #pragma unroll
-    for (int k = 0; k < kUnroll; k++) sv[k] = S.isr[sb + k];
+    for (int k = 0; k < LU; k++) sv[k] = S.isr[sb + k];
 #pragma unroll
-    for (int k = 0; k < kUnroll * NC; k++) cv[k] = cf[sb * NC + k];
-    if (LinFar<MODE>::on && block_far<kUnroll + 1>(iv, irfar)) {
+    for (int k = 0; k < LU * NC; k++) cv[k] = cf[sb * NC + k];
+    const bool far = block_far<LU + 1>(iv, irfar);
+    if (far && block_far<LU + 1>(hw, hsm)) {
+      // every gap of the block (its left one included) is below both the
+      // G-difference and the long-segment limits on every lane
 #pragma unroll
-      for (int k = 0; k < kUnroll; k++) {
+      for (int k = 0; k < LU; k++) {
         const float dxL = dxR;
-        dxR = L.kf * hv[k];
-        linseg_station<MODE, true>(sb + k, rv[k], iv[k], sv[k], dxL, dxR, L, cv + k * NC, st);
+        dxR = L.kf * hw[k + 1];
+        linseg_station<MODE, true, true>(sb + k, rv[k], iv[k], sv[k], dxL, dxR, hw[k + 1], hlim,
+                                         L, cv + k * NC, nullptr, st);
       }
     } else {
+      // One-spectrum modes fetch the block's b records here, once, rather
+      // than by a wait in each station's rare branch (not rare in arrays
+      // of long segments: batched sub-arrays 658 -> 643 ms).  The
+      // two-spectrum modes read them in the branch: the 8 more scalar
+      // registers spill (J - Y 408 -> 438 ms per 4096 dense columns).
+      constexpr bool PRE = NS == 1;
+      float bv[LU * NS];
 #pragma unroll
-      for (int k = 0; k < kUnroll; k++) {
-        const float dxL = dxR;
-        dxR = L.kf * hv[k];
-        linseg_station<MODE, false>(sb + k, rv[k], iv[k], sv[k], dxL, dxR, L, cv + k * NC, st);
+      for (int k = 0; k < LU * NS; k++) bv[k] = PRE ? cfb[(size_t)sb * NS + k] : 0.0f;
+      auto brec = [&](int k) { return PRE ? bv + k * NS : cfb + (size_t)(sb + k) * NS; };
+      if (LinFar<MODE>::on && far) {
+#pragma unroll
+        for (int k = 0; k < LU; k++) {
+          const float dxL = dxR;
+          dxR = L.kf * hw[k + 1];
+          linseg_station<MODE, true>(sb + k, rv[k], iv[k], sv[k], dxL, dxR, hw[k + 1], hlim, L,
+                                     cv + k * NC, brec(k), st);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < LU; k++) {
+          const float dxL = dxR;
+          dxR = L.kf * hw[k + 1];
+          linseg_station<MODE, false>(sb + k, rv[k], iv[k], sv[k], dxL, dxR, hw[k + 1], hlim, L,
+                                      cv + k * NC, brec(k), st);
+        }
       }
     }
-    if (((sb + kUnroll) & (kPartial - 1)) == 0) {
-      accR += (double)st.pR;
-      accI += (double)st.pI;
-      st.pR = st.pI = 0.0f;
-    }
+    if (((sb + LU) & (kPartial - 1)) == 0) flush(st);
   }
-  accR += (double)st.pR;
-  accI += (double)st.pI;
+  flush(st);
   const double inv_k2 = 1.0 / (L.q2pi * L.q2pi);
   if (MODE == MODE_ABS_J || MODE == MODE_ABS_H)
     return (float)(sqrt(accR * accR + accI * accI) * inv_k2);

@@ -140,27 +140,23 @@ __global__ void k_mwfj_stations(const float* __restrict__ r, int nr, int ntot,
 }
 
 // sqrt(2/(pi k)) * (that value) = Z0(k r) sqrt(r): the kernel-matrix entry
-// of one lane (velocity) and station.  FAR: x >= kFarX on every lane of
-// the wave (degree-2 fits, no branch); otherwise the large-argument fits
-// for x >= 8 and the double polynomials below.
+// of one lane (velocity) and station.  x >= kFarX: the modulus-phase form
+// (one transcendental; in a FAR block, where every lane of the wave has
+// x >= kFarX, with no branch); otherwise the large-argument fits for x >= 8
+// and the double polynomials below.  Both paths give an entry the same
+// value, whichever lanes share its wave.
 template <bool Y, bool FAR>
 __device__ inline float mwfj_value(double rs, float ir, const LaneK& L) {
   const float z = L.zk * ir;   // 8 / x
-  if (FAR) {   // modulus-phase form, one transcendental
+  if (FAR || z <= 8.0f / kFarX) {
     const float t = z * z;
     const float u = __builtin_fmaf(z, fitf_ph0(t), phase_rev(L.q, rs));
     return fitf_M0(t) * (Y ? sin_rev(u) : cos_rev(u));
   }
   if (z <= 1.0f) {
     const float t = z * z;
-    float p0, q0;
-    if (z <= 8.0f / kFarX) {
-      p0 = fitf_P0(t);
-      q0 = z * fitf_q0(t);
-    } else {
-      p0 = fit_P0(t);
-      q0 = z * fit_q0(t);
-    }
+    const float p0 = fit_P0(t);
+    const float q0 = z * fit_q0(t);
     float sn, co;
     sincos_rev(phase_rev(L.q, rs), &sn, &co);
     return Y ? __builtin_fmaf(p0, sn, q0 * co) : __builtin_fmaf(p0, co, -q0 * sn);

@@ -123,13 +123,12 @@ __global__ void k_label_argmax(const float* __restrict__ X, int B, int H, int W,
 // cannot check them).  Fewer than two stations: zeros, as the full grid.
 template <int MODE, int ITYPE>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj_items(
+__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE, true>::waves, 8))) void k_fj_items(
     Stations S, const float* __restrict__ coef,
                                                  const float* __restrict__ fv,
                                                  const float* __restrict__ cv,
                                                  const int* __restrict__ items, int nr, int nrp,
                                                  int nf, int nc, float* __restrict__ out) {
-  constexpr int NC = ITYPE == 0 ? ModeT<MODE>::nspec : 2 * ModeT<MODE>::nspec;
   const int* it = items + 3 * (size_t)blockIdx.x;
   const int fi = it[0], c0 = it[1];
   const int cnt = min(it[2], nc - c0);   // item clipped to the grid
@@ -143,7 +142,7 @@ __attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj_ite
     return;
   }
   const double q = (double)fv[fi] / (double)cv[j];
-  const float* __restrict__ cf = coef + (size_t)fi * nrp * NC;
+  const float* __restrict__ cf = coef + (size_t)fi * coef_block(nrp, MODE, ITYPE);
   const float v = ITYPE == 0 ? trapz_lane<MODE>(S, cf, nrp, q) : linseg_lane<MODE>(S, cf, nrp, q);
   if (store) out[(size_t)j * nf + fi] = v;
 }
