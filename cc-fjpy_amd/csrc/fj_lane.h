@@ -233,22 +233,16 @@ FJ_HD inline float long_segment_limit(float f, float cmin) {
 // to 3 waves, a few spills on its cold small-argument path included (held
 // to 4 it spills on the hot path: 490 ms against 408 ms for 19,900 pairs x
 // 4096 f x 3000 c); the other modes are fastest at their natural count.
-// The linear-segment J kernel is held to 5 waves (96 VGPRs, a few spills
-// off the smooth far path) in the dense kernel, where smooth far blocks
-// dominate: 247 ms against 266 ms at its natural 4 waves on the same grid.
-// The persistent batched kernel and the item-list kernel, whose small
-// sub-arrays spend more of their time in near and non-smooth blocks, keep
-// 4 (FjOcc's SMALL form).
-// -DFJ_OCC_I1_JMY=N / -DFJ_OCC_I1_J=N / -DFJ_OCC_I1_J_SMALL=N override for
-// tuning runs.
+// The linear-segment J kernels are held to 5 waves (96 VGPRs, a few spills
+// off the smooth far path): 19,900 pairs x 4096 f x 3000 c in 247 ms
+// against 266 ms at the natural 4 waves, and 4096 ragged sub-arrays in the
+// persistent kernel 658 ms against 732 ms.
+// -DFJ_OCC_I1_JMY=N / -DFJ_OCC_I1_J=N override for tuning runs.
 #ifndef FJ_OCC_I1_JMY
 #define FJ_OCC_I1_JMY 3
 #endif
 #ifndef FJ_OCC_I1_J
 #define FJ_OCC_I1_J 5
-#endif
-#ifndef FJ_OCC_I1_J_SMALL
-#define FJ_OCC_I1_J_SMALL 5
 #endif
 // How the linear-segment kernels hold the small-argument (x < 8) double
 // polynomials.  Measured on the MI355X (J kernel, 19,900 pairs x 512 f x
@@ -262,13 +256,10 @@ FJ_HD inline float long_segment_limit(float f, float cmin) {
 #ifndef FJ_FIT_I1
 #define FJ_FIT_I1 FIT_LITERAL
 #endif
-template <int MODE, int ITYPE, bool SMALL = false>
+template <int MODE, int ITYPE>
 struct FjOcc {
   static constexpr int waves =
-      ITYPE == 0 ? 1
-                 : (MODE == MODE_J_MINUS_Y ? FJ_OCC_I1_JMY
-                                           : (MODE == MODE_J ? (SMALL ? FJ_OCC_I1_J_SMALL : FJ_OCC_I1_J)
-                                                             : 1));
+      ITYPE == 0 ? 1 : (MODE == MODE_J_MINUS_Y ? FJ_OCC_I1_JMY : (MODE == MODE_J ? FJ_OCC_I1_J : 1));
 };
 
 // ---- coefficients of station s (one frequency) ----------------------------
@@ -691,9 +682,15 @@ FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dx
 #ifndef FJ_FAR_LINSEG_ONE
 #define FJ_FAR_LINSEG_ONE 0
 #endif
-template <int MODE>
+// Smooth far blocks in the one-spectrum modes pay in the dense kernel (J:
+// 19,900 pairs x 4096 f x 3000 c in 248 ms against 300 ms without) and not
+// in the kernels for small arrays (SMALL: the persistent batched kernel and
+// the item-list kernel), whose blocks are seldom smooth: 4096 ragged
+// sub-arrays 631 ms without against 645 ms with.
+template <int MODE, bool SMALL = false>
 struct LinFar {
   static constexpr bool on = ModeT<MODE>::nspec == 2 || FJ_FAR_LINSEG_ONE != 0;
+  static constexpr bool smooth = ModeT<MODE>::nspec == 2 || !SMALL;
 };
 
 // Stations per block of the linear-segment loop.  4, not kUnroll: with 8,
@@ -709,8 +706,9 @@ struct LinFar {
 // then the long-segment limit.  The lane sums the three kinds of term in
 // float partial sums and, every kPartial stations, adds a + k p + k^3 t to
 // a double accumulator.
-template <int MODE>
+template <int MODE, bool SMALL = false>
 FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, int nrp, double q) {
+  using LF = LinFar<MODE, SMALL>;
   constexpr int NS = ModeT<MODE>::nspec;
   constexpr int NC = 3 * NS;
   constexpr int LU = FJ_LIN_UNROLL;   // divides kUnroll, so nrp is a multiple of it
@@ -746,8 +744,8 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
     for (int k = 0; k < LU; k++) sv[k] = S.isr[sb + k];
 #pragma unroll
     for (int k = 0; k < LU * NC; k++) cv[k] = cf[sb * NC + k];
-    const bool far = block_far<LU + 1>(iv, irfar);
-    if (far && block_far<LU + 1>(hw, hsm)) {
+    const bool far = (LF::smooth || LF::on) && block_far<LU + 1>(iv, irfar);
+    if (LF::smooth && far && block_far<LU + 1>(hw, hsm)) {
       // every gap of the block (its left one included) is below both the
       // G-difference and the long-segment limits on every lane
 #pragma unroll
@@ -764,11 +762,14 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
       // two-spectrum modes read them in the branch: the 8 more scalar
       // registers spill (J - Y 408 -> 438 ms per 4096 dense columns).
       constexpr bool PRE = NS == 1;
-      float bv[LU * NS];
+      float bv[PRE ? LU * NS : 1];
+      if (PRE) {
 #pragma unroll
-      for (int k = 0; k < LU * NS; k++) bv[k] = PRE ? cfb[(size_t)sb * NS + k] : 0.0f;
-      auto brec = [&](int k) { return PRE ? bv + k * NS : cfb + (size_t)(sb + k) * NS; };
-      if (LinFar<MODE>::on && far) {
+        for (int k = 0; k < LU * NS; k++) bv[k] = cfb[(size_t)sb * NS + k];
+      }
+      const float* bsrc = PRE ? bv : cfb + (size_t)sb * NS;
+      auto brec = [&](int k) { return bsrc + k * NS; };
+      if (LF::on && far) {
 #pragma unroll
         for (int k = 0; k < LU; k++) {
           const float dxL = dxR;

@@ -123,7 +123,7 @@ __global__ void k_label_argmax(const float* __restrict__ X, int B, int H, int W,
 // cannot check them).  Fewer than two stations: zeros, as the full grid.
 template <int MODE, int ITYPE>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE, true>::waves, 8))) void k_fj_items(
+__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj_items(
     Stations S, const float* __restrict__ coef,
                                                  const float* __restrict__ fv,
                                                  const float* __restrict__ cv,
@@ -143,7 +143,7 @@ __attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE, true>::waves, 8))) void k_
   }
   const double q = (double)fv[fi] / (double)cv[j];
   const float* __restrict__ cf = coef + (size_t)fi * coef_block(nrp, MODE, ITYPE);
-  const float v = ITYPE == 0 ? trapz_lane<MODE>(S, cf, nrp, q) : linseg_lane<MODE>(S, cf, nrp, q);
+  const float v = ITYPE == 0 ? trapz_lane<MODE>(S, cf, nrp, q) : linseg_lane<MODE, true>(S, cf, nrp, q);
   if (store) out[(size_t)j * nf + fi] = v;
 }
 
