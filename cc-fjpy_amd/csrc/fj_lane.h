@@ -233,16 +233,22 @@ FJ_HD inline float long_segment_limit(float f, float cmin) {
 // to 3 waves, a few spills on its cold small-argument path included (held
 // to 4 it spills on the hot path: 490 ms against 408 ms for 19,900 pairs x
 // 4096 f x 3000 c); the other modes are fastest at their natural count.
-// The linear-segment J kernels are held to 5 waves (96 VGPRs, a few spills
-// off the smooth far path): 19,900 pairs x 4096 f x 3000 c in 247 ms
-// against 266 ms at the natural 4 waves, and 4096 ragged sub-arrays in the
-// persistent kernel 658 ms against 732 ms.
-// -DFJ_OCC_I1_JMY=N / -DFJ_OCC_I1_J=N override for tuning runs.
+// The dense linear-segment J kernel is held to 6 waves (80 VGPRs; its smooth
+// far blocks are scheduled station by station, so that they stay clear of
+// the spills): 19,900 pairs x 4096 f x 3000 c in 235 ms against 250 ms at 5
+// waves and 266 ms at the natural 4.  The kernels for small arrays (SMALL:
+// persistent batched, item list) hold 5: 4096 ragged sub-arrays 658 ms
+// against 732 ms at 4.
+// -DFJ_OCC_I1_JMY=N / -DFJ_OCC_I1_J=N / -DFJ_OCC_I1_J_SMALL=N override for
+// tuning runs.
 #ifndef FJ_OCC_I1_JMY
 #define FJ_OCC_I1_JMY 3
 #endif
 #ifndef FJ_OCC_I1_J
-#define FJ_OCC_I1_J 5
+#define FJ_OCC_I1_J 6
+#endif
+#ifndef FJ_OCC_I1_J_SMALL
+#define FJ_OCC_I1_J_SMALL 5
 #endif
 // How the linear-segment kernels hold the small-argument (x < 8) double
 // polynomials.  Measured on the MI355X (J kernel, 19,900 pairs x 512 f x
@@ -256,10 +262,13 @@ FJ_HD inline float long_segment_limit(float f, float cmin) {
 #ifndef FJ_FIT_I1
 #define FJ_FIT_I1 FIT_LITERAL
 #endif
-template <int MODE, int ITYPE>
+template <int MODE, int ITYPE, bool SMALL = false>
 struct FjOcc {
   static constexpr int waves =
-      ITYPE == 0 ? 1 : (MODE == MODE_J_MINUS_Y ? FJ_OCC_I1_JMY : (MODE == MODE_J ? FJ_OCC_I1_J : 1));
+      ITYPE == 0 ? 1
+      : MODE == MODE_J_MINUS_Y ? FJ_OCC_I1_JMY
+      : MODE == MODE_J ? (SMALL ? FJ_OCC_I1_J_SMALL : FJ_OCC_I1_J)
+      : 1;
 };
 
 // ---- coefficients of station s (one frequency) ----------------------------
@@ -701,6 +710,12 @@ struct LinFar {
 #ifndef FJ_LIN_UNROLL
 #define FJ_LIN_UNROLL 4
 #endif
+#ifndef FJ_LIN_UNROLL_SMALL
+#define FJ_LIN_UNROLL_SMALL 4
+#endif
+#ifndef FJ_PRE_SMALL
+#define FJ_PRE_SMALL 1
+#endif
 // cf: the coefficients of one frequency (coef_block floats): NC = 3 nspec
 // floats (a, p, t) per station, the b plane (nspec floats per station),
 // then the long-segment limit.  The lane sums the three kinds of term in
@@ -711,7 +726,7 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
   using LF = LinFar<MODE, SMALL>;
   constexpr int NS = ModeT<MODE>::nspec;
   constexpr int NC = 3 * NS;
-  constexpr int LU = FJ_LIN_UNROLL;   // divides kUnroll, so nrp is a multiple of it
+  constexpr int LU = SMALL ? FJ_LIN_UNROLL_SMALL : FJ_LIN_UNROLL;   // divides kUnroll (nrp's multiple)
   static_assert(kUnroll % LU == 0 && kPartial % LU == 0, "station block");
   const float* __restrict__ cfb = cf + (size_t)nrp * NC;
   const uint32_t hlim = f32_bits(cf[(size_t)nrp * 4 * NS]);
@@ -754,6 +769,11 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
         dxR = L.kf * hw[k + 1];
         linseg_station<MODE, true, true>(sb + k, rv[k], iv[k], sv[k], dxL, dxR, hw[k + 1], hlim,
                                          L, cv + k * NC, nullptr, st);
+#if defined(__HIP_DEVICE_COMPILE__)
+        // one-spectrum modes: no interleaving of stations, which at their
+        // occupancy would spill (J - Y, at 3 waves, gains nothing from it)
+        if (NS == 1) __builtin_amdgcn_sched_barrier(0);
+#endif
       }
     } else {
       // One-spectrum modes fetch the block's b records here, once, rather
@@ -761,7 +781,7 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
       // of long segments: batched sub-arrays 658 -> 643 ms).  The
       // two-spectrum modes read them in the branch: the 8 more scalar
       // registers spill (J - Y 408 -> 438 ms per 4096 dense columns).
-      constexpr bool PRE = NS == 1;
+      constexpr bool PRE = NS == 1 && (!SMALL || FJ_PRE_SMALL != 0);
       float bv[PRE ? LU * NS : 1];
       if (PRE) {
 #pragma unroll

@@ -123,7 +123,7 @@ __global__ void k_label_argmax(const float* __restrict__ X, int B, int H, int W,
 // cannot check them).  Fewer than two stations: zeros, as the full grid.
 template <int MODE, int ITYPE>
 __global__ __launch_bounds__(256)
-__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj_items(
+__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE, true>::waves, 8))) void k_fj_items(
     Stations S, const float* __restrict__ coef,
                                                  const float* __restrict__ fv,
                                                  const float* __restrict__ cv,
