@@ -108,6 +108,33 @@ def test_dense_column_vs_oracle(emul, itype):
     assert emax <= TOL
 
 
+@pytest.mark.parametrize("mode", [MODE_J, MODE_Y, MODE_JMY])
+def test_long_and_short_segments(emul, mode):
+    """Linear segments on an array that mixes clusters of short gaps with
+    gaps of tens of kilometres: over the frequency grid, k_max h runs from
+    far below to far above the long-segment limit (2), so every kind of
+    segment occurs -- in the per-station records, recorded but taking the G
+    difference on some lanes, and long (summed per lane) -- also between
+    neighbours of different kinds."""
+    rng = np.random.default_rng(11)
+    parts = [np.sort(rng.uniform(a, a + w, n)) for a, w, n in
+             ((2e3, 3e3, 40), (30e3, 1e3, 25), (75e3, 40e3, 12), (160e3, 2e3, 30))]
+    r = np.concatenate(parts).astype(np.float32)
+    nr = len(r)
+    f = np.linspace(0.0, 1.5, 24).astype(np.float32)
+    c = np.linspace(300.0, 5000.0, 40).astype(np.float32)
+    U0 = rng.normal(size=(nr, len(f))).astype(np.float32)
+    U1 = rng.normal(size=(nr, len(f))).astype(np.float32)
+    got = run(emul, mode, 1, U0, r, f, c, U1=U1)
+    if mode == MODE_JMY:
+        want = O.oracle64(U0, r, f, c, 1, 0) - O.oracle64(U1, r, f, c, 1, 1)
+    else:
+        want = O.oracle64(U0, r, f, c, 1, 1 if mode == MODE_Y else 0)
+    emax, _, mism = O.column_error(got[:, 1:], want[:, 1:])   # f = 0: NaN in both
+    assert mism == 0
+    assert emax <= TOL
+
+
 def test_unsorted_stations_same_math(emul, golden):
     """Unsorted r (the C level does not sort) still gives the integral the
     reference's formulas define (negative gaps are clamped to 0.1 m)."""
