@@ -705,8 +705,12 @@ struct LinFar {
 // Stations per block of the linear-segment loop.  4, not kUnroll: with 8,
 // the (a, p, t) records (up to 48 floats) and the station data exceed the
 // scalar registers and spill to vector lanes (J - Y: 718 ms against 447 ms
-// per 4096 dense columns; J: 280 ms against 249 ms).  -DFJ_LIN_UNROLL=N
-// overrides it (a divisor of kUnroll).
+// per 4096 dense columns; J: 280 ms against 249 ms).  The same holds in
+// the kernels for small arrays (4096 ragged sub-arrays: 632 ms with 8,
+// without the b prefetch below, against 631 ms).  Tuning switches:
+// -DFJ_LIN_UNROLL=N and -DFJ_LIN_UNROLL_SMALL=N (divisors of kUnroll) set
+// the block, -DFJ_PRE_SMALL=0 turns the b prefetch off in the small-array
+// kernels (646 ms against 631 ms).
 #ifndef FJ_LIN_UNROLL
 #define FJ_LIN_UNROLL 4
 #endif
