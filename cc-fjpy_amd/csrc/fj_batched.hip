@@ -63,7 +63,7 @@ __global__ void k_reset_counter(unsigned int* counter, unsigned int start) {
 // data of the loop -- are scalar loads, as in k_fj.
 template <int MODE, int ITYPE, int TILE>
 __global__ __launch_bounds__(TILE)
-__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE, true>::waves, 8))) void k_fj_b(
+__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj_b(
     int nf, int nc, unsigned int limit, const int64_t* __restrict__ soff,
     const int64_t* __restrict__ poff, const int64_t* __restrict__ coff,
     const int* __restrict__ nrpa, const int* __restrict__ order, const float* __restrict__ fv,

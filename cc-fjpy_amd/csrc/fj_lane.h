@@ -233,13 +233,14 @@ FJ_HD inline float long_segment_limit(float f, float cmin) {
 // to 3 waves, a few spills on its cold small-argument path included (held
 // to 4 it spills on the hot path: 490 ms against 408 ms for 19,900 pairs x
 // 4096 f x 3000 c); the other modes are fastest at their natural count.
-// The dense linear-segment J kernel is held to 6 waves (80 VGPRs; its smooth
-// far blocks are scheduled station by station, so that they stay clear of
-// the spills): 19,900 pairs x 4096 f x 3000 c in 235 ms against 250 ms at 5
-// waves and 266 ms at the natural 4.  The kernels for small arrays (SMALL:
-// persistent batched, item list) hold 5: 4096 ragged sub-arrays 658 ms
-// against 732 ms at 4.
-// -DFJ_OCC_I1_JMY=N / -DFJ_OCC_I1_J=N / -DFJ_OCC_I1_J_SMALL=N override for
+// The dense and the persistent batched linear-segment J kernels are held to
+// 6 waves (80 VGPRs; the dense kernel's smooth far blocks are scheduled
+// station by station, so that they stay clear of the spills): 19,900 pairs
+// x 4096 f x 3000 c in 235 ms against 250 ms at 5 waves and 266 ms at the
+// natural 4; 4096 ragged sub-arrays in 604 ms against 631 ms at 5 and 674
+// ms at 4.  The item-list kernel (ITEMS) holds 4: the learned bands of a
+// 300-pair field in 9.0 ms against 9.5 ms at 5 and 11.6 ms at 6.
+// -DFJ_OCC_I1_JMY=N / -DFJ_OCC_I1_J=N / -DFJ_OCC_I1_J_ITEMS=N override for
 // tuning runs.
 #ifndef FJ_OCC_I1_JMY
 #define FJ_OCC_I1_JMY 3
@@ -247,8 +248,8 @@ FJ_HD inline float long_segment_limit(float f, float cmin) {
 #ifndef FJ_OCC_I1_J
 #define FJ_OCC_I1_J 6
 #endif
-#ifndef FJ_OCC_I1_J_SMALL
-#define FJ_OCC_I1_J_SMALL 5
+#ifndef FJ_OCC_I1_J_ITEMS
+#define FJ_OCC_I1_J_ITEMS 4
 #endif
 // How the linear-segment kernels hold the small-argument (x < 8) double
 // polynomials.  Measured on the MI355X (J kernel, 19,900 pairs x 512 f x
@@ -262,12 +263,12 @@ FJ_HD inline float long_segment_limit(float f, float cmin) {
 #ifndef FJ_FIT_I1
 #define FJ_FIT_I1 FIT_LITERAL
 #endif
-template <int MODE, int ITYPE, bool SMALL = false>
+template <int MODE, int ITYPE, bool ITEMS = false>
 struct FjOcc {
   static constexpr int waves =
       ITYPE == 0 ? 1
       : MODE == MODE_J_MINUS_Y ? FJ_OCC_I1_JMY
-      : MODE == MODE_J ? (SMALL ? FJ_OCC_I1_J_SMALL : FJ_OCC_I1_J)
+      : MODE == MODE_J ? (ITEMS ? FJ_OCC_I1_J_ITEMS : FJ_OCC_I1_J)
       : 1;
 };
 
