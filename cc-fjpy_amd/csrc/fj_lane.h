@@ -696,19 +696,24 @@ FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dx
 // 19,900 pairs x 4096 f x 3000 c in 248 ms against 300 ms without) and not
 // in the kernels for small arrays (SMALL: the persistent batched kernel and
 // the item-list kernel), whose blocks are seldom smooth: 4096 ragged
-// sub-arrays 631 ms without against 645 ms with.
+// sub-arrays 602 ms without against 635 ms with, the learned bands of a
+// 300-pair field 8.9 ms against 9.6 ms (-DFJ_SMOOTH_SMALL=1 turns them on,
+// a tuning switch).
+#ifndef FJ_SMOOTH_SMALL
+#define FJ_SMOOTH_SMALL 0
+#endif
 template <int MODE, bool SMALL = false>
 struct LinFar {
   static constexpr bool on = ModeT<MODE>::nspec == 2 || FJ_FAR_LINSEG_ONE != 0;
-  static constexpr bool smooth = ModeT<MODE>::nspec == 2 || !SMALL;
+  static constexpr bool smooth = ModeT<MODE>::nspec == 2 || !SMALL || FJ_SMOOTH_SMALL != 0;
 };
 
 // Stations per block of the linear-segment loop.  4, not kUnroll: with 8,
 // the (a, p, t) records (up to 48 floats) and the station data exceed the
 // scalar registers and spill to vector lanes (J - Y: 718 ms against 447 ms
 // per 4096 dense columns; J: 280 ms against 249 ms).  The same holds in
-// the kernels for small arrays (4096 ragged sub-arrays: 632 ms with 8,
-// without the b prefetch below, against 631 ms).  Tuning switches:
+// the kernels for small arrays (4096 ragged sub-arrays: 638 ms with 8
+// against 602 ms).  Tuning switches:
 // -DFJ_LIN_UNROLL=N and -DFJ_LIN_UNROLL_SMALL=N (divisors of kUnroll) set
 // the block, -DFJ_PRE_SMALL=0 turns the b prefetch off in the small-array
 // kernels (646 ms against 631 ms).
