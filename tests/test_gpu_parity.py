@@ -89,6 +89,33 @@ def test_fused_modes_match_single_basis(ccfj, ctx, golden, itype):
 
 
 # ------------------------------------------------------- single-array workload
+@pytest.mark.parametrize("mode", ["J", "Y", "J_MINUS_Y"])
+def test_long_and_short_segments(ccfj, ctx, mode):
+    """Linear segments on an array of clusters of short gaps separated by
+    gaps of tens of kilometres, 256 velocities per frequency: the kernel
+    meets smooth far blocks, far blocks with long segments or G
+    differences, and near blocks, and every kind of segment, against the
+    FP64 oracle."""
+    rng = np.random.default_rng(11)
+    parts = [np.sort(rng.uniform(a, a + w, n)) for a, w, n in
+             ((2e3, 3e3, 40), (30e3, 1e3, 25), (75e3, 40e3, 12), (160e3, 2e3, 30))]
+    r = np.concatenate(parts).astype(np.float32)
+    f = np.linspace(0.0, 1.5, 24).astype(np.float32)
+    c = np.linspace(300.0, 5000.0, 256).astype(np.float32)
+    U0 = rng.normal(size=(len(r), len(f))).astype(np.float32)
+    U1 = rng.normal(size=(len(r), len(f))).astype(np.float32)
+    m = getattr(ccfj, "MODE_" + mode)
+    got = native(ctx, m, 1, U0, r, f, c, U1=U1)
+    if mode == "J_MINUS_Y":
+        want = O.oracle64(U0, r, f, c, 1, 0) - O.oracle64(U1, r, f, c, 1, 1)
+    else:
+        want = O.oracle64(U0, r, f, c, 1, 1 if mode == "Y" else 0)
+    emax, _, mism = O.column_error(got[:, 1:], want[:, 1:])   # f = 0: NaN in both
+    print(f"mode {mode}: emax={emax:.2e}")
+    assert mism == 0
+    assert emax <= TOL
+
+
 @pytest.fixture(scope="module")
 def single():
     from ccfj import synth
