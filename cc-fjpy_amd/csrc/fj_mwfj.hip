@@ -98,15 +98,29 @@ __device__ inline double2 rfft_bin(const double2* __restrict__ row, int k, int M
 // B layout [nf][Kp][nrhs]; rhs 2w = Re, 2w+1 = Im of window w.
 // func 0: rows k = s.  func 1: rows s (J half: Re, Im) and half + s
 // (Y half: -Im, Re), so O_re = FJ[Re] - FY[Im], O_im = FJ[Im] + FY[Re].
-__global__ void k_mwfj_coef(const double2* __restrict__ spec, const float* __restrict__ r,
-                            int nr, int w0, int cw, int nf, int fstride, int nbins, int zhalf,
-                            int func, int half, int nrhs, float* __restrict__ B) {
-  const int wl = blockIdx.x * blockDim.x + threadIdx.x;
+// A block takes one station, 64 frequencies and 64 windows of the chunk:
+// it reads the spectra along the bins (coalesced; one twiddle per bin for
+// all 64 windows), transposes through LDS and writes each B row segment of
+// 64 windows x (Re, Im) = 512 bytes in one piece.
+constexpr int kCoefT = 64;
+__global__ __launch_bounds__(256) void k_mwfj_coef(const double2* __restrict__ spec,
+                                                   const float* __restrict__ r, int nr, int w0,
+                                                   int cw, int nf, int fstride, int nbins,
+                                                   int zhalf, int func, int half, int nrhs,
+                                                   float* __restrict__ B) {
+  __shared__ float tre[kCoefT][kCoefT + 1], tim[kCoefT][kCoefT + 1];
+  __shared__ double tsn[kCoefT], tcs[kCoefT];
+  const int f0 = blockIdx.x * kCoefT;
   const int s = blockIdx.y;
-  const int f = blockIdx.z;
-  if (wl >= cw) return;
-  const double2 z = rfft_bin(spec + ((size_t)wl * nr + s) * nbins, f * fstride, nbins, zhalf);
-  const float re = (float)z.x, im = (float)z.y;   // the reference's float32 staging
+  const int wb = blockIdx.z * kCoefT;
+  const int M = nbins;
+  if (threadIdx.x < kCoefT) {
+    const int k = (f0 + threadIdx.x) * fstride;
+    double sn = 0.0, cs = 1.0;
+    if (zhalf) sincospi((double)k / (double)M, &sn, &cs);   // e^{-i pi k/M} = cs - i sn
+    tsn[threadIdx.x] = sn;
+    tcs[threadIdx.x] = cs;
+  }
   const float rs = r[s];
   double w = 0.0;
   if (s > 0) w += (double)fmaxf(rs - r[s - 1], 0.1f);
@@ -115,16 +129,46 @@ __global__ void k_mwfj_coef(const double2* __restrict__ spec, const float* __res
   // sqrt(r) Z0(k r) (its large-argument amplitude without 1/sqrt r) and the
   // contraction's epilogue multiplies by sqrt(2/(pi k))
   const double wr = 0.5 * w * sqrt((double)rs);
-  const float are = (float)(wr * (double)re), aim = (float)(wr * (double)im);
-  const int w2 = 2 * (w0 + wl);
+  __syncthreads();
+  for (int it = threadIdx.x; it < kCoefT * kCoefT; it += 256) {
+    const int fl = it & (kCoefT - 1), wl = it / kCoefT;
+    const int f = f0 + fl, wc = wb + wl;
+    float are = 0.0f, aim = 0.0f;
+    if (f < nf && wc < cw) {
+      const double2* row = spec + ((size_t)wc * nr + s) * nbins;
+      const int k = f * fstride;
+      double zr, zi;
+      if (!zhalf) {
+        const double2 z = row[k];
+        zr = z.x;
+        zi = z.y;
+      } else {   // X[k] = E[k] + e^{-i pi k/M} O[k] (rfft_bin)
+        const double2 a = row[k < M ? k : 0];
+        const double2 b = row[k == 0 ? 0 : M - k];   // Z[M-k], Z[M] = Z[0]
+        const double er = 0.5 * (a.x + b.x), ei = 0.5 * (a.y - b.y);
+        const double orr = 0.5 * (a.y + b.y), oi = -0.5 * (a.x - b.x);
+        const double cs = tcs[fl], sn = tsn[fl];
+        zr = er + cs * orr + sn * oi;
+        zi = ei + cs * oi - sn * orr;
+      }
+      const float re = (float)zr, im = (float)zi;   // the reference's float32 staging
+      are = (float)(wr * (double)re);
+      aim = (float)(wr * (double)im);
+    }
+    tre[fl][wl] = are;
+    tim[fl][wl] = aim;
+  }
+  __syncthreads();
   const int Kp = func ? 2 * half : half;
-  float* row = B + ((size_t)f * Kp + s) * nrhs;
-  row[w2] = are;
-  row[w2 + 1] = aim;
-  if (func) {
-    float* yrow = B + ((size_t)f * Kp + half + s) * nrhs;
-    yrow[w2] = -aim;
-    yrow[w2 + 1] = are;
+  for (int it = threadIdx.x; it < kCoefT * 2 * kCoefT; it += 256) {
+    const int fl = it / (2 * kCoefT), e = it & (2 * kCoefT - 1);
+    const int wl = e >> 1, part = e & 1;
+    const int f = f0 + fl, wc = wb + wl;
+    if (f >= nf || wc >= cw) continue;
+    const int col = 2 * (w0 + wc) + part;
+    const float are = tre[fl][wl], aim = tim[fl][wl];
+    B[((size_t)f * Kp + s) * nrhs + col] = part ? aim : are;
+    if (func) B[((size_t)f * Kp + half + s) * nrhs + col] = part ? are : -aim;
   }
 }
 
@@ -327,8 +371,8 @@ hipError_t launch_mwfj_stations(const MwfjDev& d, double* rr, float* ir, hipStre
 }
 
 hipError_t launch_mwfj_coef(const MwfjDev& d, int w0, int cw, hipStream_t st) {
-  dim3 grid((cw + 63) / 64, d.nr, d.nf);
-  hipLaunchKernelGGL(k_mwfj_coef, grid, dim3(64), 0, st, (const double2*)d.spec, d.r, d.nr, w0,
+  dim3 grid((d.nf + kCoefT - 1) / kCoefT, d.nr, (cw + kCoefT - 1) / kCoefT);
+  hipLaunchKernelGGL(k_mwfj_coef, grid, dim3(256), 0, st, (const double2*)d.spec, d.r, d.nr, w0,
                      cw, d.nf, d.fstride, d.nbins, d.zhalf, d.func, d.half, d.nrhs, d.B);
   return hipGetLastError();
 }
