@@ -40,40 +40,50 @@ using namespace fjl;
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // ---------------------------------------------------------------------------
+// Taper value of window description desc at sample t.  A description whose
+// xidx is outside [-1, nx) gives a zero row (the device entry point cannot
+// check descriptions on the host).
+__device__ inline double win_value(const int* __restrict__ n1, const int* __restrict__ n2,
+                                   const int* __restrict__ dl, const int* __restrict__ xidx,
+                                   const double* __restrict__ xrows, int nx, int npts, int desc,
+                                   int t) {
+  const int xi = xidx[desc];
+  if (xi >= nx || xi < -1) return 0.0;
+  if (xi >= 0) return xrows[(size_t)xi * npts + t];
+  const int a = n1[desc], b = n2[desc], d = dl[desc];
+  if (t < a || t >= b) return 0.0;
+  if (t < a + d) return sin(3.141592653589793 / 2 * (double)(t - a) / (double)d);
+  if (t >= b - d) return sin(3.141592653589793 / 2 * (double)(b - 1 - t) / (double)d);
+  return 1.0;
+}
+
 // tapered traces: xw[(wl*nr + s)*npts + t], windows w0 .. w0+cw-1.  Each
-// thread reads its sample u[s][t] once and writes it, tapered, for every
-// window of the chunk (the trace is read once per chunk, not per window).
-// A description whose xidx is outside [-1, nx) gives a zero row (the
-// device entry point cannot check descriptions on the host).
+// thread reads its PER consecutive samples of u[s] once and writes them,
+// tapered, for every window of the chunk (the trace is read once per chunk,
+// not per window).  PER = 2 (16-byte stores; npts even, so rows stay
+// 16-byte aligned) or 1.  The kernel is bound by its stores (the tapered
+// traces of a chunk of windows, zeros included).
+template <int PER>
 __global__ void k_window(const double* __restrict__ u, const int* __restrict__ n1,
                          const int* __restrict__ n2, const int* __restrict__ dl,
                          const int* __restrict__ xidx, const double* __restrict__ xrows,
                          int nx, int w0, int cw, int nr, int npts, double* __restrict__ xw) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = PER * (blockIdx.x * blockDim.x + threadIdx.x);
   const int s = blockIdx.y;
   if (t >= npts) return;
-  const double us = u[(size_t)s * npts + t];
+  double us[PER];
+#pragma unroll
+  for (int q = 0; q < PER; q++) us[q] = u[(size_t)s * npts + t + q];
   for (int wl = 0; wl < cw; wl++) {
     const int desc = (w0 + wl) * nr + s;   // uniform over the block
-    double v;
-    const int xi = xidx[desc];
-    if (xi >= nx || xi < -1) {
-      v = 0.0;
-    } else if (xi >= 0) {
-      v = xrows[(size_t)xi * npts + t];
+    double* dst = xw + ((size_t)wl * nr + s) * npts + t;
+    if (PER == 2) {
+      const double2 v = make_double2(us[0] * win_value(n1, n2, dl, xidx, xrows, nx, npts, desc, t),
+                                     us[1] * win_value(n1, n2, dl, xidx, xrows, nx, npts, desc, t + 1));
+      *reinterpret_cast<double2*>(dst) = v;
     } else {
-      const int a = n1[desc], b = n2[desc], d = dl[desc];
-      if (t < a || t >= b) {
-        v = 0.0;
-      } else if (t < a + d) {
-        v = sin(3.141592653589793 / 2 * (double)(t - a) / (double)d);
-      } else if (t >= b - d) {
-        v = sin(3.141592653589793 / 2 * (double)(b - 1 - t) / (double)d);
-      } else {
-        v = 1.0;
-      }
+      dst[0] = us[0] * win_value(n1, n2, dl, xidx, xrows, nx, npts, desc, t);
     }
-    xw[((size_t)wl * nr + s) * npts + t] = us * v;
   }
 }
 
@@ -358,9 +368,15 @@ __global__ void k_to_double(const float* __restrict__ a, size_t n, double* __res
 
 // ---------------------------------------------------------------------------
 hipError_t launch_window(const MwfjDev& d, int w0, int cw, hipStream_t st) {
-  dim3 grid((d.npts + 255) / 256, d.nr);
-  hipLaunchKernelGGL(k_window, grid, dim3(256), 0, st, d.u, d.n1, d.n2, d.dl, d.xidx, d.xrows,
-                     d.nx, w0, cw, d.nr, d.npts, d.xw);
+  if (d.npts % 2 == 0) {
+    dim3 grid((d.npts / 2 + 255) / 256, d.nr);
+    hipLaunchKernelGGL(k_window<2>, grid, dim3(256), 0, st, d.u, d.n1, d.n2, d.dl, d.xidx,
+                       d.xrows, d.nx, w0, cw, d.nr, d.npts, d.xw);
+  } else {
+    dim3 grid((d.npts + 255) / 256, d.nr);
+    hipLaunchKernelGGL(k_window<1>, grid, dim3(256), 0, st, d.u, d.n1, d.n2, d.dl, d.xidx,
+                       d.xrows, d.nx, w0, cw, d.nr, d.npts, d.xw);
+  }
   return hipGetLastError();
 }
 
