@@ -237,7 +237,12 @@ constexpr int kMwK = 32;    // K rows per LDS chunk
 #define FJ_MWFJ_WAVES 3
 #endif
 
-template <int FUNC>
+// Station arrays of up to kMwLdsSt rows are copied into LDS (dynamic shared
+// memory, 12 bytes a row) at the start, so that the VALU phase reads them
+// there instead of waiting on a global load per entry; larger arrays read
+// them from global memory.
+constexpr int kMwLdsSt = 1536;
+template <int FUNC, bool LDS_ST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_MWFJ_WAVES, 8))) void k_mwfj_mfma(const float* __restrict__ B,
                                                    const double* __restrict__ rr,
                                                    const float* __restrict__ irr,
@@ -248,6 +253,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_MWFJ_WAV
   // B chunks (kMwK rows x kMwR right-hand sides) double-buffered in LDS:
   // chunk c+1 is fetched into registers while chunk c is contracted
   __shared__ float Bs[2][kMwK][kMwR + 4];
+  extern __shared__ double st_lds[];   // [half] r, then [half] 1/r (LDS_ST)
   constexpr int kPer = kMwK * kMwR / 256;   // B elements per thread and chunk
   const int fi = blockIdx.y;
   const int rg = blockIdx.z;   // right-hand-side group
@@ -283,6 +289,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_MWFJ_WAV
       Bs[buf][kr][cc] = nb[u];
     }
   };
+  if (LDS_ST) {
+    float* irl = reinterpret_cast<float*>(st_lds + half);
+    for (int q = threadIdx.x; q < half; q += 256) {
+      st_lds[q] = rr[q];
+      irl[q] = irr[q];
+    }
+  }
   fetch(0);
   stash(0);
   __syncthreads();
@@ -299,8 +312,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_MWFJ_WAV
 #pragma unroll
     for (int h = 0; h < kMwK / 2; h++) {
       const int s = sbase + 2 * h + kq;   // < half: padded station arrays
-      const double rs = rr[s];
-      const float ir = irr[s];
+      const double rs = LDS_ST ? st_lds[s] : rr[s];
+      const float ir = LDS_ST ? reinterpret_cast<const float*>(st_lds + half)[s] : irr[s];
       // far when x >= kFarX on every lane (a wave-uniform branch)
       const bool far = !wave_any(L.zk * ir > 8.0f / kFarX);
       if (ypart)
@@ -395,12 +408,12 @@ hipError_t launch_mwfj_coef(const MwfjDev& d, int w0, int cw, hipStream_t st) {
 
 hipError_t launch_mwfj_mfma(const MwfjDev& d, hipStream_t st) {
   dim3 grid((d.nc + kMwC - 1) / kMwC, d.nf, (d.nrhs + kMwR - 1) / kMwR);
-  if (d.func)
-    hipLaunchKernelGGL(k_mwfj_mfma<1>, grid, dim3(256), 0, st, d.B, d.rr, d.ir, d.f, d.c, d.nr,
-                       d.half, d.nf, d.nc, d.nrhs, d.nwin, d.T);
-  else
-    hipLaunchKernelGGL(k_mwfj_mfma<0>, grid, dim3(256), 0, st, d.B, d.rr, d.ir, d.f, d.c, d.nr,
-                       d.half, d.nf, d.nc, d.nrhs, d.nwin, d.T);
+  const bool lds = d.half <= kMwLdsSt;
+  const size_t shm = lds ? (sizeof(double) + sizeof(float)) * (size_t)d.half : 0;
+  auto k = d.func ? (lds ? k_mwfj_mfma<1, true> : k_mwfj_mfma<1, false>)
+                  : (lds ? k_mwfj_mfma<0, true> : k_mwfj_mfma<0, false>);
+  hipLaunchKernelGGL(k, grid, dim3(256), shm, st, d.B, d.rr, d.ir, d.f, d.c, d.nr, d.half, d.nf,
+                     d.nc, d.nrhs, d.nwin, d.T);
   return hipGetLastError();
 }
 
