@@ -221,6 +221,22 @@ def test_mwfj_wrapper(ccfj, itype, func):
         assert mism == 0 and emax <= TOL, (w, emax)
 
 
+@pytest.mark.parametrize("func", [0, 1])
+def test_mwfj_many_receivers(ccfj, func):
+    """More receivers (1600) than the contraction kernel stages in LDS
+    (1536 rows): its station arrays are then read from global memory."""
+    u, r, c, f, Fs = _quake(nr=1600, npts=256)
+    nwin = 2
+    winl = np.stack([r / 4000.0, r / 3000.0])
+    winr = np.stack([r / 2500.0 + 8.0, r / 2000.0 + 8.0])
+    got = ccfj.MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=0, func=func)
+    want = O.MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=0, func=func,
+                  engine="fp64")
+    for w in range(nwin):
+        emax, _, mism = O.column_error(got[w], want[w])
+        assert mism == 0 and emax <= TOL, (w, emax)
+
+
 @pytest.mark.parametrize("npts", [511, 512])
 def test_mwfj_trace_length_parity(ccfj, npts):
     """Odd trace lengths take the full rFFT (D2Z), even ones the half-length
