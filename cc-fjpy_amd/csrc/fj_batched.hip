@@ -38,7 +38,7 @@ __global__ void k_stations_b(BatchDev d) {
   station_entry(d.r + s0, nr, s, d.rr + o, d.hh + o, d.ir + o, d.isr + o);
 }
 
-template <int ITYPE, int NSPEC>
+template <int ITYPE, int NSPEC, bool POLAR>
 __global__ __launch_bounds__(256) void k_prep_b(BatchDev d) {
   const int b = blockIdx.z;
   const int64_t s00 = d.soff[b];
@@ -46,7 +46,7 @@ __global__ __launch_bounds__(256) void k_prep_b(BatchDev d) {
   const int nrp = d.nrp[b];
   const int s0 = blockIdx.y * 64;
   if (s0 >= nrp || nr <= 0) return;   // uniform per block: no barrier skipped partially
-  prep_tile<ITYPE, NSPEC>(d.U0 + s00 * d.nf, d.U1 + s00 * d.nf, d.r + s00, d.f, d.c, nr, nrp,
+  prep_tile<ITYPE, NSPEC, POLAR>(d.U0 + s00 * d.nf, d.U1 + s00 * d.nf, d.r + s00, d.f, d.c, nr, nrp,
                           d.nf, d.nc, d.coef + d.coff[b], blockIdx.x * 64, s0);
 }
 
@@ -104,10 +104,10 @@ __attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj_b(
 }
 
 // ---------------------------------------------------------------------------
-template <int ITYPE, int NSPEC>
+template <int ITYPE, int NSPEC, bool POLAR = false>
 static hipError_t prep_t(const BatchDev& d, int max_nrp, hipStream_t st) {
   dim3 grid((d.nf + 63) / 64, (max_nrp + 63) / 64, d.nb);
-  hipLaunchKernelGGL((k_prep_b<ITYPE, NSPEC>), grid, dim3(256), 0, st, d);
+  hipLaunchKernelGGL((k_prep_b<ITYPE, NSPEC, POLAR>), grid, dim3(256), 0, st, d);
   return hipGetLastError();
 }
 
@@ -118,6 +118,7 @@ hipError_t launch_prep_batched(int mode, int itype, const BatchDev& d, int max_n
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const bool two = !(mode == MODE_J || mode == MODE_Y);
+  if (polar_records(mode, itype)) return prep_t<0, 2, true>(d, max_nrp, st);
   if (itype == 0) return two ? prep_t<0, 2>(d, max_nrp, st) : prep_t<0, 1>(d, max_nrp, st);
   return two ? prep_t<1, 2>(d, max_nrp, st) : prep_t<1, 1>(d, max_nrp, st);
 }

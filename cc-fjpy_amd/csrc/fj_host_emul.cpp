@@ -49,6 +49,7 @@ void run_mode(int itype, const float* U0, const float* U1, const float* r, const
       if (itype == 0) {
         if (NS == 1) station_coef<0, 1>(s, nr, rp, r[s], rn, gp, gc, gn, hlim, rec);
         else station_coef<0, 2>(s, nr, rp, r[s], rn, gp, gc, gn, hlim, rec);
+        if (polar_records(MODE, 0)) to_polar(rec);
         if (NS == 1) coef_store<0, 1>(cf.data(), nrp, s, rec);
         else coef_store<0, 2>(cf.data(), nrp, s, rec);
       } else {

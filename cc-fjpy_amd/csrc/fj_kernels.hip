@@ -10,7 +10,8 @@
 //   hh[nrp + 8]       float    r_s - r_{s-1} (hh[0] = 0)
 //   ir, isr[nrp + 8]  float    1 / r_s, 1 / sqrt(r_s)
 //   coef              float    per frequency, fjl::coef_block floats:
-//       itype 0: [nrp][nspec] A_s / sqrt(r_s) of each spectrum
+//       itype 0: [nrp][nspec] A_s / sqrt(r_s) of each spectrum (J - Y: in
+//                polar form, fjl::to_polar)
 //       itype 1: [nrp][3 nspec] (a_s, p_s, t_s) per spectrum, then
 //                [nrp][nspec] b_s, then the long-segment limit
 //                (fjl::station_coef, fjl::coef_store)
@@ -63,15 +64,15 @@ __global__ void k_stations(const float* __restrict__ r, int nr, int ntot, double
 }
 
 // Coefficients + [nr][nf] -> per-frequency records (fj_prep.h).
-template <int ITYPE, int NSPEC>
+template <int ITYPE, int NSPEC, bool POLAR>
 __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ U0,
                                               const float* __restrict__ U1,
                                               const float* __restrict__ r,
                                               const float* __restrict__ f,
                                               const float* __restrict__ c, int nr, int nrp,
                                               int nf, int nc, float* __restrict__ coef) {
-  prep_tile<ITYPE, NSPEC>(U0, U1, r, f, c, nr, nrp, nf, nc, coef, blockIdx.x * 64,
-                          blockIdx.y * 64);
+  prep_tile<ITYPE, NSPEC, POLAR>(U0, U1, r, f, c, nr, nrp, nf, nc, coef, blockIdx.x * 64,
+                                 blockIdx.y * 64);
 }
 
 template <int MODE, int ITYPE>
@@ -172,10 +173,10 @@ static hipError_t launch_main(int itype, const Problem& p, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <int ITYPE, int NSPEC>
+template <int ITYPE, int NSPEC, bool POLAR = false>
 static hipError_t launch_prep_t(const Problem& p, hipStream_t st) {
   dim3 grid((p.nf + 63) / 64, (p.nrp + 63) / 64);
-  hipLaunchKernelGGL((k_prep<ITYPE, NSPEC>), grid, dim3(256), 0, st, p.U0, p.U1, p.r, p.f, p.c,
+  hipLaunchKernelGGL((k_prep<ITYPE, NSPEC, POLAR>), grid, dim3(256), 0, st, p.U0, p.U1, p.r, p.f, p.c,
                      p.nr, p.nrp, p.nf, p.nc, p.coef);
   return hipGetLastError();
 }
@@ -193,6 +194,7 @@ hipError_t launch_prep(int mode, int itype, const Problem& p, hipStream_t st) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const bool two = !(mode == MODE_J || mode == MODE_Y);
+  if (polar_records(mode, itype)) return launch_prep_t<0, 2, true>(p, st);
   if (itype == 0) return two ? launch_prep_t<0, 2>(p, st) : launch_prep_t<0, 1>(p, st);
   return two ? launch_prep_t<1, 2>(p, st) : launch_prep_t<1, 1>(p, st);
 }

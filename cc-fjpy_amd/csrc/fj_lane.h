@@ -334,6 +334,21 @@ FJ_HD inline void station_coef(int s, int nr, float rp, float rs, float rn,
   }
 }
 
+// itype 0, J - Y: the station's two coefficients (c0 for the J sum of U,
+// c1 for the Y sum of its Hilbert transform) in polar form, R = |c0 + i c1|
+// and psi = arg(c0 + i c1) in revolutions.  Then c0 J0 - c1 Y0 = amp R M0
+// cos(th + phi0 + psi) in the far form: one cosine a term instead of a sine
+// and a cosine (trapz_far); the other argument ranges rotate the phase by
+// psi as well, or rebuild c0 and c1 below x = 8.
+FJ_HD constexpr bool polar_records(int mode, int itype) {
+  return itype == 0 && mode == MODE_J_MINUS_Y;
+}
+FJ_HD inline void to_polar(float* rec) {
+  const double c0 = rec[0], c1 = rec[1];
+  rec[0] = (float)sqrt(c0 * c0 + c1 * c1);
+  rec[1] = (float)(atan2(c1, c0) / kTwoPi);
+}
+
 // Stores station s's record into the coefficients of one frequency (cf,
 // nrp stations): itype 0 at cf[s NSPEC]; itype 1 the hot part at
 // cf[s 3 NSPEC] and b_s at cf[nrp 3 NSPEC + s NSPEC].
@@ -397,12 +412,32 @@ FJ_HD inline void far_unit(double rs, const LaneK& L, float z, float& vj, float&
   }
 }
 
+// J - Y with polar records (c = R, psi): R M0 cos(u + psi), x >= kFarX.
+FJ_HD inline float far_polar(double rs, const LaneK& L, float z, float psi) {
+  const float t = z * z;
+  const float u = __builtin_fmaf(z, fitf_ph0(t), phase_rev(L.q, rs) + psi);
+  return fitf_M0(t) * cos_rev(u);
+}
+
 template <int MODE>
 FJ_HD inline void trapz_station(double rs, float ir, const LaneK& L, const float* c, float& pR,
                                 float& pI) {
   using M = ModeT<MODE>;
   float vj = 0.0f, vy = 0.0f;
   const float z = L.zk * ir;  // 8 / x; inf when k = 0 or r = 0
+  if (MODE == MODE_J_MINUS_Y && z <= 1.0f) {   // polar records, x >= 8
+    float v;
+    if (z <= 8.0f / kFarX) {
+      v = far_polar(rs, L, z, c[1]);
+    } else {
+      const float t = z * z;
+      float s, co;
+      sincos_rev(phase_rev(L.q, rs) + c[1], &s, &co);
+      v = __builtin_fmaf(fit_P0(t), co, -(z * fit_q0(t)) * s);
+    }
+    pR = __builtin_fmaf(c[0], v, pR);
+    return;
+  }
   if (z <= 8.0f / kFarX) {    // x >= kFarX
     far_unit<MODE>(rs, L, z, vj, vy);
   } else if (z <= 1.0f) {     // 8 <= x < kFarX
@@ -421,6 +456,13 @@ FJ_HD inline void trapz_station(double rs, float ir, const LaneK& L, const float
     vj = b.j0 * ia;
     vy = b.y0 * ia;
   }
+  if (MODE == MODE_J_MINUS_Y) {   // polar records: c0 = R cos psi, c1 = R sin psi
+    float s, co;
+    sincos_rev(c[1], &s, &co);
+    pR = __builtin_fmaf(c[0] * co, vj, pR);
+    pR = __builtin_fmaf(-(c[0] * s), vy, pR);
+    return;
+  }
   trapz_add<MODE>(c, vj, vy, pR, pI);
 }
 
@@ -428,6 +470,10 @@ FJ_HD inline void trapz_station(double rs, float ir, const LaneK& L, const float
 template <int MODE>
 FJ_HD inline void trapz_far(double rs, float ir, const LaneK& L, const float* c, float& pR,
                             float& pI) {
+  if (MODE == MODE_J_MINUS_Y) {   // polar records
+    pR = __builtin_fmaf(c[0], far_polar(rs, L, L.zk * ir, c[1]), pR);
+    return;
+  }
   float vj = 0.0f, vy = 0.0f;
   far_unit<MODE>(rs, L, L.zk * ir, vj, vy);
   trapz_add<MODE>(c, vj, vy, pR, pI);

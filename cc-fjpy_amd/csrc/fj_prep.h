@@ -22,7 +22,8 @@ namespace fjk {
 // segments out by min |c|); coef: nf blocks of fjl::coef_block floats; tile
 // origin (f0, s0).  Stations nr..nrp-1 (padding) get zero records; the
 // tiles with s0 = 0 write the itype-1 header of their frequencies.
-template <int ITYPE, int NSPEC>
+// POLAR: itype 0, J - Y records in polar form (fjl::to_polar).
+template <int ITYPE, int NSPEC, bool POLAR = false>
 __device__ inline void prep_tile(const float* __restrict__ U0, const float* __restrict__ U1,
                                  const float* __restrict__ r, const float* __restrict__ fgrid,
                                  const float* __restrict__ cgrid, int nr, int nrp, int nf, int nc,
@@ -81,6 +82,7 @@ __device__ inline void prep_tile(const float* __restrict__ U0, const float* __re
       const float rn = s + 1 < nr ? r[s + 1] : 0.0f;
       fjl::station_coef<ITYPE, NSPEC>(s, nr, rp, r[s], rn, gp, gc, gn,
                                       ITYPE == 1 ? hlim_s[fl] : 0.0f, rec);
+      if (POLAR) fjl::to_polar(rec);
     }
     fjl::coef_store<ITYPE, NSPEC>(coef + (size_t)f * block, nrp, s, rec);
   }
