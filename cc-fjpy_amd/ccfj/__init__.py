@@ -34,7 +34,8 @@ except ImportError as exc:  # the native extension is the product; no fallback
     ) from exc
 
 __all__ = ["fj_noise", "fj_earthquake", "MWFJ", "fj", "fh", "fhr", "fhi", "win",
-           "GetStationPairs", "CC", "hilbert_rows", "device_count", "fj_noise_batch"]
+           "GetStationPairs", "CC", "hilbert_rows", "device_count", "fj_noise_batch",
+           "fj_full", "fH_full", "fHr_full", "fHi_full", "CC_full"]
 
 MODE_J = _ccfj.MODE_J
 MODE_Y = _ccfj.MODE_Y
@@ -102,15 +103,50 @@ def _normalize_columns(out):
         out[:, i] = out[:, i] / max(np.abs(out[:, i]))
 
 
-def fj(uf, r, c, f, fstride=1, itype=1, num=-1):
-    """J0-basis transform of uf (no normalisation), float32 [nc, nf]."""
+# ---- the reference's module-level native shims (typed float32 / int32
+# memoryviews in, results written into `out` / ncfsr, ncfsi; a native error
+# raises RuntimeError).  They run on the current device.
+
+def fj_full(U_f, r, f, out, c, nc, nr, nf, type, nThread=20):
+    """J0 basis into out[nc * nf] (src/ccfj.pyx:72-90 CPU form with
+    nThread, accepted and ignored; src/ccfj-gpu.pyx:75-92 GPU form)."""
+    _ccfj.fj_full(U_f, r, f, out, c, nc, nr, nf, type)
+
+
+def fH_full(U_f, r, f, out, c, nc, nr, nf, type, nThread=20):
+    """Y0 basis into out[nc * nf] (src/ccfj.pyx:95-113; nThread ignored)."""
+    _ccfj.fH_full(U_f, r, f, out, c, nc, nr, nf, type, nThread)
+
+
+def fHr_full(U_f, r, f, out, c, nc, nr, nf, type):
+    """J0 basis into out[nc * nf] (src/ccfj-gpu.pyx:97-114)."""
+    _ccfj.fHr_full(U_f, r, f, out, c, nc, nr, nf, type)
+
+
+def fHi_full(U_f, r, f, out, c, nc, nr, nf, type):
+    """Y0 basis into out[nc * nf] (src/ccfj-gpu.pyx:118-135)."""
+    _ccfj.fHi_full(U_f, r, f, out, c, nc, nr, nf, type)
+
+
+def CC_full(npts, nsta, nf, fstride, fftlen, steplen, ifonebit, ifspecwhitenning, nThreads,
+            npairs, Pairs, startend, data, ncfsr, ncfsi):
+    """Cross-correlation into ncfsr/ncfsi[npairs * nf], added to what they
+    hold (src/ccfj.pyx:44-68); nThreads ignored."""
+    _ccfj.CC_full(npts, nsta, nf, fstride, fftlen, steplen, ifonebit, ifspecwhitenning,
+                  nThreads, npairs, Pairs, startend, data, ncfsr, ncfsi)
+
+
+def fj(uf, r, c, f, fstride=1, itype=1, nThread=20, num=-1):
+    """J0-basis transform of uf (no normalisation), float32 [nc, nf]
+    (src/ccfj-gpu.pyx:181-197; the CPU build's nThread, src/ccfj.pyx:161,
+    is accepted and ignored)."""
     U, r, c, f = _stage(uf, r, c, f, fstride)
     return _native(MODE_J, itype, False, U, None, r, c, f, num)
 
 
 def fhr(uf, r, c, f, fstride=1, itype=1, num=-1):
     """J0-basis transform (the GPU build's FP64 "real Hankel" part)."""
-    return fj(uf, r, c, f, fstride, itype, num)
+    return fj(uf, r, c, f, fstride, itype, num=num)
 
 
 def fhi(uf, r, c, f, fstride=1, itype=1, num=-1):
@@ -219,7 +255,10 @@ def win(npts, Fs, T1, T2, taper=0.8):
         out[0:i1] = 0
         out[i2:int(npts)] = 0
         if d > 0:
-            ramp = np.sin(np.pi / 2 * np.arange(d) / delta)
+            # np.pi/2*i/delta in the reference's scalar arithmetic: a numpy
+            # float delta (float32 window edges give one) sets the precision
+            dt = delta.dtype if isinstance(delta, np.floating) else np.dtype(np.float64)
+            ramp = np.sin((np.pi / 2 * np.arange(d)).astype(dt) / dt.type(delta))
             out[i1:i1 + d] = ramp
             out[i2 - d:i2] = ramp[::-1]
         return out
@@ -234,16 +273,35 @@ def win(npts, Fs, T1, T2, taper=0.8):
     return out
 
 
+def _window_params(T1, T2, npts, Fs, taper):
+    """(n1, n2, delta) of win() for arrays of window edges, in the
+    arithmetic of the reference's scalar expressions (src/ccfj.pyx:149-152):
+    numpy promotes them as it promotes the arrays, so float32 edges give
+    float32 products, differences and floors (which can differ from float64
+    by one sample); max(0, x) and min(npts, x) keep the Python int unless x
+    is larger / smaller, and with both ends clamped delta is computed from
+    those ints, in double."""
+    p1 = np.floor(T1 * Fs)
+    p2 = np.floor(T2 * Fs)
+    c1 = ~(p1 > 0)
+    c2 = ~(p2 < npts)
+    n1 = np.where(c1, np.zeros((), p1.dtype), p1)
+    n2 = np.where(c2, np.asarray(npts, p2.dtype), p2)
+    d = np.floor((n2 - n1) * (1 - taper) / 2)
+    both = c1 & c2
+    if np.any(both):
+        d = np.where(both, np.floor((npts - 0) * (1 - taper) / 2), d)
+    return n1, n2, d
+
+
 def _window_descriptions(winl, winr, indx, npts, Fs, taper):
     """Integer form of win() for every (window, sorted station): zero
     outside [n1, n2), sine ramps of delta samples.  Windows that win()
     builds through its degenerate branches (reversed, out of range,
     overlapping ramps) are returned as explicit rows instead."""
-    T1 = np.asarray(winl, np.float64)[:, indx]
-    T2 = np.asarray(winr, np.float64)[:, indx]
-    a = np.maximum(0.0, np.floor(T1 * Fs))
-    b = np.minimum(float(npts), np.floor(T2 * Fs))
-    d = np.floor((b - a) * (1 - taper) / 2)
+    T1 = np.asarray(winl)[:, indx]
+    T2 = np.asarray(winr)[:, indx]
+    a, b, d = _window_params(T1, T2, npts, Fs, taper)
     regular = (a >= 0) & (a <= b) & (b <= npts) & (2 * d <= b - a)
     n1 = np.ascontiguousarray(a, dtype=np.int32)
     n2 = np.ascontiguousarray(b, dtype=np.int32)
@@ -354,7 +412,11 @@ def CC(npts, nsta, nf, fftlen, Pairs, startend, data, overlaprate=0.0, nThreads=
                                             np.asarray(Pairs, np.int32)[:2 * npairs],
                                             np.asarray(startend, np.int32), data)
     if ifonebit:
-        np.copyto(data, np.sign(data))
+        # the reference's sign() (src/CrossCorr.cpp:22-30): a NaN sample
+        # fails both comparisons and becomes 0
+        with np.errstate(invalid="ignore"):
+            np.copyto(data, np.where(data > 0, np.float32(1),
+                                     np.where(data < 0, np.float32(-1), np.float32(0))))
     return out
 
 

@@ -23,6 +23,23 @@ cdef extern from "ccfj.h" nogil:
     int FHr(float *u_f, float *r, float *f, float *out, float *c, int nc, int nr, int nf, int type)
     int FHi(float *u_f, float *r, float *f, float *out, float *c, int nc, int nr, int nf, int type)
     int FH(float *u_f, float *r, float *f, float *out, float *c, int nc, int nr, int nf, int type, int nThread)
+    ctypedef struct CC_data:
+        int nsta
+        int npts
+        int nf
+        int npairs
+        int fstride
+        int fftlen
+        int steplen
+        int *Pairs
+        int *startend
+        float *data
+        float *ncfsr
+        float *ncfsi
+        int ifonebit
+        int ifspecwhitenning
+        int nThreads
+    int CrossCorrelation(CC_data d)
     int ccfj_create(int device, ccfj_ctx **ctx)
     int ccfj_destroy(ccfj_ctx *ctx)
     const char *ccfj_last_error()
@@ -157,6 +174,35 @@ def fH_full(float[:] U_f, float[:] r, float[:] f, float[:] out, float[:] c,
     with nogil:
         rc = FH(&U_f[0], &r[0], &f[0], &out[0], &c[0], nc, nr, nf, type, nThread)
     _check(rc, "FH")
+
+
+def CC_full(int npts, int nsta, int nf, int fstride, int fftlen, int steplen, int ifonebit,
+            int ifspecwhitenning, int nThreads, int npairs, int[:] Pairs, int[:] startend,
+            float[:] data, float[:] ncfsr, float[:] ncfsi):
+    """The reference's CC_full (src/ccfj.pyx:44-68, src/ccfj-gpu.pyx:46-70):
+    fills the CC_data block and calls CrossCorrelation, which adds each
+    pair's mean cross-spectrum to ncfsr/ncfsi [npairs * nf] and, with
+    ifonebit, replaces data by its signs in place."""
+    cdef CC_data d
+    d.npts = npts
+    d.nsta = nsta
+    d.nf = nf
+    d.fstride = fstride
+    d.fftlen = fftlen
+    d.steplen = steplen
+    d.ifonebit = ifonebit
+    d.ifspecwhitenning = ifspecwhitenning
+    d.nThreads = nThreads
+    d.npairs = npairs
+    d.Pairs = &Pairs[0]
+    d.startend = &startend[0]
+    d.data = &data[0]
+    d.ncfsr = &ncfsr[0]
+    d.ncfsi = &ncfsi[0]
+    cdef int rc
+    with nogil:
+        rc = CrossCorrelation(d)
+    _check(rc, "CrossCorrelation")
 
 
 def plain_fp64(int device, int ybasis, int itype, float[:, ::1] U, float[:] r,

@@ -23,6 +23,8 @@ descriptions leave open are fixed here.
                          tiling 1.5-4.5 km/s; nc=2000 (1500-4500 m/s),
                          f = k Fs/npts for k < 820 (0-5 Hz).
 """
+import os
+
 import numpy as np
 from scipy.special import j0
 
@@ -92,22 +94,41 @@ def subarrays(seed=3, nb=4096, nmin=20, nmax=400, nf=512, nc=1024, real_only=Tru
     rise in [500, 2500] m/s, decay in [0.1, 0.6] Hz, amplitudes 1 and
     U[0.3, 0.7]) plus noise sigma 0.1.  Returns lists of per-sub-array
     spectra and distances, and the shared grids."""
+    from concurrent.futures import ThreadPoolExecutor
     rng = np.random.default_rng(seed)
     f = np.linspace(0.02, 1.0, nf)
     c = np.linspace(1000.0, 5000.0, nc)
     ns = rng.integers(nmin, nmax + 1, nb)
-    ufs, rs = [], []
+    # the random draws stay in one sequence (sub-array by sub-array); the
+    # Bessel sums, which release the GIL, run on worker threads
+    jobs, rs = [], []
     for n in ns:
         r = np.sort(rng.uniform(1e3, 150e3, n))
-        acc = np.zeros((n, nf))
+        modes = []
         for amp in (1.0, rng.uniform(0.3, 0.7)):
             cv = rng.uniform(1800, 3600) + rng.uniform(500, 2500) * np.exp(-f / rng.uniform(0.1, 0.6))
-            acc += amp * j0(2 * np.pi * f[None, :] * r[:, None] / cv[None, :])
-        acc += rng.normal(0, 0.1, acc.shape)
-        if not real_only:
-            acc = acc + 1j * rng.normal(0, 0.1, acc.shape)
-        ufs.append(acc.astype(np.float32 if real_only else np.complex64))
+            modes.append((amp, cv))
+        noise = rng.normal(0, 0.1, (n, nf))
+        inoise = None if real_only else rng.normal(0, 0.1, (n, nf))
+        jobs.append((r, modes, noise, inoise))
         rs.append(r)
+
+    def one(job):
+        r, modes, noise, inoise = job
+        acc = np.zeros((len(r), nf))
+        for amp, cv in modes:
+            acc += amp * j0(2 * np.pi * f[None, :] * r[:, None] / cv[None, :])
+        acc += noise
+        if inoise is not None:
+            acc = acc + 1j * inoise
+        return acc.astype(np.float32 if inoise is None else np.complex64)
+
+    try:
+        workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    except AttributeError:
+        workers = max(1, min(16, os.cpu_count() or 1))
+    with ThreadPoolExecutor(workers) as ex:
+        ufs = list(ex.map(one, jobs))
     return dict(ufs=ufs, rs=rs, c=c, f=f, ns=ns)
 
 

@@ -13,8 +13,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/ccfj.h"
@@ -81,6 +83,8 @@ struct ccfj_ctx {
   std::recursive_mutex mu;              // host-pointer entry points (ctx_lock)
   void* pin[2] = {nullptr, nullptr};   // pinned staging ring (ccfj_stage.hip)
   hipEvent_t pin_ev[2] = {nullptr, nullptr};
+  std::mutex fft_mu;
+  std::map<std::tuple<int, int, int>, hipfftHandle> fft_plans;  // (n, batch, type)
 };
 
 namespace {
@@ -256,6 +260,30 @@ int ctx_pinned_ring(ccfj_ctx* ctx, void* buf[2], hipEvent_t ev[2], size_t* chunk
   return CCFJ_OK;
 }
 
+int cached_fft_plan(ccfj_ctx* ctx, int n, int batch, hipfftType type, hipStream_t st,
+                    hipfftHandle* h) {
+  std::lock_guard<std::mutex> lk(ctx->fft_mu);
+  const auto key = std::make_tuple(n, batch, (int)type);
+  auto it = ctx->fft_plans.find(key);
+  if (it == ctx->fft_plans.end()) {
+    hipfftHandle p = 0;
+    int nn[1] = {n};
+    const int nb = n / 2 + 1;
+    const bool fwd = type == HIPFFT_D2Z || type == HIPFFT_R2C;
+    const bool c2c = type == HIPFFT_Z2Z || type == HIPFFT_C2C;
+    const hipfftResult r = hipfftPlanMany(&p, 1, nn, nullptr, 1, (fwd || c2c) ? n : nb, nullptr,
+                                          1, (fwd && !c2c) ? nb : n, type, batch);
+    if (r != HIPFFT_SUCCESS)
+      return fail(CCFJ_EDEVICE, "hipfftPlanMany (n=%d, batch=%d, type=%d) failed", n, batch,
+                  (int)type);
+    it = ctx->fft_plans.emplace(key, p).first;
+  }
+  if (hipfftSetStream(it->second, st) != HIPFFT_SUCCESS)
+    return fail(CCFJ_EDEVICE, "hipfftSetStream failed");
+  *h = it->second;
+  return CCFJ_OK;
+}
+
 int timing_begin(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t>* ev) {
   if (!ctx->timing) return CCFJ_OK;
   hipError_t e;
@@ -319,6 +347,10 @@ int ccfj_destroy(ccfj_ctx* ctx) {
   if (!ctx) return CCFJ_OK;
   set_device(ctx);
   (void)hipStreamSynchronize(ctx->stream);
+  // work of the device-pointer entry points may still run on callers' streams
+  (void)hipDeviceSynchronize();
+  for (auto& kv : ctx->fft_plans) (void)hipfftDestroy(kv.second);
+  ctx->fft_plans.clear();
   for (int s = 0; s < S_COUNT; s++)
     if (ctx->buf[s]) (void)hipFree(ctx->buf[s]);
   for (auto& ev : ctx->events) {

@@ -123,12 +123,12 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
       return hip_fail(e, "gather kernel");
     hipfftHandle plan = 0;
     if (zhalf) {
-      if ((rc = ccfj_internal::cached_fft_plan(L / 2, n * nsta, HIPFFT_C2C, st, &plan))) return rc;
+      if ((rc = ccfj_internal::cached_fft_plan(ctx, L / 2, n * nsta, HIPFFT_C2C, st, &plan))) return rc;
       if (hipfftExecC2C(plan, (hipfftComplex*)dseg, (hipfftComplex*)dspec, HIPFFT_FORWARD) !=
           HIPFFT_SUCCESS)
         return fail(CCFJ_EDEVICE, "hipfftExecC2C failed");
     } else {
-      if ((rc = ccfj_internal::cached_fft_plan(L, n * nsta, HIPFFT_R2C, st, &plan))) return rc;
+      if ((rc = ccfj_internal::cached_fft_plan(ctx, L, n * nsta, HIPFFT_R2C, st, &plan))) return rc;
       if (hipfftExecR2C(plan, (hipfftReal*)dseg, (hipfftComplex*)dspec) != HIPFFT_SUCCESS)
         return fail(CCFJ_EDEVICE, "hipfftExecR2C failed");
     }

@@ -25,9 +25,12 @@ hipStream_t ctx_stream(const ccfj_ctx* ctx);
 // entry points use it).
 int default_context(ccfj_ctx** out);
 // A hipFFT plan for batch packed rows of length n (R2C/D2Z forward,
-// C2R/Z2D backward, C2C/Z2Z either way), cached for the process, bound to
-// stream st.
-int cached_fft_plan(int n, int batch, hipfftType type, hipStream_t st, hipfftHandle* h);
+// C2R/Z2D backward, C2C/Z2Z either way), bound to stream st.  Plans are
+// cached in the context (a plan's work area must not be shared by two
+// contexts, whose calls may run at once on different streams) and
+// destroyed with it.
+int cached_fft_plan(ccfj_ctx* ctx, int n, int batch, hipfftType type, hipStream_t st,
+                    hipfftHandle* h);
 // With the context's timing on (ccfj_set_timing), events around a main
 // kernel, summed by ccfj_get_timing; no-ops otherwise.
 int timing_begin(ccfj_ctx* ctx, hipStream_t st, std::pair<hipEvent_t, hipEvent_t>* ev);

@@ -111,13 +111,13 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
     if ((e = fjk::launch_window(d, w0, cwn, st)) != hipSuccess) return hip_fail(e, "window kernel");
     hipfftHandle plan = 0;
     if (d.zhalf) {
-      if ((rc = ccfj_internal::cached_fft_plan(npts / 2, cwn * nr, HIPFFT_Z2Z, st, &plan)))
+      if ((rc = ccfj_internal::cached_fft_plan(ctx, npts / 2, cwn * nr, HIPFFT_Z2Z, st, &plan)))
         return rc;
       if (hipfftExecZ2Z(plan, (hipfftDoubleComplex*)dxw, (hipfftDoubleComplex*)dspec,
                         HIPFFT_FORWARD) != HIPFFT_SUCCESS)
         return fail(CCFJ_EDEVICE, "hipfftExecZ2Z failed");
     } else {
-      if ((rc = ccfj_internal::cached_fft_plan(npts, cwn * nr, HIPFFT_D2Z, st, &plan))) return rc;
+      if ((rc = ccfj_internal::cached_fft_plan(ctx, npts, cwn * nr, HIPFFT_D2Z, st, &plan))) return rc;
       if (hipfftExecD2Z(plan, (hipfftDoubleReal*)dxw, (hipfftDoubleComplex*)dspec) !=
           HIPFFT_SUCCESS)
         return fail(CCFJ_EDEVICE, "hipfftExecD2Z failed");

@@ -78,44 +78,11 @@ struct Bufs {
 
 }  // namespace
 
-namespace ccfj_internal {
-
-// hipFFT plans, cached per (device, length, batch, type) for the life of the
-// process (a plan holds only its small work area); packed rows: real rows of
-// n, complex rows of n/2 + 1.
-static std::mutex g_plan_mu;
-static std::map<std::tuple<int, int, int, int>, hipfftHandle> g_plans;
-
-int cached_fft_plan(int n, int batch, hipfftType type, hipStream_t st, hipfftHandle* h) {
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  std::lock_guard<std::mutex> lk(g_plan_mu);
-  auto key = std::make_tuple(dev, n, batch, (int)type);
-  auto it = g_plans.find(key);
-  if (it == g_plans.end()) {
-    hipfftHandle p = 0;
-    int nn[1] = {n};
-    const int nb = n / 2 + 1;
-    const bool fwd = type == HIPFFT_D2Z || type == HIPFFT_R2C;
-    const bool c2c = type == HIPFFT_Z2Z || type == HIPFFT_C2C;
-    const hipfftResult r = hipfftPlanMany(&p, 1, nn, nullptr, 1, (fwd || c2c) ? n : nb, nullptr,
-                                          1, (fwd && !c2c) ? nb : n, type, batch);
-    if (r != HIPFFT_SUCCESS)
-      return fail(CCFJ_EDEVICE, "hipfftPlanMany (n=%d, batch=%d, type=%d) failed", n, batch,
-                  (int)type);
-    it = g_plans.emplace(key, p).first;
-  }
-  if (hipfftSetStream(it->second, st) != HIPFFT_SUCCESS) return fail(CCFJ_EDEVICE, "hipfftSetStream failed");
-  *h = it->second;
-  return CCFJ_OK;
-}
-
-}  // namespace ccfj_internal
 
 namespace {
 
-int get_plan(int n, int batch, bool forward, hipStream_t st, hipfftHandle* h) {
-  return ccfj_internal::cached_fft_plan(n, batch, forward ? HIPFFT_D2Z : HIPFFT_Z2D, st, h);
+int get_plan(ccfj_ctx* ctx, int n, int batch, bool forward, hipStream_t st, hipfftHandle* h) {
+  return ccfj_internal::cached_fft_plan(ctx, n, batch, forward ? HIPFFT_D2Z : HIPFFT_Z2D, st, h);
 }
 
 // Hilbert transform of rows x -> y (device, float32), double inside; the
@@ -132,7 +99,8 @@ int hilbert_rows(ccfj_ctx* ctx, const float* x, float* y, int rows, int n, hipSt
   for (int r0 = 0; r0 < rows; r0 += chunk) {
     const int m = std::min(chunk, rows - r0);
     hipfftHandle fwd, inv;
-    if ((rc = get_plan(n, m, true, st, &fwd)) || (rc = get_plan(n, m, false, st, &inv))) return rc;
+    if ((rc = get_plan(ctx, n, m, true, st, &fwd)) || (rc = get_plan(ctx, n, m, false, st, &inv)))
+      return rc;
     const int64_t nr_ = (int64_t)m * n, nc_ = (int64_t)m * nb;
     hipLaunchKernelGGL(k_to_double_rows, dim3((unsigned)((nr_ + 255) / 256)), dim3(256), 0, st,
                        x + (size_t)r0 * n, nr_, (double*)dr);

@@ -33,12 +33,20 @@ def used(startend, k, steplen, fftlen):
     return (t0 >= se[:, 0]) & (t0 + fftlen < se[:, 1])
 
 
+def sign_ref(x):
+    """The reference's sign() (src/CrossCorr.cpp:22-30): 1 for x > 0, -1
+    for x < 0, else 0 -- a NaN sample fails both comparisons and becomes 0
+    (np.sign would keep NaN)."""
+    x = np.asarray(x, np.float32)
+    return np.where(x > 0, np.float32(1), np.where(x < 0, np.float32(-1), np.float32(0)))
+
+
 def cross_correlate(data, nsta, npts, nf, fftlen, steplen, pairs, startend, fstride=1,
                     onebit=False, whiten=True):
     """-> (ncfs complex128 [npairs, nf], count [npairs])."""
     x = np.asarray(data, np.float32).reshape(nsta, npts)
     if onebit:
-        x = np.sign(x).astype(np.float32)
+        x = sign_ref(x)
     fftlen, nshifts = windows(npts, fftlen, steplen)
     p = np.asarray(pairs).reshape(-1, 2)
     acc = np.zeros((len(p), nf), np.complex128)

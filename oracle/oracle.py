@@ -148,9 +148,10 @@ def _normalize(out):
     return out
 
 
-def fj_noise(uf, r, c, f, fstride=1, itype=1, func=0, engine="fp64"):
+def fj_noise(uf, r, c, f, fstride=1, itype=1, func=0, engine="fp64", normalize=True):
     """src/ccfj.pyx:202-227.  The Hilbert transform along frequency is
-    scipy.fftpack.hilbert row by row, as there."""
+    scipy.fftpack.hilbert row by row, as there.  normalize=False returns
+    the combination before the per-column division (for linearity checks)."""
     from scipy import fftpack
     indx = np.argsort(r)
     r = r[indx]
@@ -163,12 +164,15 @@ def fj_noise(uf, r, c, f, fstride=1, itype=1, func=0, engine="fp64"):
             ufi[i, :] = np.real(fftpack.hilbert(uf[i, :]))
         Ui, _, _, _ = _stage(ufi, r, c, f, fstride)
         out = out - _engine(Ui, rr, ff, cc, itype, 1, engine)
+    if not normalize:
+        return out
     with np.errstate(invalid="ignore", divide="ignore"):
         return _normalize(out)
 
 
-def fj_earthquake(u, r, c, f, fstride=1, itype=1, func=0, engine="fp64"):
-    """src/ccfj.pyx:229-250."""
+def fj_earthquake(u, r, c, f, fstride=1, itype=1, func=0, engine="fp64", normalize=True):
+    """src/ccfj.pyx:229-250 (normalize=False: before the per-column
+    division)."""
     indx = np.argsort(r)
     r = r[indx]
     u1 = u[indx]
@@ -182,6 +186,8 @@ def fj_earthquake(u, r, c, f, fstride=1, itype=1, func=0, engine="fp64"):
         outr = _engine(Ur, rr, ff, cc, itype, 0, engine) - _engine(Ui, rr, ff, cc, itype, 1, engine)
         outi = _engine(Ui, rr, ff, cc, itype, 0, engine) + _engine(Ur, rr, ff, cc, itype, 1, engine)
     out = np.sqrt(outr ** 2 + outi ** 2)
+    if not normalize:
+        return out
     with np.errstate(invalid="ignore", divide="ignore"):
         return _normalize(out)
 

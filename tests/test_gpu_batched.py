@@ -83,3 +83,68 @@ def test_batch_graph_replay(ccfj, batch):
         assert np.max(np.abs(got[fin] - eager[fin])) <= 1e-6, rep
         _ = (tU * 2).sum().item()      # unrelated eager work between replays
     assert plan.tile in (64, 128, 256)
+
+
+# ------------------------------------------------ the batched workload at size
+@pytest.fixture(scope="module")
+def cfg4():
+    """4096 sub-arrays of 20-400 pairs (848,879 in all), r sorted in
+    [1, 150] km, nf 512 (0.02-1 Hz), nc 1024 (1000-5000 m/s), two modes each."""
+    from ccfj import synth
+    print("cfg4: generating 4096 sub-arrays", flush=True)
+    d = synth.subarrays(seed=3)
+    Us = [np.asarray(u, np.float32) for u in d["ufs"]]
+    Rs = [np.asarray(r, np.float32) for r in d["rs"]]
+    soff = np.zeros(len(Us) + 1, np.int64)
+    soff[1:] = np.cumsum([len(r) for r in Rs])
+    return dict(d, U=np.ascontiguousarray(np.concatenate(Us)), R=np.concatenate(Rs), soff=soff)
+
+
+def test_batched_full_size_graph_vs_oracle(ccfj, cfg4):
+    """The batched workload at its configured size (4096 sub-arrays, 512 f
+    x 1024 c, itype 1, func 0) through one plan replayed from a HIP graph:
+    two replays agree bit for bit; 24 (sub-array, column) samples meet the
+    FP64-oracle bar; 8 sampled sub-arrays equal single-array fj_noise
+    within 1e-6 (src/ccfj.pyx:202-227 per sub-array)."""
+    import torch
+    d = cfg4
+    nb, nf, nc = len(d["ns"]), len(d["f"]), len(d["c"])
+    assert nb == 4096 and nf == 512 and nc == 1024 and d["soff"][-1] == 848879
+    ctx = ccfj._context(0)
+    dev = torch.device("cuda:0")
+    f32 = lambda a: np.ascontiguousarray(a, np.float32)
+    tU, tR, tf, tc = (torch.from_numpy(a).to(dev) for a in (d["U"], d["R"], f32(d["f"]), f32(d["c"])))
+    out = torch.empty((nb, nc, nf), dtype=torch.float32, device=dev)
+    plan = ctx.batch_plan(ccfj.MODE_J, 1, True, d["soff"], nf, nc)
+    s = torch.cuda.Stream(device=dev)
+    torch.cuda.synchronize()
+    ns = d["ns"]
+    picks = [0, nb - 1, int(np.argmin(ns)), int(np.argmax(ns)), 1000, 2047, 3001, 4000]
+    cols = [0, 255, 511]
+    first = None
+    for rep in range(2):
+        out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        print(f"cfg4: graph replay {rep}", flush=True)
+        plan.run(tU.data_ptr(), tU.data_ptr(), tR.data_ptr(), tf.data_ptr(), tc.data_ptr(),
+                 out.data_ptr(), s.cuda_stream, True)
+        s.synchronize()
+        got = {b: out[b].cpu().numpy() for b in picks}
+        if first is None:
+            first = got
+        else:
+            for b in picks:
+                assert np.array_equal(got[b].view(np.uint32), first[b].view(np.uint32)), b
+    assert plan.tile in (64, 128, 256)
+    worst = 0.0
+    for b in picks:
+        a, e = d["soff"][b], d["soff"][b + 1]
+        U, r = d["U"][a:e], d["R"][a:e]
+        want = O.oracle64(U, r, d["f"], d["c"], 1, 0, fidx=cols)
+        want = want / np.max(np.abs(want), axis=0)
+        emax, erms, mism = O.column_error(first[b][:, cols], want)
+        worst = max(worst, emax)
+        assert mism == 0 and emax <= TOL, (b, ns[b], emax)
+        one = ccfj.fj_noise(d["ufs"][b], d["rs"][b], d["c"], d["f"], itype=1, func=0)
+        assert np.max(np.abs(one - first[b])) <= 1e-6, b
+    print(f"cfg4: worst sampled error vs FP64 oracle {worst:.2e}")

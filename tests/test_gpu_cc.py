@@ -181,3 +181,22 @@ def test_cc_bin_stride_fold_and_plain(ccfj, fftlen, fstride):
     assert np.all(count > 0)
     err = np.max(np.abs(got - want)) / np.max(np.abs(want))
     assert err <= TOL, err
+
+
+def test_cc_onebit_nan_samples(ccfj):
+    """One-bit with NaN samples: the reference's sign() turns them into 0
+    (src/CrossCorr.cpp:22-30), in the correlation and in the caller's
+    float32 array, which CC overwrites in place."""
+    nsta, npts, fftlen, nf = 4, 3000, 512, 80
+    x, se = _case(9, nsta, npts)
+    x[0, 100] = np.nan
+    x[2, 2500:2510] = np.nan
+    pairs = ccfj.GetStationPairs(nsta)
+    data = x.reshape(-1).copy()
+    got = ccfj.CC(npts, nsta, nf, fftlen, pairs, se, data, overlaprate=0.5, ifonebit=1)
+    want, _ = C.cross_correlate(x.reshape(-1), nsta, npts, nf, fftlen, fftlen // 2, pairs, se,
+                                onebit=True, whiten=True)
+    assert np.all(np.isfinite(got)) and np.all(np.isfinite(want))
+    assert np.max(np.abs(got - want)) <= TOL * np.max(np.abs(want))
+    assert np.array_equal(data, C.sign_ref(x.reshape(-1)))
+    assert data[100] == 0 and np.all(data[2 * npts + 2500:2 * npts + 2510] == 0)

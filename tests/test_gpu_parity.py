@@ -433,23 +433,38 @@ def dense():
                 f=d["f"].astype(np.float32), c=d["c"].astype(np.float32))
 
 
-DENSE_COLS = [7, 1500, 4000]
+# eight columns stratified over 0.005-2 Hz, the last (2 Hz, the largest
+# arguments k r) included
+DENSE_COLS = [7, 600, 1200, 1800, 2400, 3000, 3600, 4095]
+
+
+@pytest.fixture(scope="module")
+def dense_oracle(ccfj, dense):
+    """FP64 oracle at DENSE_COLS: J0 basis of U and Y0 basis of the
+    Hilbert-transformed U, per itype (computed once, shared by func 0/1)."""
+    d = dense
+    H = ccfj.hilbert_rows(d["U"])
+    cache = {"H": H}
+    for itype in (0, 1):
+        print(f"dense oracle itype={itype}", flush=True)
+        cache[itype, 0] = O.oracle64(d["U"], d["r"], d["f"], d["c"], itype, 0, fidx=DENSE_COLS)
+        cache[itype, 1] = O.oracle64(H, d["r"], d["f"], d["c"], itype, 1, fidx=DENSE_COLS)
+    return cache
 
 
 @pytest.mark.parametrize("itype", [0, 1])
 @pytest.mark.parametrize("func", [0, 1])
-def test_dense_full_size_sampled_columns(ccfj, ctx, dense, itype, func):
-    """19,900 pairs x 4096 f x 3000 c at full size on the GPU; three
-    frequency columns (low, mid, high) checked against the FP64 oracle."""
+def test_dense_full_size_sampled_columns(ccfj, ctx, dense, dense_oracle, itype, func):
+    """19,900 pairs x 4096 f x 3000 c at full size on the GPU; eight
+    frequency columns from 0.005 to 2 Hz checked against the FP64 oracle."""
     d = dense
     if func == 0:
         got = native(ctx, ccfj.MODE_J, itype, d["U"], d["r"], d["f"], d["c"])
-        want = O.oracle64(d["U"], d["r"], d["f"], d["c"], itype, 0, fidx=DENSE_COLS)
+        want = dense_oracle[itype, 0]
     else:
-        H = ccfj.hilbert_rows(d["U"])
-        got = native(ctx, ccfj.MODE_J_MINUS_Y, itype, d["U"], d["r"], d["f"], d["c"], U1=H)
-        want = (O.oracle64(d["U"], d["r"], d["f"], d["c"], itype, 0, fidx=DENSE_COLS)
-                - O.oracle64(H, d["r"], d["f"], d["c"], itype, 1, fidx=DENSE_COLS))
+        got = native(ctx, ccfj.MODE_J_MINUS_Y, itype, d["U"], d["r"], d["f"], d["c"],
+                     U1=dense_oracle["H"])
+        want = dense_oracle[itype, 0] - dense_oracle[itype, 1]
     emax, erms, mism = O.column_error(got[:, DENSE_COLS], want)
     print(f"dense itype={itype} func={func}: emax={emax:.2e} erms={erms:.2e}")
     assert mism == 0 and emax <= TOL

@@ -80,3 +80,20 @@ def test_bin_stride_fold_identity():
         folded = np.fft.rfft(x.reshape(s, n // s).sum(axis=0))
         m = min(len(full), len(folded))
         assert np.allclose(full[:m], folded[:m], rtol=0, atol=1e-9 * np.abs(full).max())
+
+
+def test_sign_matches_reference_rule():
+    """The reference's sign() (src/CrossCorr.cpp:22-30): 1 above zero, -1
+    below, 0 otherwise -- NaN fails both comparisons and gives 0, where
+    np.sign would keep NaN; signed zeros give 0."""
+    x = np.array([2.5, -0.1, 0.0, -0.0, np.nan, np.inf, -np.inf, 1e-45], np.float32)
+    want = np.array([1, -1, 0, 0, 0, 1, -1, 1], np.float32)
+    got = C.sign_ref(x)
+    assert got.dtype == np.float32 and np.array_equal(got, want)
+    # one-bit data with a NaN sample: the oracle's spectra stay finite
+    rng = np.random.default_rng(5)
+    y = rng.normal(size=(2, 300)).astype(np.float32)
+    y[0, 17] = np.nan
+    a, _ = C.cross_correlate(y.reshape(-1), 2, 300, 10, 128, 64, [0, 1], [0, 400, 0, 400],
+                             onebit=True, whiten=False)
+    assert np.all(np.isfinite(a))
