@@ -127,29 +127,45 @@ int set_device(ccfj_ctx* ctx) {
   return e == hipSuccess ? CCFJ_OK : hip_fail(e, "hipSetDevice");
 }
 
-// Enqueue prep + main kernel (+ normalisation) on st.  Inputs are device
-// pointers; workspace comes from ctx.
-int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U0,
-               const float* U1, const float* r, const float* f, const float* c, int nr,
-               int nf, int nc, float* out, hipStream_t st) {
+// Device work arrays of one transform (stations, records, results, column
+// scale); rr..coef are unused when nr < 2, m when not normalising.
+struct Work {
+  double* rr = nullptr;
+  float* hh = nullptr;
+  float* ir = nullptr;
+  float* isr = nullptr;
+  float* coef = nullptr;
+  float* T = nullptr;
+  float* m = nullptr;
+};
+
+// Bytes of each work array, in Work's order (256-byte multiples).
+void work_sizes(int mode, int itype, int normalize, int nr, int nf, int nc, size_t sz[7]) {
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  const int nrp = fjk::padded(nr);
+  const bool seg = nr >= 2;
+  sz[0] = seg ? up(sizeof(double) * (nrp + 8)) : 0;
+  sz[1] = sz[2] = sz[3] = seg ? up(sizeof(float) * (nrp + 8)) : 0;
+  sz[4] = seg ? up(sizeof(float) * fjk::coef_floats(mode, itype, nrp, nf)) : 0;
+  sz[5] = up(sizeof(float) * (size_t)nf * nc);
+  sz[6] = normalize ? up(sizeof(float) * (size_t)nf) : 0;
+}
+
+// Enqueue prep + main kernel (+ normalisation) on st.  Inputs and work
+// arrays are device pointers; the main kernel is timed when `timed` and the
+// context's timing is on.
+int enqueue_transform(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U0,
+                      const float* U1, const float* r, const float* f, const float* c, int nr,
+                      int nf, int nc, float* out, hipStream_t st, const Work& w, bool timed) {
   if (nf == 0 || nc == 0) return CCFJ_OK;
   hipError_t e;
   int rc;
-  if ((rc = ensure(ctx, S_T, sizeof(float) * (size_t)nf * nc))) return rc;
-  if (normalize && (rc = ensure(ctx, S_M, sizeof(float) * (size_t)nf))) return rc;
-  float* T = (float*)ctx->buf[S_T];
   if (nr < 2) {
     // no segment: the reference's sums are empty and stay 0 (even at f = 0,
     // where a segment would divide by k = 0)
-    e = fjk::launch_fill(T, (size_t)nf * nc, 0.0f, st);
+    e = fjk::launch_fill(w.T, (size_t)nf * nc, 0.0f, st);
     if (e != hipSuccess) return hip_fail(e, "fill kernel launch");
   } else {
-    const int nrp = fjk::padded(nr);
-    if ((rc = ensure(ctx, S_RR, sizeof(double) * (nrp + 8)))) return rc;
-    if ((rc = ensure(ctx, S_HH, sizeof(float) * (nrp + 8)))) return rc;
-    if ((rc = ensure(ctx, S_IR, sizeof(float) * (nrp + 8)))) return rc;
-    if ((rc = ensure(ctx, S_ISR, sizeof(float) * (nrp + 8)))) return rc;
-    if ((rc = ensure(ctx, S_COEF, sizeof(float) * fjk::coef_floats(mode, itype, nrp, nf)))) return rc;
     fjk::Problem p;
     p.U0 = U0;
     p.U1 = two_spectra(mode) ? U1 : U0;
@@ -157,26 +173,94 @@ int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U
     p.f = f;
     p.c = c;
     p.nr = nr;
-    p.nrp = nrp;
+    p.nrp = fjk::padded(nr);
     p.nf = nf;
     p.nc = nc;
-    p.rr = (double*)ctx->buf[S_RR];
-    p.hh = (float*)ctx->buf[S_HH];
-    p.ir = (float*)ctx->buf[S_IR];
-    p.isr = (float*)ctx->buf[S_ISR];
-    p.coef = (float*)ctx->buf[S_COEF];
-    p.T = T;
+    p.rr = w.rr;
+    p.hh = w.hh;
+    p.ir = w.ir;
+    p.isr = w.isr;
+    p.coef = w.coef;
+    p.T = w.T;
     e = fjk::launch_prep(mode, itype, p, st);
     if (e != hipSuccess) return hip_fail(e, "prep kernel launch");
     std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
-    if ((rc = timing_begin(ctx, st, &ev))) return rc;
+    if (timed && (rc = timing_begin(ctx, st, &ev))) return rc;
     e = fjk::launch_fj(mode, itype, p, st);
     if (e != hipSuccess) return hip_fail(e, "F-J kernel launch");
-    timing_end(ctx, st, ev);
+    if (timed) timing_end(ctx, st, ev);
   }
-  e = fjk::launch_finish(T, normalize ? (float*)ctx->buf[S_M] : nullptr, 1, nf, nc, out, st);
+  e = fjk::launch_finish(w.T, normalize ? w.m : nullptr, 1, nf, nc, out, st);
   if (e != hipSuccess) return hip_fail(e, "finish kernel launch");
   return CCFJ_OK;
+}
+
+// The same with the work arrays taken from the context's grow-only slots.
+int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U0,
+               const float* U1, const float* r, const float* f, const float* c, int nr,
+               int nf, int nc, float* out, hipStream_t st) {
+  if (nf == 0 || nc == 0) return CCFJ_OK;
+  size_t sz[7];
+  work_sizes(mode, itype, normalize, nr, nf, nc, sz);
+  const Slot slots[7] = {S_RR, S_HH, S_IR, S_ISR, S_COEF, S_T, S_M};
+  void* ptr[7] = {};
+  for (int i = 0; i < 7; i++) {
+    if (!sz[i]) continue;
+    int rc = ensure(ctx, slots[i], sz[i]);
+    if (rc) return rc;
+    ptr[i] = ctx->buf[slots[i]];
+  }
+  Work w;
+  w.rr = (double*)ptr[0];
+  w.hh = (float*)ptr[1];
+  w.ir = (float*)ptr[2];
+  w.isr = (float*)ptr[3];
+  w.coef = (float*)ptr[4];
+  w.T = (float*)ptr[5];
+  w.m = (float*)ptr[6];
+  return enqueue_transform(ctx, mode, itype, normalize, U0, U1, r, f, c, nr, nf, nc, out, st, w,
+                           true);
+}
+
+// ---- work arrays of ccfj_transform_dev_noalloc ----------------------------
+// One array per GPU, held by that GPU from the library's load to the end of
+// the process.  In host code `scratch` names a host-side stand-in: its
+// device address on each GPU comes from hipGetSymbolAddress, resolved once
+// per device when the first context on it is created, and only that
+// address is handed to kernels.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+std::vector<void*> g_scratch_ptr;      // per device; nullptr until resolved
+std::vector<std::string> g_scratch_err;  // per device; why resolving failed
+
+// With `device` current: resolve its copy of `scratch` unless done.
+void resolve_scratch(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= device) {
+    g_scratch_ptr.resize(device + 1, nullptr);
+    g_scratch_err.resize(device + 1);
+  }
+  if (g_scratch_ptr[device]) return;
+  void* p = nullptr;
+  hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    g_scratch_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return;
+  }
+  g_scratch_ptr[device] = p;
+}
+
+int scratch_of(int device, void** p) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < (int)g_scratch_ptr.size() && g_scratch_ptr[device]) {
+    *p = g_scratch_ptr[device];
+    return CCFJ_OK;
+  }
+  const bool known = device < (int)g_scratch_err.size() && !g_scratch_err[device].empty();
+  return fail(CCFJ_EDEVICE, "no-allocation work array of device %d unavailable: %s", device,
+              known ? g_scratch_err[device].c_str() : "not resolved");
 }
 
 std::mutex g_default_mu;
@@ -339,6 +423,7 @@ int ccfj_create(int device, ccfj_ctx** out) {
     delete ctx;
     return hip_fail(e, "hipStreamCreate");
   }
+  resolve_scratch(device);   // a failure is reported by the entry point that needs it
   *out = ctx;
   return CCFJ_OK;
 }
@@ -379,6 +464,44 @@ int ccfj_transform_dev(ccfj_ctx* ctx, int mode, int itype, int normalize, const 
   if ((rc = set_device(ctx))) return rc;
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
   return run_device(ctx, mode, itype, normalize, U0, U1, r, f, c, nr, nf, nc, out, st);
+}
+
+int ccfj_transform_dev_noalloc(ccfj_ctx* ctx, int mode, int itype, int normalize,
+                               const float* U0, const float* U1, const float* r, const float* f,
+                               const float* c, int nr, int nf, int nc, float* out,
+                               void* stream) {
+  if (!ctx) return fail(CCFJ_EINVAL, "null context");
+  int rc = check_args(mode, itype, U0, U1, r, f, c, nr, nf, nc, out);
+  if (rc) return rc;
+  if (nf == 0 || nc == 0) return CCFJ_OK;
+  size_t sz[7], total = 0;
+  work_sizes(mode, itype, normalize, nr, nf, nc, sz);
+  for (int i = 0; i < 7; i++) total += sz[i];
+  if (total > kNoAllocScratchBytes)
+    return fail(CCFJ_EINVAL,
+                "ccfj_transform_dev_noalloc: nr=%d nf=%d nc=%d needs %zu bytes of work arrays, "
+                "more than the %zu bytes it holds per GPU (use ccfj_transform_dev)",
+                nr, nf, nc, total, kNoAllocScratchBytes);
+  if ((rc = set_device(ctx))) return rc;
+  void* base = nullptr;
+  if ((rc = scratch_of(ctx->device, &base))) return rc;
+  char* q = (char*)base;
+  void* ptr[7] = {};
+  for (int i = 0; i < 7; i++) {
+    if (sz[i]) ptr[i] = q;
+    q += sz[i];
+  }
+  Work w;
+  w.rr = (double*)ptr[0];
+  w.hh = (float*)ptr[1];
+  w.ir = (float*)ptr[2];
+  w.isr = (float*)ptr[3];
+  w.coef = (float*)ptr[4];
+  w.T = (float*)ptr[5];
+  w.m = (float*)ptr[6];
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  return enqueue_transform(ctx, mode, itype, normalize, U0, U1, r, f, c, nr, nf, nc, out, st, w,
+                           false);
 }
 
 int ccfj_transform(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U0,

@@ -85,6 +85,23 @@ int ccfj_transform_dev(ccfj_ctx *ctx, int mode, int itype, int normalize,
                        const float *f, const float *c, int nr, int nf, int nc,
                        float *out, void *stream);
 
+/* The same transform with no device allocation: no counterpart in the
+ * reference (its GPU code allocates on every call, src/FJgpu.cu:398-417).
+ * For callers that cannot allocate -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  Its work arrays (station
+ * arrays, records, [nf][nc] results, column scales) live in one 64 MiB
+ * device array per GPU that the library holds for the life of the process;
+ * a size needing more returns CCFJ_EINVAL with the bytes it would need (the
+ * workload of 300 pairs x 512 f x 500 c needs about 3.6 MB).  Calls on one
+ * GPU share that array, so two calls whose work overlaps in time on one GPU
+ * (other streams, other contexts, other threads) race: serialise them.  Every
+ * work array is written before it is read; the timing of ccfj_set_timing is
+ * not recorded.  Same arguments and results as ccfj_transform_dev. */
+int ccfj_transform_dev_noalloc(ccfj_ctx *ctx, int mode, int itype, int normalize,
+                               const float *U0, const float *U1, const float *r,
+                               const float *f, const float *c, int nr, int nf, int nc,
+                               float *out, void *stream);
+
 int ccfj_synchronize(ccfj_ctx *ctx);
 
 /* Device memory on the context's device (for callers without their own
