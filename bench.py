@@ -76,6 +76,9 @@ def parse():
     p.add_argument("--epochs", type=int, default=3, help="learned: training epochs")
     p.add_argument("--copy-calib", action="store_true",
                    help="only run a device copy of known size (PMC byte-counter calibration)")
+    p.add_argument("--valu-calib", action="store_true",
+                   help="only run kernels of known instruction counts per class (issue cost "
+                        "and SQ counter calibration)")
     return p.parse_args()
 
 
@@ -113,6 +116,49 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes_read": 4 * n, "bytes_written": 4 * n, "launches": 5}))
 
 
+VALU_KINDS = ["v_fma_f32", "v_mul_f32", "v_add_f32", "v_sin_f32", "v_rcp_f32", "v_fma_f64",
+              "v_fract_f64", "v_add_u32", "v_cndmask_b32", "v_cvt_f32_f64"]
+
+
+def valu_calibration():
+    """Issue cost per instruction class: kernels of 32 * iters instructions
+    of one class per wave (cc-fjpy_amd/csrc/valu_calib.hip), timed with HIP
+    events, at 1, 2, 4 and 8 waves per SIMD; the engine clock from
+    s_memtime against the 100 MHz s_memrealtime counter.  Prints one JSON
+    line: per class and occupancy, cycles per wave64 instruction per SIMD."""
+    import ctypes
+    import torch
+    lib = ctypes.CDLL(os.path.join(ROOT, "cc-fjpy_amd", "ccfj", "libccfj_calib.so"))
+    lib.calib_run.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                              ctypes.POINTER(ctypes.c_double)]
+    lib.calib_clock.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    assert lib.calib_kinds() == len(VALU_KINDS)
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    simds = 4 * cus
+    ck = torch.zeros(3, dtype=torch.int64, device="cuda:0")
+    spins = 2_000_000
+    assert lib.calib_clock(ck.data_ptr(), spins) == 0
+    ticks, real = (int(v) for v in ck.cpu().tolist()[:2])
+    clock_ghz = ticks / (real / 100e6) / 1e9
+    iters = 4000
+    res = {"cus": cus, "simds": simds, "clock_ghz_s_memtime": round(clock_ghz, 4),
+           "instructions_per_wave": 32 * iters, "classes": {}}
+    for wps in (1, 2, 4, 8):
+        nblocks = simds * wps // 4          # 256 lanes = 4 waves per workgroup
+        out = torch.empty(nblocks * 256, dtype=torch.float32, device="cuda:0")
+        waves = nblocks * 4
+        for k, name in enumerate(VALU_KINDS):
+            ms = ctypes.c_double()
+            for _ in range(2):              # the second run is the one kept
+                assert lib.calib_run(k, nblocks, iters, out.data_ptr(), ctypes.byref(ms)) == 0
+            per_simd = waves * 32.0 * iters / simds
+            cyc = ms.value * 1e-3 * clock_ghz * 1e9 / per_simd
+            res["classes"].setdefault(name, {})[f"{wps}_waves_per_simd"] = {
+                "ms": round(ms.value, 4), "cycles_per_instruction_per_simd": round(cyc, 3)}
+        print(f"bench: valu calibration at {wps} waves/SIMD done", file=sys.stderr, flush=True)
+    print(json.dumps(res), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -122,6 +168,9 @@ def main():
         sys.exit(2)
     if args.copy_calib:
         copy_calibration()
+        return
+    if args.valu_calib:
+        valu_calibration()
         return
     if args.workload != "dense":
         {"batched": bench_batched, "mwfj": bench_mwfj, "learned": bench_learned,
@@ -145,7 +194,9 @@ def main():
     tU = torch.from_numpy(U).to(dev)
     tH = torch.empty_like(tU)
     tr, tf, tc = (torch.from_numpy(a).to(dev) for a in (r, f, c))
-    tout = torch.empty((nc, nf), dtype=torch.float32, device=dev)
+    # one result array per variant, so that the timed outputs stay for the
+    # accuracy check after the timed region
+    touts = {v: torch.empty((nc, nf), dtype=torch.float32, device=dev) for v in variants}
     ctx = ccfj._context(0)
     # a dedicated stream (default priority): the library launches on it and
     # the step events are recorded on it
@@ -158,7 +209,8 @@ def main():
             ctx.hilbert_dev(tU.data_ptr(), tH.data_ptr(), nr, nf, stream)
         mode = ccfj.MODE_J if func == 0 else ccfj.MODE_J_MINUS_Y
         ctx.transform_dev(mode, itype, True, tU.data_ptr(), tH.data_ptr(), tr.data_ptr(),
-                          tf.data_ptr(), tc.data_ptr(), nr, nf, nc, tout.data_ptr(), stream)
+                          tf.data_ptr(), tc.data_ptr(), nr, nf, nc,
+                          touts[(itype, func)].data_ptr(), stream)
 
     def step():
         for it, fn in variants:
@@ -180,6 +232,9 @@ def main():
         print(f"bench: step {k + 1}/{args.steps} enqueued", file=sys.stderr, flush=True)
     e1.record(tstream)
     torch.cuda.synchronize()
+    # the last timed step's outputs at the accuracy columns (0.49 and 2 Hz)
+    acc_cols = [1365, nf - 1]
+    timed_cols = {v: touts[v][:, acc_cols].cpu().numpy() for v in variants}
     wall = time.perf_counter() - t0
     gpu_ms = e0.elapsed_time(e1)
     ms_per_step = gpu_ms / args.steps
@@ -248,7 +303,9 @@ def main():
 
     if not args.no_plain:
         from ccfj import _ccfj
-        cols = np.arange(0, nf, 128)
+        # every 16th column: 256 frequencies x 3000 velocities in 32 x 16
+        # blocks is 8 x 188 = 1504 workgroups, enough to fill the 256 CUs
+        cols = np.arange(0, nf, 16)
         Us, Hs, fs = np.ascontiguousarray(U[:, cols]), np.ascontiguousarray(H[:, cols]), f[cols]
         pt, pms = 0.0, 0.0
         print("bench: plain FP64 GPU baseline", file=sys.stderr, flush=True)
@@ -262,8 +319,10 @@ def main():
                 pt += synth.term_evals(nr, len(cols), nc)
         prate = pt / (pms / 1e3)
         out["plain_fp64_gpu"] = {"value": prate, "unit": "term-evals/s",
-                                 "sample": f"{len(cols)} of {nf} frequency columns (every 128th), "
-                                           "all variants, kernel time"}
+                                 "sample": f"{len(cols)} of {nf} frequency columns (every 16th), "
+                                           f"all {nc} velocities and {nr} pairs, all variants, "
+                                           "kernel time",
+                                 "workgroups": ((len(cols) + 31) // 32) * ((nc + 15) // 16)}
         out["speedup_vs_plain_fp64_gpu"] = round(value / prate, 2)
 
     if not args.no_cpu_baseline:
@@ -289,6 +348,32 @@ def main():
                                          f"every 4th of {nc} velocities, all {nr} pairs, every "
                                          f"variant ({terms:.3e} term-evals in {dt:.1f} s)",
                                "cpu": cpu_model()}
+        # the same CPU leg checks the timed outputs: e = |X - X*| / max_c |X*[:, f]|
+        # against the FP64 oracle on the same float32 inputs, with the Hilbert
+        # transform of the oracle side taken on the host (numpy, float64)
+        print("bench: accuracy of the timed outputs against the FP64 oracle", file=sys.stderr,
+              flush=True)
+        Hh = ccfj.hilbert_rows(U)
+        ref = {}
+        acc = {"columns": acc_cols, "f_hz": [float(f[i]) for i in acc_cols], "bar": 1e-5,
+               "reference": "FP64 oracle (oracle/fj_oracle64.c) on the float32 inputs, "
+                            "per-column normalised"}
+        for it, fn in variants:
+            if (it, 0) not in ref:
+                ref[it, 0] = O.oracle64(U, r, f, c, it, 0, fidx=acc_cols, nthreads=cores)
+            want = ref[it, 0]
+            if fn == 1:
+                if (it, 1) not in ref:
+                    ref[it, 1] = O.oracle64(Hh, r, f, c, it, 1, fidx=acc_cols, nthreads=cores)
+                want = want - ref[it, 1]
+            want = want / np.max(np.abs(want), axis=0)
+            emax, erms, mism = O.column_error(timed_cols[(it, fn)], want)
+            acc[f"itype{it}_func{fn}"] = {"e_max": float(f"{emax:.3e}"), "e_rms": float(f"{erms:.3e}"),
+                                          "nonfinite_mismatch": mism}
+        acc["all_within_bar"] = all(acc[f"itype{a}_func{b}"]["e_max"] <= 1e-5 and
+                                    acc[f"itype{a}_func{b}"]["nonfinite_mismatch"] == 0
+                                    for a, b in variants)
+        out["accuracy"] = acc
     print(json.dumps(out), flush=True)
 
 

@@ -6,6 +6,7 @@ Produces, next to the Python package so they travel with the source tree:
   ccfj/libccfj.so          HIP kernels + C ABI (include/ccfj.h), gfx950
   ccfj/_ccfj.*.so          Cython shim linked against libccfj.so
   ccfj/libccfj_emul.so     host build of the kernel arithmetic (CPU tests)
+  ccfj/libccfj_calib.so    instruction-issue calibration kernels (bench.py --valu-calib)
 """
 import os
 import subprocess
@@ -42,6 +43,14 @@ def build_hip():
     return out
 
 
+def build_calib():
+    """Instruction-issue calibration kernels (bench.py --valu-calib); a
+    measurement aid, kept out of libccfj.so."""
+    out = os.path.join(PKG, "libccfj_calib.so")
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", out, os.path.join(CSRC, "valu_calib.hip")])
+    return out
+
+
 def build_cython():
     import numpy
     pyx = os.path.join(PKG, "_ccfj.pyx")
@@ -69,6 +78,7 @@ def build_all():
     build_hip()
     build_cython()
     build_emul()
+    build_calib()
 
 
 if __name__ == "__main__":

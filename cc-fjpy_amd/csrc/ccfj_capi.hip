@@ -65,7 +65,7 @@ bool injected_alloc_failure() {
   return fail_at > 0 && n == fail_at;
 }
 
-enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_IR, S_ISR, S_COEF, S_T, S_M, S_X0,
+enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_IR, S_ISR, S_BW, S_COEF, S_T, S_M, S_X0,
             S_COUNT = S_X0 + ccfj_internal::kScratchSlots };
 
 }  // namespace
@@ -115,7 +115,8 @@ int check_args(int mode, int itype, const void* U0, const void* U1, const void* 
   if (mode < CCFJ_MODE_J || mode > CCFJ_MODE_ABS_H) return fail(CCFJ_EINVAL, "unknown mode %d", mode);
   if (itype != 0 && itype != 1) return fail(CCFJ_EINVAL, "itype must be 0 or 1, got %d", itype);
   if (nr < 0 || nf < 0 || nc < 0) return fail(CCFJ_EINVAL, "negative size (nr=%d nf=%d nc=%d)", nr, nf, nc);
-  if ((long long)nf * ((nc + 255) / 256) > (1LL << 31) - 8) return fail(CCFJ_EINVAL, "grid too large");
+  // one workgroup per 64 velocities of a frequency (the main kernel's tile)
+  if ((long long)nf * ((nc + 63) / 64) > (1LL << 31) - 8) return fail(CCFJ_EINVAL, "grid too large");
   if (nr > 0 && nf > 0 && (!U0 || !r || !f)) return fail(CCFJ_EINVAL, "null input pointer");
   if (nr > 0 && nf > 0 && two_spectra(mode) && !U1) return fail(CCFJ_EINVAL, "mode %d needs U1", mode);
   if (nf > 0 && nc > 0 && (!c || !out)) return fail(CCFJ_EINVAL, "null c/out pointer");
@@ -134,21 +135,36 @@ struct Work {
   float* hh = nullptr;
   float* ir = nullptr;
   float* isr = nullptr;
+  uint32_t* bw = nullptr;
   float* coef = nullptr;
   float* T = nullptr;
   float* m = nullptr;
 };
+constexpr int kWork = 8;
 
 // Bytes of each work array, in Work's order (256-byte multiples).
-void work_sizes(int mode, int itype, int normalize, int nr, int nf, int nc, size_t sz[7]) {
+void work_sizes(int mode, int itype, int normalize, int nr, int nf, int nc, size_t sz[kWork]) {
   auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const int nrp = fjk::padded(nr);
   const bool seg = nr >= 2;
   sz[0] = seg ? up(sizeof(double) * (nrp + 8)) : 0;
-  sz[1] = sz[2] = sz[3] = seg ? up(sizeof(float) * (nrp + 8)) : 0;
-  sz[4] = seg ? up(sizeof(float) * fjk::coef_floats(mode, itype, nrp, nf)) : 0;
-  sz[5] = up(sizeof(float) * (size_t)nf * nc);
-  sz[6] = normalize ? up(sizeof(float) * (size_t)nf) : 0;
+  sz[1] = sz[2] = sz[3] = sz[4] = seg ? up(sizeof(float) * (nrp + 8)) : 0;
+  sz[5] = seg ? up(sizeof(float) * fjk::coef_floats(mode, itype, nrp, nf)) : 0;
+  sz[6] = up(sizeof(float) * (size_t)nf * nc);
+  sz[7] = normalize ? up(sizeof(float) * (size_t)nf) : 0;
+}
+
+Work work_from(void* const ptr[kWork]) {
+  Work w;
+  w.rr = (double*)ptr[0];
+  w.hh = (float*)ptr[1];
+  w.ir = (float*)ptr[2];
+  w.isr = (float*)ptr[3];
+  w.bw = (uint32_t*)ptr[4];
+  w.coef = (float*)ptr[5];
+  w.T = (float*)ptr[6];
+  w.m = (float*)ptr[7];
+  return w;
 }
 
 // Enqueue prep + main kernel (+ normalisation) on st.  Inputs and work
@@ -180,6 +196,7 @@ int enqueue_transform(ccfj_ctx* ctx, int mode, int itype, int normalize, const f
     p.hh = w.hh;
     p.ir = w.ir;
     p.isr = w.isr;
+    p.bw = w.bw;
     p.coef = w.coef;
     p.T = w.T;
     e = fjk::launch_prep(mode, itype, p, st);
@@ -200,24 +217,17 @@ int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U
                const float* U1, const float* r, const float* f, const float* c, int nr,
                int nf, int nc, float* out, hipStream_t st) {
   if (nf == 0 || nc == 0) return CCFJ_OK;
-  size_t sz[7];
+  size_t sz[kWork];
   work_sizes(mode, itype, normalize, nr, nf, nc, sz);
-  const Slot slots[7] = {S_RR, S_HH, S_IR, S_ISR, S_COEF, S_T, S_M};
-  void* ptr[7] = {};
-  for (int i = 0; i < 7; i++) {
+  const Slot slots[kWork] = {S_RR, S_HH, S_IR, S_ISR, S_BW, S_COEF, S_T, S_M};
+  void* ptr[kWork] = {};
+  for (int i = 0; i < kWork; i++) {
     if (!sz[i]) continue;
     int rc = ensure(ctx, slots[i], sz[i]);
     if (rc) return rc;
     ptr[i] = ctx->buf[slots[i]];
   }
-  Work w;
-  w.rr = (double*)ptr[0];
-  w.hh = (float*)ptr[1];
-  w.ir = (float*)ptr[2];
-  w.isr = (float*)ptr[3];
-  w.coef = (float*)ptr[4];
-  w.T = (float*)ptr[5];
-  w.m = (float*)ptr[6];
+  const Work w = work_from(ptr);
   return enqueue_transform(ctx, mode, itype, normalize, U0, U1, r, f, c, nr, nf, nc, out, st, w,
                            true);
 }
@@ -474,9 +484,9 @@ int ccfj_transform_dev_noalloc(ccfj_ctx* ctx, int mode, int itype, int normalize
   int rc = check_args(mode, itype, U0, U1, r, f, c, nr, nf, nc, out);
   if (rc) return rc;
   if (nf == 0 || nc == 0) return CCFJ_OK;
-  size_t sz[7], total = 0;
+  size_t sz[kWork], total = 0;
   work_sizes(mode, itype, normalize, nr, nf, nc, sz);
-  for (int i = 0; i < 7; i++) total += sz[i];
+  for (int i = 0; i < kWork; i++) total += sz[i];
   if (total > kNoAllocScratchBytes)
     return fail(CCFJ_EINVAL,
                 "ccfj_transform_dev_noalloc: nr=%d nf=%d nc=%d needs %zu bytes of work arrays, "
@@ -486,19 +496,12 @@ int ccfj_transform_dev_noalloc(ccfj_ctx* ctx, int mode, int itype, int normalize
   void* base = nullptr;
   if ((rc = scratch_of(ctx->device, &base))) return rc;
   char* q = (char*)base;
-  void* ptr[7] = {};
-  for (int i = 0; i < 7; i++) {
+  void* ptr[kWork] = {};
+  for (int i = 0; i < kWork; i++) {
     if (sz[i]) ptr[i] = q;
     q += sz[i];
   }
-  Work w;
-  w.rr = (double*)ptr[0];
-  w.hh = (float*)ptr[1];
-  w.ir = (float*)ptr[2];
-  w.isr = (float*)ptr[3];
-  w.coef = (float*)ptr[4];
-  w.T = (float*)ptr[5];
-  w.m = (float*)ptr[6];
+  const Work w = work_from(ptr);
   hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
   return enqueue_transform(ctx, mode, itype, normalize, U0, U1, r, f, c, nr, nf, nc, out, st, w,
                            false);
@@ -564,6 +567,7 @@ int ccfj_transform_items(ccfj_ctx* ctx, int mode, int itype, const float* U0, co
   if ((rc = ensure(ctx, S_HH, sizeof(float) * (nrp + 8)))) return rc;
   if ((rc = ensure(ctx, S_IR, sizeof(float) * (nrp + 8)))) return rc;
   if ((rc = ensure(ctx, S_ISR, sizeof(float) * (nrp + 8)))) return rc;
+  if ((rc = ensure(ctx, S_BW, sizeof(uint32_t) * (nrp + 8)))) return rc;
   if ((rc = ensure(ctx, S_COEF, sizeof(float) * fjk::coef_floats(mode, itype, nrp, nf)))) return rc;
   fjk::Problem p;
   p.U0 = U0;
@@ -579,6 +583,7 @@ int ccfj_transform_items(ccfj_ctx* ctx, int mode, int itype, const float* U0, co
   p.hh = (float*)ctx->buf[S_HH];
   p.ir = (float*)ctx->buf[S_IR];
   p.isr = (float*)ctx->buf[S_ISR];
+  p.bw = (uint32_t*)ctx->buf[S_BW];
   p.coef = (float*)ctx->buf[S_COEF];
   p.T = nullptr;   // the item kernel writes out[c][f] directly
   hipError_t e = fjk::launch_prep(mode, itype, p, st);

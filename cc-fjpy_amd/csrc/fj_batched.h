@@ -25,6 +25,7 @@ struct BatchDev {
   float* hh;
   float* ir;
   float* isr;
+  uint32_t* bw;          // block words (fjl::station_block_word), at poff like rr
   float* coef;
   float* T;              // [nb][nf][nc] persistent-kernel results
   float* m;              // [nb][nf] column scales (normalisation)

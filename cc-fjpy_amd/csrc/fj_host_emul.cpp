@@ -27,8 +27,12 @@ void run_mode(int itype, const float* U0, const float* U1, const float* r, const
   const int ntot = nrp + kUnroll;
   std::vector<double> rr(ntot);
   std::vector<float> hh(ntot), ir(ntot), isr(ntot);
-  for (int s = 0; s < ntot; s++) station_entry(r, nr, s, rr.data(), hh.data(), ir.data(), isr.data());
-  const Stations S{rr.data(), hh.data(), ir.data(), isr.data()};
+  std::vector<uint32_t> bw(ntot);
+  for (int s = 0; s < ntot; s++) {
+    station_entry(r, nr, s, rr.data(), hh.data(), ir.data(), isr.data());
+    bw[s] = station_block_word(r, nr, s);
+  }
+  const Stations S{rr.data(), hh.data(), ir.data(), isr.data(), bw.data()};
   const size_t block = coef_block(nrp, MODE, itype);
   std::vector<float> cf(block, 0.0f);
   float cmin = INFINITY;

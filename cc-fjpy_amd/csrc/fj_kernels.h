@@ -21,6 +21,7 @@ struct Problem {
   float* hh;     // [nrp + 8] workspace
   float* ir;     // [nrp + 8] workspace, 1 / r
   float* isr;    // [nrp + 8] workspace, 1 / sqrt(r)
+  uint32_t* bw;  // [nrp + 8] workspace, block words (fjl::station_block_word)
   float* coef;   // coef_floats(...) workspace
   float* T;      // [nf][nc] main-kernel results (k_finish forms [nc][nf])
 };

@@ -9,6 +9,8 @@
 //   rr[nrp + 8]       double   station distances, ascending
 //   hh[nrp + 8]       float    r_s - r_{s-1} (hh[0] = 0)
 //   ir, isr[nrp + 8]  float    1 / r_s, 1 / sqrt(r_s)
+//   bw[nrp + 8]       uint32   block words of the far / smooth tests
+//                              (fjl::station_block_word)
 //   coef              float    per frequency, fjl::coef_block floats:
 //       itype 0: [nrp][nspec] A_s / sqrt(r_s) of each spectrum (J - Y: in
 //                polar form, fjl::to_polar)
@@ -20,9 +22,9 @@
 //                              formed from T by k_finish (optionally
 //                              normalised per frequency)
 //
-// Launch: a workgroup is 256 lanes = 256 consecutive velocities of ONE
-// frequency.  The station data of the loop are then wave-uniform and are
-// read with scalar loads (one record per station per wave); each
+// Launch: a workgroup is one wave of 64 lanes = 64 consecutive velocities
+// of ONE frequency.  The station data of the loop are then wave-uniform and
+// are read with scalar loads (one record per station per wave); each
 // lane owns one (f,c) output and keeps the whole station loop in registers,
 // so the main kernels are VALU-bound by construction.  Workgroups are
 // numbered XCD-major so that the workgroups of one frequency share an XCD
@@ -41,7 +43,12 @@ namespace fjk {
 
 using namespace fjl;
 
-constexpr int kBlock = 256;
+// Lanes per workgroup of the dense kernels: one wave.  Measured on the
+// MI355X (19,900 pairs x 4096 f x 3000 c): trapezoid J 96.9 ms against 102.8
+// with 128 lanes and 105.2 with 256, J - Y 139.5 / 148.9 / 155.1 ms; linear
+// segments J 222.8 / 229.0 / 231.2, J - Y 396.0 / 408.5 / 412.3 ms.  A
+// workgroup of several waves holds its slots until its slowest wave ends.
+constexpr int kBlock = 64;
 
 // Hardware places workgroup w on XCD (w mod 8); tile T = (w mod 8) * per +
 // w / 8 keeps consecutive tiles (same frequency) on one XCD.  Returns false
@@ -58,9 +65,12 @@ __device__ inline bool tile_of(int ntiles, int* tile) {
 // distance and zero gaps
 __global__ void k_stations(const float* __restrict__ r, int nr, int ntot, double* __restrict__ rr,
                            float* __restrict__ hh, float* __restrict__ ir,
-                           float* __restrict__ isr) {
+                           float* __restrict__ isr, uint32_t* __restrict__ bw) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < ntot) station_entry(r, nr, s, rr, hh, ir, isr);
+  if (s < ntot) {
+    station_entry(r, nr, s, rr, hh, ir, isr);
+    bw[s] = station_block_word(r, nr, s);
+  }
 }
 
 // Coefficients + [nr][nf] -> per-frequency records (fj_prep.h).
@@ -157,7 +167,7 @@ __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
 }
 
 // ---------------------------------------------------------------------------
-Stations stations(const Problem& p) { return Stations{p.rr, p.hh, p.ir, p.isr}; }
+Stations stations(const Problem& p) { return Stations{p.rr, p.hh, p.ir, p.isr, p.bw}; }
 
 template <int MODE>
 static hipError_t launch_main(int itype, const Problem& p, hipStream_t st) {
@@ -190,7 +200,7 @@ int padded(int nr) { return padded_stations(nr); }
 hipError_t launch_prep(int mode, int itype, const Problem& p, hipStream_t st) {
   const int ntot = p.nrp + kUnroll;
   hipLaunchKernelGGL(k_stations, dim3((ntot + 255) / 256), dim3(256), 0, st, p.r, p.nr, ntot,
-                     p.rr, p.hh, p.ir, p.isr);
+                     p.rr, p.hh, p.ir, p.isr, p.bw);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const bool two = !(mode == MODE_J || mode == MODE_Y);

@@ -69,16 +69,25 @@ struct Stations {
   const float* hh;   // r_s - r_{s-1} (hh[0] = 0)
   const float* ir;   // 1 / r_s      (inf at r = 0, where only x < 8 code runs)
   const float* isr;  // 1 / sqrt(r_s)
+  const uint32_t* bw;  // block words (station_block_word)
 };
+
+// 1/r_s and the gap left of station s as the station arrays hold them
+// (entries past the last station repeat its distance, with a zero gap).
+FJ_HD inline float station_ir(const float* r, int nr, int s) {
+  return (float)(1.0 / (double)r[s < nr ? s : nr - 1]);
+}
+FJ_HD inline float station_hh(const float* r, int nr, int s) {
+  return (s < nr && s > 0) ? r[s] - r[s - 1] : 0.0f;
+}
 
 // Entry s of the station arrays from the sorted float distances r[0..nr).
 FJ_HD inline void station_entry(const float* r, int nr, int s, double* rr, float* hh, float* ir,
                                 float* isr) {
-  const bool in = s < nr;
-  const double rs = (double)r[in ? s : nr - 1];
+  const double rs = (double)r[s < nr ? s : nr - 1];
   rr[s] = rs;
-  hh[s] = (in && s > 0) ? r[s] - r[s - 1] : 0.0f;
-  ir[s] = (float)(1.0 / rs);
+  hh[s] = station_hh(r, nr, s);
+  ir[s] = station_ir(r, nr, s);
   isr[s] = (float)(1.0 / sqrt(rs));
 }
 
@@ -132,15 +141,36 @@ FJ_HD inline uint32_t far_threshold(double q2pi) {
 #endif
 }
 
-template <int N>
-FJ_HD inline bool block_far(const float* iv, uint32_t irfar) {
-  uint32_t m = f32_bits(iv[0]);
-#pragma unroll
-  for (int k = 1; k < N; k++) {
-    const uint32_t b = f32_bits(iv[k]);
-    m = b > m ? b : m;
+// Block words, one uint32 per station slot, so that a block's far and
+// smooth tests are one wave-uniform load and one compare (the bits of
+// non-negative floats order as unsigned integers):
+//   bw[sb]     (sb % 4 == 0)  max bits of 1/r over stations sb .. sb+4
+//                             (a block of 4 and the next block's first)
+//   bw[sb + 1]                max bits of the gaps hh[sb .. sb+4]
+//   bw[sb + 2] (sb % 8 == 0)  max bits of 1/r over stations sb .. sb+7
+// other slots 0.  Built from r with the station arrays' own expressions.
+// (Measured on the MI355X, the trapezoid J kernel at 19,900 pairs x 4096 f
+// x 3000 c: 107.3 ms against 108.6 ms with the maximum taken per block.)
+FJ_HD inline uint32_t station_block_word(const float* r, int nr, int s) {
+  const int k = s & 7;
+  uint32_t m = 0;
+  if ((k & 3) == 0) {
+    for (int i = 0; i <= 4; i++) {
+      const uint32_t b = f32_bits(station_ir(r, nr, s + i));
+      m = b > m ? b : m;
+    }
+  } else if ((k & 3) == 1) {
+    for (int i = 0; i <= 4; i++) {
+      const uint32_t b = f32_bits(station_hh(r, nr, s - 1 + i));
+      m = b > m ? b : m;
+    }
+  } else if (k == 2) {
+    for (int i = 0; i < 8; i++) {
+      const uint32_t b = f32_bits(station_ir(r, nr, s - 2 + i));
+      m = b > m ? b : m;
+    }
   }
-  return m <= irfar;
+  return m;
 }
 
 // true when the predicate holds on any active lane of the wave (uniform;
@@ -164,7 +194,7 @@ constexpr float kKappa = 0.25f;
 // or takes the G difference in a far block: min(h_limit, 0.99 kTau / k_max)
 // with k_max the largest |k| of the wave (the margin covers the rounding of
 // dx = k h, so every lane's own test agrees).  A block whose gaps are all at
-// or below it compares them with block_far.
+// or below it (its block word, station_block_word) is smooth.
 FJ_HD inline uint32_t smooth_threshold(double q2pi, uint32_t hlim) {
   double kmax = fabs(q2pi);
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -479,6 +509,47 @@ FJ_HD inline void trapz_far(double rs, float ir, const LaneK& L, const float* c,
   trapz_add<MODE>(c, vj, vy, pR, pI);
 }
 
+// A far block of kUnroll stations evaluated stage by stage (all phases, then
+// all fits, then all cosines, then the sums in station order), so that each
+// stage offers kUnroll independent instructions to hide the latency of the
+// double-precision phase and the transcendental.  The same operations as
+// trapz_far per station, summed in the same order: bit-identical results.
+// Measured on the MI355X at 19,900 pairs x 4096 f x 3000 c: J 104.6 ms
+// against 107.3 ms station by station, J - Y 153.6 against 157.3 ms.
+template <int MODE>
+FJ_HD inline void trapz_far_block(const double* rv, const float* iv, const LaneK& L,
+                                  const float* cv, float& pR, float& pI) {
+  constexpr int NC = ModeT<MODE>::nspec;
+  if (MODE == MODE_J || MODE == MODE_J_MINUS_Y) {
+    float ph[kUnroll], z[kUnroll], t[kUnroll], u[kUnroll], v[kUnroll];
+    double w[kUnroll];
+    // phase_rev in three stages of kUnroll independent instructions
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) w[k] = __builtin_fma(L.q, rv[k], -0.125);
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) w[k] = fract_d(w[k]);
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) ph[k] = (float)w[k];
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) {
+      z[k] = L.zk * iv[k];
+      t[k] = z[k] * z[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) {
+      const float p = MODE == MODE_J_MINUS_Y ? ph[k] + cv[k * NC + 1] : ph[k];
+      u[k] = __builtin_fmaf(z[k], fitf_ph0(t[k]), p);
+    }
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) v[k] = fitf_M0(t[k]) * cos_rev(u[k]);
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) pR = __builtin_fmaf(cv[k * NC], v[k], pR);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kUnroll; k++) trapz_far<MODE>(rv[k], iv[k], L, cv + k * NC, pR, pI);
+  }
+}
+
 template <int MODE>
 FJ_HD inline float trapz_lane(const Stations& S, const float* __restrict__ cf, int nrp, double q) {
   constexpr int NC = ModeT<MODE>::nspec;
@@ -498,9 +569,8 @@ FJ_HD inline float trapz_lane(const Stations& S, const float* __restrict__ cf, i
       for (int k = 0; k < kUnroll; k++) iv[k] = S.ir[sb + k];
 #pragma unroll
       for (int k = 0; k < kUnroll * NC; k++) cv[k] = cf[sb * NC + k];
-      if (block_far<kUnroll>(iv, irfar)) {
-#pragma unroll
-        for (int k = 0; k < kUnroll; k++) trapz_far<MODE>(rv[k], iv[k], L, cv + k * NC, pR, pI);
+      if (S.bw[sb + 2] <= irfar) {   // every station of the block has x >= kFarX
+        trapz_far_block<MODE>(rv, iv, L, cv, pR, pI);
       } else {
 #pragma unroll
         for (int k = 0; k < kUnroll; k++)
@@ -754,21 +824,14 @@ struct LinFar {
   static constexpr bool smooth = ModeT<MODE>::nspec == 2 || !SMALL || FJ_SMOOTH_SMALL != 0;
 };
 
-// Stations per block of the linear-segment loop.  4, not kUnroll: with 8,
+// Stations per block of the linear-segment loop: 4, not kUnroll.  With 8,
 // the (a, p, t) records (up to 48 floats) and the station data exceed the
 // scalar registers and spill to vector lanes (J - Y: 718 ms against 447 ms
-// per 4096 dense columns; J: 280 ms against 249 ms).  The same holds in
-// the kernels for small arrays (4096 ragged sub-arrays: 638 ms with 8
-// against 602 ms).  Tuning switches:
-// -DFJ_LIN_UNROLL=N and -DFJ_LIN_UNROLL_SMALL=N (divisors of kUnroll) set
-// the block, -DFJ_PRE_SMALL=0 turns the b prefetch off in the small-array
-// kernels (646 ms against 631 ms).
-#ifndef FJ_LIN_UNROLL
-#define FJ_LIN_UNROLL 4
-#endif
-#ifndef FJ_LIN_UNROLL_SMALL
-#define FJ_LIN_UNROLL_SMALL 4
-#endif
+// per 4096 dense columns; J: 280 ms against 249 ms; 4096 ragged sub-arrays
+// 638 ms against 602 ms).  The block words are laid out for blocks of 4.
+// -DFJ_PRE_SMALL=0 turns the b prefetch off in the small-array kernels
+// (646 ms against 631 ms), a tuning switch.
+constexpr int kLinBlock = 4;
 #ifndef FJ_PRE_SMALL
 #define FJ_PRE_SMALL 1
 #endif
@@ -782,8 +845,8 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
   using LF = LinFar<MODE, SMALL>;
   constexpr int NS = ModeT<MODE>::nspec;
   constexpr int NC = 3 * NS;
-  constexpr int LU = SMALL ? FJ_LIN_UNROLL_SMALL : FJ_LIN_UNROLL;   // divides kUnroll (nrp's multiple)
-  static_assert(kUnroll % LU == 0 && kPartial % LU == 0, "station block");
+  constexpr int LU = kLinBlock;
+  static_assert(kUnroll % LU == 0 && kPartial % LU == 0 && LU == 4, "station block");
   const float* __restrict__ cfb = cf + (size_t)nrp * NC;
   const uint32_t hlim = f32_bits(cf[(size_t)nrp * 4 * NS]);
   const LaneK L = lane_consts(q);
@@ -803,20 +866,20 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
   float dxR = 0.0f;
   for (int sb = 0; sb < nrp; sb += LU) {
     double rv[LU];
-    float hw[LU + 1], iv[LU + 1], sv[LU];   // hw[k + 1]: gap right of station sb + k
+    float hw[LU + 1], iv[LU], sv[LU];   // hw[k + 1]: gap right of station sb + k
     float cv[LU * NC];
 #pragma unroll
     for (int k = 0; k < LU; k++) rv[k] = S.rr[sb + k];
 #pragma unroll
-    for (int k = 0; k <= LU; k++) hw[k] = S.hh[sb + k];
+    for (int k = 1; k <= LU; k++) hw[k] = S.hh[sb + k];
 #pragma unroll
-    for (int k = 0; k <= LU; k++) iv[k] = S.ir[sb + k];   // + the next block's first
+    for (int k = 0; k < LU; k++) iv[k] = S.ir[sb + k];
 #pragma unroll
     for (int k = 0; k < LU; k++) sv[k] = S.isr[sb + k];
 #pragma unroll
     for (int k = 0; k < LU * NC; k++) cv[k] = cf[sb * NC + k];
-    const bool far = (LF::smooth || LF::on) && block_far<LU + 1>(iv, irfar);
-    if (LF::smooth && far && block_far<LU + 1>(hw, hsm)) {
+    const bool far = (LF::smooth || LF::on) && S.bw[sb] <= irfar;
+    if (LF::smooth && far && S.bw[sb + 1] <= hsm) {
       // every gap of the block (its left one included) is below both the
       // G-difference and the long-segment limits on every lane
 #pragma unroll

@@ -238,16 +238,20 @@ FJ_HD inline float reduce_phase(double q, double r) {
   return (float)(t - rint(t));
 }
 
+// t - floor(t) for finite t (one v_fract_f64 on the device).
+FJ_HD inline double fract_d(double t) {
+#if FJ_USE_HW
+  return __builtin_amdgcn_fract(t);
+#else
+  return t - floor(t);
+#endif
+}
+
 // The same phase reduced to [0, 1) (one v_fract_f64 on the device).  The
 // float rounding of the result is at most 2^-25 revolutions, as for
 // reduce_phase, and sincos_rev accepts either range.
 FJ_HD inline float phase_rev(double q, double r) {
-  const double t = __builtin_fma(q, r, -0.125);
-#if FJ_USE_HW
-  return (float)__builtin_amdgcn_fract(t);
-#else
-  return (float)(t - floor(t));
-#endif
+  return (float)fract_d(__builtin_fma(q, r, -0.125));
 }
 
 }  // namespace fjm
