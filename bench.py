@@ -29,10 +29,13 @@ Other workloads (--workload), each printing its own JSON line:
 Prints one JSON line.  Besides the contract fields it carries
   roofline      the dominant kernel's algorithmic FP32 rate (34 flop per
                 trapezoid term, 92 per linear-segment term and basis call)
-                from HIP-event timing of that kernel, against 157.3 TFLOP/s
+                from HIP-event timing of that kernel, against 157.3 TFLOP/s,
+                and the FP32 rate it executes (committed SQ counter summary)
   cpu_baseline  the FJcpu restatement (oracle/, OpenMP, the host's cores)
-                on a bounded sample of frequency columns
-  plain_fp64_gpu  the plain single-instance FP64 GPU F-J on a column sample
+                on a bounded sample of frequency columns; the same CPU leg
+                checks the timed outputs against the FP64 oracle (accuracy)
+  plain_fp64_gpu  the plain single-instance FP64 GPU F-J on 256 frequency
+                columns (enough workgroups to fill the device)
 """
 import argparse
 import json
@@ -54,6 +57,7 @@ FLOP_PER_TERM = {0: 34, 1: 92}                # per basis call
 FP32_PEAK = 157.3                             # TFLOP/s, MI355X
 HBM_PEAK = 8000.0                             # GB/s
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_fj_dense_noise_array_pmc.md")
+VALU_SUMMARY = os.path.join(ROOT, "profiles", "k_fj_dense_noise_array_valu.md")
 
 
 def calls(func):
@@ -68,6 +72,9 @@ def parse():
     p.add_argument("--variants", default="all", help="'all' or e.g. '1:1,0:0' (itype:func)")
     p.add_argument("--kernel-reps", type=int, default=2)
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-scale", type=int, default=1,
+                   help="dense: frequency columns per thread in the CPU baseline sample "
+                        "(1 takes about 25 s on 16 cores)")
     p.add_argument("--no-plain", action="store_true")
     p.add_argument("--no-host", action="store_true", help="dense: skip the host end-to-end call")
     p.add_argument("--workload", default="dense",
@@ -82,19 +89,30 @@ def parse():
     return p.parse_args()
 
 
+def _summary_value(path, tag, key, conv):
+    try:
+        with open(path) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == tag and parts[1] == key:
+                    return conv(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def measured_traffic(key):
     """HBM bytes per launch of the F-J kernel for variant key, from the
     committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or
     None when that variant was not profiled."""
-    try:
-        with open(PMC_SUMMARY) as fh:
-            for line in fh:
-                parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == key:
-                    return int(parts[2])
-    except OSError:
-        pass
-    return None
+    return _summary_value(PMC_SUMMARY, "traffic_bytes_per_launch", key, int)
+
+
+def executed_tflops(key):
+    """FP32 TFLOP/s the F-J kernel of variant key executes (2 FMA + ADD + MUL
+    instructions x 64 lanes over its duration), from the committed rocprofv3
+    SQ counter summary, or None."""
+    return _summary_value(VALU_SUMMARY, "executed_fp32_tflops", key, float)
 
 
 def cpu_model():
@@ -103,6 +121,14 @@ def cpu_model():
             return next(l.split(":", 1)[1].strip() for l in fh if l.startswith("model name"))
     except Exception:
         return ""
+
+
+def host_info():
+    """CPU model, the cores of this process's affinity mask and the host's
+    OMP_NUM_THREADS, beside the threads a CPU baseline used ("cores")."""
+    from oracle import oracle as O
+    return {"cpu": cpu_model(), "host_cores": O.host_cores(),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def copy_calibration():
@@ -261,9 +287,14 @@ def main():
     dom_key = max(kernels, key=lambda k: kernels[k]["avg_ms"])
     dom = kernels[dom_key]
     achieved = dom["tflops_alg"]
+    ex = executed_tflops(dom_key)
     roofline = {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK, "unit": "TFLOP/s",
                 "frac": round(achieved / FP32_PEAK, 4), "traffic": measured_traffic(dom_key),
-                "kernel": f"k_fj ({dom_key})", "avg_ms": dom["avg_ms"]}
+                "kernel": f"k_fj ({dom_key})", "avg_ms": dom["avg_ms"],
+                "achieved_is": "algorithmic flops (92 per linear-segment term and basis call) per "
+                               "launch / HIP-event launch time",
+                "executed_fp32_tflops": ex,
+                "executed_frac": None if ex is None else round(ex / FP32_PEAK, 4)}
     print(f"bench: kernels {json.dumps(kernels)}", file=sys.stderr, flush=True)
 
     out = {
@@ -330,7 +361,7 @@ def main():
         cores = O.nthreads_default()
         # the restatement parallelises over frequency columns (OpenMP over f,
         # as FJcpu): one column per thread, every 4th velocity
-        cols = np.linspace(128, nf - 1, cores).astype(np.int32)
+        cols = np.linspace(128, nf - 1, cores * max(1, args.cpu_scale)).astype(np.int32)
         cs = np.ascontiguousarray(c[::4])
         print(f"bench: CPU baseline on {cores} threads", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
@@ -344,10 +375,11 @@ def main():
         dt = time.perf_counter() - t0
         out["cpu_baseline"] = {"value": terms / dt, "unit": "term-evals/s", "cores": cores,
                                "kind": "port",
-                               "sample": f"{len(cols)} frequency columns of {nf} (one per thread), "
+                               "sample": f"{len(cols)} frequency columns of {nf} "
+                                         f"({len(cols) // cores} per thread), "
                                          f"every 4th of {nc} velocities, all {nr} pairs, every "
                                          f"variant ({terms:.3e} term-evals in {dt:.1f} s)",
-                               "cpu": cpu_model()}
+                               **host_info()}
         # the same CPU leg checks the timed outputs: e = |X - X*| / max_c |X*[:, f]|
         # against the FP64 oracle on the same float32 inputs, with the Hilbert
         # transform of the oracle side taken on the host (numpy, float64)
@@ -461,7 +493,7 @@ def bench_batched(args):
         out["cpu_baseline"] = {"value": ct / dt, "unit": "term-evals/s", "cores": cores,
                                "kind": "port",
                                "sample": f"sub-arrays {picks} in full ({ct:.3e} term-evals in "
-                                         f"{dt:.1f} s)", "cpu": cpu_model()}
+                                         f"{dt:.1f} s)", **host_info()}
     print(json.dumps(out), flush=True)
 
 
@@ -578,7 +610,7 @@ def bench_mwfj(args):
                                          f"{nf} frequencies and {nc} velocities ({ct:.3e} "
                                          f"term-evals in {dt:.1f} s; F-J only, no windowing or "
                                          f"FFT)",
-                               "cpu": cpu_model()}
+                               **host_info()}
     print(json.dumps(out), flush=True)
 
 
@@ -633,7 +665,7 @@ def bench_learned(args):
                                "kind": "port",
                                "sample": f"one field's full dense grid ({m.nr} pairs x {m.W} f x "
                                          f"{m.nc_dense} c, itype {m.itype}: {ct:.3e} term-evals "
-                                         f"in {dt:.1f} s)", "cpu": cpu_model()}
+                                         f"in {dt:.1f} s)", **host_info()}
     print(json.dumps(out), flush=True)
 
 
@@ -706,19 +738,21 @@ def bench_cc(args):
                                       "device work, D2H of the complex64 result"}
     res["output_shape"] = list(out.shape)
     if not args.no_cpu_baseline:
-        nw = 64   # the restatement on every station and pair, first 64 windows
-        npts_s = fftlen + (nw - 1) * steplen
-        ds = np.ascontiguousarray(data.reshape(nsta, npts)[:, :npts_s]).reshape(-1)
-        ses = np.tile(np.array([0, npts_s + 1], np.int32), nsta)
+        # the restatement on the whole job: every station, pair and window,
+        # FFTs and pair sums on the host's threads
+        from oracle import oracle as O
+        cores = O.nthreads_default()
+        print(f"bench: CPU baseline on {cores} threads", file=sys.stderr, flush=True)
         t0 = time.perf_counter()
-        _, cnt = C.cross_correlate(ds, nsta, npts_s, nf, fftlen, steplen, pairs, ses,
-                                   fstride=fstride)
+        _, cnt = C.cross_correlate_threaded(data, nsta, npts, nf, fftlen, steplen, pairs, se,
+                                            fstride=fstride, nthreads=cores)
         dt = time.perf_counter() - t0
         res["cpu_baseline"] = {"value": float(cnt.sum()) / dt, "unit": "pair-windows/s",
-                               "cores": 1, "kind": "port",
-                               "sample": f"numpy restatement (float64 FFT) on all {nsta} stations "
-                                         f"and {npairs} pairs, first {nw} windows ({dt:.1f} s)",
-                               "cpu": cpu_model()}
+                               "cores": cores, "kind": "port",
+                               "sample": f"threaded restatement (scipy float64 FFT) on all {nsta} "
+                                         f"stations, {npairs} pairs and {nshifts} windows "
+                                         f"({int(cnt.sum())} pair-windows in {dt:.1f} s)",
+                               **host_info()}
     print(json.dumps(res), flush=True)
 
 

@@ -87,6 +87,8 @@ cdef extern from "ccfj.h" nogil:
                           const float *modes, float halfwidth, float amin, float *Y, void *stream)
     int ccfj_pool_abs_max(ccfj_ctx *ctx, const float *inp, int B, int H, int S, int W, float *out,
                           void *stream)
+    int ccfj_pool_abs_max_ranges(ccfj_ctx *ctx, const float *inp, int B, int N, int H, int W,
+                                 const int *ranges, float *out, void *stream)
     int ccfj_label_argmax(ccfj_ctx *ctx, const float *X, int B, int H, int W, float *Y,
                           void *stream)
     int ccfj_cross_correlate(ccfj_ctx *ctx, int nsta, int npts, int nf, int fstride, int fftlen,
@@ -555,6 +557,12 @@ def synth_labels(Context ctx, int B, int H, int nf, size_t cgrid, size_t f, size
 def pool_abs_max(Context ctx, size_t x, int B, int H, int S, int W, size_t out):
     _check(ccfj_pool_abs_max(ctx.ctx, <const float *>x, B, H, S, W, <float *>out, NULL),
            "ccfj_pool_abs_max")
+
+
+def pool_abs_max_ranges(Context ctx, size_t x, int B, int N, int H, int W, size_t ranges,
+                        size_t out):
+    _check(ccfj_pool_abs_max_ranges(ctx.ctx, <const float *>x, B, N, H, W, <const int *>ranges,
+                                    <float *>out, NULL), "ccfj_pool_abs_max_ranges")
 
 
 def label_argmax(Context ctx, size_t x, int B, int H, int W, size_t y):

@@ -631,6 +631,17 @@ int ccfj_pool_abs_max(ccfj_ctx* ctx, const float* in, int B, int H, int S, int W
   return e == hipSuccess ? CCFJ_OK : hip_fail(e, "pool kernel launch");
 }
 
+int ccfj_pool_abs_max_ranges(ccfj_ctx* ctx, const float* in, int B, int N, int H, int W,
+                             const int* ranges, float* out, void* stream) {
+  if (!ctx || !in || !ranges || !out) return fail(CCFJ_EINVAL, "null argument");
+  if (B < 1 || N < 1 || H < 1 || W < 1) return fail(CCFJ_EINVAL, "bad sizes");
+  int rc = set_device(ctx);
+  if (rc) return rc;
+  hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+  hipError_t e = fjk::launch_pool_abs_max_ranges(in, B, N, H, W, ranges, out, st);
+  return e == hipSuccess ? CCFJ_OK : hip_fail(e, "pool kernel launch");
+}
+
 int ccfj_label_argmax(ccfj_ctx* ctx, const float* X, int B, int H, int W, float* Y,
                       void* stream) {
   if (!ctx || !X || !Y) return fail(CCFJ_EINVAL, "null argument");

@@ -15,6 +15,8 @@ hipError_t launch_synth_fields(const float* r, const float* f, const float* mode
 hipError_t launch_synth_labels(const float* cgrid, const float* f, const float* modes, int B,
                                int H, int nf, float halfwidth, float amin, float* Y,
                                hipStream_t st);
+hipError_t launch_pool_abs_max_ranges(const float* in, int B, int N, int H, int W,
+                                      const int* ranges, float* out, hipStream_t st);
 hipError_t launch_pool_abs_max(const float* in, int B, int H, int S, int W, float* out,
                                hipStream_t st);
 hipError_t launch_label_argmax(const float* X, int B, int H, int W, float* Y, hipStream_t st);

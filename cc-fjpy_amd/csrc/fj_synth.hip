@@ -97,6 +97,25 @@ __global__ void k_pool_abs_max(const float* __restrict__ in, int H, int S, int W
   out[((size_t)b * H + h) * W + w] = m;
 }
 
+// out[b][h][w] = max over s in [lo_h, hi_h) of |in[b][s][w]|, in [B][N][W],
+// ranges[h] = (lo_h, hi_h): the coarse spectrum as the envelope of the
+// sub-velocities that fall in each cell (any number per cell).  Ranges are
+// clipped to [0, N); an empty one gives 0.
+__global__ void k_pool_abs_max_ranges(const float* __restrict__ in, int N, int H, int W,
+                                      const int* __restrict__ ranges, float* __restrict__ out) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  const int h = blockIdx.y;
+  const int b = blockIdx.z;
+  if (w >= W) return;
+  int lo = ranges[2 * h], hi = ranges[2 * h + 1];
+  lo = lo < 0 ? 0 : lo;
+  hi = hi > N ? N : hi;
+  const float* src = in + (size_t)b * N * W + w;
+  float m = 0.0f;
+  for (int s = lo; s < hi; s++) m = fmaxf(m, fabsf(src[(size_t)s * W]));
+  out[((size_t)b * H + h) * W + w] = m;
+}
+
 // Y[b][h*][w] = 1 at the velocity cell h* holding the column maximum of X[b]
 // (first on ties); other entries unchanged.
 __global__ void k_label_argmax(const float* __restrict__ X, int B, int H, int W,
@@ -160,6 +179,13 @@ hipError_t launch_synth_labels(const float* cgrid, const float* f, const float* 
                                hipStream_t st) {
   hipLaunchKernelGGL(k_synth_labels, dim3((nf + 255) / 256, H, B), dim3(256), 0, st, cgrid, f,
                      modes, H, nf, halfwidth, amin, Y);
+  return hipGetLastError();
+}
+
+hipError_t launch_pool_abs_max_ranges(const float* in, int B, int N, int H, int W,
+                                      const int* ranges, float* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_pool_abs_max_ranges, dim3((W + 255) / 256, H, B), dim3(256), 0, st, in, N,
+                     H, W, ranges, out);
   return hipGetLastError();
 }
 

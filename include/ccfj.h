@@ -287,6 +287,12 @@ int ccfj_synth_labels(ccfj_ctx *ctx, int B, int H, int nf, const float *cgrid, c
  * as the envelope of S sub-velocities per cell. */
 int ccfj_pool_abs_max(ccfj_ctx *ctx, const float *in, int B, int H, int S, int W, float *out,
                       void *stream);
+/* out[B][H][W] = max over s in [ranges[2h], ranges[2h+1]) of |in[B][s][W]|
+ * (in[B][N][W]; ranges: device int32 [H][2], clipped to [0, N), an empty
+ * range giving 0): the coarse spectrum as the envelope of the sub-velocities
+ * inside each cell, any number per cell. */
+int ccfj_pool_abs_max_ranges(ccfj_ctx *ctx, const float *in, int B, int N, int H, int W,
+                             const int *ranges, float *out, void *stream);
 /* Y[b][h*][w] = 1 where h* is the cell of the column maximum of X[b][:][w]
  * (X >= 0; first on ties); other entries of Y unchanged. */
 int ccfj_label_argmax(ccfj_ctx *ctx, const float *X, int B, int H, int W, float *Y, void *stream);

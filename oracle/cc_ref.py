@@ -65,3 +65,45 @@ def cross_correlate(data, nsta, npts, nf, fftlen, steplen, pairs, startend, fstr
     has = count > 0
     out[has] = acc[has] / count[has, None]
     return out, count
+
+
+def cross_correlate_threaded(data, nsta, npts, nf, fftlen, steplen, pairs, startend, fstride=1,
+                             onebit=False, whiten=True, nthreads=1):
+    """cross_correlate on nthreads host threads, as the reference spreads the
+    station FFTs and the pair sums over OpenMP threads (CrossCorr.cpp:86-138):
+    the window FFTs of all stations run in scipy's threaded real FFT and the
+    pairs are summed in nthreads contiguous chunks.  Same result as
+    cross_correlate up to float64 rounding."""
+    import scipy.fft
+    from concurrent.futures import ThreadPoolExecutor
+    x = np.asarray(data, np.float32).reshape(nsta, npts)
+    if onebit:
+        x = sign_ref(x)
+    fftlen, nshifts = windows(npts, fftlen, steplen)
+    p = np.asarray(pairs).reshape(-1, 2)
+    acc = np.zeros((len(p), nf), np.complex128)
+    count = np.zeros(len(p), np.int64)
+    bounds = np.linspace(0, len(p), nthreads + 1).astype(np.int64)
+
+    def pair_sum(q, spec, ok):
+        lo, hi = bounds[q], bounds[q + 1]
+        pa, pb = p[lo:hi, 0], p[lo:hi, 1]
+        both = ok[pa] & ok[pb]
+        idx = np.nonzero(both)[0]
+        acc[lo + idx] += np.conj(spec[pa[idx]]) * spec[pb[idx]]
+        count[lo + idx] += 1
+
+    with ThreadPoolExecutor(nthreads) as pool:
+        for k in range(nshifts):
+            ok = used(startend, k, steplen, fftlen)
+            seg = x[:, k * steplen:k * steplen + fftlen].astype(np.float64)
+            spec = scipy.fft.rfft(seg, n=fftlen, axis=1, workers=nthreads)[:, ::fstride][:, :nf]
+            if whiten:
+                mag = np.maximum(np.abs(spec).astype(np.float32), np.float32(1e-8))
+                spec = spec / mag
+            spec = np.ascontiguousarray(spec)
+            list(pool.map(lambda q: pair_sum(q, spec, ok), range(nthreads)))
+    out = np.zeros_like(acc)
+    has = count > 0
+    out[has] = acc[has] / count[has, None]
+    return out, count

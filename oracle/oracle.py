@@ -63,11 +63,22 @@ def _f32(a):
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
-def nthreads_default():
+def host_cores():
+    """Cores this process may run on (its affinity mask)."""
     try:
-        return max(1, min(16, len(os.sched_getaffinity(0))))
+        return max(1, len(os.sched_getaffinity(0)))
     except AttributeError:
-        return max(1, min(16, os.cpu_count() or 1))
+        return max(1, os.cpu_count() or 1)
+
+
+def nthreads_default():
+    """OpenMP threads of the restatements: OMP_NUM_THREADS when the host sets
+    it (a share of a larger machine), else every core of the affinity mask."""
+    try:
+        n = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        n = 0
+    return max(1, min(n, host_cores())) if n > 0 else host_cores()
 
 
 def port_native(U, r, f, c, itype, ybasis, nthreads=None):

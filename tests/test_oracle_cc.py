@@ -97,3 +97,24 @@ def test_sign_matches_reference_rule():
     a, _ = C.cross_correlate(y.reshape(-1), 2, 300, 10, 128, 64, [0, 1], [0, 400, 0, 400],
                              onebit=True, whiten=False)
     assert np.all(np.isfinite(a))
+
+
+def test_threaded_restatement_equals_serial():
+    """The threaded CPU baseline computes what the serial restatement does,
+    with uneven station spans, one-bit and a bin stride."""
+    rng = np.random.default_rng(4)
+    nsta, npts, fftlen, steplen, nf = 9, 700, 128, 40, 30
+    x = rng.normal(size=(nsta, npts)).astype(np.float32).reshape(-1)
+    se = np.tile(np.array([0, npts], np.int32), nsta)
+    se[2 * 2] = 150
+    se[2 * 5 + 1] = 420
+    iu = np.triu_indices(nsta, 1)
+    pairs = np.stack(iu, 1).astype(np.int32).reshape(-1)
+    for onebit in (False, True):
+        a, ca = C.cross_correlate(x, nsta, npts, nf, fftlen, steplen, pairs, se, fstride=2,
+                                  onebit=onebit)
+        for n in (1, 3, 8):
+            b, cb = C.cross_correlate_threaded(x, nsta, npts, nf, fftlen, steplen, pairs, se,
+                                               fstride=2, onebit=onebit, nthreads=n)
+            assert np.array_equal(ca, cb)
+            np.testing.assert_allclose(b, a, rtol=0, atol=1e-12)
