@@ -628,7 +628,7 @@ def bench_learned(args):
                      log=lambda s: print("bench:", s, file=sys.stderr, flush=True))
     m.ctx.synchronize()
     t_train = time.perf_counter() - t0
-    res = m.evaluate(64, seed=7, levels=(0.5, 0.3))
+    res = m.evaluate(64, seed=7, levels=(0.5, 0.3, 0.2))
     cnn = res["cnn"]
     full_ms = res["full_kernel_ms"]
     # the banded path's cost: coarse F-J + network + banded dense kernel

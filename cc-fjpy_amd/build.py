@@ -24,10 +24,16 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 instructions are kept out of generated code
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-HIP_SOURCES = ["fj_kernels.hip", "ccfj_capi.hip", "fj_plain64.hip", "fj_batched.hip",
-               "ccfj_batch.hip", "fj_mwfj.hip", "ccfj_mwfj.hip", "fj_hilbert.hip",
-               "fj_cnn.hip", "ccfj_cnn.hip", "fj_synth.hip", "fj_cc.hip", "ccfj_cc.hip",
-               "ccfj_stage.hip"]
+HIP_SOURCES = ["fj_kernels.hip", "fj_kernels_linseg.hip", "ccfj_capi.hip", "fj_plain64.hip",
+               "fj_batched.hip", "ccfj_batch.hip", "fj_mwfj.hip", "ccfj_mwfj.hip",
+               "fj_hilbert.hip", "fj_cnn.hip", "ccfj_cnn.hip", "fj_synth.hip", "fj_cc.hip",
+               "ccfj_cc.hip", "ccfj_stage.hip"]
+# per-source options on top of HIP_FLAGS: the dense linear-segment kernels
+# are compiled without machine-level loop-invariant code motion, which
+# otherwise keeps the small-argument polynomial constants live in vector
+# registers through the station loop (fj_kernels_linseg.hip)
+HIP_SOURCE_FLAGS = {"fj_kernels_linseg.hip": ["-mllvm", "-disable-machine-licm"]}
+BUILD_DIR = os.path.join(HERE, "_build")
 
 
 def _run(cmd, **kw):
@@ -35,10 +41,22 @@ def _run(cmd, **kw):
     subprocess.run(cmd, check=True, **kw)
 
 
-def build_hip():
+def build_hip(jobs=8):
+    """Each source to an object (in parallel, with its own options), then
+    one shared library."""
+    from concurrent.futures import ThreadPoolExecutor
     out = os.path.join(PKG, "libccfj.so")
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-I", INCLUDE, "-o", out, *srcs,
+    os.makedirs(BUILD_DIR, exist_ok=True)
+
+    def compile_one(src):
+        obj = os.path.join(BUILD_DIR, os.path.splitext(src)[0] + ".o")
+        _run([HIPCC, *HIP_FLAGS, *HIP_SOURCE_FLAGS.get(src, []), "-I", INCLUDE, "-c",
+              "-o", obj, os.path.join(CSRC, src)])
+        return obj
+
+    with ThreadPoolExecutor(max(1, jobs)) as pool:
+        objs = list(pool.map(compile_one, HIP_SOURCES))
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", out, *objs,
           "-L/opt/rocm/lib", "-lhipfft", "-Wl,-rpath,/opt/rocm/lib"])
     return out
 

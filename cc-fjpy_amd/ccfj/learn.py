@@ -67,6 +67,28 @@ def pooling_ranges(c_sub, c_cells):
     return np.stack([lo, hi], axis=1).astype(np.int32)
 
 
+def mode_windows(modes, f, c_dense, dc):
+    """Dense-velocity windows [lo, hi) around every mode's curve: for each
+    mode of amplitude > 0 (modes [MAX_MODES, 4] = (cinf, d, phi, amp),
+    c_m(f) = cinf + d exp(-f / phi)) and each frequency index whose c_m(f)
+    lies on [C_MIN, C_MAX], the indices of the ascending dense grid c_dense
+    within dc of c_m(f).  Returns int arrays (frequency, lo, hi)."""
+    f = np.asarray(f, np.float64)
+    c_dense = np.asarray(c_dense)
+    fi, lo, hi = [], [], []
+    for cinf, d, phi, amp in np.asarray(modes, np.float64):
+        if amp <= 0.0:
+            continue
+        cm = cinf + d * np.exp(-f / phi)
+        i = np.nonzero((cm >= C_MIN) & (cm <= C_MAX))[0]
+        fi.append(i)
+        lo.append(np.searchsorted(c_dense, cm[i] - dc, side="left"))
+        hi.append(np.searchsorted(c_dense, cm[i] + dc, side="right"))
+    if not fi:
+        return np.zeros(0, int), np.zeros(0, int), np.zeros(0, int)
+    return np.concatenate(fi), np.concatenate(lo), np.concatenate(hi)
+
+
 def sample_fields(rng, B, nr=300):
     """Host-side random parameters of B fields: r [B, nr] (sorted, metres),
     modes [B, 4, 4] (cinf, d, phi, amplitude; amplitude 0 = unused),
@@ -254,6 +276,19 @@ class Coarse2Fine:
         self.ctx.synchronize()
         return out.download()
 
+    def mode_windows(self, modes):
+        """mode_windows on this model's grids."""
+        return mode_windows(modes, self.f, self.c_dense, self.c_coarse[1] - self.c_coarse[0])
+
+    @staticmethod
+    def mode_picks(spec, windows):
+        """|spec| argmax over velocity inside each mode window (spec
+        [nc_dense, W]) -> dense velocity indices."""
+        fi, lo, hi = windows
+        a = np.abs(spec)
+        return np.array([lo[k] + int(np.argmax(a[lo[k]:hi[k], fi[k]])) for k in range(len(fi))],
+                        np.int64)
+
     def evaluate(self, n, seed, thresh=0.0, dilate=1, levels=(), use_net=True):
         """Fresh fields: coarse F-J -> bands -> banded dense F-J, against the
         full grid.  Bands come from the network (method "cnn", unless
@@ -263,7 +298,10 @@ class Coarse2Fine:
         one dense step of the full grid's), fraction of dense terms
         evaluated, and main-kernel time (HIP events); plus the full grid's
         kernel time and the wall time of the coarse F-J and of the network
-        for all n fields."""
+        for all n fields.  Mode agreement: the same one-step test for every
+        mode of the field, on the |F-J| argmax within one coarse cell of the
+        mode's curve (mode_windows), so that weaker modes, whose ridges the
+        global argmax never picks, count too."""
         rng = np.random.default_rng(seed)
         r, modes, sigma = sample_fields(rng, n, self.nr)
         U, d_r, _ = self._fields(r, modes, sigma, seed * 7919 + 1)
@@ -279,7 +317,7 @@ class Coarse2Fine:
         lg = self.logits(X, n) if use_net else None
         t3 = time.perf_counter()
         methods = (["cnn"] if use_net else []) + [f"coarse@{v:g}" for v in levels]
-        stats = {m: {"agree": [], "frac": [], "ms": 0.0} for m in methods}
+        stats = {m: {"agree": [], "frac": [], "mode": [], "ms": 0.0} for m in methods}
         full_ms = 0.0
         self.ctx.set_timing(True)
         self.ctx.get_timing()
@@ -290,6 +328,8 @@ class Coarse2Fine:
                 full = self.dense(rows, rb)
                 full_ms += self.ctx.get_timing()[0]
                 pf = np.argmax(np.abs(full), axis=0)
+                win = self.mode_windows(modes[b])
+                mf = self.mode_picks(full, win)
                 for m in methods:
                     if m == "cnn":
                         items, terms = self.propose_items(lg[b], Xh[b], thresh, dilate)
@@ -299,6 +339,8 @@ class Coarse2Fine:
                     stats[m]["ms"] += self.ctx.get_timing()[0]
                     pb = np.argmax(np.abs(band), axis=0)
                     stats[m]["agree"].append(np.mean(np.abs(pf - pb) <= 1))
+                    if len(mf):
+                        stats[m]["mode"].append(np.mean(np.abs(mf - self.mode_picks(band, win)) <= 1))
                     stats[m]["frac"].append(terms / (self.nc_dense * self.W))
         finally:
             self.ctx.set_timing(False)
@@ -308,6 +350,8 @@ class Coarse2Fine:
         for m, v in stats.items():
             out[m] = {"ridge_agreement": float(np.mean(v["agree"])),
                       "worst_field_agreement": float(np.min(v["agree"])),
+                      "mode_agreement": float(np.mean(v["mode"])) if v["mode"] else 1.0,
+                      "worst_field_mode_agreement": float(np.min(v["mode"])) if v["mode"] else 1.0,
                       "evaluated_fraction": float(np.mean(v["frac"])),
                       "kernel_ms": v["ms"]}
         return out

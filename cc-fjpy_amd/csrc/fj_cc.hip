@@ -226,6 +226,13 @@ typedef float ccx16 __attribute__((ext_vector_type(16)));
 
 // One wave per (32 x 32 station tile, bin); 4 waves (4 bins) per block.
 // acc[p][j] += G_j[a][b] for the listed pairs of the tile.
+// Workgroup numbering is XCD-major: workgroups go to the 8 XCDs in turn
+// (w mod 8), and the k-th workgroup of XCD x takes bin group
+// x + 8 (k / ntile^2) and tile k mod ntile^2, so that every tile of a bin
+// group runs on one XCD and reads the group's spectra through that XCD's
+// L2.  Numbered tile-major within each bin group (all tiles of a group on
+// consecutive workgroups, spread over the XCDs), each XCD fetched the
+// spectra again: 3.0 GB per call at the 200-station size, 8x the spectra.
 // v_mfma_f32_32x32x2_f32 operand layout (lane l): A[i = l%32][k = l/32],
 // B[k = l/32][n = l%32]; result register v holds row 8(v/4) + 4(l/32) + v%4,
 // column l%32.
@@ -235,9 +242,12 @@ __global__ __launch_bounds__(256) void k_cc_gram(const float2* __restrict__ S2, 
                                                  const int* __restrict__ pidx,
                                                  const int* __restrict__ nonfinite,
                                                  float2* __restrict__ acc) {
-  const int ta = blockIdx.x / ntile, tb = blockIdx.x - ta * ntile;
-  if (!tiles[blockIdx.x] || *nonfinite) return;   // uniform over the block
-  const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int nt2 = ntile * ntile;
+  const int k = blockIdx.x >> 3, g = (blockIdx.x & 7) + 8 * (k / nt2), t = k % nt2;
+  if (g * 4 >= nf) return;                        // uniform over the block
+  const int ta = t / ntile, tb = t - ta * ntile;
+  if (!tiles[t] || *nonfinite) return;            // uniform over the block
+  const int j = g * 4 + (threadIdx.x >> 6);
   if (j >= nf) return;                            // uniform over the wave
   const int lane = threadIdx.x & 63, i = lane & 31, kq = lane >> 5;
   const int a = ta * 32 + i, b = tb * 32 + i;
@@ -403,7 +413,10 @@ hipError_t launch_cc_pair_map(const int* pairs, int npairs, int nsta, int ntile,
 hipError_t launch_cc_gram(const void* S2, int nsta, int nf, int cw, int ntile,
                           const unsigned char* tiles, const int* pidx, const int* nonfinite,
                           void* acc, hipStream_t st) {
-  hipLaunchKernelGGL(k_cc_gram, dim3(ntile * ntile, (nf + 3) / 4), dim3(256), 0, st,
+  const int groups = (nf + 3) / 4;                 // 4 bins per workgroup
+  const long long nwg = 8LL * ((groups + 7) / 8) * ntile * ntile;
+  if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_cc_gram, dim3((unsigned)nwg), dim3(256), 0, st,
                      (const float2*)S2, nsta, nf, cw, ntile, tiles, pidx, nonfinite,
                      (float2*)acc);
   return hipGetLastError();

@@ -4,7 +4,21 @@
 #define FJ_HD
 #endif
 /* double tables: __constant__ on the device, plain on the host */
-enum { FIT_LITERAL = 0, FIT_TABLE = 1 };
+enum { FIT_LITERAL = 0, FIT_TABLE = 1, FIT_SCALAR = 2 };
+/* FJ_SCONST(v, lo, hi): the double v (bits hi:lo); on the device two
+   s_mov_b32 into a scalar register pair where the value is used (a
+   volatile asm, so it is neither hoisted nor kept live across a loop) */
+#if defined(__HIP_DEVICE_COMPILE__)
+template <unsigned LO, unsigned HI>
+__device__ inline double fj_sconst() {
+  unsigned lo, hi;
+  asm volatile("s_mov_b32 %0, %2\n\ts_mov_b32 %1, %3" : "=s"(lo), "=s"(hi) : "n"(LO), "n"(HI));
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+#define FJ_SCONST(v, lo, hi) fj_sconst<lo, hi>()
+#else
+#define FJ_SCONST(v, lo, hi) (v)
+#endif
 #if defined(__HIPCC__)
 #define FJ_TABLE(name, n, ...) static __constant__ double name##_d[n] = __VA_ARGS__; static const double name##_h[n] = __VA_ARGS__;
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -162,6 +176,21 @@ FJ_HD inline double fit_J0s(double y) {
     p = __builtin_fma(p, y, c[11]);
     return p;
   }
+  if (TAB == FIT_SCALAR) {
+    double p = FJ_SCONST(-0.005618801482900782, 0x89c9d4c1u, 0xbf7703bdu);
+    p = __builtin_fma(p, y, FJ_SCONST(0.07079209695405728, 0x4d3ba1c6u, 0x3fb21f6eu));
+    p = __builtin_fma(p, y, FJ_SCONST(-0.5037809434408522, 0x368da5c6u, 0xbfe01ef9u));
+    p = __builtin_fma(p, y, FJ_SCONST(2.625220559644334, 0xa303ac82u, 0x40050073u));
+    p = __builtin_fma(p, y, FJ_SCONST(-10.557357884793742, 0x033eb6b6u, 0xc0251d5eu));
+    p = __builtin_fma(p, y, FJ_SCONST(32.35918804582165, 0xdfb6f585u, 0x40402df9u));
+    p = __builtin_fma(p, y, FJ_SCONST(-72.81660855664654, 0x50891b90u, 0xc0523443u));
+    p = __builtin_fma(p, y, FJ_SCONST(113.77757421102329, 0xc69fa3b7u, 0x405c71c3u));
+    p = __builtin_fma(p, y, FJ_SCONST(-113.77775682787487, 0xc492fda6u, 0xc05c71c6u));
+    p = __builtin_fma(p, y, FJ_SCONST(63.99999888465921, 0xf6a4d2ebu, 0x404fffffu));
+    p = __builtin_fma(p, y, FJ_SCONST(-15.999999976722172, 0xff380b84u, 0xc02fffffu));
+    p = __builtin_fma(p, y, FJ_SCONST(0.9999999999195268, 0xfff4f09au, 0x3fefffffu));
+    return p;
+  }
   double p = -0.005618801482900782;
   p = __builtin_fma(p, y, 0.07079209695405728);
   p = __builtin_fma(p, y, -0.5037809434408522);
@@ -193,6 +222,20 @@ FJ_HD inline double fit_J1s(double y) {
     p = __builtin_fma(p, y, c[8]);
     p = __builtin_fma(p, y, c[9]);
     p = __builtin_fma(p, y, c[10]);
+    return p;
+  }
+  if (TAB == FIT_SCALAR) {
+    double p = FJ_SCONST(0.001933119992191311, 0xc9533cfau, 0x3f5fac17u);
+    p = __builtin_fma(p, y, FJ_SCONST(-0.022146255813530545, 0x11cab6f6u, 0xbf96ad82u));
+    p = __builtin_fma(p, y, FJ_SCONST(0.1417755820614683, 0xc829a5bfu, 0x3fc225b3u));
+    p = __builtin_fma(p, y, FJ_SCONST(-0.6564575115820559, 0x2eeeabccu, 0xbfe501b3u));
+    p = __builtin_fma(p, y, FJ_SCONST(2.309572547363542, 0x2bf5539bu, 0x40027a01u));
+    p = __builtin_fma(p, y, FJ_SCONST(-6.06743715646006, 0x3ef61ff1u, 0xc018450eu));
+    p = __builtin_fma(p, y, FJ_SCONST(11.377627299279547, 0x5d88f705u, 0x4026c158u));
+    p = __builtin_fma(p, y, FJ_SCONST(-14.222203592242003, 0xab53775cu, 0xc02c71c4u));
+    p = __builtin_fma(p, y, FJ_SCONST(10.66666548155003, 0x2d91424au, 0x40255555u));
+    p = __builtin_fma(p, y, FJ_SCONST(-3.9999999705566402, 0xfc0c5578u, 0xc00fffffu));
+    p = __builtin_fma(p, y, FJ_SCONST(0.4999999998789634, 0xffdebaceu, 0x3fdfffffu));
     return p;
   }
   double p = 0.001933119992191311;
@@ -228,6 +271,21 @@ FJ_HD inline double fit_Y0s(double y) {
     p = __builtin_fma(p, y, c[11]);
     return p;
   }
+  if (TAB == FIT_SCALAR) {
+    double p = FJ_SCONST(0.011007520614961696, 0x68691346u, 0x3f868b1cu);
+    p = __builtin_fma(p, y, FJ_SCONST(-0.13607675855273377, 0x95dd6626u, 0xbfc16af6u));
+    p = __builtin_fma(p, y, FJ_SCONST(0.9417493466708342, 0x869f6112u, 0x3fee22cfu));
+    p = __builtin_fma(p, y, FJ_SCONST(-4.732348709939299, 0xd1f9d40du, 0xc012edecu));
+    p = __builtin_fma(p, y, FJ_SCONST(18.20275922795894, 0x075d0bbdu, 0x403233e8u));
+    p = __builtin_fma(p, y, FJ_SCONST(-52.85781250766169, 0xccdd40dcu, 0xc04a6dccu));
+    p = __builtin_fma(p, y, FJ_SCONST(111.22092650701482, 0xa8ee9cafu, 0x405bce23u));
+    p = __builtin_fma(p, y, FJ_SCONST(-159.2993419475141, 0x35905c9cu, 0xc063e994u));
+    p = __builtin_fma(p, y, FJ_SCONST(141.1914151817568, 0x12bb334fu, 0x4061a620u));
+    p = __builtin_fma(p, y, FJ_SCONST(-65.8389707815695, 0xb28148ffu, 0xc05075b1u));
+    p = __builtin_fma(p, y, FJ_SCONST(11.366785032612231, 0x3f6f76b2u, 0x4026bbcbu));
+    p = __builtin_fma(p, y, FJ_SCONST(-0.07380429494619678, 0x99192720u, 0xbfb2e4d6u));
+    return p;
+  }
   double p = 0.011007520614961696;
   p = __builtin_fma(p, y, -0.13607675855273377);
   p = __builtin_fma(p, y, 0.9417493466708342);
@@ -259,6 +317,20 @@ FJ_HD inline double fit_Y1s(double y) {
     p = __builtin_fma(p, y, c[8]);
     p = __builtin_fma(p, y, c[9]);
     p = __builtin_fma(p, y, c[10]);
+    return p;
+  }
+  if (TAB == FIT_SCALAR) {
+    double p = FJ_SCONST(-0.0037279760680778515, 0xe95d8314u, 0xbf6e8a21u);
+    p = __builtin_fma(p, y, FJ_SCONST(0.04184985392300427, 0x13e223f7u, 0x3fa56d58u));
+    p = __builtin_fma(p, y, FJ_SCONST(-0.2599904328974209, 0xe9a44ebbu, 0xbfd0a3aeu));
+    p = __builtin_fma(p, y, FJ_SCONST(1.1572271941470782, 0xa98e74eau, 0x3ff28400u));
+    p = __builtin_fma(p, y, FJ_SCONST(-3.8770978479644107, 0xe0633168u, 0xc00f044bu));
+    p = __builtin_fma(p, y, FJ_SCONST(9.589115060130059, 0x7d39adeau, 0x40232da0u));
+    p = __builtin_fma(p, y, FJ_SCONST(-16.654007220047983, 0x04657437u, 0xc030a76du));
+    p = __builtin_fma(p, y, FJ_SCONST(18.780663548716326, 0x90faea7au, 0x4032c7d9u));
+    p = __builtin_fma(p, y, FJ_SCONST(-12.104928295564273, 0x295bbf45u, 0xc02835b9u));
+    p = __builtin_fma(p, y, FJ_SCONST(3.478315983625302, 0x54969913u, 0x400bd397u));
+    p = __builtin_fma(p, y, FJ_SCONST(-0.19605709040518532, 0x13b83742u, 0xbfc91866u));
     return p;
   }
   double p = -0.0037279760680778515;
@@ -293,6 +365,20 @@ FJ_HD inline double fit_GJs(double y) {
     p = __builtin_fma(p, y, c[10]);
     return p;
   }
+  if (TAB == FIT_SCALAR) {
+    double p = FJ_SCONST(8.887397533409992e-05, 0x45256592u, 0x3f174c3bu);
+    p = __builtin_fma(p, y, FJ_SCONST(-0.0010796865524819253, 0x9bb3aa23u, 0xbf51b088u));
+    p = __builtin_fma(p, y, FJ_SCONST(0.007517728519888622, 0xe2214af3u, 0x3f7ecae8u));
+    p = __builtin_fma(p, y, FJ_SCONST(-0.03868451112092575, 0xcc409361u, 0xbfa3ce74u));
+    p = __builtin_fma(p, y, FJ_SCONST(0.1540241037174691, 0xd4219124u, 0x3fc3b70fu));
+    p = __builtin_fma(p, y, FJ_SCONST(-0.4667509709541983, 0x76e7f843u, 0xbfdddf3fu));
+    p = __builtin_fma(p, y, FJ_SCONST(1.034337151432889, 0x1ce712abu, 0x3ff08ca5u));
+    p = __builtin_fma(p, y, FJ_SCONST(-1.5802461351138604, 0x2bdf1c60u, 0xbff948b0u));
+    p = __builtin_fma(p, y, FJ_SCONST(1.5238094742576516, 0x0b145857u, 0x3ff86186u));
+    p = __builtin_fma(p, y, FJ_SCONST(-0.7999999987682918, 0x98f050b8u, 0xbfe99999u));
+    p = __builtin_fma(p, y, FJ_SCONST(0.16666666666160115, 0x55528c6du, 0x3fc55555u));
+    return p;
+  }
   double p = 8.887397533409992e-05;
   p = __builtin_fma(p, y, -0.0010796865524819253);
   p = __builtin_fma(p, y, 0.007517728519888622);
@@ -324,6 +410,21 @@ FJ_HD inline double fit_GYs(double y) {
     p = __builtin_fma(p, y, c[9]);
     p = __builtin_fma(p, y, c[10]);
     p = __builtin_fma(p, y, c[11]);
+    return p;
+  }
+  if (TAB == FIT_SCALAR) {
+    double p = FJ_SCONST(-0.010493016794276316, 0xdfbe0b96u, 0xbf857d5cu);
+    p = __builtin_fma(p, y, FJ_SCONST(0.12932686955425834, 0x699d6375u, 0x3fc08dc8u));
+    p = __builtin_fma(p, y, FJ_SCONST(-0.8908749111050289, 0x1a00bc78u, 0xbfec820cu));
+    p = __builtin_fma(p, y, FJ_SCONST(4.447592307093985, 0xa343a3b1u, 0x4011ca55u));
+    p = __builtin_fma(p, y, FJ_SCONST(-16.958832589184595, 0x0d74e308u, 0xc030f576u));
+    p = __builtin_fma(p, y, FJ_SCONST(48.66961558750545, 0xf6ac9d26u, 0x404855b5u));
+    p = __builtin_fma(p, y, FJ_SCONST(-100.72670076329833, 0x43eb160du, 0xc0592e82u));
+    p = __builtin_fma(p, y, FJ_SCONST(140.7051499670384, 0x96a9e693u, 0x40619690u));
+    p = __builtin_fma(p, y, FJ_SCONST(-119.54298053751586, 0x3170bfb0u, 0xc05de2c0u));
+    p = __builtin_fma(p, y, FJ_SCONST(51.0414296530347, 0x911e6f30u, 0x4049854du));
+    p = __builtin_fma(p, y, FJ_SCONST(-6.446088190525072, 0x57b5bf7du, 0xc019c8cbu));
+    p = __builtin_fma(p, y, FJ_SCONST(-0.6366197725231723, 0x6ddf2ae1u, 0xbfe45f30u));
     return p;
   }
   double p = -0.010493016794276316;

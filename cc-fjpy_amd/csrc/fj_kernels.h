@@ -31,6 +31,8 @@ size_t coef_floats(int mode, int itype, int nrp, int nf);  // coef workspace
 int padded(int nr);  // station count padded to the kernels' block
 hipError_t launch_prep(int mode, int itype, const Problem& p, hipStream_t st);
 hipError_t launch_fj(int mode, int itype, const Problem& p, hipStream_t st);
+// the itype 1 part of launch_fj (fj_kernels_linseg.hip)
+hipError_t launch_fj_linseg(int mode, const Problem& p, hipStream_t st);
 // out[b][c][f] = T[b][f][c], divided by the column max |.| when m (workspace
 // [nb][nf]) is given; nb stacked problems of nf x nc.
 hipError_t launch_finish(const float* T, float* m, int nb, int nf, int nc, float* out,

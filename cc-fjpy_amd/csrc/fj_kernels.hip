@@ -37,29 +37,12 @@
 
 #include "fj_kernels.h"
 #include "fj_lane.h"
+#include "fj_main.h"
 #include "fj_prep.h"
 
 namespace fjk {
 
 using namespace fjl;
-
-// Lanes per workgroup of the dense kernels: one wave.  Measured on the
-// MI355X (19,900 pairs x 4096 f x 3000 c): trapezoid J 96.9 ms against 102.8
-// with 128 lanes and 105.2 with 256, J - Y 139.5 / 148.9 / 155.1 ms; linear
-// segments J 222.8 / 229.0 / 231.2, J - Y 396.0 / 408.5 / 412.3 ms.  A
-// workgroup of several waves holds its slots until its slowest wave ends.
-constexpr int kBlock = 64;
-
-// Hardware places workgroup w on XCD (w mod 8); tile T = (w mod 8) * per +
-// w / 8 keeps consecutive tiles (same frequency) on one XCD.  Returns false
-// for the padding workgroups of the last round.
-__device__ inline bool tile_of(int ntiles, int* tile) {
-  const int w = blockIdx.x;
-  const int per = (ntiles + 7) >> 3;
-  const int T = (w & 7) * per + (w >> 3);
-  *tile = T;
-  return T < ntiles;
-}
 
 // station arrays (fjl::station_entry), padded to ntot entries with the last
 // distance and zero gaps
@@ -83,29 +66,6 @@ __global__ __launch_bounds__(256) void k_prep(const float* __restrict__ U0,
                                               int nf, int nc, float* __restrict__ coef) {
   prep_tile<ITYPE, NSPEC, POLAR>(U0, U1, r, f, c, nr, nrp, nf, nc, coef, blockIdx.x * 64,
                                  blockIdx.y * 64);
-}
-
-template <int MODE, int ITYPE>
-__global__ __launch_bounds__(kBlock)
-__attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj(Stations S, const float* __restrict__ coef,
-                                              const float* __restrict__ fv,
-                                              const float* __restrict__ cv, int nrp, int nf,
-                                              int nc, float* __restrict__ T_out) {
-  const int ncb = (nc + kBlock - 1) / kBlock;
-  int T;
-  if (!tile_of(nf * ncb, &T)) return;
-  const int fi = T / ncb;
-  const int j = (T - fi * ncb) * kBlock + threadIdx.x;
-  // lanes past the last velocity compute a copy of it (the lane code needs
-  // every lane of the wave active) and store nothing
-  const double q = (double)fv[fi] / (double)cv[j < nc ? j : nc - 1];
-  const float* __restrict__ cf = coef + (size_t)fi * coef_block(nrp, MODE, ITYPE);
-  float v;
-  if (ITYPE == 0)
-    v = trapz_lane<MODE>(S, cf, nrp, q);
-  else
-    v = linseg_lane<MODE>(S, cf, nrp, q);
-  if (j < nc) T_out[(size_t)fi * nc + j] = v;
 }
 
 // Per-frequency scale for the normalisation (src/ccfj.pyx:225-226,
@@ -171,15 +131,9 @@ Stations stations(const Problem& p) { return Stations{p.rr, p.hh, p.ir, p.isr, p
 
 template <int MODE>
 static hipError_t launch_main(int itype, const Problem& p, hipStream_t st) {
-  const int ncb = (p.nc + kBlock - 1) / kBlock;
-  const long ntiles = (long)p.nf * ncb;
-  const long nblk = ((ntiles + 7) / 8) * 8;
-  if (itype == 0)
-    hipLaunchKernelGGL((k_fj<MODE, 0>), dim3((unsigned)nblk), dim3(kBlock), 0, st, stations(p),
-                       p.coef, p.f, p.c, p.nrp, p.nf, p.nc, p.T);
-  else
-    hipLaunchKernelGGL((k_fj<MODE, 1>), dim3((unsigned)nblk), dim3(kBlock), 0, st, stations(p),
-                       p.coef, p.f, p.c, p.nrp, p.nf, p.nc, p.T);
+  if (itype != 0) return launch_fj_linseg(MODE, p, st);   // fj_kernels_linseg.hip
+  hipLaunchKernelGGL((k_fj<MODE, 0>), dim3((unsigned)main_blocks(p.nf, p.nc)), dim3(kBlock), 0,
+                     st, stations(p), p.coef, p.f, p.c, p.nrp, p.nf, p.nc, p.T);
   return hipGetLastError();
 }
 

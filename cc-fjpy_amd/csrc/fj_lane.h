@@ -258,17 +258,18 @@ FJ_HD inline float long_segment_limit(float f, float cmin) {
 }
 
 // Occupancy floor of the F-J kernels (waves per SIMD; the compiler then
-// keeps VGPRs within 512 / waves).  Measured on the MI355X: the fused J - Y
-// linear-segment kernel (natural 217 VGPRs, 2 waves) runs 16% faster held
-// to 3 waves, a few spills on its cold small-argument path included (held
-// to 4 it spills on the hot path: 490 ms against 408 ms for 19,900 pairs x
-// 4096 f x 3000 c); the other modes are fastest at their natural count.
-// The dense and the persistent batched linear-segment J kernels are held to
-// 6 waves (80 VGPRs; the dense kernel's smooth far blocks are scheduled
-// station by station, so that they stay clear of the spills): 19,900 pairs
-// x 4096 f x 3000 c in 235 ms against 250 ms at 5 waves and 266 ms at the
-// natural 4; 4096 ragged sub-arrays in 604 ms against 631 ms at 5 and 674
-// ms at 4.  The item-list kernel (ITEMS) holds 4: the learned bands of a
+// keeps VGPRs within 512 / waves).  The dense linear-segment kernels are
+// compiled without machine loop-invariant code motion
+// (fj_kernels_linseg.hip) and need no floor there: J - Y takes 62 VGPRs
+// and 101 SGPRs (7 waves), J 43 VGPRs (8 waves), with no scratch.  With
+// that code motion on, J - Y took 217 VGPRs (2 waves) unheld and ran 16%
+// faster held to 3 (168 VGPRs, 116 bytes of spills per lane): 19,900 pairs
+// x 4096 f x 3000 c in 392 ms, against 323 ms now.  Measured without the
+// code motion, occupancy limited by LDS: J - Y 451 / 371 / 353 / 332 ms at
+// 3 / 4 / 5 / 6 waves, 323 ms at its natural 7; J 257 / 241 ms at 5 / 6,
+// 222 ms at 8.  The persistent batched linear-segment J kernel is held to
+// 6 waves (4096 ragged sub-arrays in 604 ms against 631 ms at 5 and 674 ms
+// at 4); the item-list kernel (ITEMS) holds 4: the learned bands of a
 // 300-pair field in 9.0 ms against 9.5 ms at 5 and 11.6 ms at 6.
 // -DFJ_OCC_I1_JMY=N / -DFJ_OCC_I1_J=N / -DFJ_OCC_I1_J_ITEMS=N override for
 // tuning runs.

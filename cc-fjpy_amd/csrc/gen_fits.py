@@ -33,6 +33,8 @@ cancellation for float):
 import os
 
 import mpmath as mp
+import struct
+
 import numpy as np
 
 mp.mp.dps = 40
@@ -114,12 +116,19 @@ def emit(fname, coefs, ctype, err, var, top=1.0):
     suf = "f" if ctype == "float" else ""
     head = f"/* max abs error of the fit on (0,{top:g}]: {err:.2e} */"
     if ctype == "double":
-        # two forms of one polynomial, chosen per kernel by TAB: literal
-        # coefficients (FIT_LITERAL) or a table (FIT_TABLE; on the device a
+        # three forms of one polynomial, chosen per kernel by TAB: literal
+        # coefficients (FIT_LITERAL), a table (FIT_TABLE; on the device a
         # __constant__ array whose loads the compiler hoists into scalar
-        # registers at kernel entry).  Which is faster depends on the
-        # kernel's register pressure.
+        # registers at kernel entry), or scalar moves at the point of use
+        # (FIT_SCALAR; FJ_SCONST, which the compiler cannot hoist out of the
+        # branch that evaluates the polynomial).  Which is faster depends on
+        # the kernel's register pressure.
         vals = ", ".join(f"{float(c)!r}" for c in coefs[::-1])
+
+        def sconst(c):
+            bits = struct.unpack("<Q", struct.pack("<d", float(c)))[0]
+            return f"FJ_SCONST({float(c)!r}, 0x{bits & 0xffffffff:08x}u, 0x{bits >> 32:08x}u)"
+
         lines = [head,
                  f"FJ_TABLE(k_{fname}, {len(coefs)}, {{{vals}}})",
                  "template <int TAB = FIT_LITERAL>",
@@ -129,6 +138,10 @@ def emit(fname, coefs, ctype, err, var, top=1.0):
                  "    double p = c[0];"]
         for k in range(1, len(coefs)):
             lines.append(f"    p = {fma}(p, {var}, c[{k}]);")
+        lines += ["    return p;", "  }", "  if (TAB == FIT_SCALAR) {",
+                  f"    double p = {sconst(coefs[-1])};"]
+        for c in coefs[-2::-1]:
+            lines.append(f"    p = {fma}(p, {var}, {sconst(c)});")
         lines += ["    return p;", "  }", f"  double p = {float(coefs[-1])!r};"]
         for c in coefs[-2::-1]:
             lines.append(f"  p = {fma}(p, {var}, {float(c)!r});")
@@ -157,7 +170,21 @@ def main():
            "#define FJ_HD",
            "#endif",
            "/* double tables: __constant__ on the device, plain on the host */",
-           "enum { FIT_LITERAL = 0, FIT_TABLE = 1 };",
+           "enum { FIT_LITERAL = 0, FIT_TABLE = 1, FIT_SCALAR = 2 };",
+           "/* FJ_SCONST(v, lo, hi): the double v (bits hi:lo); on the device two",
+           "   s_mov_b32 into a scalar register pair where the value is used (a",
+           "   volatile asm, so it is neither hoisted nor kept live across a loop) */",
+           "#if defined(__HIP_DEVICE_COMPILE__)",
+           "template <unsigned LO, unsigned HI>",
+           "__device__ inline double fj_sconst() {",
+           "  unsigned lo, hi;",
+           "  asm volatile(\"s_mov_b32 %0, %2\\n\\ts_mov_b32 %1, %3\" : \"=s\"(lo), \"=s\"(hi) : \"n\"(LO), \"n\"(HI));",
+           "  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);",
+           "}",
+           "#define FJ_SCONST(v, lo, hi) fj_sconst<lo, hi>()",
+           "#else",
+           "#define FJ_SCONST(v, lo, hi) (v)",
+           "#endif",
            "#if defined(__HIPCC__)",
            "#define FJ_TABLE(name, n, ...) static __constant__ double name##_d[n] = __VA_ARGS__; "
            "static const double name##_h[n] = __VA_ARGS__;",
