@@ -58,6 +58,9 @@ FP32_PEAK = 157.3                             # TFLOP/s, MI355X
 HBM_PEAK = 8000.0                             # GB/s
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "k_fj_dense_noise_array_pmc.md")
 VALU_SUMMARY = os.path.join(ROOT, "profiles", "k_fj_dense_noise_array_valu.md")
+CC_SUMMARY = os.path.join(ROOT, "profiles", "k_cc_gram_noise_day.md")
+MWFJ_SUMMARY = os.path.join(ROOT, "profiles", "k_mwfj_mfma_active_source.md")
+BATCH_SUMMARY = os.path.join(ROOT, "profiles", "k_fj_b_batched_subarrays.md")
 
 
 def calls(func):
@@ -89,7 +92,7 @@ def parse():
     return p.parse_args()
 
 
-def _summary_value(path, tag, key, conv):
+def summary_value(path, tag, key, conv):
     try:
         with open(path) as fh:
             for line in fh:
@@ -105,14 +108,14 @@ def measured_traffic(key):
     """HBM bytes per launch of the F-J kernel for variant key, from the
     committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-calibrated), or
     None when that variant was not profiled."""
-    return _summary_value(PMC_SUMMARY, "traffic_bytes_per_launch", key, int)
+    return summary_value(PMC_SUMMARY, "traffic_bytes_per_launch", key, int)
 
 
 def executed_tflops(key):
     """FP32 TFLOP/s the F-J kernel of variant key executes (2 FMA + ADD + MUL
     instructions x 64 lanes over its duration), from the committed rocprofv3
     SQ counter summary, or None."""
-    return _summary_value(VALU_SUMMARY, "executed_fp32_tflops", key, float)
+    return summary_value(VALU_SUMMARY, "executed_fp32_tflops", key, float)
 
 
 def cpu_model():
@@ -464,8 +467,14 @@ def bench_batched(args):
                     "sub_arrays": nb, "pairs_total": int(ns.sum()), "nf": nf, "nc": nc,
                     "tile": plan.tile, "term_evals_per_step": terms})
     out["roofline"] = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK,
-                       "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK, 4), "traffic": None,
-                       "kernel": "whole graph (stations, prep, persistent k_fj_b, finish)"}
+                       "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK, 4),
+                       "traffic": summary_value(BATCH_SUMMARY, "traffic_bytes_per_launch",
+                                                "k_fj_b", int),
+                       "kernel": "whole graph (stations, prep, persistent k_fj_b, finish)",
+                       "achieved_is": "algorithmic flops (92 per linear-segment term) / the "
+                                      "graph's time; traffic is the persistent kernel's",
+                       "executed_fp32_tflops": summary_value(BATCH_SUMMARY, "executed_fp32_tflops",
+                                                             "k_fj_b", float)}
     if not args.no_plain:
         print("bench: plain FP64 GPU baseline (every 64th sub-array)", file=sys.stderr, flush=True)
         pt, pms = 0.0, 0.0
@@ -567,7 +576,11 @@ def bench_mwfj(args):
                           "mfma_fp32_tflops": round(mfma_tflops, 2)}
     out["roofline"] = {"bound": "mfma", "achieved": round(mfma_tflops, 2), "peak": FP32_PEAK,
                        "unit": "TFLOP/s", "frac": round(mfma_tflops / FP32_PEAK, 4),
-                       "traffic": None, "kernel": "k_mwfj_mfma"}
+                       "traffic": summary_value(MWFJ_SUMMARY, "traffic_bytes_per_launch",
+                                                "k_mwfj_mfma", int),
+                       "kernel": "k_mwfj_mfma", "avg_ms": round(kms / max(nk / max(args.steps, 1), 1), 4),
+                       "achieved_is": "2 flops per (f, c, station, right-hand side) / HIP-event "
+                                      "time of the contraction"}
     if not args.no_plain:
         print("bench: plain FP64 GPU baseline (4 windows)", file=sys.stderr, flush=True)
         idx = np.argsort(r)
@@ -711,6 +724,8 @@ def bench_cc(args):
         out = ccfj.CC(npts, nsta, nf, fftlen, pairs, se, data, overlaprate=0.9, fstride=fstride)
     st.synchronize()
     assert np.array_equal(tre.cpu().numpy(), out.real) and np.array_equal(tim.cpu().numpy(), out.imag)
+    ctx.set_timing(True)
+    ctx.get_timing()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -718,13 +733,17 @@ def bench_cc(args):
     st.synchronize()
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / args.steps
+    gms, ng = ctx.get_timing()     # the Gram pair kernel (HIP events on the launch stream)
+    ctx.set_timing(False)
+    gms /= max(args.steps, 1)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         out = ccfj.CC(npts, nsta, nf, fftlen, pairs, se, data, overlaprate=0.9, fstride=fstride)
     wall_host = (time.perf_counter() - t0) / args.steps
-    # pair-window products actually formed (both stations hold the window)
-    from oracle import cc_ref as C
-    use = np.stack([C.used(se, k, steplen, fftlen) for k in range(nshifts)])
+    # pair-window products actually formed: window k of station i is used
+    # when k*steplen >= start_i and k*steplen + fftlen < end_i
+    t_k = np.arange(nshifts)[:, None] * steplen
+    use = (t_k >= se[0::2][None, :]) & (t_k + fftlen < se[1::2][None, :])
     pw = int(np.sum(use[:, pairs[0::2]] & use[:, pairs[1::2]]))
     res = contract("cross-correlation pair-window spectra/s (nf bins each)", pw / wall,
                    "pair-windows/s", args, wall * 1e3,
@@ -732,6 +751,18 @@ def bench_cc(args):
                                 "float32 re/im [npairs, nf] out in HBM", "stations": nsta,
                     "npts": npts, "fftlen": fftlen, "windows": nshifts, "pairs": npairs,
                     "nf": nf, "fstride": fstride, "pair_windows": pw})
+    # dominant kernel: the Gram pair stage, 8 flops (one complex multiply-
+    # add) per (pair, shared window, bin), on the FP32 matrix cores
+    gram_tflops = 8.0 * pw * nf / (gms / 1e3) / 1e12 if gms > 0 else 0.0
+    res["roofline"] = {"bound": "mfma", "achieved": round(gram_tflops, 2), "peak": FP32_PEAK,
+                       "unit": "TFLOP/s", "frac": round(gram_tflops / FP32_PEAK, 4),
+                       "traffic": summary_value(CC_SUMMARY, "traffic_bytes_per_launch",
+                                                "cc_gram", int),
+                       "kernel": "k_cc_gram", "avg_ms": round(gms, 4),
+                       "launches_per_step": ng / max(args.steps, 1),
+                       "achieved_is": "8 flops per (pair, shared window, bin) / HIP-event time",
+                       "executed_mfma_tflops": summary_value(CC_SUMMARY, "executed_mfma_tflops",
+                                                             "cc_gram", float)}
     res["end_to_end_host"] = {"value": pw / wall_host, "unit": "pair-windows/s",
                               "ms_per_step": round(wall_host * 1e3, 3),
                               "what": "CC() with numpy in and out: H2D of the records, the same "
@@ -740,6 +771,7 @@ def bench_cc(args):
     if not args.no_cpu_baseline:
         # the restatement on the whole job: every station, pair and window,
         # FFTs and pair sums on the host's threads
+        from oracle import cc_ref as C
         from oracle import oracle as O
         cores = O.nthreads_default()
         print(f"bench: CPU baseline on {cores} threads", file=sys.stderr, flush=True)

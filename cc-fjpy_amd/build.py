@@ -29,10 +29,12 @@ HIP_SOURCES = ["fj_kernels.hip", "fj_kernels_linseg.hip", "ccfj_capi.hip", "fj_p
                "fj_hilbert.hip", "fj_cnn.hip", "ccfj_cnn.hip", "fj_synth.hip", "fj_cc.hip",
                "ccfj_cc.hip", "ccfj_stage.hip"]
 # per-source options on top of HIP_FLAGS: the dense linear-segment kernels
-# are compiled without machine-level loop-invariant code motion, which
-# otherwise keeps the small-argument polynomial constants live in vector
-# registers through the station loop (fj_kernels_linseg.hip)
-HIP_SOURCE_FLAGS = {"fj_kernels_linseg.hip": ["-mllvm", "-disable-machine-licm"]}
+# and the persistent batched kernels are compiled without machine-level
+# loop-invariant code motion, which otherwise keeps the small-argument
+# polynomial constants live in vector registers through the station loop
+# and spills them (fj_kernels_linseg.hip, fj_batched.hip)
+NO_MACHINE_LICM = ["-mllvm", "-disable-machine-licm"]
+HIP_SOURCE_FLAGS = {"fj_kernels_linseg.hip": NO_MACHINE_LICM, "fj_batched.hip": NO_MACHINE_LICM}
 BUILD_DIR = os.path.join(HERE, "_build")
 
 

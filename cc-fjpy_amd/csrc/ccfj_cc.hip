@@ -137,8 +137,13 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
       if (e == hipSuccess)
         e = fjk::launch_cc_spec_t(dspec, n * nsta, nsta, nbins, zhalf, nf, bstride, whiten, duse,
                                   (int)w0, dS, dflag, st);
-      if (e == hipSuccess)
+      if (e == hipSuccess) {
+        // the pair stage's main kernel, timed when the context's timing is on
+        std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
+        if ((rc = ccfj_internal::timing_begin(ctx, st, &ev))) return rc;
         e = fjk::launch_cc_gram(dS, nsta, nf, n, ntile, dtiles, dpidx, dflag, dacc, st);
+        ccfj_internal::timing_end(ctx, st, ev);
+      }
       if (e == hipSuccess)
         e = fjk::launch_cc_pairs_t(dS, duse, pairs, npairs, nsta, nf, (int)w0, n, ddup, dflag,
                                    dacc, st);
