@@ -88,13 +88,15 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
     void *dB, *dT;
     if ((rc = ccfj_internal::ctx_scratch(ctx, 5, sizeof(float) * nf * Kp * d.nrhs, &dB))) return rc;
     const size_t tb = align256(sizeof(float) * (size_t)nwin * nf * nc);
-    if ((rc = ccfj_internal::ctx_scratch(ctx, 6, tb + sizeof(float) * (size_t)nwin * nf, &dT)))
+    const size_t mb = align256(sizeof(float) * (size_t)nwin * nf);
+    if ((rc = ccfj_internal::ctx_scratch(ctx, 6, tb + mb + sizeof(float) * (size_t)nf, &dT)))
       return rc;
     e = fjk::launch_fill((float*)dB, nf * Kp * d.nrhs, 0.0f, st);
     if (e != hipSuccess) return hip_fail(e, "fill kernel launch");
     d.B = (float*)dB;
     d.T = (float*)dT;
     d.M = (float*)((char*)dT + tb);
+    d.bsc = (float*)((char*)dT + tb + mb);
   } else {
     // per-window staging: Re and Im spectra [nr][nf], one [nc][nf] result
     const size_t a = align256(sizeof(float) * (size_t)nr * nf);

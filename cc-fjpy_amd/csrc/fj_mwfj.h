@@ -28,6 +28,8 @@ struct MwfjDev {
   float* B;            // [nf][Kp][nrhs] coefficients (itype 0)
   float* T;            // [nwin][nf][nc] |O| before normalisation (itype 0)
   float* M;            // [nwin][nf] column maxima (itype 0)
+  float* bsc;          // [nf] per-frequency scale of the split-product contraction
+                       // (itype 0; nullptr: FP32 products throughout)
   double* out;         // [nwin][nc][nf]
 };
 

@@ -292,6 +292,32 @@ def test_mwfj_many_windows_mfma_tiles(ccfj):
         assert mism == 0 and emax <= TOL, (w, emax)
 
 
+@pytest.mark.parametrize("func", [0, 1])
+def test_mwfj_zero_frequency_and_nonfinite_coefficients(ccfj, func):
+    """The contraction takes fp16 split products where a frequency's scale
+    allows and FP32 products where it does not: at f = 0 (entries sqrt(r) Z0(0),
+    -inf for the Y basis) and where a coefficient is not finite.  Every
+    column, f = 0 included, against the oracle; a NaN sample in one trace
+    makes every output NaN, as in the reference."""
+    u, r, c, f, Fs = _quake(nr=40, npts=256, seed=3)
+    f = np.concatenate([[0.0], f[:40]])
+    nwin = 3
+    winl = np.stack([r / 4500.0 + w for w in range(nwin)])
+    winr = winl + 12.0
+    got = ccfj.MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=0, func=func)
+    want = O.MWFJ(u, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=0, func=func, engine="fp64")
+    for w in range(nwin):
+        emax, _, mism = O.column_error(got[w], want[w])
+        assert mism == 0 and emax <= TOL, (w, emax)
+    bad = u.copy()
+    bad[7, 100] = np.nan
+    got = ccfj.MWFJ(bad, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=0, func=func)
+    want = O.MWFJ(bad, r, c, f, Fs, nwin, winl, winr, taper=0.9, itype=0, func=func,
+                  engine="fp64")
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    assert np.isnan(got).all()
+
+
 @pytest.mark.parametrize("itype", [0, 1])
 def test_mwfj_device_pointers_match_host_entry(ccfj, ctx, itype):
     """ccfj_mwfj_dev on device arrays gives the host entry point's result
