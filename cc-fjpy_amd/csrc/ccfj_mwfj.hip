@@ -96,7 +96,9 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
     d.B = (float*)dB;
     d.T = (float*)dT;
     d.M = (float*)((char*)dT + tb);
-    d.bsc = (float*)((char*)dT + tb + mb);
+    d.bmax = (unsigned int*)((char*)dT + tb + mb);
+    e = fjk::launch_fill((float*)d.bmax, nf, 0.0f, st);   // bits 0: max |B| = 0
+    if (e != hipSuccess) return hip_fail(e, "fill kernel launch");
   } else {
     // per-window staging: Re and Im spectra [nr][nf], one [nc][nf] result
     const size_t a = align256(sizeof(float) * (size_t)nr * nf);

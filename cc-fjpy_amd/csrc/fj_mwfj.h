@@ -28,8 +28,8 @@ struct MwfjDev {
   float* B;            // [nf][Kp][nrhs] coefficients (itype 0)
   float* T;            // [nwin][nf][nc] |O| before normalisation (itype 0)
   float* M;            // [nwin][nf] column maxima (itype 0)
-  float* bsc;          // [nf] per-frequency scale of the split-product contraction
-                       // (itype 0; nullptr: FP32 products throughout)
+  unsigned int* bmax;  // [nf] bits of max |B| per frequency (itype 0; zeroed before
+                       // the coefficient kernels, which raise it by atomicMax)
   double* out;         // [nwin][nc][nf]
 };
 

@@ -20,8 +20,10 @@
 //   k_mwfj_mfma     itype 0: the windows become right-hand sides of one
 //                   contraction per frequency, O[c][rhs] = sum_k Z(k_c r_k)
 //                   B[k][rhs]; the kernel matrix Z (J0 or Y0) is generated by
-//                   VALU and contracted with v_mfma_f32_32x32x2_f32 (full
-//                   FP32), 32 velocities x 128 right-hand sides per wave
+//                   VALU and contracted on the matrix cores, in fp16 split
+//                   products (v_mfma_f32_32x32x16_f16) or in FP32
+//                   (v_mfma_f32_32x32x2_f32), 32 velocities x 128
+//                   right-hand sides per wave
 //   finish          |Re + i Im| was formed in the MFMA epilogue; per-column
 //                   max (k_rowmax), divide and transpose to [nwin][nc][nf]
 //                   in float64 through LDS tiles (k_finish<double>)
@@ -117,7 +119,8 @@ __global__ __launch_bounds__(256) void k_mwfj_coef(const double2* __restrict__ s
                                                    const float* __restrict__ r, int nr, int w0,
                                                    int cw, int nf, int fstride, int nbins,
                                                    int zhalf, int func, int half, int nrhs,
-                                                   float* __restrict__ B) {
+                                                   float* __restrict__ B,
+                                                   unsigned int* __restrict__ bmax) {
   __shared__ float tre[kCoefT][kCoefT + 1], tim[kCoefT][kCoefT + 1];
   __shared__ double tsn[kCoefT], tcs[kCoefT];
   const int f0 = blockIdx.x * kCoefT;
@@ -169,6 +172,16 @@ __global__ __launch_bounds__(256) void k_mwfj_coef(const double2* __restrict__ s
     tim[fl][wl] = aim;
   }
   __syncthreads();
+  // max |B| of each frequency, as float bits (inf and NaN order above every
+  // finite value), for the contraction's choice of form and scale
+  if (threadIdx.x < kCoefT && f0 + (int)threadIdx.x < nf) {
+    unsigned int m = 0u;
+    for (int wl = 0; wl < kCoefT; wl++) {
+      m = max(m, __float_as_uint(fabsf(tre[threadIdx.x][wl])));
+      m = max(m, __float_as_uint(fabsf(tim[threadIdx.x][wl])));
+    }
+    if (m) atomicMax(&bmax[f0 + threadIdx.x], m);
+  }
   const int Kp = func ? 2 * half : half;
   for (int it = threadIdx.x; it < kCoefT * 2 * kCoefT; it += 256) {
     const int fl = it / (2 * kCoefT), e = it & (2 * kCoefT - 1);
@@ -223,13 +236,45 @@ __device__ inline float mwfj_value(double rs, float ir, const LaneK& L) {
 }
 
 // O[c][rhs] = sum_k Z(k_c r_k) B[k][rhs] for one frequency, 128 velocities
-// (4 waves x 32) and 128 right-hand sides (4 MFMA tiles) per workgroup.
-// v_mfma_f32_32x32x2_f32 operand layout (lane l): A[i = l%32][k = l/32],
-// B[k = l/32][n = l%32]; result register v holds row 8(v/4) + 4(l/32) + v%4,
-// column l%32.
+// (4 waves x 32) and 128 right-hand sides (4 MFMA tiles of 32) per workgroup,
+// K in chunks of 32 stations double-buffered in LDS: chunk c+1 is fetched
+// into registers while chunk c is contracted (one barrier per chunk).  Each
+// wave first computes its lane's 16 kernel-matrix values of the chunk as
+// independent chains, then issues the chunk's MFMAs back to back, so one
+// wave's Bessel work overlaps another's matrix work.
+//
+// Two forms, chosen per workgroup (one frequency) from bmax[f], the bits of
+// max |B| over the frequency's coefficients (atomicMax in k_mwfj_coef; the
+// bits of inf and NaN order above every finite value):
+//   split (f > 0 and every coefficient finite): each FP32 operand held as
+//     two fp16 numbers, v = v_hi + v_lo (v_hi = v rounded to fp16, v_lo the
+//     rounded remainder; 22 significand bits together), each product as
+//     three fp16 products a_hi b_hi + a_hi b_lo + a_lo b_hi accumulated in
+//     FP32 by v_mfma_f32_32x32x16_f16; the dropped a_lo b_lo and the
+//     rounding of the low parts leave about 2^-21 of each product.  The
+//     entries sqrt(r) Z0(k r) are at most about 2.5 in size for k > 0; B is
+//     scaled by the power of two 2^(13 - floor(log2 max |B|)) (exact), so
+//     that |B| < 2^14.  A 32x32x16 fp16 MFMA takes 16 stations in 32 cycles
+//     against 2 in 64 for the FP32 form: the three products need 5.3x less
+//     matrix time.
+//   FP32 (f = 0, where the entries are sqrt(r) Z0(0), unbounded in r and
+//     -inf for Y; or a non-finite coefficient, whose NaN pattern this form
+//     keeps): v_mfma_f32_32x32x2_f32 on the FP32 values.
+// Operand layouts (lane l): FP32 A[i = l%32][k = l/32], B[k = l/32][n =
+// l%32]; fp16 A[i = l%32][k = 8(l/32) + q], B[k = 8(l/32) + q][n = l%32],
+// q = 0..7 (A and B share the k mapping, which is all the product needs).
+// Both write the result layout of every 32x32 MFMA: register v holds row
+// 8(v/4) + 4(l/32) + v%4, column l%32.
 constexpr int kMwC = 128;   // velocities per workgroup
 constexpr int kMwR = 128;   // right-hand sides per workgroup
 constexpr int kMwK = 32;    // K rows per LDS chunk
+constexpr int kHP = 40;     // row pitch of the split form's transposed B planes, in
+                            // halves (80 bytes: the 8 lanes of a 16-byte read phase
+                            // hit distinct banks)
+// static LDS of a workgroup: the FP32 form's [2][kMwK][kMwR + 4] floats
+// (33,792 B) or the split form's [2][2][kMwR][kHP] halves (40,960 B)
+constexpr int kMwSmem = 2 * 2 * kMwR * kHP * 2;
+static_assert(2 * kMwK * (kMwR + 4) * 4 <= kMwSmem, "LDS of the FP32 form");
 
 // waves per SIMD the contraction kernel is held to at least (a tuning
 // switch: -DFJ_MWFJ_WAVES=N)
@@ -242,23 +287,74 @@ constexpr int kMwK = 32;    // K rows per LDS chunk
 // there instead of waiting on a global load per entry; larger arrays read
 // them from global memory.
 constexpr int kMwLdsSt = 1536;
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+
+// The lane's 16 kernel-matrix values of a chunk at stations s(h), h = 0..15.
+// When x >= kFarX at every one of them on every lane, they are 16
+// independent straight-line chains; otherwise each station takes its own
+// wave-uniform branch.  Both give an entry the same value.
+template <bool Y, typename SF>
+__device__ inline void mwfj_chunk_values(const double* __restrict__ rrs,
+                                         const float* __restrict__ irs, const LaneK& L,
+                                         SF station, float* av) {
+  float irm = 0.0f;
+#pragma unroll
+  for (int h = 0; h < 16; h++) irm = fmaxf(irm, irs[station(h)]);
+  if (!wave_any(!(L.zk * irm <= 8.0f / kFarX))) {
+#pragma unroll
+    for (int h = 0; h < 16; h++) av[h] = mwfj_value<Y, true>(rrs[station(h)], irs[station(h)], L);
+  } else {
+#pragma unroll
+    for (int h = 0; h < 16; h++) {
+      const int s = station(h);
+      const float ir = irs[s];
+      const bool far = !wave_any(L.zk * ir > 8.0f / kFarX);
+      av[h] = far ? mwfj_value<Y, true>(rrs[s], ir, L) : mwfj_value<Y, false>(rrs[s], ir, L);
+    }
+  }
+}
+
+// |Re + i Im| per window (Re in even columns, Im in odd) times the amplitude
+// sqrt(2/(pi k)) of the row's velocity (1 at k = 0) and `unscale`
+__device__ inline void mwfj_epilogue(const floatx16* acc, const float* __restrict__ fv,
+                                     const float* __restrict__ cv, int fi, int rg, int wave,
+                                     int i, int kq, int nf, int nc, int nwin, float unscale,
+                                     float* __restrict__ T) {
+  float amp[16];
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    const int m = 8 * (v >> 2) + 4 * kq + (v & 3);
+    const int c = blockIdx.x * kMwC + wave * 32 + m;
+    const double k = (double)fv[fi] / (double)cv[c < nc ? c : nc - 1] * kTwoPi;
+    amp[v] = ((float)k > 0.0f ? (float)sqrt(2.0 / (kPiD * k)) : 1.0f) * unscale;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+      const float mine = acc[t][v];
+      const float other = __shfl_xor(mine, 1);
+      const int m = 8 * (v >> 2) + 4 * kq + (v & 3);
+      const int col = rg * kMwR + t * 32 + i;
+      const int c = blockIdx.x * kMwC + wave * 32 + m;
+      if ((i & 1) == 0 && c < nc && col < 2 * nwin) {
+        const int w = col >> 1;
+        T[((size_t)w * nf + fi) * nc + c] = sqrtf(mine * mine + other * other) * amp[v];
+      }
+    }
+  }
+}
+
 template <int FUNC, bool LDS_ST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_MWFJ_WAVES, 8))) void k_mwfj_mfma(const float* __restrict__ B,
-                                                   const double* __restrict__ rr,
-                                                   const float* __restrict__ irr,
-                                                   const float* __restrict__ fv,
-                                                   const float* __restrict__ cv, int nr,
-                                                   int half, int nf, int nc, int nrhs,
-                                                   int nwin, const float* __restrict__ bsc,
-                                                   float* __restrict__ T) {
-  // B chunks (kMwK rows x kMwR right-hand sides) double-buffered in LDS:
-  // chunk c+1 is fetched into registers while chunk c is contracted
-  __shared__ float Bs[2][kMwK][kMwR + 4];
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_MWFJ_WAVES, 8))) void k_mwfj_mfma(
+    const float* __restrict__ B, const unsigned int* __restrict__ bmax,
+    const double* __restrict__ rr, const float* __restrict__ irr, const float* __restrict__ fv,
+    const float* __restrict__ cv, int nr, int half, int nf, int nc, int nrhs, int nwin,
+    float* __restrict__ T) {
+  __shared__ __attribute__((aligned(16))) unsigned char smem[kMwSmem];
   extern __shared__ double st_lds[];   // [half] r, then [half] 1/r (LDS_ST)
-  constexpr int kPer = kMwK * kMwR / 256;   // B elements per thread and chunk
   const int fi = blockIdx.y;
-  // frequencies with a scale are done by the split-product kernel
-  if (bsc && bsc[fi] != 0.0f) return;   // uniform over the block
   const int rg = blockIdx.z;   // right-hand-side group
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -269,29 +365,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_MWFJ_WAV
   const int Kp = FUNC ? 2 * half : half;   // a multiple of kMwK
   const float* Bf = B + (size_t)fi * Kp * nrhs + (size_t)rg * kMwR;
   const int rcols = min(kMwR, nrhs - rg * kMwR);   // multiple of 32
-  floatx16 acc[4];
-#pragma unroll
-  for (int t = 0; t < 4; t++)
-#pragma unroll
-    for (int v = 0; v < 16; v++) acc[t][v] = 0.0f;
-
-  float nb[kPer];
-  auto fetch = [&](int k0) {
-#pragma unroll
-    for (int u = 0; u < kPer; u++) {
-      const int e = threadIdx.x + u * 256;
-      const int kr = e / kMwR, cc = e - kr * kMwR;
-      nb[u] = cc < rcols ? Bf[(size_t)(k0 + kr) * nrhs + cc] : 0.0f;
-    }
-  };
-  auto stash = [&](int buf) {
-#pragma unroll
-    for (int u = 0; u < kPer; u++) {
-      const int e = threadIdx.x + u * 256;
-      const int kr = e / kMwR, cc = e - kr * kMwR;
-      Bs[buf][kr][cc] = nb[u];
-    }
-  };
   if (LDS_ST) {
     float* irl = reinterpret_cast<float*>(st_lds + half);
     for (int q = threadIdx.x; q < half; q += 256) {
@@ -299,289 +372,132 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_MWFJ_WAV
       irl[q] = irr[q];
     }
   }
-  fetch(0);
-  stash(0);
-  __syncthreads();
-  for (int k0 = 0, buf = 0; k0 < Kp; k0 += kMwK, buf ^= 1) {
-    const bool more = k0 + kMwK < Kp;
-    if (more) fetch(k0 + kMwK);
-    const bool ypart = FUNC && k0 >= half;
-    const int sbase = ypart ? k0 - half : k0;
-    // VALU phase: the chunk's 16 kernel-matrix values of this lane, as
-    // independent chains; then the MFMA phase back to back.  Waves of the
-    // block alternate between the two, so one wave's Bessel work overlaps
-    // another's matrix work.
-    float av[kMwK / 2];
+  const double* rrs = LDS_ST ? st_lds : rr;
+  const float* irs = LDS_ST ? reinterpret_cast<const float*>(st_lds + half) : irr;
+  floatx16 acc[4];
 #pragma unroll
-    for (int h = 0; h < kMwK / 2; h++) {
-      const int s = sbase + 2 * h + kq;   // < half: padded station arrays
-      const double rs = LDS_ST ? st_lds[s] : rr[s];
-      const float ir = LDS_ST ? reinterpret_cast<const float*>(st_lds + half)[s] : irr[s];
-      // far when x >= kFarX on every lane (a wave-uniform branch)
-      const bool far = !wave_any(L.zk * ir > 8.0f / kFarX);
+  for (int t = 0; t < 4; t++)
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc[t][v] = 0.0f;
+  const unsigned int mb = bmax[fi];
+  const bool split = mb < 0x7f800000u && fv[fi] > 0.0f;   // uniform over the block
+
+  if (split) {
+    auto Bs = reinterpret_cast<_Float16 (*)[2][kMwR][kHP]>(smem);   // [buf][hi/lo][rhs][k]
+    const float sc = mb == 0u ? 1.0f : ldexpf(1.0f, 13 - ilogbf(__uint_as_float(mb)));
+    // staging: thread -> column cc of the chunk, rows 16 kg .. 16 kg + 15
+    const int cc = threadIdx.x & (kMwR - 1), kg = threadIdx.x >> 7;
+    float nb[16];
+    auto fetch = [&](int k0) {
+#pragma unroll
+      for (int j = 0; j < 16; j++)
+        nb[j] = cc < rcols ? Bf[(size_t)(k0 + 16 * kg + j) * nrhs + cc] : 0.0f;
+    };
+    auto stash = [&](int buf) {
+#pragma unroll
+      for (int j0 = 0; j0 < 16; j0 += 8) {
+        half8 hi, lo;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const float v = nb[j0 + q] * sc;
+          const _Float16 h = (_Float16)v;
+          hi[q] = h;
+          lo[q] = (_Float16)(v - (float)h);
+        }
+        *reinterpret_cast<half8*>(&Bs[buf][0][cc][16 * kg + j0]) = hi;
+        *reinterpret_cast<half8*>(&Bs[buf][1][cc][16 * kg + j0]) = lo;
+      }
+    };
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    for (int k0 = 0, buf = 0; k0 < Kp; k0 += kMwK, buf ^= 1) {
+      const bool more = k0 + kMwK < Kp;
+      if (more) fetch(k0 + kMwK);
+      const bool ypart = FUNC && k0 >= half;
+      const int sbase = ypart ? k0 - half : k0;
+      // value h = 8p + q at station 16p + 8kq + q (K-half p); < half: padded arrays
+      auto station = [&](int h) { return sbase + 16 * (h >> 3) + 8 * kq + (h & 7); };
+      float av[16];
       if (ypart)
-        av[h] = far ? mwfj_value<true, true>(rs, ir, L) : mwfj_value<true, false>(rs, ir, L);
+        mwfj_chunk_values<true>(rrs, irs, L, station, av);
       else
-        av[h] = far ? mwfj_value<false, true>(rs, ir, L) : mwfj_value<false, false>(rs, ir, L);
-    }
+        mwfj_chunk_values<false>(rrs, irs, L, station, av);
+      half8 ah[2], al[2];
 #pragma unroll
-    for (int h = 0; h < kMwK / 2; h++) {
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        const float b = Bs[buf][2 * h + kq][t * 32 + i];
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[h], b, acc[t], 0, 0, 0);
+      for (int h = 0; h < 16; h++) {
+        const _Float16 x = (_Float16)av[h];
+        ah[h >> 3][h & 7] = x;
+        al[h >> 3][h & 7] = (_Float16)(av[h] - (float)x);
       }
-    }
-    // the other buffer was last read before the previous barrier
-    if (more) stash(buf ^ 1);
-    __syncthreads();
-  }
-  // epilogue: |Re + i Im| per window (Re in even columns, Im in odd) times
-  // the amplitude sqrt(2/(pi k)) of the row's velocity (1 at k = 0)
-  float amp[16];
-#pragma unroll
-  for (int v = 0; v < 16; v++) {
-    const int m = 8 * (v >> 2) + 4 * kq + (v & 3);
-    const int c = blockIdx.x * kMwC + wave * 32 + m;
-    const double k = (double)fv[fi] / (double)cv[c < nc ? c : nc - 1] * kTwoPi;
-    amp[v] = (float)k > 0.0f ? (float)sqrt(2.0 / (kPiD * k)) : 1.0f;
-  }
-#pragma unroll
-  for (int t = 0; t < 4; t++) {
-#pragma unroll
-    for (int v = 0; v < 16; v++) {
-      const float mine = acc[t][v];
-      const float other = __shfl_xor(mine, 1);
-      const int m = 8 * (v >> 2) + 4 * kq + (v & 3);
-      const int col = rg * kMwR + t * 32 + i;
-      const int c = blockIdx.x * kMwC + wave * 32 + m;
-      if ((i & 1) == 0 && c < nc && col < 2 * nwin) {
-        const int w = col >> 1;
-        T[((size_t)w * nf + fi) * nc + c] = sqrtf(mine * mine + other * other) * amp[v];
-      }
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Split-product contraction on the fp16 matrix cores.  Each FP32 operand is
-// held as two fp16 numbers, v = v_hi + v_lo (v_hi = v rounded to fp16, v_lo =
-// the rounded remainder, 22 significand bits together), and each product as
-// three fp16 products, a_hi b_hi + a_hi b_lo + a_lo b_hi, accumulated in FP32
-// by v_mfma_f32_32x32x16_f16: the dropped a_lo b_lo and the rounding of the
-// low parts leave about 2^-21 of each product.  A 32x32x16 fp16 MFMA issues
-// 16 stations in 32 cycles against 2 in 64 for v_mfma_f32_32x32x2_f32, so
-// the three products take 5.3x less matrix time per station than one FP32
-// product.  The kernel-matrix entries sqrt(r) Z0(k r) (as mwfj_value) are at
-// most about 2.5 in size for k > 0; the coefficients B are scaled per
-// frequency by a power of two, bsc[f] (exact), so that max |B| < 2^14.
-// k_mwfj_bscale sets bsc[f] = 0 where a frequency must stay in FP32: f not
-// positive (the entries there are sqrt(r) Z0(0), unbounded, or -inf for Y)
-// or a non-finite coefficient (whose NaN pattern the FP32 kernel keeps).
-__global__ __launch_bounds__(256) void k_mwfj_bscale(const float* __restrict__ B, int Kp, int nrhs,
-                                                     const float* __restrict__ fv,
-                                                     float* __restrict__ bsc) {
-  __shared__ float red_m[256];
-  __shared__ int red_bad[256];
-  const int fi = blockIdx.x;
-  const float* Bf = B + (size_t)fi * Kp * nrhs;
-  const size_t n = (size_t)Kp * nrhs;
-  float m = 0.0f;
-  int bad = 0;
-  for (size_t k = threadIdx.x; k < n; k += 256) {
-    const float a = fabsf(Bf[k]);
-    if (a <= 3.402823466e38f)
-      m = fmaxf(m, a);
-    else
-      bad = 1;   // inf or NaN
-  }
-  red_m[threadIdx.x] = m;
-  red_bad[threadIdx.x] = bad;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) {
-      red_m[threadIdx.x] = fmaxf(red_m[threadIdx.x], red_m[threadIdx.x + o]);
-      red_bad[threadIdx.x] |= red_bad[threadIdx.x + o];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const float mx = red_m[0];
-    float sc;
-    if (red_bad[0] || !(fv[fi] > 0.0f))
-      sc = 0.0f;
-    else if (mx == 0.0f)
-      sc = 1.0f;
-    else   // mx < 2^(e+1): mx * 2^(13-e) < 2^14
-      sc = ldexpf(1.0f, 13 - ilogbf(mx));
-    bsc[fi] = sc;
-  }
-}
-
-typedef _Float16 half8 __attribute__((ext_vector_type(8)));
-constexpr int kHP = 40;   // LDS row pitch of the transposed B planes, in halves (80
-                          // bytes: the 8 lanes of a 16-byte read phase hit distinct banks)
-
-// Same tiling as k_mwfj_mfma (workgroup: one frequency, 128 velocities as 4
-// waves of 32, 128 right-hand sides as 4 tiles of 32), 32 stations per chunk
-// as two K-halves of 16.  Operand layout of v_mfma_f32_32x32x16_f16 (lane
-// l): A[i = l%32][k = 8(l/32) + q], B[k = 8(l/32) + q][n = l%32], q = 0..7;
-// A and B share the k mapping, which is all the product needs.  The result
-// layout is that of every 32x32 MFMA (row 8(v/4) + 4(l/32) + v%4, column
-// l%32).  B chunks are staged transposed, Bs[buf][hi/lo][rhs][k], so that a
-// lane's 8 consecutive k of one column are one 16-byte LDS read.
-template <int FUNC, bool LDS_ST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FJ_MWFJ_WAVES, 8))) void k_mwfj_h3(
-    const float* __restrict__ B, const float* __restrict__ bsc, const double* __restrict__ rr,
-    const float* __restrict__ irr, const float* __restrict__ fv, const float* __restrict__ cv,
-    int nr, int half, int nf, int nc, int nrhs, int nwin, float* __restrict__ T) {
-  __shared__ __attribute__((aligned(16))) _Float16 Bs[2][2][kMwR][kHP];
-  extern __shared__ double st_lds[];   // [half] r, then [half] 1/r (LDS_ST)
-  const int fi = blockIdx.y;
-  const float sc = bsc[fi];
-  if (sc == 0.0f) return;   // left to the FP32 kernel; uniform over the block
-  const int rg = blockIdx.z;
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int i = lane & 31, kq = lane >> 5;
-  const int crow = blockIdx.x * kMwC + wave * 32 + i;
-  const int cidx = crow < nc ? crow : nc - 1;
-  const LaneK L = lane_consts((double)fv[fi] / (double)cv[cidx]);
-  const int Kp = FUNC ? 2 * half : half;   // a multiple of kMwK
-  const float* Bf = B + (size_t)fi * Kp * nrhs + (size_t)rg * kMwR;
-  const int rcols = min(kMwR, nrhs - rg * kMwR);   // multiple of 32
-  floatx16 acc[4];
-#pragma unroll
-  for (int t = 0; t < 4; t++)
-#pragma unroll
-    for (int v = 0; v < 16; v++) acc[t][v] = 0.0f;
-
-  // staging: thread -> column cc of the chunk, rows 16 kg .. 16 kg + 15
-  const int cc = threadIdx.x & (kMwR - 1), kg = threadIdx.x >> 7;
-  float nb[16];
-  auto fetch = [&](int k0) {
-#pragma unroll
-    for (int j = 0; j < 16; j++)
-      nb[j] = cc < rcols ? Bf[(size_t)(k0 + 16 * kg + j) * nrhs + cc] : 0.0f;
-  };
-  auto stash = [&](int buf) {
-#pragma unroll
-    for (int j0 = 0; j0 < 16; j0 += 8) {
-      half8 hi, lo;
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const float v = nb[j0 + q] * sc;
-        const _Float16 h = (_Float16)v;
-        hi[q] = h;
-        lo[q] = (_Float16)(v - (float)h);
-      }
-      *reinterpret_cast<half8*>(&Bs[buf][0][cc][16 * kg + j0]) = hi;
-      *reinterpret_cast<half8*>(&Bs[buf][1][cc][16 * kg + j0]) = lo;
-    }
-  };
-  if (LDS_ST) {
-    float* irl = reinterpret_cast<float*>(st_lds + half);
-    for (int q = threadIdx.x; q < half; q += 256) {
-      st_lds[q] = rr[q];
-      irl[q] = irr[q];
-    }
-  }
-  fetch(0);
-  stash(0);
-  __syncthreads();
-  for (int k0 = 0, buf = 0; k0 < Kp; k0 += kMwK, buf ^= 1) {
-    const bool more = k0 + kMwK < Kp;
-    if (more) fetch(k0 + kMwK);
-    const bool ypart = FUNC && k0 >= half;
-    const int sbase = ypart ? k0 - half : k0;
-    // the lane's 16 kernel-matrix values: K-half p, stations 16p + 8kq + q.
-    // When x >= kFarX for every station of the chunk on every lane (the two
-    // lane halves cover its 32 stations), the 16 values are 16 independent
-    // straight-line chains; otherwise each station takes its own
-    // wave-uniform branch.  Both give an entry the same value.
-    const float* irl = LDS_ST ? reinterpret_cast<const float*>(st_lds + half) : irr;
-    float irm = 0.0f;
-#pragma unroll
-    for (int p = 0; p < 2; p++)
-#pragma unroll
-      for (int q = 0; q < 8; q++) irm = fmaxf(irm, irl[sbase + 16 * p + 8 * kq + q]);
-    const bool chunk_far = !wave_any(!(L.zk * irm <= 8.0f / kFarX));
-    float av[16];
-    if (chunk_far) {
+      // two tiles at a time, so that consecutive MFMAs use different accumulators
 #pragma unroll
       for (int p = 0; p < 2; p++)
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const int s = sbase + 16 * p + 8 * kq + q;
-          const double rs = LDS_ST ? st_lds[s] : rr[s];
-          av[8 * p + q] = ypart ? mwfj_value<true, true>(rs, irl[s], L)
-                                : mwfj_value<false, true>(rs, irl[s], L);
+        for (int t = 0; t < 4; t += 2) {
+          const half8 bh0 = *reinterpret_cast<const half8*>(&Bs[buf][0][t * 32 + i][16 * p + 8 * kq]);
+          const half8 bh1 = *reinterpret_cast<const half8*>(&Bs[buf][0][t * 32 + 32 + i][16 * p + 8 * kq]);
+          const half8 bl0 = *reinterpret_cast<const half8*>(&Bs[buf][1][t * 32 + i][16 * p + 8 * kq]);
+          const half8 bl1 = *reinterpret_cast<const half8*>(&Bs[buf][1][t * 32 + 32 + i][16 * p + 8 * kq]);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[p], bh0, acc[t], 0, 0, 0);
+          acc[t + 1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[p], bh1, acc[t + 1], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[p], bh0, acc[t], 0, 0, 0);
+          acc[t + 1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[p], bh1, acc[t + 1], 0, 0, 0);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[p], bl0, acc[t], 0, 0, 0);
+          acc[t + 1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[p], bl1, acc[t + 1], 0, 0, 0);
         }
-    } else {
-#pragma unroll
-      for (int p = 0; p < 2; p++)
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const int s = sbase + 16 * p + 8 * kq + q;   // < half: padded station arrays
-          const double rs = LDS_ST ? st_lds[s] : rr[s];
-          const float ir = irl[s];
-          const bool far = !wave_any(L.zk * ir > 8.0f / kFarX);
-          float a;
-          if (ypart)
-            a = far ? mwfj_value<true, true>(rs, ir, L) : mwfj_value<true, false>(rs, ir, L);
-          else
-            a = far ? mwfj_value<false, true>(rs, ir, L) : mwfj_value<false, false>(rs, ir, L);
-          av[8 * p + q] = a;
-        }
+      // the other buffer was last read before the previous barrier
+      if (more) stash(buf ^ 1);
+      __syncthreads();
     }
-    half8 ah[2], al[2];
+    mwfj_epilogue(acc, fv, cv, fi, rg, wave, i, kq, nf, nc, nwin, 1.0f / sc, T);
+  } else {
+    auto Bs = reinterpret_cast<float (*)[kMwK][kMwR + 4]>(smem);
+    constexpr int kPer = kMwK * kMwR / 256;   // B elements per thread and chunk
+    float nb[kPer];
+    auto fetch = [&](int k0) {
 #pragma unroll
-    for (int p = 0; p < 2; p++)
-#pragma unroll
-      for (int q = 0; q < 8; q++) {
-        const float v = av[8 * p + q];
-        const _Float16 h = (_Float16)v;
-        ah[p][q] = h;
-        al[p][q] = (_Float16)(v - (float)h);
+      for (int u = 0; u < kPer; u++) {
+        const int e = threadIdx.x + u * 256;
+        const int kr = e / kMwR, cc = e - kr * kMwR;
+        nb[u] = cc < rcols ? Bf[(size_t)(k0 + kr) * nrhs + cc] : 0.0f;
       }
+    };
+    auto stash = [&](int buf) {
 #pragma unroll
-    for (int p = 0; p < 2; p++)
-#pragma unroll
-      for (int t = 0; t < 4; t++) {
-        const half8 bh = *reinterpret_cast<const half8*>(&Bs[buf][0][t * 32 + i][16 * p + 8 * kq]);
-        const half8 bl = *reinterpret_cast<const half8*>(&Bs[buf][1][t * 32 + i][16 * p + 8 * kq]);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[p], bh, acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah[p], bl, acc[t], 0, 0, 0);
-        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al[p], bh, acc[t], 0, 0, 0);
+      for (int u = 0; u < kPer; u++) {
+        const int e = threadIdx.x + u * 256;
+        const int kr = e / kMwR, cc = e - kr * kMwR;
+        Bs[buf][kr][cc] = nb[u];
       }
-    // the other buffer was last read before the previous barrier
-    if (more) stash(buf ^ 1);
+    };
+    fetch(0);
+    stash(0);
     __syncthreads();
-  }
-  // epilogue: |Re + i Im| per window times sqrt(2/(pi k)) / bsc
-  const float unscale = 1.0f / sc;   // a power of two: exact
-  float amp[16];
+    for (int k0 = 0, buf = 0; k0 < Kp; k0 += kMwK, buf ^= 1) {
+      const bool more = k0 + kMwK < Kp;
+      if (more) fetch(k0 + kMwK);
+      const bool ypart = FUNC && k0 >= half;
+      const int sbase = ypart ? k0 - half : k0;
+      // value h at station 2h + kq (k = l/32 of each 32x32x2 MFMA)
+      auto station = [&](int h) { return sbase + 2 * h + kq; };
+      float av[16];
+      if (ypart)
+        mwfj_chunk_values<true>(rrs, irs, L, station, av);
+      else
+        mwfj_chunk_values<false>(rrs, irs, L, station, av);
 #pragma unroll
-  for (int v = 0; v < 16; v++) {
-    const int m = 8 * (v >> 2) + 4 * kq + (v & 3);
-    const int c = blockIdx.x * kMwC + wave * 32 + m;
-    const double k = (double)fv[fi] / (double)cv[c < nc ? c : nc - 1] * kTwoPi;
-    amp[v] = (float)sqrt(2.0 / (kPiD * k)) * unscale;
-  }
+      for (int h = 0; h < kMwK / 2; h++) {
 #pragma unroll
-  for (int t = 0; t < 4; t++) {
-#pragma unroll
-    for (int v = 0; v < 16; v++) {
-      const float mine = acc[t][v];
-      const float other = __shfl_xor(mine, 1);
-      const int m = 8 * (v >> 2) + 4 * kq + (v & 3);
-      const int col = rg * kMwR + t * 32 + i;
-      const int c = blockIdx.x * kMwC + wave * 32 + m;
-      if ((i & 1) == 0 && c < nc && col < 2 * nwin) {
-        const int w = col >> 1;
-        T[((size_t)w * nf + fi) * nc + c] = sqrtf(mine * mine + other * other) * amp[v];
+        for (int t = 0; t < 4; t++) {
+          const float b = Bs[buf][2 * h + kq][t * 32 + i];
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x2f32(av[h], b, acc[t], 0, 0, 0);
+        }
       }
+      if (more) stash(buf ^ 1);
+      __syncthreads();
     }
+    mwfj_epilogue(acc, fv, cv, fi, rg, wave, i, kq, nf, nc, nwin, 1.0f, T);
   }
 }
 
@@ -627,7 +543,7 @@ hipError_t launch_mwfj_stations(const MwfjDev& d, double* rr, float* ir, hipStre
 hipError_t launch_mwfj_coef(const MwfjDev& d, int w0, int cw, hipStream_t st) {
   dim3 grid((d.nf + kCoefT - 1) / kCoefT, d.nr, (cw + kCoefT - 1) / kCoefT);
   hipLaunchKernelGGL(k_mwfj_coef, grid, dim3(256), 0, st, (const double2*)d.spec, d.r, d.nr, w0,
-                     cw, d.nf, d.fstride, d.nbins, d.zhalf, d.func, d.half, d.nrhs, d.B);
+                     cw, d.nf, d.fstride, d.nbins, d.zhalf, d.func, d.half, d.nrhs, d.B, d.bmax);
   return hipGetLastError();
 }
 
@@ -635,21 +551,10 @@ hipError_t launch_mwfj_mfma(const MwfjDev& d, hipStream_t st) {
   dim3 grid((d.nc + kMwC - 1) / kMwC, d.nf, (d.nrhs + kMwR - 1) / kMwR);
   const bool lds = d.half <= kMwLdsSt;
   const size_t shm = lds ? (sizeof(double) + sizeof(float)) * (size_t)d.half : 0;
-  if (d.bsc) {   // split products where the scale allows, FP32 for the rest
-    const int Kp = (d.func ? 2 : 1) * d.half;
-    hipLaunchKernelGGL(k_mwfj_bscale, dim3(d.nf), dim3(256), 0, st, d.B, Kp, d.nrhs, d.f, d.bsc);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    auto h = d.func ? (lds ? k_mwfj_h3<1, true> : k_mwfj_h3<1, false>)
-                    : (lds ? k_mwfj_h3<0, true> : k_mwfj_h3<0, false>);
-    hipLaunchKernelGGL(h, grid, dim3(256), shm, st, d.B, (const float*)d.bsc, d.rr, d.ir, d.f,
-                       d.c, d.nr, d.half, d.nf, d.nc, d.nrhs, d.nwin, d.T);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-  }
   auto k = d.func ? (lds ? k_mwfj_mfma<1, true> : k_mwfj_mfma<1, false>)
                   : (lds ? k_mwfj_mfma<0, true> : k_mwfj_mfma<0, false>);
-  hipLaunchKernelGGL(k, grid, dim3(256), shm, st, d.B, d.rr, d.ir, d.f, d.c, d.nr, d.half, d.nf,
-                     d.nc, d.nrhs, d.nwin, (const float*)d.bsc, d.T);
+  hipLaunchKernelGGL(k, grid, dim3(256), shm, st, d.B, (const unsigned int*)d.bmax, d.rr, d.ir,
+                     d.f, d.c, d.nr, d.half, d.nf, d.nc, d.nrhs, d.nwin, d.T);
   return hipGetLastError();
 }
 
