@@ -139,8 +139,8 @@ FJ_HD inline float fitf_R1(float t) {
   return p;
 }
 
-/* far argument, modulus and phase of the order-0 functions: M0(t), and
-   the phase shift in revolutions over 8/x */
+/* far argument, modulus and phase of the order-0 and order-1 functions:
+   Mn(t), and the phase shift in revolutions over 8/x */
 /* max abs error of the fit on (0,0.16]: 2.52e-10 */
 FJ_HD inline float fitf_M0(float t) {
   float p = 2.47902815240584e-05f;
@@ -152,6 +152,19 @@ FJ_HD inline float fitf_M0(float t) {
 FJ_HD inline float fitf_ph0(float t) {
   float p = 2.0077624210811506e-05f;
   p = __builtin_fmaf(p, t, -0.002486792805295915f);
+  return p;
+}
+/* max abs error of the fit on (0,0.16]: 3.76e-10 */
+FJ_HD inline float fitf_M1(float t) {
+  float p = -4.648938447752519e-05f;
+  p = __builtin_fmaf(p, t, 0.002929644837346829f);
+  p = __builtin_fmaf(p, t, 1.0000000003775265f);
+  return p;
+}
+/* max abs error of the fit on (0,0.16]: 5.63e-09 */
+FJ_HD inline float fitf_ph1(float t) {
+  float p = -5.071470336918935e-05f;
+  p = __builtin_fmaf(p, t, 0.007460382304125815f);
   return p;
 }
 

@@ -42,7 +42,7 @@ void run_mode(int itype, const float* U0, const float* U1, const float* r, const
     const float hlim = itype == 1 ? long_segment_limit(f[i], cmin) : 0.0f;
     if (itype == 1) cf[(size_t)nrp * NC] = hlim;
     for (int s = 0; s < nr; s++) {
-      float gp[2] = {0.f, 0.f}, gc[2] = {0.f, 0.f}, gn[2] = {0.f, 0.f}, rec[8];
+      float gp[2] = {0.f, 0.f}, gc[2] = {0.f, 0.f}, gn[2] = {0.f, 0.f}, rec[12];
       for (int m = 0; m < NS; m++) {
         const float* U = m == 0 ? U0 : U1;
         gc[m] = U[(size_t)s * nf + i];

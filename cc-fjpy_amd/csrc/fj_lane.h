@@ -36,12 +36,23 @@
 // sqrt(2/(pi k)) once.
 #pragma once
 
+#include <float.h>
 #include <math.h>
 #include <stdint.h>
 #include <string.h>
 
 #include "../../include/ccfj.h"
 #include "fj_math.h"
+
+// No floating-point contraction in this file: every fused multiply-add is
+// written as one, and no other product and sum is fused.  A term's
+// arithmetic then does not depend on how the compiler happens to shape the
+// code around it (one or two velocities per lane, which kernel inlines it),
+// so the item-list, batched and dense kernels agree bit for bit where they
+// take the same branches.
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
 
 namespace fjl {
 
@@ -81,24 +92,41 @@ FJ_HD inline float station_hh(const float* r, int nr, int s) {
   return (s < nr && s > 0) ? r[s] - r[s - 1] : 0.0f;
 }
 
+// 1/sqrt(r_s) as the station arrays hold it: held to the largest float at
+// r = 0, so that the linear-segment records, which carry sqrt(r_s), come
+// back as zeros there (and not as 0 * inf) when multiplied by it.
+FJ_HD inline float station_isr(float r) {
+  const float v = (float)(1.0 / sqrt((double)r));
+  return v > FLT_MAX ? FLT_MAX : v;
+}
+
+// (2/pi) a_s for the Y basis of a far station, from the hat record ah =
+// a sqrt r and isr = 1/sqrt r: computed the same way by the coefficient
+// preparation (the kappa plane, read by smooth blocks) and by the lanes of
+// the other blocks, so both give the same float.
+FJ_HD inline float kappa_of(float ah, float isr) { return (ah * isr) * kTwoOverPi; }
+
 // Entry s of the station arrays from the sorted float distances r[0..nr).
 FJ_HD inline void station_entry(const float* r, int nr, int s, double* rr, float* hh, float* ir,
                                 float* isr) {
-  const double rs = (double)r[s < nr ? s : nr - 1];
-  rr[s] = rs;
+  const float rf = r[s < nr ? s : nr - 1];
+  rr[s] = (double)rf;
   hh[s] = station_hh(r, nr, s);
   ir[s] = station_ir(r, nr, s);
-  isr[s] = (float)(1.0 / sqrt(rs));
+  isr[s] = station_isr(rf);
 }
 
 // Per-lane constants of one (f, c).
 struct LaneK {
-  double q;     // f / c, cycles per metre
-  double q2pi;  // k = 2 pi f / c
-  float kf;     // k in float
-  float zk;     // 8 / k           (z = 8/x = zk / r)
-  float ak;     // sqrt(2/(pi k))   (Hankel amplitude = ak / sqrt r)
+  double q;      // f / c, cycles per metre
+  double q2pi;   // k = 2 pi f / c
+  float kf;      // k in float
+  float zk;      // 8 / k           (z = 8/x = zk / r)
+  float ak;      // sqrt(2/(pi k))   (Hankel amplitude = ak / sqrt r)
+  uint32_t irf;  // bits of k / kFarX: x >= kFarX where the bits of 1/r are at most this
 };
+
+FJ_HD inline uint32_t f32_bits(float v);
 
 FJ_HD inline LaneK lane_consts(double q) {
   LaneK L;
@@ -107,6 +135,7 @@ FJ_HD inline LaneK lane_consts(double q) {
   L.kf = (float)L.q2pi;
   L.zk = (float)(8.0 / L.q2pi);
   L.ak = (float)sqrt(2.0 / (kPiD * L.q2pi));
+  L.irf = f32_bits((float)(L.q2pi / (double)kFarX));
   return L;
 }
 
@@ -227,18 +256,23 @@ template <> struct ModeT<MODE_ABS_J> { static constexpr int nspec = 2; static co
 template <> struct ModeT<MODE_ABS_H> { static constexpr int nspec = 2; static constexpr bool J = true, Y = true; };
 
 // Floats of coefficient record per station and frequency.  itype 0: A'_s per
-// spectrum.  itype 1: (a_s, p_s, t_s) per spectrum in the station's record
-// and b_s per spectrum in a second plane after the nrp records (see
-// station_coef and coef_store).
+// spectrum.  itype 1, per spectrum: the hat record (ah, ph, th, tt) in the
+// station's record, b_s in a second plane after the nrp records and the
+// kappa record in a third (see station_coef and coef_store).
 FJ_HD constexpr int coef_count(int mode, int itype) {
-  return ((mode == MODE_J || mode == MODE_Y) ? 1 : 2) * (itype == 0 ? 1 : 4);
+  return ((mode == MODE_J || mode == MODE_Y) ? 1 : 2) * (itype == 0 ? 1 : 6);
 }
 // Floats of the coefficients of one frequency: nrp records, and for itype 1
-// the b plane and a header of 8 floats whose first is that frequency's
-// long-segment threshold (long_segment_limit).
+// the b and kappa planes and a header of 8 floats whose first is that
+// frequency's long-segment threshold (long_segment_limit).
 FJ_HD constexpr size_t coef_block(int nrp, int mode, int itype) {
   return (size_t)nrp * coef_count(mode, itype) + (itype == 0 ? 0 : 8);
 }
+// itype 1 offsets inside a frequency's coefficients (ns spectra): the b
+// plane [nrp][ns], the kappa plane [ns][nrp], the header.
+FJ_HD constexpr size_t lin_b_offset(int nrp, int ns) { return (size_t)nrp * 4 * ns; }
+FJ_HD constexpr size_t lin_kappa_offset(int nrp, int ns) { return (size_t)nrp * 5 * ns; }
+FJ_HD constexpr size_t lin_header(int nrp, int ns) { return (size_t)nrp * 6 * ns; }
 
 // itype 1: segments with k_max h > kLongDx at some frequency ("long": k_max
 // = 2 pi |f| / min |c| over the velocity grid) are left out of the (a, p, t)
@@ -306,9 +340,12 @@ struct FjOcc {
 // ---- coefficients of station s (one frequency) ----------------------------
 // g_prev/g_cur/g_next: spectrum values at stations s-1, s, s+1 (only those
 // that exist are read).  rp, rs, rn: r_{s-1}, r_s, r_{s+1} (likewise).
-// itype 0 writes A'_s per spectrum.  itype 1 writes, per spectrum m,
-// rec[m] = a_s, rec[NSPEC + m] = p_s, rec[2 NSPEC + m] = t_s and
-// rec[3 NSPEC + m] = b_s:
+// itype 0 writes A'_s per spectrum.  itype 1 writes, per spectrum m, the
+// hat record rec[m] = a_s sqrt r_s, rec[NSPEC + m] = p_s sqrt r_s,
+// rec[2 NSPEC + m] = t_s / sqrt r_s, rec[3 NSPEC + m] = t_s sqrt r_s (each
+// formed in double and rounded once; see the hat form before far_units),
+// rec[4 NSPEC + m] = b_s and rec[5 NSPEC + m] = kappa_of(rec[m],
+// station_isr(r_s)).  a_s, p_s and t_s are
 //   the station's share of sum_s alpha_s g_s - sum_s b_s E_s with every E_s
 //   (= 2 D_s, segment s between stations s-1 and s) taken by the
 //   Euler-Maclaurin rule,
@@ -347,6 +384,7 @@ FJ_HD inline void station_coef(int s, int nr, float rp, float rs, float rn,
     const double hr = right ? (double)(rn - rs) : 0.0;
     const double ratl = left ? hl / (double)fmaxf(rs - rp, 0.1f) : 0.0;
     const double ratr = right ? hr / (double)fmaxf(rn - rs, 0.1f) : 0.0;
+    const double sq = sqrt((double)rs), isq = 1.0 / sq;
     for (int k = 0; k < NSPEC; k++) {
       const double g2 = g_cur[k];
       const double g1 = left ? (double)g_prev[k] : 0.0;
@@ -357,10 +395,15 @@ FJ_HD inline void station_coef(int s, int nr, float rp, float rs, float rn,
       const double er = right ? g2 : 0.0;             // right limit at r_s
       const double alpha = el - er;
       const double cl = inl ? bl : 0.0, cr = inr ? br : 0.0;
-      rec[k] = (float)(alpha - cl - cr);
-      rec[NSPEC + k] = (float)((cl * hl - cr * hr) / 6.0);
-      rec[2 * NSPEC + k] = (float)(-(cl * hl * hl * hl - cr * hr * hr * hr) / 360.0);
-      rec[3 * NSPEC + k] = (float)bl;
+      const double a = alpha - cl - cr;
+      const double p = (cl * hl - cr * hr) / 6.0;
+      const double t = -(cl * hl * hl * hl - cr * hr * hr * hr) / 360.0;
+      rec[k] = (float)(a * sq);
+      rec[NSPEC + k] = (float)(p * sq);
+      rec[2 * NSPEC + k] = (float)(t * isq);
+      rec[3 * NSPEC + k] = (float)(t * sq);
+      rec[4 * NSPEC + k] = (float)bl;
+      rec[5 * NSPEC + k] = kappa_of(rec[k], station_isr(rs));
     }
   }
 }
@@ -381,15 +424,19 @@ FJ_HD inline void to_polar(float* rec) {
 }
 
 // Stores station s's record into the coefficients of one frequency (cf,
-// nrp stations): itype 0 at cf[s NSPEC]; itype 1 the hot part at
-// cf[s 3 NSPEC] and b_s at cf[nrp 3 NSPEC + s NSPEC].
+// nrp stations): itype 0 at cf[s NSPEC]; itype 1 the hat record at
+// cf[s 4 NSPEC], b_s at cf[lin_b_offset + s NSPEC + m] and kappa at
+// cf[lin_kappa_offset + m nrp + s].
 template <int ITYPE, int NSPEC>
 FJ_HD inline void coef_store(float* cf, int nrp, int s, const float* rec) {
   if (ITYPE == 0) {
     for (int k = 0; k < NSPEC; k++) cf[(size_t)s * NSPEC + k] = rec[k];
   } else {
-    for (int k = 0; k < 3 * NSPEC; k++) cf[(size_t)s * 3 * NSPEC + k] = rec[k];
-    for (int k = 0; k < NSPEC; k++) cf[(size_t)nrp * 3 * NSPEC + (size_t)s * NSPEC + k] = rec[3 * NSPEC + k];
+    for (int k = 0; k < 4 * NSPEC; k++) cf[(size_t)s * 4 * NSPEC + k] = rec[k];
+    for (int k = 0; k < NSPEC; k++) {
+      cf[lin_b_offset(nrp, NSPEC) + (size_t)s * NSPEC + k] = rec[4 * NSPEC + k];
+      cf[lin_kappa_offset(nrp, NSPEC) + (size_t)k * nrp + s] = rec[5 * NSPEC + k];
+    }
   }
 }
 
@@ -594,9 +641,9 @@ FJ_HD inline float trapz_lane(const Stations& S, const float* __restrict__ cf, i
 // ~1/(x^2 ln x) at small x.  G is stored shifted (G + 2x/pi) where it comes
 // from the small-argument branch (sh = 1) and unshifted otherwise.
 // E = 2 D (and 2 D~) with b = beta / 2 (station_coef); the rule's form of
-// every E is collected per station into the (a, p, t) records, so the
-// station loop adds three products per spectrum and evaluates E itself
-// only on the segments that take the G difference.
+// every E is collected per station into the records, so the station loop
+// adds a few products per spectrum and evaluates E itself only on the
+// segments that take the G difference.
 struct Ends {  // per-basis values at a station
   float g, gp, g3, G, sh;
 };
@@ -666,15 +713,63 @@ FJ_HD inline void large_station(const LaneK& L, double rs, float ir, float isr, 
   }
 }
 
+// ---- far stations: the hat form --------------------------------------------
+// For x >= kFarX the Bessel functions are written over their Hankel
+// amplitude amp = sqrt(2/(pi x)) = ak / sqrt(r), ak = sqrt(2/(pi k)):
+//   U0 = J0 / amp, U1 = J1 / amp, V0 = Y0 / amp, V1 = Y1 / amp.
+// With g = x Z1, g' = x Z0, g''' = -(Z1 + x Z0) and x = k r, a station's
+// record terms become
+//   a g + k p g' + k^3 t g''' = k ak [(ah - k^2 th) U1 + k (ph - k^2 tt) U0]
+// with the hat records ah = a sqrt r, ph = p sqrt r, th = t / sqrt r and
+// tt = t sqrt r (station_coef).  The lane keeps four float sums, of
+// ah U1, th U1, ph U0 and tt U0 (one product each: a multiply-add reads
+// one scalar operand), scaled by k ak = sqrt(2k/pi), -k^2 times that, k
+// times that and -k^3 times that when flushed: no x, no amplitude and no g
+// products per station.  The Y basis's shifted
+// g~ = x Y1 + 2/pi adds (2/pi) a_s on top, the kappa record (kappa_of).
+// Every far station takes this form, in every kind of block, so a
+// station's terms do not depend on which lanes share its wave.
+struct FarUnits {
+  float u0, u1, v0, v1;
+};
+
+// Bases with J only: the modulus-phase form, one transcendental per order
+// (U0 = M0 cos(th + phi0), U1 = M1 sin(th + phi1), th = x - pi/4).  With
+// Y: one sine/cosine pair of th through the P, Q fits serves all four.
+template <bool NY>
+FJ_HD inline FarUnits far_units(double rs, const LaneK& L, float z) {
+  const float t = z * z;
+  const float u = phase_rev(L.q, rs);
+  FarUnits w;
+  if (!NY) {
+    w.u0 = fitf_M0(t) * cos_rev(__builtin_fmaf(z, fitf_ph0(t), u));
+    w.u1 = fitf_M1(t) * sin_rev(__builtin_fmaf(z, fitf_ph1(t), u));
+    w.v0 = w.v1 = 0.0f;
+  } else {
+    float sn, co;
+    sincos_rev(u, &sn, &co);
+    const float P0 = fitf_P0(t), Q0 = z * fitf_q0(t);
+    const float P1 = fitf_P1(t), Q1 = z * fitf_q1(t);
+    w.u0 = __builtin_fmaf(P0, co, -Q0 * sn);
+    w.v0 = __builtin_fmaf(P0, sn, Q0 * co);
+    w.u1 = __builtin_fmaf(P1, sn, Q1 * co);
+    w.v1 = __builtin_fmaf(Q1, sn, -P1 * co);
+  }
+  return w;
+}
+
 template <int MODE>
 struct LinState {
   Ends pj = {0.f, 0.f, 0.f, 0.f, 0.f}, py = {0.f, 0.f, 0.f, 0.f, 0.f};
   float xprev = 0.0f;
   bool gnext = false;  // the segment right of the previous station uses the G difference
   bool lnext = false;  // ... is long (not in the records; uniform)
-  // float partial sums of the a g, p g' and t g''' terms (R and, for the
-  // modulus modes, I)
+  // float partial sums of the near stations' a g, p g' and t g''' terms and
+  // of the corrections (R and, for the modulus modes, I) ...
   float aR = 0.0f, aI = 0.0f, pR = 0.0f, pI = 0.0f, tR = 0.0f, tI = 0.0f;
+  // ... and of the far stations' hat form: ah U1, th U1, ph U0, tt U0
+  float haR = 0.0f, haI = 0.0f, htR = 0.0f, htI = 0.0f, hpR = 0.0f, hpI = 0.0f, hqR = 0.0f,
+        hqI = 0.0f;
 };
 
 // R (+= w . v) and I for one quantity v per basis (vJ, vY) and one
@@ -702,34 +797,138 @@ FJ_HD inline void lin_acc(const float* w, float vJ, float vY, float& R, float& I
   }
 }
 
-// End of a station: its g, g' and g''' per basis go into the three partial
-// sums with the station's (a, p, t) record c.  The Euler-Maclaurin form of
-// every segment but the long ones is already in those records.  Two rare
-// cases need the segment to the left (a wave-uniform branch; the b record
-// cb is used only there): a lane whose segment takes the G difference adds
-// b (E^rule - E^G) per basis, and a long segment (longL, uniform) adds
-// -b E on every lane, E by the rule or the G difference as the lane chose.
+// The (2/pi) a_s of the shifted Y terms, kap[m] per spectrum, with the
+// signs lin_acc gives vY.
 template <int MODE>
-FJ_HD inline void linseg_tail(int s, float x, float dxL, bool useG_L, bool longL, float j0,
-                              float j1, float y0, float y1, float gy, Ends& cj, Ends& cy,
-                              const float* c, const float* cb, LinState<MODE>& st) {
+FJ_HD inline void kappa_acc(const float* kap, float& R, float& I) {
+  if (MODE == MODE_Y) {
+    R += kap[0];
+  } else if (MODE == MODE_J_MINUS_Y) {
+    R -= kap[1];
+  } else if (MODE == MODE_ABS_H) {
+    R -= kap[1];
+    I += kap[0];
+  }
+}
+
+// A far station's record terms (hat records h: ah, ph, th, tt, nspec floats
+// each) into the four hat sums.
+template <int MODE>
+FJ_HD inline void far_acc(const float* h, const FarUnits& w, LinState<MODE>& st) {
+  constexpr int NS = ModeT<MODE>::nspec;
+  lin_acc<MODE>(h, w.u1, w.v1, st.haR, st.haI);
+  lin_acc<MODE>(h + 2 * NS, w.u1, w.v1, st.htR, st.htI);
+  lin_acc<MODE>(h + NS, w.u0, w.v0, st.hpR, st.hpI);
+  lin_acc<MODE>(h + 3 * NS, w.u0, w.v0, st.hqR, st.hqI);
+}
+
+// One station of the linear-segment sum.  dxL = k (r_s - r_{s-1}),
+// dxR = k (r_{s+1} - r_s) (0 past the last station), hR = r_{s+1} - r_s,
+// hlim = the bits of the frequency's long-segment limit, h = the station's
+// hat record, kap = its kappa record (smooth blocks; the others form it),
+// cb = its b record.  FAR: x >= kFarX on every lane of the wave.  SMOOTH
+// (with FAR): no segment at this station is long or takes the G
+// difference on any lane, so no per-station test is made.
+template <int MODE, bool FAR, bool SMOOTH = false>
+FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dxL, float dxR,
+                                 float hR, uint32_t hlim, const LaneK& L, const float* h,
+                                 const float* kap, const float* cb,
+                                 LinState<MODE>& st) {
   using M = ModeT<MODE>;
   constexpr bool NJ = M::J, NY = M::Y;
   constexpr int NS = M::nspec;
-  if (NJ) {
-    cj.g = x * j1;
-    cj.gp = x * j0;
-    cj.g3 = -(j1 + cj.gp);
+  const float z = L.zk * ir;   // 8 / x; inf when k = 0 or r = 0
+  if (SMOOTH) {
+    far_acc<MODE>(h, far_units<NY>(rs, L, z), st);
+    if (NY) kappa_acc<MODE>(kap, st.aR, st.aI);
+    return;
   }
-  if (NY) {
-    cy.g = gy;
-    cy.gp = x * y0;
-    cy.g3 = -(y1 + cy.gp);
+  const double xd = L.q2pi * rs;
+  const float x = (float)xd;
+  const bool useG_L = s > 0 && st.gnext;
+  const bool longL = st.lnext;
+  st.lnext = f32_bits(hR) > hlim;
+  // in a far block both ends of every segment have x >= kFarX, where the
+  // rule's threshold min(kTau, kKappa min(x_a, x_b)) is kTau
+  const bool gR = FAR ? fabsf(dxR) >= kTau : use_gdiff_right(x, dxR);
+  st.gnext = gR;
+  const bool needG = useG_L || gR;
+  Ends cj = {0.f, 0.f, 0.f, 0.f, 0.f}, cy = {0.f, 0.f, 0.f, 0.f, 0.f};
+  // the lane's own far test, implied by the block's (far_threshold)
+  if (FAR || f32_bits(ir) <= L.irf) {
+    const FarUnits w = far_units<NY>(rs, L, z);
+    far_acc<MODE>(h, w, st);
+    if (NY) {
+      float kp[NS];
+#pragma unroll
+      for (int m = 0; m < NS; m++) kp[m] = kappa_of(h[m], isr);
+      kappa_acc<MODE>(kp, st.aR, st.aI);
+    }
+    // the ends, for a correction of this segment or the next
+    if (needG || longL || st.lnext) {
+      const float amp = L.ak * isr;
+      const float j0 = amp * w.u0, j1 = amp * w.u1, y0 = amp * w.v0, y1 = amp * w.v1;
+      if (NJ) {
+        cj.g = x * j1;
+        cj.gp = x * j0;
+        cj.g3 = -(j1 + cj.gp);
+      }
+      if (NY) {
+        cy.g = __builtin_fmaf(x, y1, kTwoOverPi);
+        cy.gp = x * y0;
+        cy.g3 = -(y1 + cy.gp);
+      }
+      if (needG) {
+        const float t = z * z;
+        const float r0 = fitf_R0(t), xr1 = x * fitf_R1(t);
+        if (NJ) cj.G = __builtin_fmaf(r0, j1, __builtin_fmaf(-xr1, j0, 1.0f));
+        if (NY) cy.G = __builtin_fmaf(r0, y1, -xr1 * y0);
+      }
+    }
+  } else {
+    float j0, j1, y0 = 0.0f, y1 = 0.0f, gy = 0.0f;
+    if (x >= kSplit) {
+      large_station<NJ, NY>(L, rs, ir, isr, x, needG, &j0, &j1, &y0, &y1, &cj.G, &cy.G);
+      if (NY) gy = __builtin_fmaf(x, y1, kTwoOverPi);
+    } else {
+      double y, Lg;
+      const Bess b = bessel_small<true, NY, FJ_FIT_I1>(xd, &y, &Lg);
+      j0 = b.j0;
+      j1 = b.j1;
+      y0 = b.y0;
+      y1 = b.y1;
+      gy = b.gy;
+      if (needG) g_small<FJ_FIT_I1>(xd, y, Lg, NJ, NY, &cj.G, &cy.G);
+      cy.sh = 1.0f;
+    }
+    if (NJ) {
+      cj.g = x * j1;
+      cj.gp = x * j0;
+      cj.g3 = -(j1 + cj.gp);
+    }
+    if (NY) {
+      cy.g = gy;
+      cy.gp = x * y0;
+      cy.g3 = -(y1 + cy.gp);
+    }
+    // the (a, p, t) records back from the hat records: a = ah / sqrt r,
+    // p = ph / sqrt r, t = tt / sqrt r (isr is finite: station_isr)
+    float ca[NS], cp[NS], ct[NS];
+#pragma unroll
+    for (int m = 0; m < NS; m++) {
+      ca[m] = h[m] * isr;
+      cp[m] = h[NS + m] * isr;
+      ct[m] = h[3 * NS + m] * isr;
+    }
+    lin_acc<MODE>(ca, cj.g, cy.g, st.aR, st.aI);
+    lin_acc<MODE>(cp, cj.gp, cy.gp, st.pR, st.pI);
+    lin_acc<MODE>(ct, cj.g3, cy.g3, st.tR, st.tI);
   }
-  lin_acc<MODE>(c, cj.g, cy.g, st.aR, st.aI);
-  lin_acc<MODE>(c + NS, cj.gp, cy.gp, st.pR, st.pI);
-  lin_acc<MODE>(c + 2 * NS, cj.g3, cy.g3, st.tR, st.tI);
-  // never in a SMOOTH block; G is formed only on the lanes that use it
+  // Two rare cases need the segment to the left (a wave-uniform branch; the
+  // b record cb is used only there): a lane whose segment takes the G
+  // difference adds b (E^rule - E^G) per basis, and a long segment (longL,
+  // uniform) adds -b E on every lane, E by the rule or the G difference as
+  // the lane chose.
   if (s > 0 && (longL || wave_any(useG_L))) {
     float dJ = 0.0f, dY = 0.0f;
     if (useG_L) {   // per lane: the G difference (and the rule, for a short segment)
@@ -753,51 +952,6 @@ FJ_HD inline void linseg_tail(int s, float x, float dxL, bool useG_L, bool longL
   st.xprev = x;
   st.pj = cj;
   st.py = cy;
-}
-
-// One station of the linear-segment sum.  dxL = k (r_s - r_{s-1}),
-// dxR = k (r_{s+1} - r_s) (0 past the last station), hR = r_{s+1} - r_s,
-// hlim = the bits of the frequency's long-segment limit, c = the station's
-// (a, p, t) record, cb = its b record.  FAR: x >= kFarX on every lane of
-// the wave.  SMOOTH (with FAR): no segment at this station is long or takes
-// the G difference on any lane, so no per-station test is made.
-template <int MODE, bool FAR, bool SMOOTH = false>
-FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dxL, float dxR,
-                                 float hR, uint32_t hlim, const LaneK& L, const float* c,
-                                 const float* cb, LinState<MODE>& st) {
-  using M = ModeT<MODE>;
-  constexpr bool NJ = M::J, NY = M::Y;
-  const double xd = L.q2pi * rs;
-  const float x = (float)xd;
-  const bool useG_L = !SMOOTH && s > 0 && st.gnext;
-  const bool longL = !SMOOTH && st.lnext;
-  st.lnext = !SMOOTH && f32_bits(hR) > hlim;
-  // in a far block both ends of every segment have x >= kFarX, where the
-  // rule's threshold min(kTau, kKappa min(x_a, x_b)) is kTau
-  const bool gR = SMOOTH ? false : FAR ? fabsf(dxR) >= kTau : use_gdiff_right(x, dxR);
-  st.gnext = gR;
-  const bool needG = useG_L || gR;
-  Ends cj = {0.f, 0.f, 0.f, 0.f, 0.f}, cy = {0.f, 0.f, 0.f, 0.f, 0.f};
-  float j0, j1, y0 = 0.0f, y1 = 0.0f, gy = 0.0f;
-  if (FAR) {
-    large_station<NJ, NY, true>(L, rs, ir, isr, x, !SMOOTH && wave_any(needG), &j0, &j1, &y0,
-                                &y1, &cj.G, &cy.G);
-    if (NY) gy = __builtin_fmaf(x, y1, kTwoOverPi);
-  } else if (x >= kSplit) {
-    large_station<NJ, NY>(L, rs, ir, isr, x, needG, &j0, &j1, &y0, &y1, &cj.G, &cy.G);
-    if (NY) gy = __builtin_fmaf(x, y1, kTwoOverPi);
-  } else {
-    double y, Lg;
-    const Bess b = bessel_small<true, NY, FJ_FIT_I1>(xd, &y, &Lg);
-    j0 = b.j0;
-    j1 = b.j1;
-    y0 = b.y0;
-    y1 = b.y1;
-    gy = b.gy;
-    if (needG) g_small<FJ_FIT_I1>(xd, y, Lg, NJ, NY, &cj.G, &cy.G);
-    cy.sh = 1.0f;
-  }
-  linseg_tail<MODE>(s, x, dxL, useG_L, longL, j0, j1, y0, y1, gy, cj, cy, c, cb, st);
 }
 
 // Far blocks in the linear-segment loop that still test every station (a
@@ -826,76 +980,98 @@ struct LinFar {
 };
 
 // Stations per block of the linear-segment loop: 4, not kUnroll.  With 8,
-// the (a, p, t) records (up to 48 floats) and the station data exceed the
-// scalar registers and spill to vector lanes (J - Y: 718 ms against 447 ms
-// per 4096 dense columns; J: 280 ms against 249 ms; 4096 ragged sub-arrays
-// 638 ms against 602 ms).  The block words are laid out for blocks of 4.
-// -DFJ_PRE_SMALL=0 turns the b prefetch off in the small-array kernels
-// (646 ms against 631 ms), a tuning switch.
+// the records and the station data exceed the scalar registers and spill to
+// vector lanes (J - Y: 718 ms against 447 ms per 4096 dense columns; J:
+// 280 ms against 249 ms; 4096 ragged sub-arrays 638 ms against 602 ms).
+// The block words are laid out for blocks of 4.  -DFJ_PRE_SMALL=0 turns the
+// b prefetch off in the small-array kernels (646 ms against 631 ms), a
+// tuning switch.
 constexpr int kLinBlock = 4;
 #ifndef FJ_PRE_SMALL
 #define FJ_PRE_SMALL 1
 #endif
-// cf: the coefficients of one frequency (coef_block floats): NC = 3 nspec
-// floats (a, p, t) per station, the b plane (nspec floats per station),
-// then the long-segment limit.  The lane sums the three kinds of term in
-// float partial sums and, every kPartial stations, adds a + k p + k^3 t to
-// a double accumulator.
+// cf: the coefficients of one frequency (coef_block floats): the hat
+// records (4 nspec floats per station), the b plane (nspec per station),
+// the kappa plane (nrp per spectrum), then the long-segment limit
+// (lin_header).  The lane sums the near stations' three kinds of term and
+// the far stations' four in float partial sums and, every kPartial
+// stations, adds them with their scales to a double accumulator.
 template <int MODE, bool SMALL = false>
 FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, int nrp, double q) {
+  static_assert(kUnroll % kLinBlock == 0 && kPartial % kLinBlock == 0 && kLinBlock == 4,
+                "station block");
   using LF = LinFar<MODE, SMALL>;
-  constexpr int NS = ModeT<MODE>::nspec;
-  constexpr int NC = 3 * NS;
+  using M = ModeT<MODE>;
+  constexpr int NS = M::nspec;
+  constexpr int NH = 4 * NS;
   constexpr int LU = kLinBlock;
-  static_assert(kUnroll % LU == 0 && kPartial % LU == 0 && LU == 4, "station block");
-  const float* __restrict__ cfb = cf + (size_t)nrp * NC;
-  const uint32_t hlim = f32_bits(cf[(size_t)nrp * 4 * NS]);
+  const float* __restrict__ cfb = cf + lin_b_offset(nrp, NS);
+  const float* __restrict__ cfk = cf + lin_kappa_offset(nrp, NS);
+  const uint32_t hlim = f32_bits(cf[lin_header(nrp, NS)]);
   const LaneK L = lane_consts(q);
   const uint32_t irfar = far_threshold(L.q2pi);
   const uint32_t hsm = smooth_threshold(L.q2pi, hlim);
   LinState<MODE> st;
   double accR = 0.0, accI = 0.0;
-  const double k1 = L.q2pi, k3 = L.q2pi * L.q2pi * L.q2pi;
+  const double k1 = L.q2pi, k2 = k1 * k1, k3 = k2 * k1;
+  const double sk = sqrt(2.0 * L.q2pi / kPiD);   // k ak; 0 at k = 0
+  auto hat = [&](float a, float t, float p, float qq) {
+    return sk * (((double)a - k2 * (double)t) + k1 * ((double)p - k2 * (double)qq));
+  };
   auto flush = [&](LinState<MODE>& t) {
-    accR += (double)t.aR + k1 * (double)t.pR + k3 * (double)t.tR;
-    t.aR = t.pR = t.tR = 0.0f;
+    accR += (double)t.aR + k1 * (double)t.pR + k3 * (double)t.tR + hat(t.haR, t.htR, t.hpR, t.hqR);
+    t.aR = t.pR = t.tR = t.haR = t.htR = t.hpR = t.hqR = 0.0f;
     if (MODE == MODE_ABS_J || MODE == MODE_ABS_H) {
-      accI += (double)t.aI + k1 * (double)t.pI + k3 * (double)t.tI;
-      t.aI = t.pI = t.tI = 0.0f;
+      accI += (double)t.aI + k1 * (double)t.pI + k3 * (double)t.tI +
+              hat(t.haI, t.htI, t.hpI, t.hqI);
+      t.aI = t.pI = t.tI = t.haI = t.htI = t.hpI = t.hqI = 0.0f;
     }
   };
-  float dxR = 0.0f;
   for (int sb = 0; sb < nrp; sb += LU) {
     double rv[LU];
-    float hw[LU + 1], iv[LU], sv[LU];   // hw[k + 1]: gap right of station sb + k
-    float cv[LU * NC];
+    float iv[LU];
+    float hv[LU * NH];
 #pragma unroll
     for (int k = 0; k < LU; k++) rv[k] = S.rr[sb + k];
 #pragma unroll
-    for (int k = 1; k <= LU; k++) hw[k] = S.hh[sb + k];
-#pragma unroll
     for (int k = 0; k < LU; k++) iv[k] = S.ir[sb + k];
 #pragma unroll
-    for (int k = 0; k < LU; k++) sv[k] = S.isr[sb + k];
-#pragma unroll
-    for (int k = 0; k < LU * NC; k++) cv[k] = cf[sb * NC + k];
-    const bool far = (LF::smooth || LF::on) && S.bw[sb] <= irfar;
-    if (LF::smooth && far && S.bw[sb + 1] <= hsm) {
+    for (int k = 0; k < LU * NH; k++) hv[k] = cf[sb * NH + k];
+    // both block words in one scalar load: a test of the second after the
+    // first would wait for memory twice per block (sb is a multiple of 4
+    // and the array 8-byte aligned; written by another kernel, so the wider
+    // read aliases no store of this one)
+#if defined(__HIP_DEVICE_COMPILE__)
+    const uint64_t bwp = reinterpret_cast<const uint64_t*>(S.bw)[sb >> 1];
+#else
+    uint64_t bwp;
+    memcpy(&bwp, S.bw + sb, 8);
+#endif
+    const uint32_t bw0 = (uint32_t)bwp, bw1 = (uint32_t)(bwp >> 32);
+    const bool far = (LF::smooth || LF::on) && bw0 <= irfar;
+    if (LF::smooth && far && bw1 <= hsm) {
       // every gap of the block (its left one included) is below both the
       // G-difference and the long-segment limits on every lane
+      float kv[M::Y ? LU * NS : 1];
+      if (M::Y) {
+#pragma unroll
+        for (int m = 0; m < NS; m++)
+#pragma unroll
+          for (int k = 0; k < LU; k++) kv[k * NS + m] = cfk[(size_t)m * nrp + sb + k];
+      }
 #pragma unroll
       for (int k = 0; k < LU; k++) {
-        const float dxL = dxR;
-        dxR = L.kf * hw[k + 1];
-        linseg_station<MODE, true, true>(sb + k, rv[k], iv[k], sv[k], dxL, dxR, hw[k + 1], hlim,
-                                         L, cv + k * NC, nullptr, st);
-#if defined(__HIP_DEVICE_COMPILE__)
-        // one-spectrum modes: no interleaving of stations, which at their
-        // occupancy would spill (J - Y, at 3 waves, gains nothing from it)
-        if (NS == 1) __builtin_amdgcn_sched_barrier(0);
-#endif
+        linseg_station<MODE, true, true>(sb + k, rv[k], iv[k], 0.0f, 0.0f, 0.0f, 0.0f, hlim, L,
+                                         hv + k * NH, kv + (M::Y ? k * NS : 0), nullptr, st);
       }
+      // smooth: the block's last segment is neither long nor a G difference
+      st.gnext = st.lnext = false;
     } else {
+      float hw[LU + 1], sv[LU];   // hw[k]: gap left of station sb + k
+#pragma unroll
+      for (int k = 0; k <= LU; k++) hw[k] = S.hh[sb + k];
+#pragma unroll
+      for (int k = 0; k < LU; k++) sv[k] = S.isr[sb + k];
       // One-spectrum modes fetch the block's b records here, once, rather
       // than by a wait in each station's rare branch (not rare in arrays
       // of long segments: batched sub-arrays 658 -> 643 ms).  The
@@ -911,20 +1087,15 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
       auto brec = [&](int k) { return bsrc + k * NS; };
       if (LF::on && far) {
 #pragma unroll
-        for (int k = 0; k < LU; k++) {
-          const float dxL = dxR;
-          dxR = L.kf * hw[k + 1];
-          linseg_station<MODE, true>(sb + k, rv[k], iv[k], sv[k], dxL, dxR, hw[k + 1], hlim, L,
-                                     cv + k * NC, brec(k), st);
-        }
+        for (int k = 0; k < LU; k++)
+          linseg_station<MODE, true>(sb + k, rv[k], iv[k], sv[k], L.kf * hw[k], L.kf * hw[k + 1],
+                                     hw[k + 1], hlim, L, hv + k * NH, nullptr, brec(k), st);
       } else {
 #pragma unroll
-        for (int k = 0; k < LU; k++) {
-          const float dxL = dxR;
-          dxR = L.kf * hw[k + 1];
-          linseg_station<MODE, false>(sb + k, rv[k], iv[k], sv[k], dxL, dxR, hw[k + 1], hlim, L,
-                                      cv + k * NC, brec(k), st);
-        }
+        for (int k = 0; k < LU; k++)
+          linseg_station<MODE, false>(sb + k, rv[k], iv[k], sv[k], L.kf * hw[k],
+                                      L.kf * hw[k + 1], hw[k + 1], hlim, L, hv + k * NH,
+                                      nullptr, brec(k), st);
       }
     }
     if (((sb + LU) & (kPartial - 1)) == 0) flush(st);
@@ -937,3 +1108,8 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
 }
 
 }  // namespace fjl
+
+// back to the build's default (contraction where the language allows it)
+#if defined(__clang__)
+#pragma clang fp contract(fast)
+#endif

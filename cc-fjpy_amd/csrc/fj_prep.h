@@ -29,7 +29,7 @@ __device__ inline void prep_tile(const float* __restrict__ U0, const float* __re
                                  const float* __restrict__ cgrid, int nr, int nrp, int nf, int nc,
                                  float* __restrict__ coef, int f0, int s0) {
   constexpr int TS = 64, TF = 64, LD = TF + 1, ROWS = TS + 2;
-  constexpr int NREC = ITYPE == 0 ? NSPEC : 4 * NSPEC;
+  constexpr int NREC = ITYPE == 0 ? NSPEC : 6 * NSPEC;
   const size_t block = fjl::coef_block(nrp, NSPEC == 1 ? fjl::MODE_J : fjl::MODE_J_MINUS_Y, ITYPE);
   __shared__ float tile[NSPEC][ROWS][LD];
   __shared__ float hlim_s[TF];

@@ -16,11 +16,12 @@ Large argument, x >= 8, variable t = (8/x)^2 in (0,1], evaluated in float:
 Far argument, x >= 20 (t <= 0.16): the same six functions at lower degree,
 fitf_P0 ... fitf_R1 (degree 2 for P and Q, 3 for R; relative error below
 1e-8 on that range).
-Far argument, modulus-phase form of the order-0 functions (one cosine per
-J0 instead of a sine and a cosine):
-    J_0(x) = sqrt(2/(pi x)) M_0 cos(th_0 + phi_0),  Y_0 = ... sin(th_0 + phi_0),
-    M_0 = sqrt(P_0^2 + Q_0^2),  phi_0 = atan2(Q_0, P_0)
-  fitf_M0(t) = M_0,  fitf_ph0(t) = phi_0 / (2 pi) * x/8  (degree 2 and 1).
+Far argument, modulus-phase form (one transcendental per function instead
+of a sine and a cosine):
+    J_n(x) = sqrt(2/(pi x)) M_n cos(th_n + phi_n),  Y_n = ... sin(th_n + phi_n),
+    M_n = sqrt(P_n^2 + Q_n^2),  phi_n = atan2(Q_n, P_n)
+  fitf_M0(t) = M_0,  fitf_ph0(t) = phi_0 / (2 pi) * x/8  (degree 2 and 1),
+  fitf_M1, fitf_ph1 likewise for n = 1.
 Small argument, x < 8, variable y = (x/8)^2 in [0,1), evaluated in double
 (the monomial coefficients alternate in sign up to ~1e2, too much
 cancellation for float):
@@ -67,7 +68,9 @@ def far_mp_truth(x):
     v = large_truth(x)
     z = 8 / mp.mpf(x)
     P0, Q0 = v[0], v[1] * z
-    return [mp.sqrt(P0 ** 2 + Q0 ** 2), mp.atan2(Q0, P0) / (2 * mp.pi) / z]
+    P1, Q1 = v[2], v[3] * z
+    return [mp.sqrt(P0 ** 2 + Q0 ** 2), mp.atan2(Q0, P0) / (2 * mp.pi) / z,
+            mp.sqrt(P1 ** 2 + Q1 ** 2), mp.atan2(Q1, P1) / (2 * mp.pi) / z]
 
 
 def small_truth(x):
@@ -87,7 +90,7 @@ LARGE = [("fit_P0", 4), ("fit_q0", 4), ("fit_P1", 4), ("fit_q1", 4),
 FAR_X = 20.0
 FAR = [("fitf_P0", 2), ("fitf_q0", 2), ("fitf_P1", 2), ("fitf_q1", 2),
        ("fitf_R0", 3), ("fitf_R1", 3)]
-FAR_MP = [("fitf_M0", 2), ("fitf_ph0", 1)]
+FAR_MP = [("fitf_M0", 2), ("fitf_ph0", 1), ("fitf_M1", 2), ("fitf_ph1", 1)]
 SMALL = [("fit_J0s", 11), ("fit_J1s", 10), ("fit_Y0s", 11), ("fit_Y1s", 10),
          ("fit_GJs", 10), ("fit_GYs", 11)]
 
@@ -207,8 +210,8 @@ def main():
     for name, coef, err in far:
         out.append(emit(name, coef, "float", err, "t", far_top))
     out.append("")
-    out.append(f"/* far argument, modulus and phase of the order-0 functions: M0(t), and")
-    out.append(f"   the phase shift in revolutions over 8/x */")
+    out.append(f"/* far argument, modulus and phase of the order-0 and order-1 functions:")
+    out.append(f"   Mn(t), and the phase shift in revolutions over 8/x */")
     for name, coef, err in far_mp:
         out.append(emit(name, coef, "float", err, "t", far_top))
     out.append("")
