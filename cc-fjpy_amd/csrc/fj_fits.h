@@ -168,6 +168,70 @@ FJ_HD inline float fitf_ph1(float t) {
   return p;
 }
 
+/* x >= 8 (t in (0, 1]), the F-J lane's far forms: modulus and phase, and P, Q */
+/* max abs error of the fit on (0,1]: 1.62e-09 */
+FJ_HD inline float fitm_M0(float t) {
+  float p = -1.5624218298133712e-06f;
+  p = __builtin_fmaf(p, t, 2.4986901327827262e-05f);
+  p = __builtin_fmaf(p, t, -0.0009765082817761566f);
+  p = __builtin_fmaf(p, t, 0.9999999983557427f);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 2.75e-09 */
+FJ_HD inline float fitm_ph0(float t) {
+  float p = -8.67582837049008e-07f;
+  p = __builtin_fmaf(p, t, 2.018513326570912e-05f);
+  p = __builtin_fmaf(p, t, -0.0024867931705683244f);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 2.21e-09 */
+FJ_HD inline float fitm_M1(float t) {
+  float p = 2.3870479130173432e-06f;
+  p = __builtin_fmaf(p, t, -4.681973077723062e-05f);
+  p = __builtin_fmaf(p, t, 0.002929613733974215f);
+  p = __builtin_fmaf(p, t, 1.00000000224122f);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 4.11e-09 */
+FJ_HD inline float fitm_ph1(float t) {
+  float p = 1.5796516822635373e-06f;
+  p = __builtin_fmaf(p, t, -5.09207762010708e-05f);
+  p = __builtin_fmaf(p, t, 0.00746038375925628f);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 1.69e-09 */
+FJ_HD inline float fitm_P0(float t) {
+  float p = -1.6545902499551725e-06f;
+  p = __builtin_fmaf(p, t, 2.708336367553825e-05f);
+  p = __builtin_fmaf(p, t, -0.0010985762789425106f);
+  p = __builtin_fmaf(p, t, 0.9999999982850651f);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 7.58e-10 */
+FJ_HD inline float fitm_q0(float t) {
+  float p = 5.751186319100864e-07f;
+  p = __builtin_fmaf(p, t, -6.7942766767205345e-06f);
+  p = __builtin_fmaf(p, t, 0.000143025501811536f);
+  p = __builtin_fmaf(p, t, -0.015624999227061483f);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 1.94e-09 */
+FJ_HD inline float fitm_P1(float t) {
+  float p = 1.976842868909807e-06f;
+  p = __builtin_fmaf(p, t, -3.486353870672794e-05f);
+  p = __builtin_fmaf(p, t, 0.0018309899146891602f);
+  p = __builtin_fmaf(p, t, 1.000000001966288f);
+  return p;
+}
+/* max abs error of the fit on (0,1]: 8.57e-10 */
+FJ_HD inline float fitm_q1(float t) {
+  float p = -6.707130042021766e-07f;
+  p = __builtin_fmaf(p, t, 8.316961712685902e-06f);
+  p = __builtin_fmaf(p, t, -0.0002002426392543822f);
+  p = __builtin_fmaf(p, t, 0.04687499912633409f);
+  return p;
+}
+
 /* small argument x < 8, y = (x/8)^2, double */
 /* max abs error of the fit on (0,1]: 8.04e-11 */
 FJ_TABLE(k_fit_J0s, 12, {-0.005618801482900782, 0.07079209695405728, -0.5037809434408522, 2.625220559644334, -10.557357884793742, 32.35918804582165, -72.81660855664654, 113.77757421102329, -113.77775682787487, 63.99999888465921, -15.999999976722172, 0.9999999999195268})

@@ -116,6 +116,16 @@ FJ_HD inline void station_entry(const float* r, int nr, int s, double* rr, float
   isr[s] = station_isr(rf);
 }
 
+// Far argument in the F-J lanes: x >= 8, the small/large split itself.  Far
+// stations take the modulus-phase (or P, Q) forms with the fitm_ fits, valid
+// on t = (8/x)^2 in (0, 1] to about 2e-9; below 8 the double-precision
+// small-argument polynomials.  (The far forms once started at x = 20, with
+// degree-2 fits and a third form of degree-4 fits between 8 and 20: 5% of
+// the dense array's trapezoid blocks then held a station below 20 for some
+// lane against 2% below 8, and such a block costs the wave every form its
+// lanes need.)
+constexpr float kLaneFar = kSplit;
+
 // Per-lane constants of one (f, c).
 struct LaneK {
   double q;      // f / c, cycles per metre
@@ -123,7 +133,7 @@ struct LaneK {
   float kf;      // k in float
   float zk;      // 8 / k           (z = 8/x = zk / r)
   float ak;      // sqrt(2/(pi k))   (Hankel amplitude = ak / sqrt r)
-  uint32_t irf;  // bits of k / kFarX: x >= kFarX where the bits of 1/r are at most this
+  uint32_t irf;  // bits of k / kLaneFar: x >= kLaneFar where the bits of 1/r are at most this
 };
 
 FJ_HD inline uint32_t f32_bits(float v);
@@ -135,13 +145,13 @@ FJ_HD inline LaneK lane_consts(double q) {
   L.kf = (float)L.q2pi;
   L.zk = (float)(8.0 / L.q2pi);
   L.ak = (float)sqrt(2.0 / (kPiD * L.q2pi));
-  L.irf = f32_bits((float)(L.q2pi / (double)kFarX));
+  L.irf = f32_bits((float)(L.q2pi / (double)kLaneFar));
   return L;
 }
 
 // ---- wave-uniform far blocks -----------------------------------------------
-// A block of kUnroll stations is "far" when x >= kFarX for every lane of the
-// wave, i.e. when every station of the block has 1/r <= kmin / kFarX with
+// A block of kUnroll stations is "far" when x >= kLaneFar for every lane of
+// the wave, i.e. when every station of the block has 1/r <= kmin / kLaneFar with
 // kmin the smallest k of the wave.  The test is one scalar comparison per
 // block (1/r are non-negative floats, so their bit patterns order as
 // unsigned integers; r = 0 gives +inf, r < 0 a sign bit: never far), and a
@@ -164,9 +174,9 @@ FJ_HD inline uint32_t far_threshold(double q2pi) {
 #if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) kmin = fmin(kmin, __shfl_xor(kmin, o));
-  return __builtin_amdgcn_readfirstlane(f32_bits((float)(kmin / (double)kFarX)));
+  return __builtin_amdgcn_readfirstlane(f32_bits((float)(kmin / (double)kLaneFar)));
 #else
-  return f32_bits((float)(kmin / (double)kFarX));
+  return f32_bits((float)(kmin / (double)kLaneFar));
 #endif
 }
 
@@ -468,7 +478,7 @@ FJ_HD inline void trapz_add(const float* c, float vj, float vy, float& pR, float
   }
 }
 
-// x >= kFarX, z = 8/x: J0 / amp and Y0 / amp in the modulus-phase form
+// x >= kLaneFar, z = 8/x: J0 / amp and Y0 / amp in the modulus-phase form
 // J0 = amp M0 cos(th + phi), Y0 = amp M0 sin(th + phi), one transcendental
 // per basis.  Far blocks and the per-lane path both use it, so a term does
 // not depend on which lanes share its wave.
@@ -476,8 +486,8 @@ template <int MODE>
 FJ_HD inline void far_unit(double rs, const LaneK& L, float z, float& vj, float& vy) {
   using M = ModeT<MODE>;
   const float t = z * z;
-  const float m0 = fitf_M0(t);
-  const float u = __builtin_fmaf(z, fitf_ph0(t), phase_rev(L.q, rs));
+  const float m0 = fitm_M0(t);
+  const float u = __builtin_fmaf(z, fitm_ph0(t), phase_rev(L.q, rs));
   if (M::J && M::Y) {
     float s, co;
     sincos_rev(u, &s, &co);
@@ -490,11 +500,11 @@ FJ_HD inline void far_unit(double rs, const LaneK& L, float z, float& vj, float&
   }
 }
 
-// J - Y with polar records (c = R, psi): R M0 cos(u + psi), x >= kFarX.
+// J - Y with polar records (c = R, psi): R M0 cos(u + psi), x >= kLaneFar.
 FJ_HD inline float far_polar(double rs, const LaneK& L, float z, float psi) {
   const float t = z * z;
-  const float u = __builtin_fmaf(z, fitf_ph0(t), phase_rev(L.q, rs) + psi);
-  return fitf_M0(t) * cos_rev(u);
+  const float u = __builtin_fmaf(z, fitm_ph0(t), phase_rev(L.q, rs) + psi);
+  return fitm_M0(t) * cos_rev(u);
 }
 
 template <int MODE>
@@ -503,34 +513,19 @@ FJ_HD inline void trapz_station(double rs, float ir, const LaneK& L, const float
   using M = ModeT<MODE>;
   float vj = 0.0f, vy = 0.0f;
   const float z = L.zk * ir;  // 8 / x; inf when k = 0 or r = 0
-  if (MODE == MODE_J_MINUS_Y && z <= 1.0f) {   // polar records, x >= 8
-    float v;
-    if (z <= 8.0f / kFarX) {
-      v = far_polar(rs, L, z, c[1]);
-    } else {
-      const float t = z * z;
-      float s, co;
-      sincos_rev(phase_rev(L.q, rs) + c[1], &s, &co);
-      v = __builtin_fmaf(fit_P0(t), co, -(z * fit_q0(t)) * s);
-    }
-    pR = __builtin_fmaf(c[0], v, pR);
+  // x >= kLaneFar: the lane's own far test, implied by the block's
+  const bool far = f32_bits(ir) <= L.irf;
+  if (MODE == MODE_J_MINUS_Y && far) {   // polar records
+    pR = __builtin_fmaf(c[0], far_polar(rs, L, z, c[1]), pR);
     return;
   }
-  if (z <= 8.0f / kFarX) {    // x >= kFarX
+  if (far) {
     far_unit<MODE>(rs, L, z, vj, vy);
-  } else if (z <= 1.0f) {     // 8 <= x < kFarX
-    const float t = z * z;
-    const float p0 = fit_P0(t);
-    const float q0 = z * fit_q0(t);
-    float s, co;
-    sincos_rev(phase_rev(L.q, rs), &s, &co);
-    if (M::J) vj = __builtin_fmaf(p0, co, -q0 * s);
-    if (M::Y) vy = __builtin_fmaf(p0, s, q0 * co);
   } else {
     const double xd = L.q2pi * rs;
     double y, Lg;
     const Bess b = bessel_small<false, M::Y, FIT_TABLE>(xd, &y, &Lg);
-    const float ia = L.kf > 0.0f ? sqrtf(kPiOver2 * (float)xd) : sqrtf((float)rs);
+    const float ia = L.kf > 0.0f ? sqrt_f(kPiOver2 * (float)xd) : sqrt_f((float)rs);
     vj = b.j0 * ia;
     vy = b.y0 * ia;
   }
@@ -544,7 +539,7 @@ FJ_HD inline void trapz_station(double rs, float ir, const LaneK& L, const float
   trapz_add<MODE>(c, vj, vy, pR, pI);
 }
 
-// x >= kFarX on every lane (a far block): no branch on the argument.
+// x >= kLaneFar on every lane (a far block): no branch on the argument.
 template <int MODE>
 FJ_HD inline void trapz_far(double rs, float ir, const LaneK& L, const float* c, float& pR,
                             float& pI) {
@@ -586,10 +581,10 @@ FJ_HD inline void trapz_far_block(const double* rv, const float* iv, const LaneK
 #pragma unroll
     for (int k = 0; k < kUnroll; k++) {
       const float p = MODE == MODE_J_MINUS_Y ? ph[k] + cv[k * NC + 1] : ph[k];
-      u[k] = __builtin_fmaf(z[k], fitf_ph0(t[k]), p);
+      u[k] = __builtin_fmaf(z[k], fitm_ph0(t[k]), p);
     }
 #pragma unroll
-    for (int k = 0; k < kUnroll; k++) v[k] = fitf_M0(t[k]) * cos_rev(u[k]);
+    for (int k = 0; k < kUnroll; k++) v[k] = fitm_M0(t[k]) * cos_rev(u[k]);
 #pragma unroll
     for (int k = 0; k < kUnroll; k++) pR = __builtin_fmaf(cv[k * NC], v[k], pR);
   } else {
@@ -617,7 +612,7 @@ FJ_HD inline float trapz_lane(const Stations& S, const float* __restrict__ cf, i
       for (int k = 0; k < kUnroll; k++) iv[k] = S.ir[sb + k];
 #pragma unroll
       for (int k = 0; k < kUnroll * NC; k++) cv[k] = cf[sb * NC + k];
-      if (S.bw[sb + 2] <= irfar) {   // every station of the block has x >= kFarX
+      if (S.bw[sb + 2] <= irfar) {   // every station of the block has x >= kLaneFar
         trapz_far_block<MODE>(rv, iv, L, cv, pR, pI);
       } else {
 #pragma unroll
@@ -714,7 +709,7 @@ FJ_HD inline void large_station(const LaneK& L, double rs, float ir, float isr, 
 }
 
 // ---- far stations: the hat form --------------------------------------------
-// For x >= kFarX the Bessel functions are written over their Hankel
+// For x >= kLaneFar the Bessel functions are written over their Hankel
 // amplitude amp = sqrt(2/(pi x)) = ak / sqrt(r), ak = sqrt(2/(pi k)):
 //   U0 = J0 / amp, U1 = J1 / amp, V0 = Y0 / amp, V1 = Y1 / amp.
 // With g = x Z1, g' = x Z0, g''' = -(Z1 + x Z0) and x = k r, a station's
@@ -742,14 +737,14 @@ FJ_HD inline FarUnits far_units(double rs, const LaneK& L, float z) {
   const float u = phase_rev(L.q, rs);
   FarUnits w;
   if (!NY) {
-    w.u0 = fitf_M0(t) * cos_rev(__builtin_fmaf(z, fitf_ph0(t), u));
-    w.u1 = fitf_M1(t) * sin_rev(__builtin_fmaf(z, fitf_ph1(t), u));
+    w.u0 = fitm_M0(t) * cos_rev(__builtin_fmaf(z, fitm_ph0(t), u));
+    w.u1 = fitm_M1(t) * sin_rev(__builtin_fmaf(z, fitm_ph1(t), u));
     w.v0 = w.v1 = 0.0f;
   } else {
     float sn, co;
     sincos_rev(u, &sn, &co);
-    const float P0 = fitf_P0(t), Q0 = z * fitf_q0(t);
-    const float P1 = fitf_P1(t), Q1 = z * fitf_q1(t);
+    const float P0 = fitm_P0(t), Q0 = z * fitm_q0(t);
+    const float P1 = fitm_P1(t), Q1 = z * fitm_q1(t);
     w.u0 = __builtin_fmaf(P0, co, -Q0 * sn);
     w.v0 = __builtin_fmaf(P0, sn, Q0 * co);
     w.u1 = __builtin_fmaf(P1, sn, Q1 * co);
@@ -826,7 +821,7 @@ FJ_HD inline void far_acc(const float* h, const FarUnits& w, LinState<MODE>& st)
 // dxR = k (r_{s+1} - r_s) (0 past the last station), hR = r_{s+1} - r_s,
 // hlim = the bits of the frequency's long-segment limit, h = the station's
 // hat record, kap = its kappa record (smooth blocks; the others form it),
-// cb = its b record.  FAR: x >= kFarX on every lane of the wave.  SMOOTH
+// cb = its b record.  FAR: x >= kLaneFar on every lane of the wave.  SMOOTH
 // (with FAR): no segment at this station is long or takes the G
 // difference on any lane, so no per-station test is made.
 template <int MODE, bool FAR, bool SMOOTH = false>
@@ -848,7 +843,7 @@ FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dx
   const bool useG_L = s > 0 && st.gnext;
   const bool longL = st.lnext;
   st.lnext = f32_bits(hR) > hlim;
-  // in a far block both ends of every segment have x >= kFarX, where the
+  // in a far block both ends of every segment have x >= kLaneFar, where the
   // rule's threshold min(kTau, kKappa min(x_a, x_b)) is kTau
   const bool gR = FAR ? fabsf(dxR) >= kTau : use_gdiff_right(x, dxR);
   st.gnext = gR;
@@ -880,27 +875,18 @@ FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dx
       }
       if (needG) {
         const float t = z * z;
-        const float r0 = fitf_R0(t), xr1 = x * fitf_R1(t);
+        const float r0 = fit_R0(t), xr1 = x * fit_R1(t);
         if (NJ) cj.G = __builtin_fmaf(r0, j1, __builtin_fmaf(-xr1, j0, 1.0f));
         if (NY) cy.G = __builtin_fmaf(r0, y1, -xr1 * y0);
       }
     }
   } else {
-    float j0, j1, y0 = 0.0f, y1 = 0.0f, gy = 0.0f;
-    if (x >= kSplit) {
-      large_station<NJ, NY>(L, rs, ir, isr, x, needG, &j0, &j1, &y0, &y1, &cj.G, &cy.G);
-      if (NY) gy = __builtin_fmaf(x, y1, kTwoOverPi);
-    } else {
-      double y, Lg;
-      const Bess b = bessel_small<true, NY, FJ_FIT_I1>(xd, &y, &Lg);
-      j0 = b.j0;
-      j1 = b.j1;
-      y0 = b.y0;
-      y1 = b.y1;
-      gy = b.gy;
-      if (needG) g_small<FJ_FIT_I1>(xd, y, Lg, NJ, NY, &cj.G, &cy.G);
-      cy.sh = 1.0f;
-    }
+    // x < kLaneFar: the double-precision small-argument polynomials
+    double y, Lg;
+    const Bess b = bessel_small<true, NY, FJ_FIT_I1>(xd, &y, &Lg);
+    const float j0 = b.j0, j1 = b.j1, y0 = b.y0, y1 = b.y1, gy = b.gy;
+    if (needG) g_small<FJ_FIT_I1>(xd, y, Lg, NJ, NY, &cj.G, &cy.G);
+    cy.sh = 1.0f;
     if (NJ) {
       cj.g = x * j1;
       cj.gp = x * j0;

@@ -90,6 +90,15 @@ FJ_HD inline float rsqrt_f(float x) {
 #endif
 }
 
+// square root, about 1 ulp (the hardware instruction, no IEEE refinement)
+FJ_HD inline float sqrt_f(float x) {
+#if FJ_USE_HW
+  return __builtin_amdgcn_sqrtf(x);
+#else
+  return sqrtf(x);
+#endif
+}
+
 FJ_HD inline float rcp_f(float x) {
 #if FJ_USE_HW
   return __builtin_amdgcn_rcpf(x);

@@ -22,6 +22,9 @@ of a sine and a cosine):
     M_n = sqrt(P_n^2 + Q_n^2),  phi_n = atan2(Q_n, P_n)
   fitf_M0(t) = M_0,  fitf_ph0(t) = phi_0 / (2 pi) * x/8  (degree 2 and 1),
   fitf_M1, fitf_ph1 likewise for n = 1.
+From x = 8 on (t in (0, 1]), for the F-J kernels' far stations: fitm_M0 ...
+fitm_ph1 (degree 3 and 2) and fitm_P0 ... fitm_q1 (degree 3), errors about
+2e-9.
 Small argument, x < 8, variable y = (x/8)^2 in [0,1), evaluated in double
 (the monomial coefficients alternate in sign up to ~1e2, too much
 cancellation for float):
@@ -91,6 +94,9 @@ FAR_X = 20.0
 FAR = [("fitf_P0", 2), ("fitf_q0", 2), ("fitf_P1", 2), ("fitf_q1", 2),
        ("fitf_R0", 3), ("fitf_R1", 3)]
 FAR_MP = [("fitf_M0", 2), ("fitf_ph0", 1), ("fitf_M1", 2), ("fitf_ph1", 1)]
+# the F-J lane's far forms from x = 8 on (t in (0, 1]): modulus-phase and P, Q
+MID_MP = [("fitm_M0", 3), ("fitm_ph0", 2), ("fitm_M1", 3), ("fitm_ph1", 2)]
+MID_PQ = [("fitm_P0", 3), ("fitm_q0", 3), ("fitm_P1", 3), ("fitm_q1", 3)]
 SMALL = [("fit_J0s", 11), ("fit_J1s", 10), ("fit_Y0s", 11), ("fit_Y1s", 10),
          ("fit_GJs", 10), ("fit_GYs", 11)]
 
@@ -167,6 +173,8 @@ def main():
     far = fit_all(FAR, "t", large_truth, npts=80, top=far_top)
     far_mp = fit_all(FAR_MP, "t", far_mp_truth, npts=80, top=far_top)
     small = fit_all(SMALL, "y", small_truth)
+    mid_mp = fit_all(MID_MP, "t", far_mp_truth, npts=120)
+    mid_pq = fit_all(MID_PQ, "t", large_truth, npts=120)
     out = ["/* Generated by gen_fits.py -- do not edit by hand. */",
            "#pragma once",
            "#ifndef FJ_HD",
@@ -215,13 +223,17 @@ def main():
     for name, coef, err in far_mp:
         out.append(emit(name, coef, "float", err, "t", far_top))
     out.append("")
+    out.append("/* x >= 8 (t in (0, 1]), the F-J lane's far forms: modulus and phase, and P, Q */")
+    for name, coef, err in mid_mp + mid_pq:
+        out.append(emit(name, coef, "float", err, "t"))
+    out.append("")
     out.append("/* small argument x < 8, y = (x/8)^2, double */")
     for name, coef, err in small:
         out.append(emit(name, coef, "double", err, "y"))
     path = os.path.join(HERE, "fj_fits.h")
     with open(path, "w") as fh:
         fh.write("\n".join(out) + "\n")
-    for name, coef, err in large + far + far_mp + small:
+    for name, coef, err in large + far + far_mp + mid_mp + mid_pq + small:
         print(f"{name:10s} deg {len(coef) - 1:2d} err {err:.2e}")
 
 
