@@ -169,12 +169,11 @@ FJ_HD inline float fitf_ph1(float t) {
 }
 
 /* x >= 8 (t in (0, 1]), the F-J lane's far forms: modulus and phase, and P, Q */
-/* max abs error of the fit on (0,1]: 1.62e-09 */
+/* max abs error of the fit on (0,1]: 4.96e-08 */
 FJ_HD inline float fitm_M0(float t) {
-  float p = -1.5624218298133712e-06f;
-  p = __builtin_fmaf(p, t, 2.4986901327827262e-05f);
-  p = __builtin_fmaf(p, t, -0.0009765082817761566f);
-  p = __builtin_fmaf(p, t, 0.9999999983557427f);
+  float p = 2.2643268583106974e-05f;
+  p = __builtin_fmaf(p, t, -0.0009756294194968865f);
+  p = __builtin_fmaf(p, t, 0.9999999495300606f);
   return p;
 }
 /* max abs error of the fit on (0,1]: 2.75e-09 */
@@ -184,12 +183,11 @@ FJ_HD inline float fitm_ph0(float t) {
   p = __builtin_fmaf(p, t, -0.0024867931705683244f);
   return p;
 }
-/* max abs error of the fit on (0,1]: 2.21e-09 */
+/* max abs error of the fit on (0,1]: 7.57e-08 */
 FJ_HD inline float fitm_M1(float t) {
-  float p = 2.3870479130173432e-06f;
-  p = __builtin_fmaf(p, t, -4.681973077723062e-05f);
-  p = __builtin_fmaf(p, t, 0.002929613733974215f);
-  p = __builtin_fmaf(p, t, 1.00000000224122f);
+  float p = -4.323915890770389e-05f;
+  p = __builtin_fmaf(p, t, 0.0029282710195231427f);
+  p = __builtin_fmaf(p, t, 1.0000000768364674f);
   return p;
 }
 /* max abs error of the fit on (0,1]: 4.11e-09 */
@@ -199,36 +197,32 @@ FJ_HD inline float fitm_ph1(float t) {
   p = __builtin_fmaf(p, t, 0.00746038375925628f);
   return p;
 }
-/* max abs error of the fit on (0,1]: 1.69e-09 */
+/* max abs error of the fit on (0,1]: 5.25e-08 */
 FJ_HD inline float fitm_P0(float t) {
-  float p = -1.6545902499551725e-06f;
-  p = __builtin_fmaf(p, t, 2.708336367553825e-05f);
-  p = __builtin_fmaf(p, t, -0.0010985762789425106f);
-  p = __builtin_fmaf(p, t, 0.9999999982850651f);
+  float p = 2.4601478300604937e-05f;
+  p = __builtin_fmaf(p, t, -0.00109764557192691f);
+  p = __builtin_fmaf(p, t, 0.9999999465791198f);
   return p;
 }
-/* max abs error of the fit on (0,1]: 7.58e-10 */
+/* max abs error of the fit on (0,1]: 1.83e-08 */
 FJ_HD inline float fitm_q0(float t) {
-  float p = 5.751186319100864e-07f;
-  p = __builtin_fmaf(p, t, -6.7942766767205345e-06f);
-  p = __builtin_fmaf(p, t, 0.000143025501811536f);
-  p = __builtin_fmaf(p, t, -0.015624999227061483f);
+  float p = -5.931598728855359e-06f;
+  p = __builtin_fmaf(p, t, 0.0001427019975810866f);
+  p = __builtin_fmaf(p, t, -0.015624981254604237f);
   return p;
 }
-/* max abs error of the fit on (0,1]: 1.94e-09 */
+/* max abs error of the fit on (0,1]: 6.27e-08 */
 FJ_HD inline float fitm_P1(float t) {
-  float p = 1.976842868909807e-06f;
-  p = __builtin_fmaf(p, t, -3.486353870672794e-05f);
-  p = __builtin_fmaf(p, t, 0.0018309899146891602f);
-  p = __builtin_fmaf(p, t, 1.000000001966288f);
+  float p = -3.1898274403362295e-05f;
+  p = __builtin_fmaf(p, t, 0.0018298779405753972f);
+  p = __builtin_fmaf(p, t, 1.0000000637426278f);
   return p;
 }
-/* max abs error of the fit on (0,1]: 8.57e-10 */
+/* max abs error of the fit on (0,1]: 2.14e-08 */
 FJ_HD inline float fitm_q1(float t) {
-  float p = -6.707130042021766e-07f;
-  p = __builtin_fmaf(p, t, 8.316961712685902e-06f);
-  p = __builtin_fmaf(p, t, -0.0002002426392543822f);
-  p = __builtin_fmaf(p, t, 0.04687499912633409f);
+  float p = 7.310892206382535e-06f;
+  p = __builtin_fmaf(p, t, -0.00019986536318951833f);
+  p = __builtin_fmaf(p, t, 0.04687497816655271f);
   return p;
 }
 

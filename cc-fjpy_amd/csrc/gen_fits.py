@@ -23,8 +23,9 @@ of a sine and a cosine):
   fitf_M0(t) = M_0,  fitf_ph0(t) = phi_0 / (2 pi) * x/8  (degree 2 and 1),
   fitf_M1, fitf_ph1 likewise for n = 1.
 From x = 8 on (t in (0, 1]), for the F-J kernels' far stations: fitm_M0 ...
-fitm_ph1 (degree 3 and 2) and fitm_P0 ... fitm_q1 (degree 3), errors about
-2e-9.
+fitm_ph1 and fitm_P0 ... fitm_q1, all degree 2: errors up to 8e-8 in the
+moduli and P, 2e-8 in Q/z and 4e-9 revolutions x 8/x in the phases, below
+the float rounding of the terms they enter (degree 3 is about 2e-9).
 Small argument, x < 8, variable y = (x/8)^2 in [0,1), evaluated in double
 (the monomial coefficients alternate in sign up to ~1e2, too much
 cancellation for float):
@@ -95,8 +96,8 @@ FAR = [("fitf_P0", 2), ("fitf_q0", 2), ("fitf_P1", 2), ("fitf_q1", 2),
        ("fitf_R0", 3), ("fitf_R1", 3)]
 FAR_MP = [("fitf_M0", 2), ("fitf_ph0", 1), ("fitf_M1", 2), ("fitf_ph1", 1)]
 # the F-J lane's far forms from x = 8 on (t in (0, 1]): modulus-phase and P, Q
-MID_MP = [("fitm_M0", 3), ("fitm_ph0", 2), ("fitm_M1", 3), ("fitm_ph1", 2)]
-MID_PQ = [("fitm_P0", 3), ("fitm_q0", 3), ("fitm_P1", 3), ("fitm_q1", 3)]
+MID_MP = [("fitm_M0", 2), ("fitm_ph0", 2), ("fitm_M1", 2), ("fitm_ph1", 2)]
+MID_PQ = [("fitm_P0", 2), ("fitm_q0", 2), ("fitm_P1", 2), ("fitm_q1", 2)]
 SMALL = [("fit_J0s", 11), ("fit_J1s", 10), ("fit_Y0s", 11), ("fit_Y1s", 10),
          ("fit_GJs", 10), ("fit_GYs", 11)]
 
