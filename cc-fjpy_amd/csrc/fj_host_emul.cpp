@@ -99,8 +99,18 @@ extern "C" double fjemul_special(int which, double x) {
   const LaneK L = lane_consts(x / fjm::kTwoPi);
   const float xf = (float)(L.q2pi * 1.0);
   float j0, j1, y0, y1, gj = 0.f, gy = 0.f;
-  if (xf >= fjm::kSplit) {
-    large_station<true, true>(L, 1.0, 1.0f, 1.0f, xf, true, &j0, &j1, &y0, &y1, &gj, &gy);
+  if (xf >= kLaneFar) {
+    // the far stations' forms: modulus-phase for the J-only bases, P and Q
+    // for the bases with Y; G from the Struve fits (linseg_station)
+    const float z = L.zk, t = z * z, amp = L.ak;
+    const FarUnits wj = far_units<false>(1.0, L, z), wy = far_units<true>(1.0, L, z);
+    j0 = amp * wj.u0;
+    j1 = amp * wj.u1;
+    y0 = amp * wy.v0;
+    y1 = amp * wy.v1;
+    const float r0 = fit_R0(t), xr1 = xf * fit_R1(t);
+    gj = __builtin_fmaf(r0, j1, __builtin_fmaf(-xr1, j0, 1.0f));
+    gy = __builtin_fmaf(r0, y1, -xr1 * y0);
   } else {
     double y, Lg;
     const Bess b = fjm::bessel_small<true, true>(x, &y, &Lg);

@@ -117,8 +117,9 @@ FJ_HD inline void station_entry(const float* r, int nr, int s, double* rr, float
 }
 
 // Far argument in the F-J lanes: x >= 8, the small/large split itself.  Far
-// stations take the modulus-phase (or P, Q) forms with the fitm_ fits, valid
-// on t = (8/x)^2 in (0, 1] to about 2e-9; below 8 the double-precision
+// stations take the modulus-phase (or P, Q) forms with the degree-2 fitm_
+// fits on t = (8/x)^2 in (0, 1] (absolute errors at most 7.6e-8, about the
+// float rounding of the terms they enter); below 8 the double-precision
 // small-argument polynomials.  (The far forms once started at x = 20, with
 // degree-2 fits and a third form of degree-4 fits between 8 and 20: 5% of
 // the dense array's trapezoid blocks then held a station below 20 for some
@@ -659,53 +660,6 @@ FJ_HD inline float gdiff_y2(const Ends& a, float xa, const Ends& b, float xb, fl
   const float Ga = a.sh != 0.0f ? a.G : __builtin_fmaf(kTwoOverPi, xa, a.G);
   const float Gb = b.sh != 0.0f ? b.G : __builtin_fmaf(kTwoOverPi, xb, b.G);
   return (Gb - Ga) * inv2;
-}
-
-// x >= 8 at station (rs, ir = 1/r, isr = 1/sqrt r) for lane L: J0, J1,
-// (NY) Y0, Y1, and (need_g) G_J = 1 + R0 J1 - x R1 J0, G_Y = R0 Y1 - x R1 Y0.
-// Degree-2/3 fits from x = 20 on, degree 4-7 below; FAR_ONLY: the caller
-// knows x >= 20 on every lane.
-template <bool NJ, bool NY, bool FAR_ONLY = false>
-FJ_HD inline void large_station(const LaneK& L, double rs, float ir, float isr, float x,
-                                bool need_g, float* j0, float* j1, float* y0, float* y1,
-                                float* GJ, float* GY) {
-  const float z = L.zk * ir, t = z * z;
-  const float amp = L.ak * isr, az = amp * z;
-  const bool far = FAR_ONLY || x >= kFarX;
-  float P0, Q0, P1, Q1;
-  if (far) {
-    P0 = fitf_P0(t);
-    Q0 = fitf_q0(t);
-    P1 = fitf_P1(t);
-    Q1 = fitf_q1(t);
-  } else {
-    P0 = fit_P0(t);
-    Q0 = fit_q0(t);
-    P1 = fit_P1(t);
-    Q1 = fit_q1(t);
-  }
-  float sn, co;
-  sincos_rev(phase_rev(L.q, rs), &sn, &co);
-  const float A = amp * P0, B = az * Q0, C = amp * P1, D = az * Q1;
-  *j0 = __builtin_fmaf(A, co, -B * sn);
-  *j1 = __builtin_fmaf(C, sn, D * co);
-  if (NY) {
-    *y0 = __builtin_fmaf(A, sn, B * co);
-    *y1 = __builtin_fmaf(D, sn, -C * co);
-  }
-  if (need_g) {
-    float r0, r1;
-    if (far) {
-      r0 = fitf_R0(t);
-      r1 = fitf_R1(t);
-    } else {
-      r0 = fit_R0(t);
-      r1 = fit_R1(t);
-    }
-    const float xr1 = x * r1;
-    if (NJ) *GJ = __builtin_fmaf(r0, *j1, __builtin_fmaf(-xr1, *j0, 1.0f));
-    if (NY) *GY = __builtin_fmaf(r0, *y1, -xr1 * *y0);
-  }
 }
 
 // ---- far stations: the hat form --------------------------------------------

@@ -123,36 +123,6 @@ struct Bess {
   float gy;  // x Y1(x) + 2/pi, formed without cancellation near x = 0
 };
 
-// x >= 8.  u = reduced revolutions of (x - pi/4).  Shares one sin/cos pair:
-// th1 = th0 - pi/2, so cos th1 = sin th0 and sin th1 = -cos th0.
-// Also returns t = (8/x)^2 for the Struve fits.
-template <bool NEED_J, bool NEED_Y>
-FJ_HD inline Bess bessel_large(float x, float u, float* t_out) {
-  float rs = rsqrt_f(x);
-  float w = rs * rs;         // 1/x
-  float z = 8.0f * w;        // 8/x
-  float t = z * z;
-  float amp = kSqrt2OverPi * rs;
-  float s, c;
-  sincos_rev(u, &s, &c);
-  float p0 = fit_P0(t), q0 = z * fit_q0(t);
-  float p1 = fit_P1(t), q1 = z * fit_q1(t);
-  Bess b;
-  b.j0 = b.j1 = b.y0 = b.y1 = 0.0f;
-  if (NEED_J) {
-    b.j0 = amp * __builtin_fmaf(p0, c, -q0 * s);
-    b.j1 = amp * __builtin_fmaf(p1, s, q1 * c);
-  }
-  b.gy = 0.0f;
-  if (NEED_Y) {
-    b.y0 = amp * __builtin_fmaf(p0, s, q0 * c);
-    b.y1 = amp * __builtin_fmaf(q1, s, -p1 * c);
-    b.gy = __builtin_fmaf(x, b.y1, kTwoOverPi);
-  }
-  *t_out = t;
-  return b;
-}
-
 // order-0 only (trapezoid kernels)
 template <bool NEED_J, bool NEED_Y>
 FJ_HD inline void bessel0_large(float x, float u, float* j0, float* y0) {
@@ -194,16 +164,6 @@ FJ_HD inline Bess bessel_small(double x, double* y_out, double* lnterm_out) {
   *y_out = y;
   *lnterm_out = L;
   return b;
-}
-
-// G(x) = int_0^x t Z1(t) dt, large x (Wronskian form, x >= 8):
-//   G_J = 1 + R0 J1 - x R1 J0,   G_Y = R0 Y1 - x R1 Y0
-FJ_HD inline void g_large(float x, float t, const Bess& b, float* gj, float* gy,
-                          bool need_j, bool need_y) {
-  float r0 = fit_R0(t), r1 = fit_R1(t);
-  float xr1 = x * r1;
-  if (need_j) *gj = 1.0f + __builtin_fmaf(r0, b.j1, -xr1 * b.j0);
-  if (need_y) *gy = __builtin_fmaf(r0, b.y1, -xr1 * b.y0);
 }
 
 // G(x), small x (x < 8), from the double polynomials.  For the Y basis it
