@@ -170,7 +170,7 @@ int ccfj_batch_create(ccfj_ctx* ctx, int mode, int itype, int normalize, int nb,
   std::stable_sort(horder.begin(), horder.end(), [&](int a, int b) { return hnrp[a] > hnrp[b]; });
 
   int rc = CCFJ_OK;
-  void *rr, *hh, *ir, *isr, *bw, *coef, *counter, *T, *m;
+  void *rr, *hh, *ir, *isr, *bw, *dr, *coef, *counter, *T, *m;
   if (!rc) rc = upload(p, hsoff, &p->d.soff);
   if (!rc) rc = upload(p, hpoff, &p->d.poff);
   if (!rc) rc = upload(p, hcoff, &p->d.coff);
@@ -181,6 +181,7 @@ int ccfj_batch_create(ccfj_ctx* ctx, int mode, int itype, int normalize, int nb,
   if (!rc) rc = alloc(p, sizeof(float) * (pacc + 1), &ir);
   if (!rc) rc = alloc(p, sizeof(float) * (pacc + 1), &isr);
   if (!rc) rc = alloc(p, sizeof(uint32_t) * (pacc + 1), &bw);
+  if (!rc) rc = alloc(p, sizeof(float) * (pacc + 1), &dr);
   if (!rc) rc = alloc(p, sizeof(float) * (cacc + 1), &coef);
   if (!rc) rc = alloc(p, 64, &counter);
   if (!rc) rc = alloc(p, sizeof(float) * (size_t)nb * nf * nc, &T);
@@ -197,6 +198,7 @@ int ccfj_batch_create(ccfj_ctx* ctx, int mode, int itype, int normalize, int nb,
   p->d.ir = (float*)ir;
   p->d.isr = (float*)isr;
   p->d.bw = (uint32_t*)bw;
+  p->d.dr = (float*)dr;
   p->d.coef = (float*)coef;
   p->d.counter = (unsigned int*)counter;
   p->d.T = (float*)T;

@@ -65,7 +65,8 @@ bool injected_alloc_failure() {
   return fail_at > 0 && n == fail_at;
 }
 
-enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_IR, S_ISR, S_BW, S_COEF, S_T, S_M, S_X0,
+enum Slot { S_U0, S_U1, S_R, S_F, S_C, S_OUT, S_RR, S_HH, S_IR, S_ISR, S_BW, S_COEF, S_T, S_M, S_DR,
+            S_X0,
             S_COUNT = S_X0 + ccfj_internal::kScratchSlots };
 
 }  // namespace
@@ -139,8 +140,9 @@ struct Work {
   float* coef = nullptr;
   float* T = nullptr;
   float* m = nullptr;
+  float* dr = nullptr;
 };
-constexpr int kWork = 8;
+constexpr int kWork = 9;
 
 // Bytes of each work array, in Work's order (256-byte multiples).
 void work_sizes(int mode, int itype, int normalize, int nr, int nf, int nc, size_t sz[kWork]) {
@@ -152,6 +154,7 @@ void work_sizes(int mode, int itype, int normalize, int nr, int nf, int nc, size
   sz[5] = seg ? up(sizeof(float) * fjk::coef_floats(mode, itype, nrp, nf)) : 0;
   sz[6] = up(sizeof(float) * (size_t)nf * nc);
   sz[7] = normalize ? up(sizeof(float) * (size_t)nf) : 0;
+  sz[8] = seg ? up(sizeof(float) * (nrp + 8)) : 0;
 }
 
 Work work_from(void* const ptr[kWork]) {
@@ -164,6 +167,7 @@ Work work_from(void* const ptr[kWork]) {
   w.coef = (float*)ptr[5];
   w.T = (float*)ptr[6];
   w.m = (float*)ptr[7];
+  w.dr = (float*)ptr[8];
   return w;
 }
 
@@ -197,6 +201,7 @@ int enqueue_transform(ccfj_ctx* ctx, int mode, int itype, int normalize, const f
     p.ir = w.ir;
     p.isr = w.isr;
     p.bw = w.bw;
+    p.dr = w.dr;
     p.coef = w.coef;
     p.T = w.T;
     e = fjk::launch_prep(mode, itype, p, st);
@@ -219,7 +224,7 @@ int run_device(ccfj_ctx* ctx, int mode, int itype, int normalize, const float* U
   if (nf == 0 || nc == 0) return CCFJ_OK;
   size_t sz[kWork];
   work_sizes(mode, itype, normalize, nr, nf, nc, sz);
-  const Slot slots[kWork] = {S_RR, S_HH, S_IR, S_ISR, S_BW, S_COEF, S_T, S_M};
+  const Slot slots[kWork] = {S_RR, S_HH, S_IR, S_ISR, S_BW, S_COEF, S_T, S_M, S_DR};
   void* ptr[kWork] = {};
   for (int i = 0; i < kWork; i++) {
     if (!sz[i]) continue;
@@ -568,6 +573,7 @@ int ccfj_transform_items(ccfj_ctx* ctx, int mode, int itype, const float* U0, co
   if ((rc = ensure(ctx, S_IR, sizeof(float) * (nrp + 8)))) return rc;
   if ((rc = ensure(ctx, S_ISR, sizeof(float) * (nrp + 8)))) return rc;
   if ((rc = ensure(ctx, S_BW, sizeof(uint32_t) * (nrp + 8)))) return rc;
+  if ((rc = ensure(ctx, S_DR, sizeof(float) * (nrp + 8)))) return rc;
   if ((rc = ensure(ctx, S_COEF, sizeof(float) * fjk::coef_floats(mode, itype, nrp, nf)))) return rc;
   fjk::Problem p;
   p.U0 = U0;
@@ -584,6 +590,7 @@ int ccfj_transform_items(ccfj_ctx* ctx, int mode, int itype, const float* U0, co
   p.ir = (float*)ctx->buf[S_IR];
   p.isr = (float*)ctx->buf[S_ISR];
   p.bw = (uint32_t*)ctx->buf[S_BW];
+  p.dr = (float*)ctx->buf[S_DR];
   p.coef = (float*)ctx->buf[S_COEF];
   p.T = nullptr;   // the item kernel writes out[c][f] directly
   hipError_t e = fjk::launch_prep(mode, itype, p, st);

@@ -26,6 +26,7 @@ struct BatchDev {
   float* ir;
   float* isr;
   uint32_t* bw;          // block words (fjl::station_block_word), at poff like rr
+  float* dr;             // offsets from the block anchors (fjl::station_dr), at poff
   float* coef;
   float* T;              // [nb][nf][nc] persistent-kernel results
   float* m;              // [nb][nf] column scales (normalisation)

@@ -20,7 +20,7 @@
 // Layout (device):
 //   U0/U1  f32 [sum nr_b][nf]        rows of all sub-arrays, each sorted by r
 //   r      f32 [sum nr_b]
-//   rr, hh, ir, isr, bw  per sub-array b at poff[b], nrp_b + 8 entries each (padded)
+//   rr, hh, ir, isr, bw, dr  per sub-array b at poff[b], nrp_b + 8 entries each (padded)
 //   coef   per sub-array b at coff[b]: nf frequencies of nrp_b records (fj_kernels.hip)
 //   T      f32 [nb][nf][nc]     persistent-kernel results (coalesced along c)
 //   out    f32 [nb][nc][nf]     formed from T by k_finish (fj_kernels.hip)
@@ -43,7 +43,7 @@ __global__ void k_stations_b(BatchDev d) {
   const int ntot = d.nrp[b] + kUnroll;
   if (s >= ntot || nr <= 0) return;
   const int64_t o = d.poff[b];
-  station_entry(d.r + s0, nr, s, d.rr + o, d.hh + o, d.ir + o, d.isr + o);
+  station_entry(d.r + s0, nr, s, d.rr + o, d.hh + o, d.ir + o, d.isr + o, d.dr + o);
   d.bw[o + s] = station_block_word(d.r + s0, nr, s);
 }
 
@@ -78,7 +78,8 @@ __attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj_b(
     const int* __restrict__ nrpa, const int* __restrict__ order, const float* __restrict__ fv,
     const float* __restrict__ cv, const double* __restrict__ rr, const float* __restrict__ hh,
     const float* __restrict__ ir, const float* __restrict__ isr, const uint32_t* __restrict__ bw,
-    const float* __restrict__ coef, float* __restrict__ T, unsigned int* __restrict__ counter) {
+    const float* __restrict__ dr, const float* __restrict__ coef, float* __restrict__ T,
+    unsigned int* __restrict__ counter) {
   __shared__ unsigned int item_s;
   const unsigned int ncb = (unsigned int)((nc + TILE - 1) / TILE);
   while (true) {
@@ -99,7 +100,7 @@ __attribute__((amdgpu_waves_per_eu(FjOcc<MODE, ITYPE>::waves, 8))) void k_fj_b(
       const double q = (double)fv[fi] / (double)cv[j < nc ? j : nc - 1];
       const float* __restrict__ cf = coef + coff[b] + (size_t)fi * coef_block(nrp, MODE, ITYPE);
       const int64_t o = poff[b];
-      const Stations S{rr + o, hh + o, ir + o, isr + o, bw + o};
+      const Stations S{rr + o, hh + o, ir + o, isr + o, bw + o, dr + o};
       float v;
       if (ITYPE == 0)
         v = trapz_lane<MODE>(S, cf, nrp, q);
@@ -170,13 +171,14 @@ hipError_t launch_batched(const void* kern, int tile, int grid, const BatchDev& 
   int nf = d.nf, nc = d.nc;
   const int64_t *soff = d.soff, *poff = d.poff, *coff = d.coff;
   const int *nrp = d.nrp, *order = d.order;
-  const float *f = d.f, *c = d.c, *hh = d.hh, *ir = d.ir, *isr = d.isr, *coef = d.coef;
+  const float *f = d.f, *c = d.c, *hh = d.hh, *ir = d.ir, *isr = d.isr, *dr = d.dr,
+              *coef = d.coef;
   const double* rr = d.rr;
   const uint32_t* bw = d.bw;
   float* T = d.T;
   unsigned int* counter = d.counter;
   void* args[] = {&nf, &nc, &limit, &soff, &poff, &coff, &nrp, &order, &f, &c,
-                  &rr, &hh, &ir, &isr, &bw, &coef, &T, &counter};
+                  &rr, &hh, &ir, &isr, &bw, &dr, &coef, &T, &counter};
   return hipLaunchKernel(kern, dim3(grid), dim3(tile), args, 0, st);
 }
 

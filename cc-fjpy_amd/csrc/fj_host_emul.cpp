@@ -26,13 +26,13 @@ void run_mode(int itype, const float* U0, const float* U1, const float* r, const
   // station arrays padded as on the device
   const int ntot = nrp + kUnroll;
   std::vector<double> rr(ntot);
-  std::vector<float> hh(ntot), ir(ntot), isr(ntot);
+  std::vector<float> hh(ntot), ir(ntot), isr(ntot), dr(ntot);
   std::vector<uint32_t> bw(ntot);
   for (int s = 0; s < ntot; s++) {
-    station_entry(r, nr, s, rr.data(), hh.data(), ir.data(), isr.data());
+    station_entry(r, nr, s, rr.data(), hh.data(), ir.data(), isr.data(), dr.data());
     bw[s] = station_block_word(r, nr, s);
   }
-  const Stations S{rr.data(), hh.data(), ir.data(), isr.data(), bw.data()};
+  const Stations S{rr.data(), hh.data(), ir.data(), isr.data(), bw.data(), dr.data()};
   const size_t block = coef_block(nrp, MODE, itype);
   std::vector<float> cf(block, 0.0f);
   float cmin = INFINITY;
@@ -40,7 +40,8 @@ void run_mode(int itype, const float* U0, const float* U1, const float* r, const
 #pragma omp parallel for schedule(dynamic, 1) firstprivate(cf)
   for (int i = 0; i < nf; i++) {
     const float hlim = itype == 1 ? long_segment_limit(f[i], cmin) : 0.0f;
-    if (itype == 1) cf[(size_t)nrp * NC] = hlim;
+    cf[coef_header(nrp, MODE, itype)] = hlim;
+    cf[coef_header(nrp, MODE, itype) + 1] = anchor_limit(f[i], cmin);
     for (int s = 0; s < nr; s++) {
       float gp[2] = {0.f, 0.f}, gc[2] = {0.f, 0.f}, gn[2] = {0.f, 0.f}, rec[12];
       for (int m = 0; m < NS; m++) {
@@ -103,7 +104,8 @@ extern "C" double fjemul_special(int which, double x) {
     // the far stations' forms: modulus-phase for the J-only bases, P and Q
     // for the bases with Y; G from the Struve fits (linseg_station)
     const float z = L.zk, t = z * z, amp = L.ak;
-    const FarUnits wj = far_units<false>(1.0, L, z), wy = far_units<true>(1.0, L, z);
+    const float ph = phase_rev(L.q, 1.0);
+    const FarUnits wj = far_units<false>(ph, z), wy = far_units<true>(ph, z);
     j0 = amp * wj.u0;
     j1 = amp * wj.u1;
     y0 = amp * wy.v0;

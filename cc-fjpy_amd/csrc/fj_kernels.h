@@ -22,6 +22,7 @@ struct Problem {
   float* ir;     // [nrp + 8] workspace, 1 / r
   float* isr;    // [nrp + 8] workspace, 1 / sqrt(r)
   uint32_t* bw;  // [nrp + 8] workspace, block words (fjl::station_block_word)
+  float* dr;     // [nrp + 8] workspace, offsets from the block anchors (fjl::station_dr)
   float* coef;   // coef_floats(...) workspace
   float* T;      // [nf][nc] main-kernel results (k_finish forms [nc][nf])
 };

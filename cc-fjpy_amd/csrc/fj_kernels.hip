@@ -9,14 +9,18 @@
 //   rr[nrp + 8]       double   station distances, ascending
 //   hh[nrp + 8]       float    r_s - r_{s-1} (hh[0] = 0)
 //   ir, isr[nrp + 8]  float    1 / r_s, 1 / sqrt(r_s)
-//   bw[nrp + 8]       uint32   block words of the far / smooth tests
-//                              (fjl::station_block_word)
+//   bw[nrp + 8]       uint32   block words of the far / smooth / anchor
+//                              tests (fjl::station_block_word)
+//   dr[nrp + 8]       float    r_s - r_a, a the first station of s's block
+//                              of 4 (fjl::station_dr)
 //   coef              float    per frequency, fjl::coef_block floats:
 //       itype 0: [nrp][nspec] A_s / sqrt(r_s) of each spectrum (J - Y: in
 //                polar form, fjl::to_polar)
-//       itype 1: [nrp][3 nspec] (a_s, p_s, t_s) per spectrum, then
-//                [nrp][nspec] b_s, then the long-segment limit
+//       itype 1: [nrp][4 nspec] hat records per spectrum, then
+//                [nrp][nspec] b_s and [nspec][nrp] kappa
 //                (fjl::station_coef, fjl::coef_store)
+//       then an 8-float header: the long-segment limit (itype 1) and the
+//       anchor limit (fjl::coef_header)
 //   T[nf][nc]         float    main-kernel results, one row per frequency
 //   out[nc][nf]       float    (the reference's layout, src/FJcpu.cpp:247),
 //                              formed from T by k_finish (optionally
@@ -48,10 +52,11 @@ using namespace fjl;
 // distance and zero gaps
 __global__ void k_stations(const float* __restrict__ r, int nr, int ntot, double* __restrict__ rr,
                            float* __restrict__ hh, float* __restrict__ ir,
-                           float* __restrict__ isr, uint32_t* __restrict__ bw) {
+                           float* __restrict__ isr, uint32_t* __restrict__ bw,
+                           float* __restrict__ dr) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < ntot) {
-    station_entry(r, nr, s, rr, hh, ir, isr);
+    station_entry(r, nr, s, rr, hh, ir, isr, dr);
     bw[s] = station_block_word(r, nr, s);
   }
 }
@@ -127,7 +132,7 @@ __global__ void k_fill(float* __restrict__ p, size_t n, float v) {
 }
 
 // ---------------------------------------------------------------------------
-Stations stations(const Problem& p) { return Stations{p.rr, p.hh, p.ir, p.isr, p.bw}; }
+Stations stations(const Problem& p) { return Stations{p.rr, p.hh, p.ir, p.isr, p.bw, p.dr}; }
 
 template <int MODE>
 static hipError_t launch_main(int itype, const Problem& p, hipStream_t st) {
@@ -154,7 +159,7 @@ int padded(int nr) { return padded_stations(nr); }
 hipError_t launch_prep(int mode, int itype, const Problem& p, hipStream_t st) {
   const int ntot = p.nrp + kUnroll;
   hipLaunchKernelGGL(k_stations, dim3((ntot + 255) / 256), dim3(256), 0, st, p.r, p.nr, ntot,
-                     p.rr, p.hh, p.ir, p.isr, p.bw);
+                     p.rr, p.hh, p.ir, p.isr, p.bw, p.dr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const bool two = !(mode == MODE_J || mode == MODE_Y);

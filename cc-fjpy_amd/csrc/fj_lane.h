@@ -81,6 +81,7 @@ struct Stations {
   const float* ir;   // 1 / r_s      (inf at r = 0, where only x < 8 code runs)
   const float* isr;  // 1 / sqrt(r_s)
   const uint32_t* bw;  // block words (station_block_word)
+  const float* dr;   // r_s - r_a, a = s rounded down to a multiple of 4 (station_dr)
 };
 
 // 1/r_s and the gap left of station s as the station arrays hold them
@@ -106,14 +107,23 @@ FJ_HD inline float station_isr(float r) {
 // the other blocks, so both give the same float.
 FJ_HD inline float kappa_of(float ah, float isr) { return (ah * isr) * kTwoOverPi; }
 
+// Offset of station s from the anchor of its block of 4, the block's first
+// station a: r_s - r_a, formed in double and rounded once (exact when r_a
+// and r_s are within a factor of 2).
+FJ_HD inline float station_dr(const float* r, int nr, int s) {
+  const int a = s & ~3;
+  return (float)((double)r[s < nr ? s : nr - 1] - (double)r[a < nr ? a : nr - 1]);
+}
+
 // Entry s of the station arrays from the sorted float distances r[0..nr).
 FJ_HD inline void station_entry(const float* r, int nr, int s, double* rr, float* hh, float* ir,
-                                float* isr) {
+                                float* isr, float* dr) {
   const float rf = r[s < nr ? s : nr - 1];
   rr[s] = (double)rf;
   hh[s] = station_hh(r, nr, s);
   ir[s] = station_ir(r, nr, s);
   isr[s] = station_isr(rf);
+  dr[s] = station_dr(r, nr, s);
 }
 
 // Far argument in the F-J lanes: x >= 8, the small/large split itself.  Far
@@ -130,6 +140,7 @@ constexpr float kLaneFar = kSplit;
 // Per-lane constants of one (f, c).
 struct LaneK {
   double q;      // f / c, cycles per metre
+  float qf;      // q in float (anchored block phases, block_phases)
   double q2pi;   // k = 2 pi f / c
   float kf;      // k in float
   float zk;      // 8 / k           (z = 8/x = zk / r)
@@ -142,6 +153,7 @@ FJ_HD inline uint32_t f32_bits(float v);
 FJ_HD inline LaneK lane_consts(double q) {
   LaneK L;
   L.q = q;
+  L.qf = (float)q;
   L.q2pi = q * kTwoPi;
   L.kf = (float)L.q2pi;
   L.zk = (float)(8.0 / L.q2pi);
@@ -181,13 +193,15 @@ FJ_HD inline uint32_t far_threshold(double q2pi) {
 #endif
 }
 
-// Block words, one uint32 per station slot, so that a block's far and
-// smooth tests are one wave-uniform load and one compare (the bits of
-// non-negative floats order as unsigned integers):
+// Block words, one uint32 per station slot, so that a block's far, smooth
+// and anchor tests are one wave-uniform load and one compare each (the bits
+// of non-negative floats order as unsigned integers):
 //   bw[sb]     (sb % 4 == 0)  max bits of 1/r over stations sb .. sb+4
 //                             (a block of 4 and the next block's first)
 //   bw[sb + 1]                max bits of the gaps hh[sb .. sb+4]
 //   bw[sb + 2] (sb % 8 == 0)  max bits of 1/r over stations sb .. sb+7
+//   bw[sb + 3]                max bits of |dr| over stations sb .. sb+3
+//   bw[sb + 6] (sb % 8 == 0)  max bits of |dr| over stations sb .. sb+7
 // other slots 0.  Built from r with the station arrays' own expressions.
 // (Measured on the MI355X, the trapezoid J kernel at 19,900 pairs x 4096 f
 // x 3000 c: 107.3 ms against 108.6 ms with the maximum taken per block.)
@@ -207,6 +221,12 @@ FJ_HD inline uint32_t station_block_word(const float* r, int nr, int s) {
   } else if (k == 2) {
     for (int i = 0; i < 8; i++) {
       const uint32_t b = f32_bits(station_ir(r, nr, s - 2 + i));
+      m = b > m ? b : m;
+    }
+  } else if ((k & 3) == 3 || k == 6) {
+    const int n = k == 6 ? 8 : 4;
+    for (int i = 0; i < n; i++) {
+      const uint32_t b = f32_bits(fabsf(station_dr(r, nr, (s & ~(n - 1)) + i)));
       m = b > m ? b : m;
     }
   }
@@ -273,11 +293,15 @@ template <> struct ModeT<MODE_ABS_H> { static constexpr int nspec = 2; static co
 FJ_HD constexpr int coef_count(int mode, int itype) {
   return ((mode == MODE_J || mode == MODE_Y) ? 1 : 2) * (itype == 0 ? 1 : 6);
 }
-// Floats of the coefficients of one frequency: nrp records, and for itype 1
-// the b and kappa planes and a header of 8 floats whose first is that
-// frequency's long-segment threshold (long_segment_limit).
+// Floats of the coefficients of one frequency: nrp records (for itype 1
+// with the b and kappa planes) and a header of 8 floats at coef_header:
+// [0] the long-segment limit (long_segment_limit; itype 1), [1] the anchor
+// limit (anchor_limit).
+FJ_HD constexpr size_t coef_header(int nrp, int mode, int itype) {
+  return (size_t)nrp * coef_count(mode, itype);
+}
 FJ_HD constexpr size_t coef_block(int nrp, int mode, int itype) {
-  return (size_t)nrp * coef_count(mode, itype) + (itype == 0 ? 0 : 8);
+  return coef_header(nrp, mode, itype) + 8;
 }
 // itype 1 offsets inside a frequency's coefficients (ns spectra): the b
 // plane [nrp][ns], the kappa plane [ns][nrp], the header.
@@ -300,6 +324,26 @@ constexpr float kLongDx = 2.0f;
 FJ_HD inline float long_segment_limit(float f, float cmin) {
   const double k = kTwoPi * fabs((double)f) / (double)cmin;
   return k > 0.0 ? (float)((double)kLongDx / k) : INFINITY;
+}
+
+// Anchored phases (block_phases): a block of 4 stations whose offsets from
+// its anchor satisfy |q dr| <= kAnchorRev for every velocity of the grid,
+// i.e. max |dr| <= kAnchorRev min|c| / |f|, the anchor limit of frequency
+// f (+inf at f = 0).  It depends on the frequency and the grid only, so
+// every kernel that evaluates a term takes the same phase for it.
+constexpr float kAnchorRev = 1.0f;
+// Anchored phases are taken in the one-spectrum modes.  Measured on the
+// MI355X at 19,900 pairs x 4096 f x 3000 c: trapezoid J 82.0 -> 76.4 ms,
+// linear-segment J 177.3 -> 173.4 ms; the two-spectrum kernels hold more
+// station data per block, and the offsets spilled their scalar registers
+// (trapezoid J - Y 119 -> 183 ms, linear-segment J - Y 305 -> 321 ms).
+template <int MODE>
+struct Anchored {
+  static constexpr bool on = (MODE == MODE_J || MODE == MODE_Y);
+};
+FJ_HD inline float anchor_limit(float f, float cmin) {
+  const double af = fabs((double)f);
+  return af > 0.0 ? (float)((double)kAnchorRev * (double)cmin / af) : INFINITY;
 }
 
 // Occupancy floor of the F-J kernels (waves per SIMD; the compiler then
@@ -479,16 +523,17 @@ FJ_HD inline void trapz_add(const float* c, float vj, float vy, float& pR, float
   }
 }
 
-// x >= kLaneFar, z = 8/x: J0 / amp and Y0 / amp in the modulus-phase form
-// J0 = amp M0 cos(th + phi), Y0 = amp M0 sin(th + phi), one transcendental
-// per basis.  Far blocks and the per-lane path both use it, so a term does
-// not depend on which lanes share its wave.
+// x >= kLaneFar, z = 8/x, ph = the station's reduced phase (block_phases):
+// J0 / amp and Y0 / amp in the modulus-phase form J0 = amp M0 cos(th + phi),
+// Y0 = amp M0 sin(th + phi), one transcendental per basis.  Far blocks and
+// the per-lane path both use it, so a term does not depend on which lanes
+// share its wave.
 template <int MODE>
-FJ_HD inline void far_unit(double rs, const LaneK& L, float z, float& vj, float& vy) {
+FJ_HD inline void far_unit(float ph, float z, float& vj, float& vy) {
   using M = ModeT<MODE>;
   const float t = z * z;
   const float m0 = fitm_M0(t);
-  const float u = __builtin_fmaf(z, fitm_ph0(t), phase_rev(L.q, rs));
+  const float u = __builtin_fmaf(z, fitm_ph0(t), ph);
   if (M::J && M::Y) {
     float s, co;
     sincos_rev(u, &s, &co);
@@ -502,26 +547,30 @@ FJ_HD inline void far_unit(double rs, const LaneK& L, float z, float& vj, float&
 }
 
 // J - Y with polar records (c = R, psi): R M0 cos(u + psi), x >= kLaneFar.
-FJ_HD inline float far_polar(double rs, const LaneK& L, float z, float psi) {
+FJ_HD inline float far_polar(float ph, float z, float psi) {
   const float t = z * z;
-  const float u = __builtin_fmaf(z, fitm_ph0(t), phase_rev(L.q, rs) + psi);
+  const float u = __builtin_fmaf(z, fitm_ph0(t), ph + psi);
   return fitm_M0(t) * cos_rev(u);
 }
 
-template <int MODE>
-FJ_HD inline void trapz_station(double rs, float ir, const LaneK& L, const float* c, float& pR,
-                                float& pI) {
+// One station of the trapezoid sum: rs = r_s, ph = its reduced phase
+// (block_phases; OWN_PH: formed here, the same value, for far lanes only),
+// ir = 1/r_s, c = its record.
+template <int MODE, bool OWN_PH = false>
+FJ_HD inline void trapz_station(double rs, float ph, float ir, const LaneK& L, const float* c,
+                                float& pR, float& pI) {
   using M = ModeT<MODE>;
   float vj = 0.0f, vy = 0.0f;
   const float z = L.zk * ir;  // 8 / x; inf when k = 0 or r = 0
   // x >= kLaneFar: the lane's own far test, implied by the block's
   const bool far = f32_bits(ir) <= L.irf;
+  if (OWN_PH && far) ph = phase_rev(L.q, rs);
   if (MODE == MODE_J_MINUS_Y && far) {   // polar records
-    pR = __builtin_fmaf(c[0], far_polar(rs, L, z, c[1]), pR);
+    pR = __builtin_fmaf(c[0], far_polar(ph, z, c[1]), pR);
     return;
   }
   if (far) {
-    far_unit<MODE>(rs, L, z, vj, vy);
+    far_unit<MODE>(ph, z, vj, vy);
   } else {
     const double xd = L.q2pi * rs;
     double y, Lg;
@@ -542,38 +591,65 @@ FJ_HD inline void trapz_station(double rs, float ir, const LaneK& L, const float
 
 // x >= kLaneFar on every lane (a far block): no branch on the argument.
 template <int MODE>
-FJ_HD inline void trapz_far(double rs, float ir, const LaneK& L, const float* c, float& pR,
+FJ_HD inline void trapz_far(float ph, float ir, const LaneK& L, const float* c, float& pR,
                             float& pI) {
   if (MODE == MODE_J_MINUS_Y) {   // polar records
-    pR = __builtin_fmaf(c[0], far_polar(rs, L, L.zk * ir, c[1]), pR);
+    pR = __builtin_fmaf(c[0], far_polar(ph, L.zk * ir, c[1]), pR);
     return;
   }
   float vj = 0.0f, vy = 0.0f;
-  far_unit<MODE>(rs, L, L.zk * ir, vj, vy);
+  far_unit<MODE>(ph, L.zk * ir, vj, vy);
   trapz_add<MODE>(c, vj, vy, pR, pI);
 }
 
-// A far block of kUnroll stations evaluated stage by stage (all phases, then
-// all fits, then all cosines, then the sums in station order), so that each
-// stage offers kUnroll independent instructions to hide the latency of the
-// double-precision phase and the transcendental.  The same operations as
-// trapz_far per station, summed in the same order: bit-identical results.
-// Measured on the MI355X at 19,900 pairs x 4096 f x 3000 c: J 104.6 ms
-// against 107.3 ms station by station, J - Y 153.6 against 157.3 ms.
+// Reduced phases u_s = frac(q r_s - 1/8) (revolutions) of the NB stations
+// from sb.  Anchored blocks (ANCH: max |dr| <= the frequency's anchor
+// limit) reduce in double once per block of 4, at its first station a, and
+// add q dr_s in float: u_s = u_a + q_f dr_s with |q dr_s| <= kAnchorRev, so
+// that u_s is off by at most 2^-25 (u_a) + 2^-24 kAnchorRev (q_f) +
+// 2^-24 (the sum, below 2) = 1.5e-7 revolutions.  Other blocks reduce each
+// station in double (error 2^-25).  Either way one phase per station and
+// block, used by every kind of block and lane path alike.
+template <int NB>
+FJ_HD inline void direct_phases(const double* rv, const LaneK& L, float* u) {
+  double w[NB];
+  // in three stages of NB independent instructions (latency)
+#pragma unroll
+  for (int k = 0; k < NB; k++) w[k] = __builtin_fma(L.q, rv[k], -0.125);
+#pragma unroll
+  for (int k = 0; k < NB; k++) w[k] = fract_d(w[k]);
+#pragma unroll
+  for (int k = 0; k < NB; k++) u[k] = (float)w[k];
+}
+template <int NB>
+FJ_HD inline void block_phases(const Stations& S, int sb, const LaneK& L, bool anch, float* u) {
+  if (anch) {
+    float ua[NB / 4];
+#pragma unroll
+    for (int a = 0; a < NB / 4; a++) ua[a] = phase_rev(L.q, S.rr[sb + 4 * a]);
+#pragma unroll
+    for (int k = 0; k < NB; k++) u[k] = __builtin_fmaf(L.qf, S.dr[sb + k], ua[k >> 2]);
+  } else {
+    double rv[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) rv[k] = S.rr[sb + k];
+    direct_phases<NB>(rv, L, u);
+  }
+}
+
+// A far block of kUnroll stations evaluated stage by stage (all fits, then
+// all cosines, then the sums in station order), so that each stage offers
+// kUnroll independent instructions to hide the latency of the
+// transcendental.  The same operations as trapz_far per station, summed in
+// the same order: bit-identical results.  Measured on the MI355X at 19,900
+// pairs x 4096 f x 3000 c: J 104.6 ms against 107.3 ms station by station,
+// J - Y 153.6 against 157.3 ms.
 template <int MODE>
-FJ_HD inline void trapz_far_block(const double* rv, const float* iv, const LaneK& L,
+FJ_HD inline void trapz_far_block(const float* ph, const float* iv, const LaneK& L,
                                   const float* cv, float& pR, float& pI) {
   constexpr int NC = ModeT<MODE>::nspec;
   if (MODE == MODE_J || MODE == MODE_J_MINUS_Y) {
-    float ph[kUnroll], z[kUnroll], t[kUnroll], u[kUnroll], v[kUnroll];
-    double w[kUnroll];
-    // phase_rev in three stages of kUnroll independent instructions
-#pragma unroll
-    for (int k = 0; k < kUnroll; k++) w[k] = __builtin_fma(L.q, rv[k], -0.125);
-#pragma unroll
-    for (int k = 0; k < kUnroll; k++) w[k] = fract_d(w[k]);
-#pragma unroll
-    for (int k = 0; k < kUnroll; k++) ph[k] = (float)w[k];
+    float z[kUnroll], t[kUnroll], u[kUnroll], v[kUnroll];
 #pragma unroll
     for (int k = 0; k < kUnroll; k++) {
       z[k] = L.zk * iv[k];
@@ -590,7 +666,7 @@ FJ_HD inline void trapz_far_block(const double* rv, const float* iv, const LaneK
     for (int k = 0; k < kUnroll; k++) pR = __builtin_fmaf(cv[k * NC], v[k], pR);
   } else {
 #pragma unroll
-    for (int k = 0; k < kUnroll; k++) trapz_far<MODE>(rv[k], iv[k], L, cv + k * NC, pR, pI);
+    for (int k = 0; k < kUnroll; k++) trapz_far<MODE>(ph[k], iv[k], L, cv + k * NC, pR, pI);
   }
 }
 
@@ -599,26 +675,46 @@ FJ_HD inline float trapz_lane(const Stations& S, const float* __restrict__ cf, i
   constexpr int NC = ModeT<MODE>::nspec;
   const LaneK L = lane_consts(q);
   const uint32_t irfar = far_threshold(L.q2pi);
+  const uint32_t alim = f32_bits(cf[coef_header(nrp, MODE, 0) + 1]);
   double accR = 0.0, accI = 0.0;
   for (int s0 = 0; s0 < nrp; s0 += kPartial) {
     const int s1 = nrp < s0 + kPartial ? nrp : s0 + kPartial;
     float pR = 0.0f, pI = 0.0f;
     for (int sb = s0; sb < s1; sb += kUnroll) {
-      double rv[kUnroll];
       float iv[kUnroll];
       float cv[kUnroll * NC];
-#pragma unroll
-      for (int k = 0; k < kUnroll; k++) rv[k] = S.rr[sb + k];
 #pragma unroll
       for (int k = 0; k < kUnroll; k++) iv[k] = S.ir[sb + k];
 #pragma unroll
       for (int k = 0; k < kUnroll * NC; k++) cv[k] = cf[sb * NC + k];
-      if (S.bw[sb + 2] <= irfar) {   // every station of the block has x >= kLaneFar
-        trapz_far_block<MODE>(rv, iv, L, cv, pR, pI);
-      } else {
+      const bool far = S.bw[sb + 2] <= irfar;   // every station has x >= kLaneFar
+      if (Anchored<MODE>::on) {
+        float ph[kUnroll];
+        block_phases<kUnroll>(S, sb, L, S.bw[sb + 6] <= alim, ph);
+        if (far) {
+          trapz_far_block<MODE>(ph, iv, L, cv, pR, pI);
+        } else {
 #pragma unroll
-        for (int k = 0; k < kUnroll; k++)
-          trapz_station<MODE>(rv[k], iv[k], L, cv + k * NC, pR, pI);
+          for (int k = 0; k < kUnroll; k++)
+            trapz_station<MODE>(S.rr[sb + k], ph[k], iv[k], L, cv + k * NC, pR, pI);
+        }
+      } else {
+        // the other modes form the phases in the far path only (in a block
+        // that is not far, for the lanes that are), from the distances
+        // loaded with the block: fewer scalar registers held across the
+        // branch (J - Y 123 -> 119 ms)
+        double rv[kUnroll];
+#pragma unroll
+        for (int k = 0; k < kUnroll; k++) rv[k] = S.rr[sb + k];
+        if (far) {
+          float ph[kUnroll];
+          direct_phases<kUnroll>(rv, L, ph);
+          trapz_far_block<MODE>(ph, iv, L, cv, pR, pI);
+        } else {
+#pragma unroll
+          for (int k = 0; k < kUnroll; k++)
+            trapz_station<MODE, true>(rv[k], 0.0f, iv[k], L, cv + k * NC, pR, pI);
+        }
       }
     }
     accR += (double)pR;
@@ -686,9 +782,8 @@ struct FarUnits {
 // (U0 = M0 cos(th + phi0), U1 = M1 sin(th + phi1), th = x - pi/4).  With
 // Y: one sine/cosine pair of th through the P, Q fits serves all four.
 template <bool NY>
-FJ_HD inline FarUnits far_units(double rs, const LaneK& L, float z) {
+FJ_HD inline FarUnits far_units(float u, float z) {
   const float t = z * z;
-  const float u = phase_rev(L.q, rs);
   FarUnits w;
   if (!NY) {
     w.u0 = fitm_M0(t) * cos_rev(__builtin_fmaf(z, fitm_ph0(t), u));
@@ -771,7 +866,8 @@ FJ_HD inline void far_acc(const float* h, const FarUnits& w, LinState<MODE>& st)
   lin_acc<MODE>(h + 3 * NS, w.u0, w.v0, st.hqR, st.hqI);
 }
 
-// One station of the linear-segment sum.  dxL = k (r_s - r_{s-1}),
+// One station of the linear-segment sum.  ph = its reduced phase
+// (block_phases), dxL = k (r_s - r_{s-1}),
 // dxR = k (r_{s+1} - r_s) (0 past the last station), hR = r_{s+1} - r_s,
 // hlim = the bits of the frequency's long-segment limit, h = the station's
 // hat record, kap = its kappa record (smooth blocks; the others form it),
@@ -779,7 +875,7 @@ FJ_HD inline void far_acc(const float* h, const FarUnits& w, LinState<MODE>& st)
 // (with FAR): no segment at this station is long or takes the G
 // difference on any lane, so no per-station test is made.
 template <int MODE, bool FAR, bool SMOOTH = false>
-FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dxL, float dxR,
+FJ_HD inline void linseg_station(int s, double rs, float ph, float ir, float isr, float dxL, float dxR,
                                  float hR, uint32_t hlim, const LaneK& L, const float* h,
                                  const float* kap, const float* cb,
                                  LinState<MODE>& st) {
@@ -788,7 +884,7 @@ FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dx
   constexpr int NS = M::nspec;
   const float z = L.zk * ir;   // 8 / x; inf when k = 0 or r = 0
   if (SMOOTH) {
-    far_acc<MODE>(h, far_units<NY>(rs, L, z), st);
+    far_acc<MODE>(h, far_units<NY>(ph, z), st);
     if (NY) kappa_acc<MODE>(kap, st.aR, st.aI);
     return;
   }
@@ -805,7 +901,7 @@ FJ_HD inline void linseg_station(int s, double rs, float ir, float isr, float dx
   Ends cj = {0.f, 0.f, 0.f, 0.f, 0.f}, cy = {0.f, 0.f, 0.f, 0.f, 0.f};
   // the lane's own far test, implied by the block's (far_threshold)
   if (FAR || f32_bits(ir) <= L.irf) {
-    const FarUnits w = far_units<NY>(rs, L, z);
+    const FarUnits w = far_units<NY>(ph, z);
     far_acc<MODE>(h, w, st);
     if (NY) {
       float kp[NS];
@@ -948,6 +1044,7 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
   const float* __restrict__ cfb = cf + lin_b_offset(nrp, NS);
   const float* __restrict__ cfk = cf + lin_kappa_offset(nrp, NS);
   const uint32_t hlim = f32_bits(cf[lin_header(nrp, NS)]);
+  const uint32_t alim = f32_bits(cf[lin_header(nrp, NS) + 1]);
   const LaneK L = lane_consts(q);
   const uint32_t irfar = far_threshold(L.q2pi);
   const uint32_t hsm = smooth_threshold(L.q2pi, hlim);
@@ -968,26 +1065,30 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
     }
   };
   for (int sb = 0; sb < nrp; sb += LU) {
-    double rv[LU];
     float iv[LU];
     float hv[LU * NH];
-#pragma unroll
-    for (int k = 0; k < LU; k++) rv[k] = S.rr[sb + k];
 #pragma unroll
     for (int k = 0; k < LU; k++) iv[k] = S.ir[sb + k];
 #pragma unroll
     for (int k = 0; k < LU * NH; k++) hv[k] = cf[sb * NH + k];
-    // both block words in one scalar load: a test of the second after the
-    // first would wait for memory twice per block (sb is a multiple of 4
-    // and the array 8-byte aligned; written by another kernel, so the wider
-    // read aliases no store of this one)
+    // the block's four words in one scalar load: a test of one after
+    // another would wait for memory twice per block (sb is a multiple of 4
+    // and the array 16-byte aligned; written by another kernel, so the
+    // wider read aliases no store of this one)
+    uint32_t bwv[4];
 #if defined(__HIP_DEVICE_COMPILE__)
-    const uint64_t bwp = reinterpret_cast<const uint64_t*>(S.bw)[sb >> 1];
+    {
+      const uint4 b4 = reinterpret_cast<const uint4*>(S.bw)[sb >> 2];
+      bwv[0] = b4.x;
+      bwv[1] = b4.y;
+      bwv[3] = b4.w;
+    }
 #else
-    uint64_t bwp;
-    memcpy(&bwp, S.bw + sb, 8);
+    memcpy(bwv, S.bw + sb, 16);
 #endif
-    const uint32_t bw0 = (uint32_t)bwp, bw1 = (uint32_t)(bwp >> 32);
+    const uint32_t bw0 = bwv[0], bw1 = bwv[1];
+    float ph[LU];
+    block_phases<LU>(S, sb, L, Anchored<MODE>::on && bwv[3] <= alim, ph);
     const bool far = (LF::smooth || LF::on) && bw0 <= irfar;
     if (LF::smooth && far && bw1 <= hsm) {
       // every gap of the block (its left one included) is below both the
@@ -1001,13 +1102,16 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
       }
 #pragma unroll
       for (int k = 0; k < LU; k++) {
-        linseg_station<MODE, true, true>(sb + k, rv[k], iv[k], 0.0f, 0.0f, 0.0f, 0.0f, hlim, L,
-                                         hv + k * NH, kv + (M::Y ? k * NS : 0), nullptr, st);
+        linseg_station<MODE, true, true>(sb + k, 0.0, ph[k], iv[k], 0.0f, 0.0f, 0.0f, 0.0f, hlim,
+                                         L, hv + k * NH, kv + (M::Y ? k * NS : 0), nullptr, st);
       }
       // smooth: the block's last segment is neither long nor a G difference
       st.gnext = st.lnext = false;
     } else {
+      double rv[LU];
       float hw[LU + 1], sv[LU];   // hw[k]: gap left of station sb + k
+#pragma unroll
+      for (int k = 0; k < LU; k++) rv[k] = S.rr[sb + k];
 #pragma unroll
       for (int k = 0; k <= LU; k++) hw[k] = S.hh[sb + k];
 #pragma unroll
@@ -1028,12 +1132,13 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
       if (LF::on && far) {
 #pragma unroll
         for (int k = 0; k < LU; k++)
-          linseg_station<MODE, true>(sb + k, rv[k], iv[k], sv[k], L.kf * hw[k], L.kf * hw[k + 1],
-                                     hw[k + 1], hlim, L, hv + k * NH, nullptr, brec(k), st);
+          linseg_station<MODE, true>(sb + k, rv[k], ph[k], iv[k], sv[k], L.kf * hw[k],
+                                     L.kf * hw[k + 1], hw[k + 1], hlim, L, hv + k * NH, nullptr,
+                                     brec(k), st);
       } else {
 #pragma unroll
         for (int k = 0; k < LU; k++)
-          linseg_station<MODE, false>(sb + k, rv[k], iv[k], sv[k], L.kf * hw[k],
+          linseg_station<MODE, false>(sb + k, rv[k], ph[k], iv[k], sv[k], L.kf * hw[k],
                                       L.kf * hw[k + 1], hw[k + 1], hlim, L, hv + k * NH,
                                       nullptr, brec(k), st);
       }

@@ -21,7 +21,8 @@ namespace fjk {
 // (the velocity grid the records will be used with: itype 1 leaves long
 // segments out by min |c|); coef: nf blocks of fjl::coef_block floats; tile
 // origin (f0, s0).  Stations nr..nrp-1 (padding) get zero records; the
-// tiles with s0 = 0 write the itype-1 header of their frequencies.
+// tiles with s0 = 0 write the header of their frequencies (the long-segment
+// limit of itype 1, the anchor limit).
 // POLAR: itype 0, J - Y records in polar form (fjl::to_polar).
 template <int ITYPE, int NSPEC, bool POLAR = false>
 __device__ inline void prep_tile(const float* __restrict__ U0, const float* __restrict__ U1,
@@ -36,7 +37,7 @@ __device__ inline void prep_tile(const float* __restrict__ U0, const float* __re
   __shared__ float red[4];
   const int lane = threadIdx.x & 63;
   const int grp = threadIdx.x >> 6;
-  if (ITYPE == 1) {
+  {
     // min |c| over the grid (exact, so every tile finds the same value)
     float m = INFINITY;
     for (int j = threadIdx.x; j < nc; j += 256) m = fminf(m, fabsf(cgrid[j]));
@@ -47,9 +48,14 @@ __device__ inline void prep_tile(const float* __restrict__ U0, const float* __re
     const float cmin = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
     if (threadIdx.x < TF) {
       const int fi = f0 + threadIdx.x;
-      const float h = fi < nf ? fjl::long_segment_limit(fgrid[fi], cmin) : 0.0f;
+      const float h =
+          ITYPE == 1 && fi < nf ? fjl::long_segment_limit(fgrid[fi], cmin) : 0.0f;
       hlim_s[threadIdx.x] = h;
-      if (s0 == 0 && fi < nf) coef[(size_t)fi * block + (size_t)nrp * NREC] = h;
+      if (s0 == 0 && fi < nf) {
+        float* hd = coef + (size_t)fi * block + (size_t)nrp * NREC;
+        hd[0] = h;
+        hd[1] = fjl::anchor_limit(fgrid[fi], cmin);
+      }
     }
   }
   for (int row = grp; row < ROWS; row += 4) {   // stations s0 - 1 .. s0 + TS
