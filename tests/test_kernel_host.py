@@ -71,6 +71,32 @@ def test_fused_modes(emul, golden, itype):
         assert O.column_error(got, want)[0] < 1e-6, mode
 
 
+@pytest.mark.parametrize("itype", [0, 1])
+def test_anchored_block_phases(emul, itype):
+    """One-spectrum modes take a block's phases from one double-precision
+    reduction at its first station plus q (r_s - r_a) in float when the
+    block's spread of r is within the frequency's anchor limit (1 / max q,
+    250 m at 2 Hz over c >= 500 m/s here): error at most 1.5e-7
+    revolutions per phase.  Spacings of 80 and 90 m put the blocks of 4
+    (spans 240 and 270 m) just inside and just outside the limit at 2 Hz
+    (both inside at 1 Hz); against the FP64 oracle the results stay within
+    3e-6.  At 800 m the blocks are 10 times the limit and must take the
+    per-station reduction: a limit that let them anchor would add errors of
+    several 1e-6 (|q dr| up to 10 revolutions in float), above the 1e-6
+    asserted there."""
+    rng = np.random.default_rng(3)
+    f = np.array([1.0, 2.0], np.float32)
+    c = np.linspace(500, 3000, 96).astype(np.float32)
+    for d, bound in ((80.0, 3e-6), (90.0, 3e-6), (800.0, 1e-6)):
+        r = (20e3 + d * np.arange(400)).astype(np.float32)
+        U = (rng.normal(size=(400, 2)) + 1.0).astype(np.float32)
+        for mode, ybasis in ((MODE_J, 0), (MODE_Y, 1)):
+            got = run(emul, mode, itype, U, r, f, c)
+            emax, _, mism = O.column_error(got, O.oracle64(U, r, f, c, itype, ybasis))
+            assert mism == 0
+            assert emax <= bound, (d, mode, emax)
+
+
 def test_special_functions(emul):
     """J0, J1, Y0, Y1 and G(x) = int_0^x t Z1 dt against scipy/mpmath:
     absolute error (G: relative to max(1, sqrt x)) across both branches."""
