@@ -1100,10 +1100,15 @@ FJ_HD inline float linseg_lane(const Stations& S, const float* __restrict__ cf, 
 #pragma unroll
           for (int k = 0; k < LU; k++) kv[k * NS + m] = cfk[(size_t)m * nrp + sb + k];
       }
+      // the block's far units first (independent), then the sums in
+      // station order: the same operations as linseg_station's smooth path
+      FarUnits w[LU];
+#pragma unroll
+      for (int k = 0; k < LU; k++) w[k] = far_units<M::Y>(ph[k], L.zk * iv[k]);
 #pragma unroll
       for (int k = 0; k < LU; k++) {
-        linseg_station<MODE, true, true>(sb + k, 0.0, ph[k], iv[k], 0.0f, 0.0f, 0.0f, 0.0f, hlim,
-                                         L, hv + k * NH, kv + (M::Y ? k * NS : 0), nullptr, st);
+        far_acc<MODE>(hv + k * NH, w[k], st);
+        if (M::Y) kappa_acc<MODE>(kv + k * NS, st.aR, st.aI);
       }
       // smooth: the block's last segment is neither long nor a G difference
       st.gnext = st.lnext = false;
