@@ -296,7 +296,10 @@ def main():
                 "frac": round(achieved / FP32_PEAK, 4), "traffic": measured_traffic(dom_key),
                 "kernel": f"k_fj ({dom_key})", "avg_ms": dom["avg_ms"],
                 "achieved_is": "algorithmic flops (92 per linear-segment term and basis call) per "
-                               "launch / HIP-event launch time",
+                               "launch / HIP-event launch time; a frac above 1 means the kernel "
+                               "needs fewer flops per term than that closed-form recipe (the "
+                               "station-only work is precomputed into the records): "
+                               "executed_frac is the share of the FP32 pipe in use",
                 "executed_fp32_tflops": ex,
                 "executed_frac": None if ex is None else round(ex / FP32_PEAK, 4)}
     print(f"bench: kernels {json.dumps(kernels)}", file=sys.stderr, flush=True)
