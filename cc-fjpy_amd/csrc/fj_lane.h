@@ -644,10 +644,11 @@ FJ_HD inline void block_phases(const Stations& S, int sb, const LaneK& L, bool a
 // the same order: bit-identical results.  Measured on the MI355X at 19,900
 // pairs x 4096 f x 3000 c: J 104.6 ms against 107.3 ms station by station,
 // J - Y 153.6 against 157.3 ms.
-template <int MODE>
+template <int MODE, int NB>
 FJ_HD inline void trapz_far_block(const float* ph, const float* iv, const LaneK& L,
                                   const float* cv, float& pR, float& pI) {
   constexpr int NC = ModeT<MODE>::nspec;
+  constexpr int kUnroll = NB;
   if (MODE == MODE_J || MODE == MODE_J_MINUS_Y) {
     float z[kUnroll], t[kUnroll], u[kUnroll], v[kUnroll];
 #pragma unroll
@@ -670,9 +671,16 @@ FJ_HD inline void trapz_far_block(const float* ph, const float* iv, const LaneK&
   }
 }
 
+// Stations per block of the trapezoid loop, per mode.
+template <int MODE>
+struct TrapzBlock {
+  static constexpr int n = Anchored<MODE>::on ? kUnroll : 4;
+};
+
 template <int MODE>
 FJ_HD inline float trapz_lane(const Stations& S, const float* __restrict__ cf, int nrp, double q) {
   constexpr int NC = ModeT<MODE>::nspec;
+  constexpr int TB = TrapzBlock<MODE>::n;
   const LaneK L = lane_consts(q);
   const uint32_t irfar = far_threshold(L.q2pi);
   const uint32_t alim = f32_bits(cf[coef_header(nrp, MODE, 0) + 1]);
@@ -680,22 +688,23 @@ FJ_HD inline float trapz_lane(const Stations& S, const float* __restrict__ cf, i
   for (int s0 = 0; s0 < nrp; s0 += kPartial) {
     const int s1 = nrp < s0 + kPartial ? nrp : s0 + kPartial;
     float pR = 0.0f, pI = 0.0f;
-    for (int sb = s0; sb < s1; sb += kUnroll) {
-      float iv[kUnroll];
-      float cv[kUnroll * NC];
+    for (int sb = s0; sb < s1; sb += TB) {
+      float iv[TB];
+      float cv[TB * NC];
 #pragma unroll
-      for (int k = 0; k < kUnroll; k++) iv[k] = S.ir[sb + k];
+      for (int k = 0; k < TB; k++) iv[k] = S.ir[sb + k];
 #pragma unroll
-      for (int k = 0; k < kUnroll * NC; k++) cv[k] = cf[sb * NC + k];
-      const bool far = S.bw[sb + 2] <= irfar;   // every station has x >= kLaneFar
+      for (int k = 0; k < TB * NC; k++) cv[k] = cf[sb * NC + k];
+      // every station of the block has x >= kLaneFar (block words)
+      const bool far = S.bw[TB == 8 ? sb + 2 : sb] <= irfar;
       if (Anchored<MODE>::on) {
-        float ph[kUnroll];
-        block_phases<kUnroll>(S, sb, L, S.bw[sb + 6] <= alim, ph);
+        float ph[TB];
+        block_phases<TB>(S, sb, L, S.bw[TB == 8 ? sb + 6 : sb + 3] <= alim, ph);
         if (far) {
-          trapz_far_block<MODE>(ph, iv, L, cv, pR, pI);
+          trapz_far_block<MODE, TB>(ph, iv, L, cv, pR, pI);
         } else {
 #pragma unroll
-          for (int k = 0; k < kUnroll; k++)
+          for (int k = 0; k < TB; k++)
             trapz_station<MODE>(S.rr[sb + k], ph[k], iv[k], L, cv + k * NC, pR, pI);
         }
       } else {
@@ -703,16 +712,16 @@ FJ_HD inline float trapz_lane(const Stations& S, const float* __restrict__ cf, i
         // that is not far, for the lanes that are), from the distances
         // loaded with the block: fewer scalar registers held across the
         // branch (J - Y 123 -> 119 ms)
-        double rv[kUnroll];
+        double rv[TB];
 #pragma unroll
-        for (int k = 0; k < kUnroll; k++) rv[k] = S.rr[sb + k];
+        for (int k = 0; k < TB; k++) rv[k] = S.rr[sb + k];
         if (far) {
-          float ph[kUnroll];
-          direct_phases<kUnroll>(rv, L, ph);
-          trapz_far_block<MODE>(ph, iv, L, cv, pR, pI);
+          float ph[TB];
+          direct_phases<TB>(rv, L, ph);
+          trapz_far_block<MODE, TB>(ph, iv, L, cv, pR, pI);
         } else {
 #pragma unroll
-          for (int k = 0; k < kUnroll; k++)
+          for (int k = 0; k < TB; k++)
             trapz_station<MODE, true>(rv[k], 0.0f, iv[k], L, cv + k * NC, pR, pI);
         }
       }
