@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
+#include <cstring>
 #include <utility>
 
 #include "../../include/ccfj.h"
@@ -35,10 +37,43 @@ int check_sizes(int itype, int func, int nr, int npts, int nwin, int nf, int nc,
 // Work arrays live in the context's grow-only scratch slots 2-7 (slots 0-1
 // belong to the Hilbert transform, 8-9 to the host-pointer entry point), so
 // a call allocates nothing once the slots have grown.
+// Sum of the windows' support lengths (k_mwfj_goertzel), from host copies
+// of the descriptions.
+long long support_sum(int nr, int npts, int nwin, const int* n1, const int* n2,
+                      const int* xidx, int nx) {
+  long long sum = 0;
+  for (int d = 0; d < nwin * nr; d++) {
+    if (xidx[d] >= 0 && xidx[d] < nx) {
+      sum += npts;
+    } else if (xidx[d] == -1) {
+      const int a = std::max(n1[d], 0), b = std::min(n2[d], npts);
+      if (b > a) sum += b - a;
+    }
+  }
+  return sum;
+}
+
+// The used bins come from the Goertzel recurrence over each window's
+// support when that costs less than writing and transforming the
+// zero-padded rows: 2 double operations per (support sample, bin) against
+// the FFT path's measured cost, 38 of them per row sample (window kernel
+// and length-8192 complex FFT: 2.54 ms for 64 x 480 rows of 16,384 samples
+// on the MI355X).
+// CCFJ_MWFJ_SPECTRA=fft or =goertzel in the environment forces one path (a
+// test and tuning switch; both give the same spectra to rounding).
+bool use_goertzel(long long support, int nf, int nwin, int nr, int npts) {
+  const char* force = getenv("CCFJ_MWFJ_SPECTRA");
+  if (force && strcmp(force, "fft") == 0) return false;
+  if (force && strcmp(force, "goertzel") == 0) return true;
+  return (double)support * nf <= 38.0 * (double)nwin * nr * npts;
+}
+
+// support: the sum of the windows' support lengths, or -1 to sum them on
+// the device (one wait for the stream).
 int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int nf, int nc,
              int fstride, const double* u, const float* r, const float* f, const float* c,
              const int* n1, const int* n2, const int* delta, const int* xidx,
-             const double* xrows, int nx, double* out, hipStream_t st) {
+             const double* xrows, int nx, long long support, double* out, hipStream_t st) {
   fjk::MwfjDev d{};
   d.nr = nr;
   d.npts = npts;
@@ -54,30 +89,48 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
   d.func = func;
   d.half = (nr + 31) / 32 * 32;
   d.nrhs = (2 * nwin + 31) / 32 * 32;
-  // windows per rFFT batch: all of them when the tapered traces and their
-  // spectra fit in 8 GiB (64 windows of 480 x 16384 take 8 GB)
-  const size_t per_win = (size_t)nr * npts * 8 + (size_t)nr * d.nbins * 16;
-  const int cw = (int)std::max<size_t>(1, std::min<size_t>(nwin, ((size_t)8 << 30) / per_win));
-
-  void *dxw, *dspec, *drr;
-  int rc = ccfj_internal::ctx_scratch(ctx, 2, sizeof(double) * (size_t)cw * nr * npts, &dxw);
-  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 3, 16 * (size_t)cw * nr * d.nbins, &dspec);
-  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 7, 12 * (size_t)d.half, &drr);
+  void *dxw = nullptr, *dspec = nullptr, *drr = nullptr;
+  const size_t o_sum = align256(12 * (size_t)d.half);
+  const size_t o_bad = o_sum + 256;
+  int rc = ccfj_internal::ctx_scratch(ctx, 7, o_bad + sizeof(int) * (size_t)nr, &drr);
   if (rc) return rc;
   d.r = r;
   hipError_t e = fjk::launch_mwfj_stations(d, (double*)drr, (float*)((double*)drr + d.half), st);
   if (e != hipSuccess) return hip_fail(e, "station kernel");
   d.u = u;
-  d.rr = (const double*)drr;
-  d.ir = (const float*)((const double*)drr + d.half);
-  d.f = f;
-  d.c = c;
   d.n1 = n1;
   d.n2 = n2;
   d.dl = delta;
   d.xidx = xidx;
   d.xrows = xrows;
   d.nx = xrows ? nx : 0;
+  if (support < 0) {
+    unsigned long long* dsum = (unsigned long long*)((char*)drr + o_sum);
+    unsigned long long hsum = 0;
+    if ((e = fjk::launch_fill((float*)dsum, 2, 0.0f, st)) != hipSuccess)
+      return hip_fail(e, "fill kernel launch");
+    if ((e = fjk::launch_mwfj_support(d, dsum, st)) != hipSuccess)
+      return hip_fail(e, "support kernel");
+    if ((e = hipMemcpyAsync(&hsum, dsum, sizeof(hsum), hipMemcpyDeviceToHost, st)) != hipSuccess)
+      return hip_fail(e, "support copy");
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(e, "support wait");
+    support = (long long)hsum;
+  }
+  const bool direct = use_goertzel(support, nf, nwin, nr, npts);
+  // windows per batch: all of them when the tapered traces and their
+  // spectra fit in 8 GiB (64 windows of 480 x 16384 take 8 GB); the
+  // Goertzel path keeps only the nf used bins of each row
+  const size_t row_traces = direct ? 0 : (size_t)nr * npts * 8;   // tapered traces
+  const size_t row_spec = (size_t)nr * (direct ? nf : d.nbins) * 16;  // spectra
+  const int cw = (int)std::max<size_t>(
+      1, std::min<size_t>(nwin, ((size_t)8 << 30) / (row_traces + row_spec)));
+  if (!direct) rc = ccfj_internal::ctx_scratch(ctx, 2, row_traces * cw, &dxw);
+  if (!rc) rc = ccfj_internal::ctx_scratch(ctx, 3, row_spec * cw, &dspec);
+  if (rc) return rc;
+  d.rr = (const double*)drr;
+  d.ir = (const float*)((const double*)drr + d.half);
+  d.f = f;
+  d.c = c;
   d.xw = (double*)dxw;
   d.spec = dspec;
   d.out = out;
@@ -110,11 +163,29 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
     dtmp = (float*)((char*)dst + 2 * a);
   }
 
+  // the spectra rows as the coefficient and staging kernels read them: the
+  // Goertzel path stores the used bins only
+  fjk::MwfjDev dd = d;
+  int* rowbad = (int*)((char*)drr + o_bad);
+  if (direct) {
+    dd.nbins = nf;
+    dd.fstride = 1;
+    dd.zhalf = 0;
+    if ((e = fjk::launch_mwfj_rowbad(d, rowbad, st)) != hipSuccess)
+      return hip_fail(e, "row check kernel");
+  }
   for (int w0 = 0; w0 < nwin; w0 += cw) {
     const int cwn = std::min(cw, nwin - w0);
-    if ((e = fjk::launch_window(d, w0, cwn, st)) != hipSuccess) return hip_fail(e, "window kernel");
+    if (direct) {
+      if ((e = fjk::launch_mwfj_goertzel(d, w0, cwn, rowbad, (double2*)dspec, st)) != hipSuccess)
+        return hip_fail(e, "Goertzel kernel");
+    } else if ((e = fjk::launch_window(d, w0, cwn, st)) != hipSuccess) {
+      return hip_fail(e, "window kernel");
+    }
     hipfftHandle plan = 0;
-    if (d.zhalf) {
+    if (direct) {
+      // no transform
+    } else if (d.zhalf) {
       if ((rc = ccfj_internal::cached_fft_plan(ctx, npts / 2, cwn * nr, HIPFFT_Z2Z, st, &plan)))
         return rc;
       if (hipfftExecZ2Z(plan, (hipfftDoubleComplex*)dxw, (hipfftDoubleComplex*)dspec,
@@ -127,10 +198,10 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
         return fail(CCFJ_EDEVICE, "hipfftExecD2Z failed");
     }
     if (itype == 0) {
-      if ((e = fjk::launch_mwfj_coef(d, w0, cwn, st)) != hipSuccess) return hip_fail(e, "coef kernel");
+      if ((e = fjk::launch_mwfj_coef(dd, w0, cwn, st)) != hipSuccess) return hip_fail(e, "coef kernel");
     } else {
       for (int wl = 0; wl < cwn; wl++) {
-        if ((e = fjk::launch_mwfj_stage(d, wl, dre, dim, st)) != hipSuccess)
+        if ((e = fjk::launch_mwfj_stage(dd, wl, dre, dim, st)) != hipSuccess)
           return hip_fail(e, "stage kernel");
         rc = ccfj_transform_dev(ctx, func ? CCFJ_MODE_ABS_H : CCFJ_MODE_ABS_J, 1, 1, dre, dim, d.r,
                                 d.f, d.c, nr, nf, nc, dtmp, st);
@@ -166,7 +237,7 @@ extern "C" int ccfj_mwfj_dev(ccfj_ctx* ctx, int itype, int func, int nr, int npt
   if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
   hipStream_t st = stream ? (hipStream_t)stream : ccfj_internal::ctx_stream(ctx);
   return mwfj_run(ctx, itype, func, nr, npts, nwin, nf, nc, fstride, u, r, f, c, n1, n2, delta,
-                  xidx, xrows, nx, out, st);
+                  xidx, xrows, nx, -1, out, st);
 }
 
 extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin,
@@ -223,7 +294,7 @@ extern "C" int ccfj_mwfj(ccfj_ctx* ctx, int itype, int func, int nr, int npts, i
                 (const float*)(b + o_r), (const float*)(b + o_f), (const float*)(b + o_c),
                 (const int*)(b + o_n1), (const int*)(b + o_n2), (const int*)(b + o_dl),
                 (const int*)(b + o_xi), nx > 0 ? (const double*)(b + o_xr) : nullptr, nx,
-                (double*)dout, st);
+                support_sum(nr, npts, nwin, n1, n2, xidx, nx), (double*)dout, st);
   if (rc) {
     (void)hipStreamSynchronize(st);
     return rc;

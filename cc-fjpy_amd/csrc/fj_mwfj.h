@@ -34,6 +34,14 @@ struct MwfjDev {
 };
 
 hipError_t launch_window(const MwfjDev& d, int w0, int cw, hipStream_t st);
+// the used bins of windows w0 .. w0+cw-1 by the Goertzel recurrence over
+// each window's support: spec [cw][nr][nf] complex double (k_mwfj_goertzel)
+hipError_t launch_mwfj_goertzel(const MwfjDev& d, int w0, int cw, const int* rowbad,
+                                double2* spec, hipStream_t st);
+// rowbad[s] = 1 when trace s holds a non-finite sample (device, [nr])
+hipError_t launch_mwfj_rowbad(const MwfjDev& d, int* rowbad, hipStream_t st);
+// sum of the windows' support lengths into *sum (device; zeroed by the caller)
+hipError_t launch_mwfj_support(const MwfjDev& d, unsigned long long* sum, hipStream_t st);
 hipError_t launch_mwfj_stations(const MwfjDev& d, double* rr, float* ir, hipStream_t st);
 hipError_t launch_mwfj_coef(const MwfjDev& d, int w0, int cw, hipStream_t st);
 hipError_t launch_mwfj_mfma(const MwfjDev& d, hipStream_t st);

@@ -89,6 +89,126 @@ __global__ void k_window(const double* __restrict__ u, const int* __restrict__ n
   }
 }
 
+// Used bins of the rFFT of the tapered traces by the Goertzel recurrence
+// over each window's support only: spec[(wl nr + s) nf + f] = X[f fstride]
+// for window w0 + wl and station s, where X[k] = sum_t x[t] e^{-2 pi i k t /
+// npts} and x = u[s] times the window, zero outside [a, b) = [n1, n2) (the
+// whole row for an explicit row, empty for an invalid description).  With
+// w = 2 pi k / npts and s_t = x_t + 2 cos(w) s_{t-1} - s_{t-2} over t = a ..
+// b-1 from s = 0, sum_t x_t e^{-i w t} = e^{-i w (b-1)} (s_{b-1} - e^{-i w}
+// s_{b-2}).  Two double-precision operations per sample and bin, against
+// writing and transforming the whole zero-padded row: the group-velocity
+// windows of an active-source gather cover about 1% of it.  The phase
+// factor's argument is reduced exactly in integers (k (b-1) mod npts).
+// One workgroup per (window, station); the support is staged through LDS in
+// segments, 4 bins per thread per pass over it.  A trace with a non-finite
+// sample anywhere gives NaN in every bin of every window, as the transform
+// of the whole tapered row does (NaN times a zero taper is NaN).
+constexpr int kGzSeg = 2048;
+constexpr int kGzBins = 4;
+__global__ __launch_bounds__(256) void k_mwfj_goertzel(
+    const double* __restrict__ u, const int* __restrict__ n1, const int* __restrict__ n2,
+    const int* __restrict__ dl, const int* __restrict__ xidx, const double* __restrict__ xrows,
+    int nx, int w0, int nr, int npts, int nf, int fstride, const int* __restrict__ rowbad,
+    double2* __restrict__ spec) {
+  __shared__ double xs[kGzSeg];
+  const int s = blockIdx.x, wl = blockIdx.y;
+  if (rowbad[s]) {
+    double2* __restrict__ o = spec + ((size_t)wl * nr + s) * nf;
+    for (int f = threadIdx.x; f < nf; f += 256) o[f] = make_double2(__builtin_nan(""), __builtin_nan(""));
+    return;
+  }
+  const int desc = (w0 + wl) * nr + s;
+  const int xi = xidx[desc];
+  int a = 0, b = 0;
+  if (xi >= 0 && xi < nx) {
+    b = npts;
+  } else if (xi == -1) {
+    a = max(n1[desc], 0);
+    b = min(n2[desc], npts);
+  }
+  if (b < a) b = a;
+  double2* __restrict__ out = spec + ((size_t)wl * nr + s) * nf;
+  const double* __restrict__ us = u + (size_t)s * npts;
+  for (int fb = 0; fb < nf; fb += 256 * kGzBins) {
+    double c2[kGzBins], s1[kGzBins], s2[kGzBins];
+    int kk[kGzBins];
+#pragma unroll
+    for (int i = 0; i < kGzBins; i++) {
+      const int f = min(fb + (int)threadIdx.x + 256 * i, nf - 1);
+      kk[i] = f * fstride;
+      c2[i] = 2.0 * cospi(2.0 * (double)kk[i] / (double)npts);
+      s1[i] = s2[i] = 0.0;
+    }
+    for (int t0 = a; t0 < b; t0 += kGzSeg) {
+      const int n = min(kGzSeg, b - t0);
+      __syncthreads();   // the previous segment has been read
+      for (int q = threadIdx.x; q < n; q += 256)
+        xs[q] = us[t0 + q] * win_value(n1, n2, dl, xidx, xrows, nx, npts, desc, t0 + q);
+      __syncthreads();
+      for (int q = 0; q < n; q++) {
+        const double x = xs[q];
+#pragma unroll
+        for (int i = 0; i < kGzBins; i++) {
+          const double v = __builtin_fma(c2[i], s1[i], x - s2[i]);
+          s2[i] = s1[i];
+          s1[i] = v;
+        }
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < kGzBins; i++) {
+      const int f = fb + (int)threadIdx.x + 256 * i;
+      if (f >= nf) continue;
+      double2 X = make_double2(0.0, 0.0);
+      if (b > a) {
+        const int k = kk[i];
+        double sk, ck, sm, cm;
+        sincospi(2.0 * (double)k / (double)npts, &sk, &ck);   // e^{-i w} = ck - i sk
+        const double yr = s1[i] - ck * s2[i], yi = sk * s2[i];
+        const long long m = ((long long)k * (long long)(b - 1)) % npts;
+        sincospi(2.0 * (double)m / (double)npts, &sm, &cm);   // e^{-i w (b-1)} = cm - i sm
+        X = make_double2(cm * yr + sm * yi, cm * yi - sm * yr);
+      }
+      out[f] = X;
+    }
+  }
+}
+
+// rowbad[s] = 1 when trace s has a non-finite sample (k_mwfj_goertzel)
+__global__ __launch_bounds__(256) void k_mwfj_rowbad(const double* __restrict__ u, int npts,
+                                                     int* __restrict__ rowbad) {
+  const int s = blockIdx.x;
+  int any = 0;
+  for (int t = threadIdx.x; t < npts; t += 256) {
+    const double v = u[(size_t)s * npts + t];
+    any |= !(v - v == 0.0);   // NaN or inf
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) rowbad[s] = any;
+}
+
+// Sum over window descriptions of the support length b - a (as in
+// k_mwfj_goertzel), for the choice between it and the FFT.
+__global__ void k_mwfj_support(const int* __restrict__ n1, const int* __restrict__ n2,
+                               const int* __restrict__ xidx, int nx, int npts, int ndesc,
+                               unsigned long long* __restrict__ sum) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long v = 0;
+  if (d < ndesc) {
+    const int xi = xidx[d];
+    if (xi >= 0 && xi < nx) {
+      v = (unsigned long long)npts;
+    } else if (xi == -1) {
+      const int a = max(n1[d], 0), b = min(n2[d], npts);
+      v = b > a ? (unsigned long long)(b - a) : 0ull;
+    }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(sum, v);
+}
+
 // Bin k of the real DFT of a row of npts = 2M samples.  zhalf = false: row
 // holds the rFFT itself (M + 1 bins).  zhalf = true: row holds the length-M
 // complex DFT Z of z[n] = x[2n] + i x[2n+1], and for k <= M
@@ -531,6 +651,25 @@ hipError_t launch_window(const MwfjDev& d, int w0, int cw, hipStream_t st) {
     hipLaunchKernelGGL(k_window<1>, grid, dim3(256), 0, st, d.u, d.n1, d.n2, d.dl, d.xidx,
                        d.xrows, d.nx, w0, cw, d.nr, d.npts, d.xw);
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_mwfj_rowbad(const MwfjDev& d, int* rowbad, hipStream_t st) {
+  hipLaunchKernelGGL(k_mwfj_rowbad, dim3(d.nr), dim3(256), 0, st, d.u, d.npts, rowbad);
+  return hipGetLastError();
+}
+
+hipError_t launch_mwfj_goertzel(const MwfjDev& d, int w0, int cw, const int* rowbad,
+                                double2* spec, hipStream_t st) {
+  hipLaunchKernelGGL(k_mwfj_goertzel, dim3(d.nr, cw), dim3(256), 0, st, d.u, d.n1, d.n2, d.dl,
+                     d.xidx, d.xrows, d.nx, w0, d.nr, d.npts, d.nf, d.fstride, rowbad, spec);
+  return hipGetLastError();
+}
+
+hipError_t launch_mwfj_support(const MwfjDev& d, unsigned long long* sum, hipStream_t st) {
+  const int ndesc = d.nwin * d.nr;
+  hipLaunchKernelGGL(k_mwfj_support, dim3((ndesc + 255) / 256), dim3(256), 0, st, d.n1, d.n2,
+                     d.xidx, d.nx, d.npts, ndesc, sum);
   return hipGetLastError();
 }
 

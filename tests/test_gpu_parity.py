@@ -292,6 +292,37 @@ def test_mwfj_many_windows_mfma_tiles(ccfj):
         assert mism == 0 and emax <= TOL, (w, emax)
 
 
+@pytest.mark.parametrize("itype", [0, 1])
+@pytest.mark.parametrize("func", [0, 1])
+def test_mwfj_goertzel_and_fft_spectra(ccfj, monkeypatch, itype, func):
+    """The used bins come from the Goertzel recurrence over each window's
+    support, or from the FFT of the whole tapered rows, whichever costs less
+    (CCFJ_MWFJ_SPECTRA forces one).  Both paths against the oracle, and
+    against each other, on windows of every kind: short, tapered, reaching
+    past the end of the trace, empty, and degenerate ones that travel as
+    explicit rows."""
+    u, r, c, f, Fs = _quake(nr=32, npts=512, seed=6)
+    nwin = 6
+    t0 = np.linspace(2.0, 9.0, nwin)[:, None] + r[None, :] / 4000.0
+    width = np.array([0.5, 2.0, 6.0, 40.0, 0.0, 0.01])[:, None] + 0.0 * r[None, :]
+    winl, winr = t0, t0 + width
+    winl[3] = 0.0   # the whole trace and beyond
+    out = {}
+    for path in ("goertzel", "fft"):
+        monkeypatch.setenv("CCFJ_MWFJ_SPECTRA", path)
+        out[path] = ccfj.MWFJ(u, r, c, f[::2], Fs, nwin, winl, winr, taper=0.6, itype=itype,
+                              func=func)
+    monkeypatch.delenv("CCFJ_MWFJ_SPECTRA")
+    want = O.MWFJ(u, r, c, f[::2], Fs, nwin, winl, winr, taper=0.6, itype=itype, func=func,
+                  engine="fp64")
+    for w in range(nwin):
+        for path in ("goertzel", "fft"):
+            emax, _, mism = O.column_error(out[path][w], want[w])
+            assert mism == 0 and emax <= TOL, (path, w, emax)
+        emax, _, mism = O.column_error(out["goertzel"][w], out["fft"][w])
+        assert mism == 0 and emax <= 1e-6, (w, emax)
+
+
 @pytest.mark.parametrize("func", [0, 1])
 def test_mwfj_zero_frequency_and_nonfinite_coefficients(ccfj, func):
     """The contraction takes fp16 split products where a frequency's scale
