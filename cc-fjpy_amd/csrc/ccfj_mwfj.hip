@@ -92,7 +92,8 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
   void *dxw = nullptr, *dspec = nullptr, *drr = nullptr;
   const size_t o_sum = align256(12 * (size_t)d.half);
   const size_t o_bad = o_sum + 256;
-  int rc = ccfj_internal::ctx_scratch(ctx, 7, o_bad + sizeof(int) * (size_t)nr, &drr);
+  const size_t o_tw = o_bad + align256(sizeof(int) * (size_t)nr);
+  int rc = ccfj_internal::ctx_scratch(ctx, 7, o_tw + 16 * (size_t)npts, &drr);
   if (rc) return rc;
   d.r = r;
   hipError_t e = fjk::launch_mwfj_stations(d, (double*)drr, (float*)((double*)drr + d.half), st);
@@ -167,17 +168,21 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
   // Goertzel path stores the used bins only
   fjk::MwfjDev dd = d;
   int* rowbad = (int*)((char*)drr + o_bad);
+  double2* tw = (double2*)((char*)drr + o_tw);
   if (direct) {
     dd.nbins = nf;
     dd.fstride = 1;
     dd.zhalf = 0;
     if ((e = fjk::launch_mwfj_rowbad(d, rowbad, st)) != hipSuccess)
       return hip_fail(e, "row check kernel");
+    if ((e = fjk::launch_mwfj_twiddles(npts, tw, st)) != hipSuccess)
+      return hip_fail(e, "twiddle kernel");
   }
   for (int w0 = 0; w0 < nwin; w0 += cw) {
     const int cwn = std::min(cw, nwin - w0);
     if (direct) {
-      if ((e = fjk::launch_mwfj_goertzel(d, w0, cwn, rowbad, (double2*)dspec, st)) != hipSuccess)
+      if ((e = fjk::launch_mwfj_goertzel(d, w0, cwn, rowbad, tw, (double2*)dspec, st)) !=
+          hipSuccess)
         return hip_fail(e, "Goertzel kernel");
     } else if ((e = fjk::launch_window(d, w0, cwn, st)) != hipSuccess) {
       return hip_fail(e, "window kernel");

@@ -37,7 +37,9 @@ hipError_t launch_window(const MwfjDev& d, int w0, int cw, hipStream_t st);
 // the used bins of windows w0 .. w0+cw-1 by the Goertzel recurrence over
 // each window's support: spec [cw][nr][nf] complex double (k_mwfj_goertzel)
 hipError_t launch_mwfj_goertzel(const MwfjDev& d, int w0, int cw, const int* rowbad,
-                                double2* spec, hipStream_t st);
+                                const double2* tw, double2* spec, hipStream_t st);
+// tw[m] = (cos, sin)(2 pi m / n), m < n (device, n entries)
+hipError_t launch_mwfj_twiddles(int n, double2* tw, hipStream_t st);
 // rowbad[s] = 1 when trace s holds a non-finite sample (device, [nr])
 hipError_t launch_mwfj_rowbad(const MwfjDev& d, int* rowbad, hipStream_t st);
 // sum of the windows' support lengths into *sum (device; zeroed by the caller)

@@ -101,7 +101,8 @@ __global__ void k_window(const double* __restrict__ u, const int* __restrict__ n
 // windows of an active-source gather cover about 1% of it.  The phase
 // factor's argument is reduced exactly in integers (k (b-1) mod npts).
 // One workgroup per (window, station); the support is staged through LDS in
-// segments, 4 bins per thread per pass over it.  A trace with a non-finite
+// segments, 4 bins per thread per pass over it.  tw[m] = (cos, sin) of
+// 2 pi m / npts (k_mwfj_twiddles) gives every trigonometric value.  A trace with a non-finite
 // sample anywhere gives NaN in every bin of every window, as the transform
 // of the whole tapered row does (NaN times a zero taper is NaN).
 constexpr int kGzSeg = 2048;
@@ -110,8 +111,8 @@ __global__ __launch_bounds__(256) void k_mwfj_goertzel(
     const double* __restrict__ u, const int* __restrict__ n1, const int* __restrict__ n2,
     const int* __restrict__ dl, const int* __restrict__ xidx, const double* __restrict__ xrows,
     int nx, int w0, int nr, int npts, int nf, int fstride, const int* __restrict__ rowbad,
-    double2* __restrict__ spec) {
-  __shared__ double xs[kGzSeg];
+    const double2* __restrict__ tw, double2* __restrict__ spec) {
+  __shared__ __attribute__((aligned(16))) double xs[kGzSeg];
   const int s = blockIdx.x, wl = blockIdx.y;
   if (rowbad[s]) {
     double2* __restrict__ o = spec + ((size_t)wl * nr + s) * nf;
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(256) void k_mwfj_goertzel(
     for (int i = 0; i < kGzBins; i++) {
       const int f = min(fb + (int)threadIdx.x + 256 * i, nf - 1);
       kk[i] = f * fstride;
-      c2[i] = 2.0 * cospi(2.0 * (double)kk[i] / (double)npts);
+      c2[i] = 2.0 * tw[kk[i]].x;
       s1[i] = s2[i] = 0.0;
     }
     for (int t0 = a; t0 < b; t0 += kGzSeg) {
@@ -146,7 +147,23 @@ __global__ __launch_bounds__(256) void k_mwfj_goertzel(
       for (int q = threadIdx.x; q < n; q += 256)
         xs[q] = us[t0 + q] * win_value(n1, n2, dl, xidx, xrows, nx, npts, desc, t0 + q);
       __syncthreads();
-      for (int q = 0; q < n; q++) {
+      // two samples a step, the recurrence's two values taking turns in
+      // place (no register moves): after x0, s2 holds the newest value;
+      // after x1, s1 does and s2 the one before it
+      // (the next pair is read from LDS while this one is used)
+      int q = 0;
+      const double2* xp = reinterpret_cast<const double2*>(xs);
+      double2 nxt = n > 1 ? xp[0] : make_double2(0.0, 0.0);
+      for (; q + 1 < n; q += 2) {
+        const double2 cur = nxt;
+        if (q + 3 < n) nxt = xp[(q >> 1) + 1];
+#pragma unroll
+        for (int i = 0; i < kGzBins; i++) {
+          s2[i] = __builtin_fma(c2[i], s1[i], cur.x - s2[i]);
+          s1[i] = __builtin_fma(c2[i], s2[i], cur.y - s1[i]);
+        }
+      }
+      if (q < n) {
         const double x = xs[q];
 #pragma unroll
         for (int i = 0; i < kGzBins; i++) {
@@ -163,16 +180,24 @@ __global__ __launch_bounds__(256) void k_mwfj_goertzel(
       double2 X = make_double2(0.0, 0.0);
       if (b > a) {
         const int k = kk[i];
-        double sk, ck, sm, cm;
-        sincospi(2.0 * (double)k / (double)npts, &sk, &ck);   // e^{-i w} = ck - i sk
-        const double yr = s1[i] - ck * s2[i], yi = sk * s2[i];
+        const double2 e1 = tw[k];   // e^{-i w} = e1.x - i e1.y
+        const double yr = s1[i] - e1.x * s2[i], yi = e1.y * s2[i];
         const long long m = ((long long)k * (long long)(b - 1)) % npts;
-        sincospi(2.0 * (double)m / (double)npts, &sm, &cm);   // e^{-i w (b-1)} = cm - i sm
-        X = make_double2(cm * yr + sm * yi, cm * yi - sm * yr);
+        const double2 em = tw[m];   // e^{-i w (b-1)} = em.x - i em.y
+        X = make_double2(em.x * yr + em.y * yi, em.x * yi - em.y * yr);
       }
       out[f] = X;
     }
   }
+}
+
+// tw[m] = (cos, sin)(2 pi m / n), m < n
+__global__ void k_mwfj_twiddles(int n, double2* __restrict__ tw) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  double sn, cs;
+  sincospi(2.0 * (double)m / (double)n, &sn, &cs);
+  tw[m] = make_double2(cs, sn);
 }
 
 // rowbad[s] = 1 when trace s has a non-finite sample (k_mwfj_goertzel)
@@ -659,10 +684,15 @@ hipError_t launch_mwfj_rowbad(const MwfjDev& d, int* rowbad, hipStream_t st) {
   return hipGetLastError();
 }
 
+hipError_t launch_mwfj_twiddles(int n, double2* tw, hipStream_t st) {
+  hipLaunchKernelGGL(k_mwfj_twiddles, dim3((n + 255) / 256), dim3(256), 0, st, n, tw);
+  return hipGetLastError();
+}
+
 hipError_t launch_mwfj_goertzel(const MwfjDev& d, int w0, int cw, const int* rowbad,
-                                double2* spec, hipStream_t st) {
+                                const double2* tw, double2* spec, hipStream_t st) {
   hipLaunchKernelGGL(k_mwfj_goertzel, dim3(d.nr, cw), dim3(256), 0, st, d.u, d.n1, d.n2, d.dl,
-                     d.xidx, d.xrows, d.nx, w0, d.nr, d.npts, d.nf, d.fstride, rowbad, spec);
+                     d.xidx, d.xrows, d.nx, w0, d.nr, d.npts, d.nf, d.fstride, rowbad, tw, spec);
   return hipGetLastError();
 }
 
