@@ -224,6 +224,12 @@ __global__ void k_cc_pair_map(const int* __restrict__ pairs, int npairs, int nst
 
 typedef float ccx16 __attribute__((ext_vector_type(16)));
 
+// windows per batch of the Gram loop (k_cc_gram).  Measured on the MI355X at
+// 200 stations, 231 windows and 1000 bins: 0.93 ms at 8 (124 VGPRs), 0.95 at
+// 4, 1.27 at 16 (188 VGPRs, 2 waves); 1.00 ms with four windows' loads
+// waited for at each step.
+constexpr int kGramU = 8;
+
 // One wave per (32 x 32 station tile, bin); 4 waves (4 bins) per block.
 // acc[p][j] += G_j[a][b] for the listed pairs of the tile.
 // Workgroup numbering is XCD-major: workgroups go to the 8 XCDs in turn
@@ -260,16 +266,40 @@ __global__ __launch_bounds__(256) void k_cc_gram(const float2* __restrict__ S2, 
     gre[v] = 0.0f;
     gim[v] = 0.0f;
   }
-#pragma unroll 4
-  for (int wl = 0; wl < cw; wl++) {
-    const float2* Sw = Sj + (size_t)wl * nsta;
-    const float2 xa = av ? Sw[a] : make_float2(0.0f, 0.0f);
-    const float2 xb = bv ? Sw[b] : make_float2(0.0f, 0.0f);
-    // real: ar br + ai bi;  imaginary: ar bi + (-ai) br
-    const float a1 = kq ? xa.y : xa.x, b1 = kq ? xb.y : xb.x;
-    const float a2 = kq ? -xa.y : xa.x, b2 = kq ? xb.x : xb.y;
-    gre = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, gre, 0, 0, 0);
-    gim = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b2, gim, 0, 0, 0);
+  // windows in batches of kGramU, the next batch's loads issued before the
+  // current batch's MFMAs so that their latency overlaps the matrix work
+  // (windows past cw load zeros, which add nothing)
+  auto load = [&](int w, float2& xa, float2& xb) {
+    const bool ok = w < cw;
+    const float2* Sw = Sj + (size_t)(ok ? w : 0) * nsta;
+    xa = ok && av ? Sw[a] : make_float2(0.0f, 0.0f);
+    xb = ok && bv ? Sw[b] : make_float2(0.0f, 0.0f);
+  };
+  float2 xa[kGramU], xb[kGramU];
+#pragma unroll
+  for (int u = 0; u < kGramU; u++) load(u, xa[u], xb[u]);
+  for (int w0 = 0; w0 < cw; w0 += kGramU) {
+    float2 na[kGramU], nb[kGramU];
+    const bool more = w0 + kGramU < cw;   // uniform
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < kGramU; u++) load(w0 + kGramU + u, na[u], nb[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kGramU; u++) {
+      // real: ar br + ai bi;  imaginary: ar bi + (-ai) br
+      const float a1 = kq ? xa[u].y : xa[u].x, b1 = kq ? xb[u].y : xb[u].x;
+      const float a2 = kq ? -xa[u].y : xa[u].x, b2 = kq ? xb[u].x : xb[u].y;
+      gre = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, gre, 0, 0, 0);
+      gim = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b2, gim, 0, 0, 0);
+    }
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < kGramU; u++) {
+        xa[u] = na[u];
+        xb[u] = nb[u];
+      }
+    }
   }
   if (!bv) return;
 #pragma unroll
