@@ -141,7 +141,7 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
         // the pair stage's main kernel, timed when the context's timing is on
         std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
         if ((rc = ccfj_internal::timing_begin(ctx, st, &ev))) return rc;
-        e = fjk::launch_cc_gram(dS, nsta, nf, n, ntile, dtiles, dpidx, dflag, dacc, st);
+        e = fjk::launch_cc_gram(dS, nsta, nf, n, ntile, whiten, dtiles, dpidx, dflag, dacc, st);
         ccfj_internal::timing_end(ctx, st, ev);
       }
       if (e == hipSuccess)

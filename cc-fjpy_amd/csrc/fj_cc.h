@@ -21,7 +21,9 @@ hipError_t launch_cc_spec_t(const void* spec, int rows, int nsta, int nbins, int
                             int* nonfinite, hipStream_t st);
 hipError_t launch_cc_pair_map(const int* pairs, int npairs, int nsta, int ntile, int* pidx,
                               unsigned char* dup, unsigned char* tiles, hipStream_t st);
-hipError_t launch_cc_gram(const void* S2, int nsta, int nf, int cw, int ntile,
+// whiten: the spectra are whitened (|x| <= 1), and the product takes split
+// fp16 MFMAs (k_cc_gram_split); else FP32 ones (k_cc_gram)
+hipError_t launch_cc_gram(const void* S2, int nsta, int nf, int cw, int ntile, int whiten,
                           const unsigned char* tiles, const int* pidx, const int* nonfinite,
                           void* acc, hipStream_t st);
 hipError_t launch_cc_pairs_t(const void* S2, const unsigned char* use, const int* pairs,

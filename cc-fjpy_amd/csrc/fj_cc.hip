@@ -316,6 +316,105 @@ __global__ __launch_bounds__(256) void k_cc_gram(const float2* __restrict__ S2, 
   }
 }
 
+// The Gram product of whitened spectra (|x| <= 1) in split fp16 products
+// on v_mfma_f32_32x32x16_f16: each value held as hi = x rounded to fp16 and
+// lo = the rounded remainder (22 significand bits together), each product
+// as hi hi + hi lo + lo hi with FP32 accumulation (about 2^-22 of |a||b|
+// per term).  K = 16 holds 8 windows x (re, im): lane l takes station
+// rows i = l % 32 and windows 4 (l / 32) + 0..3 of each batch of 8, A[i][k]
+// = (ar, ai) of its windows; the real part takes B[k][n] = (br, bi), the
+// imaginary part (bi, -br) against the same A.  Six 32x32x16 MFMAs per 8
+// windows against sixteen 32x32x2 FP32 ones, and each lane loads only its
+// own windows.  Same numbering, tiles and result layout as k_cc_gram.
+typedef _Float16 cch8 __attribute__((ext_vector_type(8)));
+__global__ __launch_bounds__(256) void k_cc_gram_split(const float2* __restrict__ S2, int nsta,
+                                                       int nf, int cw, int ntile,
+                                                       const unsigned char* __restrict__ tiles,
+                                                       const int* __restrict__ pidx,
+                                                       const int* __restrict__ nonfinite,
+                                                       float2* __restrict__ acc) {
+  const int nt2 = ntile * ntile;
+  const int k = blockIdx.x >> 3, g = (blockIdx.x & 7) + 8 * (k / nt2), t = k % nt2;
+  if (g * 4 >= nf) return;                        // uniform over the block
+  const int ta = t / ntile, tb = t - ta * ntile;
+  if (!tiles[t] || *nonfinite) return;            // uniform over the block
+  const int j = g * 4 + (threadIdx.x >> 6);
+  if (j >= nf) return;                            // uniform over the wave
+  const int lane = threadIdx.x & 63, i = lane & 31, kq = lane >> 5;
+  const int a = ta * 32 + i, b = tb * 32 + i;
+  const bool av = a < nsta, bv = b < nsta;
+  const size_t rows = (size_t)cw * nsta;
+  const float2* Sj = S2 + (size_t)j * rows;
+  ccx16 gre, gim;
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    gre[v] = 0.0f;
+    gim[v] = 0.0f;
+  }
+  // this lane's 4 windows of the batch from w0 (zeros past cw)
+  auto load = [&](int w0, float2* xa, float2* xb) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int w = w0 + 4 * kq + u;
+      const bool ok = w < cw;
+      const float2* Sw = Sj + (size_t)(ok ? w : 0) * nsta;
+      xa[u] = ok && av ? Sw[a] : make_float2(0.0f, 0.0f);
+      xb[u] = ok && bv ? Sw[b] : make_float2(0.0f, 0.0f);
+    }
+  };
+  auto hi16 = [](float x) { return (_Float16)x; };
+  auto lo16 = [](float x) { return (_Float16)(x - (float)(_Float16)x); };
+  float2 xa[4], xb[4];
+  load(0, xa, xb);
+  for (int w0 = 0; w0 < cw; w0 += 8) {
+    float2 na[4], nb[4];
+    const bool more = w0 + 8 < cw;   // uniform
+    if (more) load(w0 + 8, na, nb);
+    cch8 ah, al, bh, bl, ch, cl;   // A; B of the real part; B of the imaginary part
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      ah[2 * u] = hi16(xa[u].x);
+      al[2 * u] = lo16(xa[u].x);
+      ah[2 * u + 1] = hi16(xa[u].y);
+      al[2 * u + 1] = lo16(xa[u].y);
+      bh[2 * u] = hi16(xb[u].x);
+      bl[2 * u] = lo16(xb[u].x);
+      bh[2 * u + 1] = hi16(xb[u].y);
+      bl[2 * u + 1] = lo16(xb[u].y);
+      ch[2 * u] = bh[2 * u + 1];
+      cl[2 * u] = bl[2 * u + 1];
+      ch[2 * u + 1] = -bh[2 * u];
+      cl[2 * u + 1] = -bl[2 * u];
+    }
+    gre = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, gre, 0, 0, 0);
+    gim = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, ch, gim, 0, 0, 0);
+    gre = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, gre, 0, 0, 0);
+    gim = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cl, gim, 0, 0, 0);
+    gre = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, gre, 0, 0, 0);
+    gim = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, ch, gim, 0, 0, 0);
+    if (more) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        xa[u] = na[u];
+        xb[u] = nb[u];
+      }
+    }
+  }
+  if (!bv) return;
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    const int ar = ta * 32 + 8 * (v >> 2) + 4 * kq + (v & 3);
+    if (ar >= nsta) continue;
+    const int p = pidx[(size_t)ar * nsta + b];
+    if (p < 0) continue;
+    float2* o = acc + (size_t)p * nf + j;
+    float2 s = *o;
+    s.x += gre[v];
+    s.y += gim[v];
+    *o = s;
+  }
+}
+
 // The per-pair form on the S2 layout, for what the Gram form leaves: every
 // pair when the chunk holds a non-finite used value, else the duplicates.
 __global__ __launch_bounds__(256) void k_cc_pairs_t(const float2* __restrict__ S2,
@@ -440,13 +539,13 @@ hipError_t launch_cc_pair_map(const int* pairs, int npairs, int nsta, int ntile,
   return hipGetLastError();
 }
 
-hipError_t launch_cc_gram(const void* S2, int nsta, int nf, int cw, int ntile,
+hipError_t launch_cc_gram(const void* S2, int nsta, int nf, int cw, int ntile, int whiten,
                           const unsigned char* tiles, const int* pidx, const int* nonfinite,
                           void* acc, hipStream_t st) {
   const int groups = (nf + 3) / 4;                 // 4 bins per workgroup
   const long long nwg = 8LL * ((groups + 7) / 8) * ntile * ntile;
   if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(k_cc_gram, dim3((unsigned)nwg), dim3(256), 0, st,
+  hipLaunchKernelGGL(whiten ? k_cc_gram_split : k_cc_gram, dim3((unsigned)nwg), dim3(256), 0, st,
                      (const float2*)S2, nsta, nf, cw, ntile, tiles, pidx, nonfinite,
                      (float2*)acc);
   return hipGetLastError();
