@@ -766,16 +766,30 @@ def bench_cc(args):
                                 "float32 re/im [npairs, nf] out in HBM", "stations": nsta,
                     "npts": npts, "fftlen": fftlen, "windows": nshifts, "pairs": npairs,
                     "nf": nf, "fstride": fstride, "pair_windows": pw})
-    # dominant kernel: the Gram pair stage, 8 flops (one complex multiply-
-    # add) per (pair, shared window, bin), on the FP32 matrix cores
-    gram_tflops = 8.0 * pw * nf / (gms / 1e3) / 1e12 if gms > 0 else 0.0
-    res["roofline"] = {"bound": "mfma", "achieved": round(gram_tflops, 2), "peak": FP32_PEAK,
-                       "unit": "TFLOP/s", "frac": round(gram_tflops / FP32_PEAK, 4),
+    # dominant kernel: the Gram pair stage.  Per launch it reads every
+    # station's spectra of the chunk once (8 bytes a value) and writes the
+    # pair sums (8 bytes a pair and bin; a later chunk also reads them);
+    # 8 flops (one complex multiply-add) per (pair, shared window, bin) on
+    # the fp16 matrix cores (k_cc_gram_split: three split products per FP32
+    # product), 70 flops a byte, under the 312 of the fp16 ridge
+    # (2500 TFLOP/s / 8 TB/s): HBM is the bounding roof.
+    launches = ng / max(args.steps, 1)
+    sums = 8.0 * npairs * nf * (2.0 * launches - 1.0) / launches
+    gram_bytes = 8.0 * nsta * nshifts * nf / launches + sums
+    lms = gms / max(launches, 1.0)   # average launch
+    gram_gbs = gram_bytes / (lms / 1e3) / 1e9 if lms > 0 else 0.0
+    gram_tflops = 8.0 * pw * nf / launches / (lms / 1e3) / 1e12 if lms > 0 else 0.0
+    res["roofline"] = {"bound": "hbm", "achieved": round(gram_gbs, 1), "peak": HBM_PEAK,
+                       "unit": "GB/s", "frac": round(gram_gbs / HBM_PEAK, 4),
                        "traffic": summary_value(CC_SUMMARY, "traffic_bytes_per_launch",
                                                 "cc_gram", int),
-                       "kernel": "k_cc_gram", "avg_ms": round(gms, 4),
-                       "launches_per_step": ng / max(args.steps, 1),
-                       "achieved_is": "8 flops per (pair, shared window, bin) / HIP-event time",
+                       "kernel": "k_cc_gram_split", "avg_ms": round(lms, 4),
+                       "launches_per_step": launches,
+                       "algorithmic_bytes_per_launch": int(gram_bytes),
+                       "achieved_is": "spectra read once + pair sums written, per launch / "
+                                      "HIP-event time",
+                       "mfma_tflops_algorithmic": round(gram_tflops, 2),
+                       "mfma_peak_fp16": FP16_MFMA_PEAK,
                        "executed_mfma_tflops": summary_value(CC_SUMMARY, "executed_mfma_tflops",
                                                              "cc_gram", float)}
     res["end_to_end_host"] = {"value": pw / wall_host, "unit": "pair-windows/s",

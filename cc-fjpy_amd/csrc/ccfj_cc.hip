@@ -141,12 +141,13 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
         // the pair stage's main kernel, timed when the context's timing is on
         std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
         if ((rc = ccfj_internal::timing_begin(ctx, st, &ev))) return rc;
-        e = fjk::launch_cc_gram(dS, nsta, nf, n, ntile, whiten, dtiles, dpidx, dflag, dacc, st);
+        e = fjk::launch_cc_gram(dS, nsta, nf, n, ntile, whiten, dtiles, dpidx, dflag,
+                                w0 == 0 && !accumulate, dacc, st);
         ccfj_internal::timing_end(ctx, st, ev);
       }
       if (e == hipSuccess)
-        e = fjk::launch_cc_pairs_t(dS, duse, pairs, npairs, nsta, nf, (int)w0, n, ddup, dflag,
-                                   dacc, st);
+        e = fjk::launch_cc_pairs_t(dS, whiten, duse, pairs, npairs, nsta, nf, (int)w0, n, ddup,
+                                   dflag, dacc, st);
       if (e != hipSuccess) return hip_fail(e, "Gram pair stage");
     } else {
       if ((e = fjk::launch_cc_spec(dspec, n * nsta, nbins, zhalf, nf, bstride, whiten, dS, st)) !=

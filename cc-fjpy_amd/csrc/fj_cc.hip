@@ -163,9 +163,28 @@ __global__ __launch_bounds__(256) void k_cc_pairs(const float2* __restrict__ S,
 // (whose product with a masked zero would give NaN where the reference
 // skips the window), fall back to the per-pair kernel.
 
+typedef _Float16 cch2 __attribute__((ext_vector_type(2)));
+
+// A whitened value (|re|, |im| <= 1) in split fp16 form, 8 bytes as its
+// float2: x = hi + lo, hi = x rounded to fp16, lo = x - hi rounded to fp16;
+// .x = re_hi | im_hi << 16, .y = re_lo | im_lo << 16.  The pair products
+// take these words as MFMA operands directly.
+__device__ inline uint2 split16(float2 v) {
+  const _Float16 rh = (_Float16)v.x, ih = (_Float16)v.y;
+  const _Float16 rl = (_Float16)(v.x - (float)rh), il = (_Float16)(v.y - (float)ih);
+  return make_uint2(__builtin_bit_cast(unsigned, cch2{rh, ih}),
+                    __builtin_bit_cast(unsigned, cch2{rl, il}));
+}
+
+__device__ inline float2 join16(uint2 u) {
+  const cch2 h = __builtin_bit_cast(cch2, u.x), l = __builtin_bit_cast(cch2, u.y);
+  return make_float2((float)h[0] + (float)l[0], (float)h[1] + (float)l[1]);
+}
+
 // S2[j][wl][s] = spectrum bin j*fstride of window w0+wl, station s,
 // whitened, zero where the station lacks the window; flags a non-finite
 // used value.  32 x 32 (rows x bins) tiles through LDS.
+// Whitened spectra are stored in the split form of split16.
 __global__ __launch_bounds__(256) void k_cc_spec_t(const float2* __restrict__ spec, int rows,
                                                    int nsta, int nbins, int zhalf, int nf,
                                                    int fstride,
@@ -197,7 +216,12 @@ __global__ __launch_bounds__(256) void k_cc_spec_t(const float2* __restrict__ sp
   __syncthreads();
   for (int k = ty; k < 32; k += 8) {
     const int j = j0 + k, row = r0 + tx;
-    if (j < nf && row < rows) S2[(size_t)j * rows + row] = t[tx][k];
+    if (j < nf && row < rows) {
+      if (whiten)
+        reinterpret_cast<uint2*>(S2)[(size_t)j * rows + row] = split16(t[tx][k]);
+      else
+        S2[(size_t)j * rows + row] = t[tx][k];
+    }
   }
 }
 
@@ -233,12 +257,18 @@ constexpr int kGramU = 8;
 // One wave per (32 x 32 station tile, bin); 4 waves (4 bins) per block.
 // acc[p][j] += G_j[a][b] for the listed pairs of the tile.
 // Workgroup numbering is XCD-major: workgroups go to the 8 XCDs in turn
-// (w mod 8), and the k-th workgroup of XCD x takes bin group
-// x + 8 (k / ntile^2) and tile k mod ntile^2, so that every tile of a bin
-// group runs on one XCD and reads the group's spectra through that XCD's
-// L2.  Numbered tile-major within each bin group (all tiles of a group on
-// consecutive workgroups, spread over the XCDs), each XCD fetched the
-// spectra again: 3.0 GB per call at the 200-station size, 8x the spectra.
+// (w mod 8), and the m-th bin group slot of XCD x (workgroups k =
+// m ntile^2 .. (m + 1) ntile^2 - 1 of that XCD, one per tile) takes bin
+// group 4 (8 (m / 4) + x) + m % 4, so that every tile of a bin group runs
+// on one XCD and reads the group's spectra through that XCD's L2, and the
+// 16 bins of 4 consecutive groups (one 128-byte line of a pair's sums) are
+// written through one L2.  Numbered tile-major within each bin group (all
+// tiles of a group on consecutive workgroups, spread over the XCDs), each
+// XCD fetched the spectra again: 3.0 GB per call at the 200-station size,
+// 8x the spectra.  With one group per XCD in turn (4 bins, 32 bytes of a
+// line), the split form took 0.64 ms against 0.52 ms for runs of 4, at
+// 200 stations, 231 windows and 1000 bins, its sums stored on the first
+// chunk.
 // v_mfma_f32_32x32x2_f32 operand layout (lane l): A[i = l%32][k = l/32],
 // B[k = l/32][n = l%32]; result register v holds row 8(v/4) + 4(l/32) + v%4,
 // column l%32.
@@ -247,9 +277,10 @@ __global__ __launch_bounds__(256) void k_cc_gram(const float2* __restrict__ S2, 
                                                  const unsigned char* __restrict__ tiles,
                                                  const int* __restrict__ pidx,
                                                  const int* __restrict__ nonfinite,
-                                                 float2* __restrict__ acc) {
+                                                 int first, float2* __restrict__ acc) {
   const int nt2 = ntile * ntile;
-  const int k = blockIdx.x >> 3, g = (blockIdx.x & 7) + 8 * (k / nt2), t = k % nt2;
+  const int k = blockIdx.x >> 3, m = k / nt2, t = k % nt2;
+  const int g = (((m >> 2) << 3) + (blockIdx.x & 7)) * 4 + (m & 3);
   if (g * 4 >= nf) return;                        // uniform over the block
   const int ta = t / ntile, tb = t - ta * ntile;
   if (!tiles[t] || *nonfinite) return;            // uniform over the block
@@ -309,7 +340,7 @@ __global__ __launch_bounds__(256) void k_cc_gram(const float2* __restrict__ S2, 
     const int p = pidx[(size_t)ar * nsta + b];
     if (p < 0) continue;
     float2* o = acc + (size_t)p * nf + j;
-    float2 s = *o;
+    float2 s = first ? make_float2(0.0f, 0.0f) : *o;   // first: acc holds zeros
     s.x += gre[v];
     s.y += gim[v];
     *o = s;
@@ -318,87 +349,91 @@ __global__ __launch_bounds__(256) void k_cc_gram(const float2* __restrict__ S2, 
 
 // The Gram product of whitened spectra (|x| <= 1) in split fp16 products
 // on v_mfma_f32_32x32x16_f16: each value held as hi = x rounded to fp16 and
-// lo = the rounded remainder (22 significand bits together), each product
-// as hi hi + hi lo + lo hi with FP32 accumulation (about 2^-22 of |a||b|
-// per term).  K = 16 holds 8 windows x (re, im): lane l takes station
-// rows i = l % 32 and windows 4 (l / 32) + 0..3 of each batch of 8, A[i][k]
-// = (ar, ai) of its windows; the real part takes B[k][n] = (br, bi), the
-// imaginary part (bi, -br) against the same A.  Six 32x32x16 MFMAs per 8
-// windows against sixteen 32x32x2 FP32 ones, and each lane loads only its
-// own windows.  Same numbering, tiles and result layout as k_cc_gram.
+// lo = the rounded remainder (22 significand bits together; split16, written
+// by k_cc_spec_t), each product as hi hi + hi lo + lo hi with FP32
+// accumulation (about 2^-22 of |a||b| per term).  K = 16 holds 8 windows x
+// (re, im): lane l takes station rows i = l % 32 and windows 4 (l / 32) +
+// 0..3 of each batch of 8, A[i][k] = (ar, ai) of its windows; the real part
+// takes B[k][n] = (br, bi), the imaginary part (bi, -br) against the same A.
+// The stored words are the operands: A and the real part's B as loaded, the
+// imaginary part's B a half swap and a sign flip per word.  Six 32x32x16
+// MFMAs per 8 windows against sixteen 32x32x2 FP32 ones, and each lane loads
+// only its own windows; two batches in flight, the next one's loads issued
+// before the current one's products.  Same numbering, tiles and result
+// layout as k_cc_gram.
 typedef _Float16 cch8 __attribute__((ext_vector_type(8)));
-__global__ __launch_bounds__(256) void k_cc_gram_split(const float2* __restrict__ S2, int nsta,
+typedef unsigned ccu4 __attribute__((ext_vector_type(4)));
+
+// At least 6 waves per SIMD (at most 80 registers, no spills): 0.46-0.48 ms
+// against 0.52 at the 5 its 94 registers allow unbounded, 0.59 at 8 (spills
+// to scratch), and 0.49 at 6 with each batch's loads waited for at once.
+__global__ __launch_bounds__(256, 6) void k_cc_gram_split(const uint2* __restrict__ S2, int nsta,
                                                        int nf, int cw, int ntile,
                                                        const unsigned char* __restrict__ tiles,
                                                        const int* __restrict__ pidx,
                                                        const int* __restrict__ nonfinite,
-                                                       float2* __restrict__ acc) {
+                                                       int first, float2* __restrict__ acc) {
   const int nt2 = ntile * ntile;
-  const int k = blockIdx.x >> 3, g = (blockIdx.x & 7) + 8 * (k / nt2), t = k % nt2;
+  const int k = blockIdx.x >> 3, m = k / nt2, t = k % nt2;   // numbered as k_cc_gram
+  const int g = (((m >> 2) << 3) + (blockIdx.x & 7)) * 4 + (m & 3);
   if (g * 4 >= nf) return;                        // uniform over the block
   const int ta = t / ntile, tb = t - ta * ntile;
   if (!tiles[t] || *nonfinite) return;            // uniform over the block
-  const int j = g * 4 + (threadIdx.x >> 6);
+  const int j = g * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (j >= nf) return;                            // uniform over the wave
   const int lane = threadIdx.x & 63, i = lane & 31, kq = lane >> 5;
   const int a = ta * 32 + i, b = tb * 32 + i;
-  const bool av = a < nsta, bv = b < nsta;
-  const size_t rows = (size_t)cw * nsta;
-  const float2* Sj = S2 + (size_t)j * rows;
+  const bool bv = b < nsta;
+  // rows and columns of stations past nsta are never stored: they read
+  // the last station's values instead of branching round their loads
+  const int ac = min(a, nsta - 1), bc = min(b, nsta - 1);
+  const uint2* Sj = S2 + (size_t)j * cw * nsta;   // uniform over the wave
   ccx16 gre, gim;
 #pragma unroll
   for (int v = 0; v < 16; v++) {
     gre[v] = 0.0f;
     gim[v] = 0.0f;
   }
-  // this lane's 4 windows of the batch from w0 (zeros past cw)
-  auto load = [&](int w0, float2* xa, float2* xb) {
+  // this lane's 4 windows of the batch from w0; a window past cw reads
+  // window cw - 1 and its A operand is zeroed, so that it adds nothing
+  const int last = (cw - 1) * nsta;
+  auto load = [&](int w0, uint2* xa, uint2* xb) {
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int w = w0 + 4 * kq + u;
-      const bool ok = w < cw;
-      const float2* Sw = Sj + (size_t)(ok ? w : 0) * nsta;
-      xa[u] = ok && av ? Sw[a] : make_float2(0.0f, 0.0f);
-      xb[u] = ok && bv ? Sw[b] : make_float2(0.0f, 0.0f);
+      const int wq = 4 * kq + u;                  // the window's place in the batch
+      const bool ok = wq < cw - w0;
+      const int o = min(w0 * nsta + wq * nsta, last);
+      const uint2 va = Sj[(unsigned)(o + ac)];
+      xa[u] = make_uint2(ok ? va.x : 0u, ok ? va.y : 0u);
+      xb[u] = Sj[(unsigned)(o + bc)];
     }
   };
-  auto hi16 = [](float x) { return (_Float16)x; };
-  auto lo16 = [](float x) { return (_Float16)(x - (float)(_Float16)x); };
-  float2 xa[4], xb[4];
-  load(0, xa, xb);
-  for (int w0 = 0; w0 < cw; w0 += 8) {
-    float2 na[4], nb[4];
-    const bool more = w0 + 8 < cw;   // uniform
-    if (more) load(w0 + 8, na, nb);
-    cch8 ah, al, bh, bl, ch, cl;   // A; B of the real part; B of the imaginary part
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      ah[2 * u] = hi16(xa[u].x);
-      al[2 * u] = lo16(xa[u].x);
-      ah[2 * u + 1] = hi16(xa[u].y);
-      al[2 * u + 1] = lo16(xa[u].y);
-      bh[2 * u] = hi16(xb[u].x);
-      bl[2 * u] = lo16(xb[u].x);
-      bh[2 * u + 1] = hi16(xb[u].y);
-      bl[2 * u + 1] = lo16(xb[u].y);
-      ch[2 * u] = bh[2 * u + 1];
-      cl[2 * u] = bl[2 * u + 1];
-      ch[2 * u + 1] = -bh[2 * u];
-      cl[2 * u + 1] = -bl[2 * u];
-    }
+  // (re, im) word -> (im, -re)
+  auto conj_swap = [](unsigned x) { return __builtin_amdgcn_alignbit(x, x, 16) ^ 0x80000000u; };
+  auto batch = [&](const uint2* xa, const uint2* xb) {
+    const cch8 ah = __builtin_bit_cast(cch8, ccu4{xa[0].x, xa[1].x, xa[2].x, xa[3].x});
+    const cch8 al = __builtin_bit_cast(cch8, ccu4{xa[0].y, xa[1].y, xa[2].y, xa[3].y});
+    const cch8 bh = __builtin_bit_cast(cch8, ccu4{xb[0].x, xb[1].x, xb[2].x, xb[3].x});
+    const cch8 bl = __builtin_bit_cast(cch8, ccu4{xb[0].y, xb[1].y, xb[2].y, xb[3].y});
+    const cch8 ch = __builtin_bit_cast(cch8, ccu4{conj_swap(xb[0].x), conj_swap(xb[1].x),
+                                                  conj_swap(xb[2].x), conj_swap(xb[3].x)});
+    const cch8 cl = __builtin_bit_cast(cch8, ccu4{conj_swap(xb[0].y), conj_swap(xb[1].y),
+                                                  conj_swap(xb[2].y), conj_swap(xb[3].y)});
     gre = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, gre, 0, 0, 0);
     gim = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, ch, gim, 0, 0, 0);
     gre = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, gre, 0, 0, 0);
     gim = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, cl, gim, 0, 0, 0);
     gre = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, gre, 0, 0, 0);
     gim = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, ch, gim, 0, 0, 0);
-    if (more) {
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        xa[u] = na[u];
-        xb[u] = nb[u];
-      }
-    }
+  };
+  // batches in pairs; a batch wholly past cw adds nothing
+  uint2 xa[4], xb[4], na[4], nb[4];
+  load(0, xa, xb);
+  for (int w0 = 0; w0 < cw; w0 += 16) {           // uniform
+    load(w0 + 8, na, nb);
+    batch(xa, xb);
+    load(w0 + 16, xa, xb);
+    batch(na, nb);
   }
   if (!bv) return;
 #pragma unroll
@@ -408,7 +443,7 @@ __global__ __launch_bounds__(256) void k_cc_gram_split(const float2* __restrict_
     const int p = pidx[(size_t)ar * nsta + b];
     if (p < 0) continue;
     float2* o = acc + (size_t)p * nf + j;
-    float2 s = *o;
+    float2 s = first ? make_float2(0.0f, 0.0f) : *o;   // first: acc holds zeros
     s.x += gre[v];
     s.y += gim[v];
     *o = s;
@@ -417,6 +452,7 @@ __global__ __launch_bounds__(256) void k_cc_gram_split(const float2* __restrict_
 
 // The per-pair form on the S2 layout, for what the Gram form leaves: every
 // pair when the chunk holds a non-finite used value, else the duplicates.
+template <bool SPLIT>
 __global__ __launch_bounds__(256) void k_cc_pairs_t(const float2* __restrict__ S2,
                                                     const unsigned char* __restrict__ use,
                                                     const int* __restrict__ pairs, int nsta,
@@ -434,8 +470,11 @@ __global__ __launch_bounds__(256) void k_cc_pairs_t(const float2* __restrict__ S
   for (int wl = 0; wl < cw; wl++) {
     const unsigned char* u = use + (size_t)(w0 + wl) * nsta;
     if (!(u[a] && u[b])) continue;   // uniform over the block
-    const float2 xa = S2[(size_t)j * rows + (size_t)wl * nsta + a];
-    const float2 xb = S2[(size_t)j * rows + (size_t)wl * nsta + b];
+    const size_t ka = (size_t)j * rows + (size_t)wl * nsta + a;
+    const size_t kb = (size_t)j * rows + (size_t)wl * nsta + b;
+    const uint2* U2 = reinterpret_cast<const uint2*>(S2);
+    const float2 xa = SPLIT ? join16(U2[ka]) : S2[ka];
+    const float2 xb = SPLIT ? join16(U2[kb]) : S2[kb];
     re = __builtin_fmaf(xa.x, xb.x, __builtin_fmaf(xa.y, xb.y, re));
     im = __builtin_fmaf(xa.x, xb.y, __builtin_fmaf(-xa.y, xb.x, im));
   }
@@ -541,21 +580,26 @@ hipError_t launch_cc_pair_map(const int* pairs, int npairs, int nsta, int ntile,
 
 hipError_t launch_cc_gram(const void* S2, int nsta, int nf, int cw, int ntile, int whiten,
                           const unsigned char* tiles, const int* pidx, const int* nonfinite,
-                          void* acc, hipStream_t st) {
+                          int first, void* acc, hipStream_t st) {
   const int groups = (nf + 3) / 4;                 // 4 bins per workgroup
-  const long long nwg = 8LL * ((groups + 7) / 8) * ntile * ntile;
+  // 8 XCDs x runs of 4 groups
+  const long long nwg = 32LL * ((groups + 31) / 32) * ntile * ntile;
   if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(whiten ? k_cc_gram_split : k_cc_gram, dim3((unsigned)nwg), dim3(256), 0, st,
-                     (const float2*)S2, nsta, nf, cw, ntile, tiles, pidx, nonfinite,
-                     (float2*)acc);
+  if (whiten)
+    hipLaunchKernelGGL(k_cc_gram_split, dim3((unsigned)nwg), dim3(256), 0, st, (const uint2*)S2,
+                       nsta, nf, cw, ntile, tiles, pidx, nonfinite, first, (float2*)acc);
+  else
+    hipLaunchKernelGGL(k_cc_gram, dim3((unsigned)nwg), dim3(256), 0, st, (const float2*)S2, nsta,
+                       nf, cw, ntile, tiles, pidx, nonfinite, first, (float2*)acc);
   return hipGetLastError();
 }
 
-hipError_t launch_cc_pairs_t(const void* S2, const unsigned char* use, const int* pairs,
-                             int npairs, int nsta, int nf, int w0, int cw,
+hipError_t launch_cc_pairs_t(const void* S2, int whiten, const unsigned char* use,
+                             const int* pairs, int npairs, int nsta, int nf, int w0, int cw,
                              const unsigned char* dup, const int* nonfinite, void* acc,
                              hipStream_t st) {
-  hipLaunchKernelGGL(k_cc_pairs_t, dim3(npairs, (nf + 255) / 256), dim3(256), 0, st,
+  hipLaunchKernelGGL(whiten ? k_cc_pairs_t<true> : k_cc_pairs_t<false>,
+                     dim3(npairs, (nf + 255) / 256), dim3(256), 0, st,
                      (const float2*)S2, use, pairs, nsta, nf, w0, cw, dup, nonfinite,
                      (float2*)acc);
   return hipGetLastError();
