@@ -40,29 +40,78 @@ __global__ void k_cc_onebit(float* __restrict__ x, size_t n) {
 }
 
 // seg[wl][i][t] = sum over q < fold of data[i][(w0 + wl) * steplen + t + q L],
-// t < L = fftlen / fold; grid (wl, i), one workgroup per window row (a
-// workgroup per 256 samples made 878k short workgroups for 231 windows of
-// 200 stations, 0.70 ms, bound by their dispatch), V consecutive samples per
-// thread and step: V = 4 (16-byte loads and stores) when L, npts and
-// steplen are multiples of 4, else 1.  fold = 1 is the plain window.
-// Folding by the bin stride s (when s divides fftlen) is exact for the bins
-// that are kept: X[s m] = sum_n (sum_q x[n + q L]) e^{-2 pi i m n / L}, so
-// an rFFT of length L gives bins 0, s, 2s, ... of the length-fftlen one.
-template <int V>
-__global__ __launch_bounds__(256) void k_cc_gather(const float* __restrict__ data, int npts,
-                                                   int L, int fold, int steplen, int w0,
-                                                   int nsta, float* __restrict__ seg) {
-  using VT = typename std::conditional<V == 4, float4, float>::type;
-  const int wl = blockIdx.x;
-  const int i = blockIdx.y;
-  const VT* x = reinterpret_cast<const VT*>(data + (size_t)i * npts + (size_t)(w0 + wl) * steplen);
-  VT* y = reinterpret_cast<VT*>(seg + ((size_t)wl * nsta + i) * L);
-  const int n = L / V, qs = L / V;
+// t < L = fftlen / fold; one workgroup per window row (a workgroup per 256
+// samples made 878k short workgroups for 231 windows of 200 stations,
+// 0.70 ms, bound by their dispatch).  Rows are numbered station-major, (i,
+// wl) -> i cw + wl, and split into 8 contiguous ranges, one per XCD
+// (workgroup b runs on XCD b mod 8 and takes row (b mod 8) per + b / 8):
+// the overlapping windows of a station run on one XCD and share its L2.
+// Numbered window-fastest across the XCDs, each XCD read nearly every
+// record (90% overlap, 8 XCDs).  V = 4: 16-byte stores and loads, a window
+// start off the 16-byte grid read as two aligned loads and a shift (SH =
+// start mod 4, uniform over the workgroup); V = 1: one sample a step.
+// fold = 1 is the plain window.  Folding by the bin stride s (when s
+// divides fftlen) is exact for the bins that are kept: X[s m] = sum_n
+// (sum_q x[n + q L]) e^{-2 pi i m n / L}, so an rFFT of length L gives bins
+// 0, s, 2s, ... of the length-fftlen one.
+template <int SH>
+__device__ inline float4 load_shifted(const float4* __restrict__ xb, int t) {
+  const float4 A = xb[t];
+  if (SH == 0) return A;
+  // the last sample wanted, xb[t + 1] element SH - 1, is in the window, so
+  // the whole aligned 16 bytes of xb[t + 1] are in the array
+  const float4 B = xb[t + 1];
+  if (SH == 1) return make_float4(A.y, A.z, A.w, B.x);
+  if (SH == 2) return make_float4(A.z, A.w, B.x, B.y);
+  return make_float4(A.w, B.x, B.y, B.z);
+}
+
+template <int SH>
+__device__ inline void gather_row4(const float4* __restrict__ xb, float4* __restrict__ y, int n,
+                                   int fold) {
 #pragma unroll 4
   for (int t = threadIdx.x; t < n; t += 256) {
-    VT v = x[t];
-    for (int q = 1; q < fold; q++) v = v + x[t + q * qs];
+    float4 v = load_shifted<SH>(xb, t);
+    for (int q = 1; q < fold; q++) {
+      const float4 u = load_shifted<SH>(xb, t + q * n);
+      v.x += u.x;
+      v.y += u.y;
+      v.z += u.z;
+      v.w += u.w;
+    }
     y[t] = v;
+  }
+}
+
+template <int V>
+__global__ __launch_bounds__(256) void k_cc_gather(const float* __restrict__ data, int npts,
+                                                   int L, int fold, int steplen, int w0, int cw,
+                                                   int nsta, float* __restrict__ seg) {
+  const int per = gridDim.x >> 3;
+  const long long row = (long long)(blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (row >= (long long)cw * nsta) return;         // uniform over the block
+  const int i = (int)(row / cw), wl = (int)(row - (long long)i * cw);
+  const size_t s0 = (size_t)i * npts + (size_t)(w0 + wl) * steplen;
+  float* y = seg + ((size_t)wl * nsta + i) * L;
+  if (V == 4) {
+    const int sh = (int)(s0 & 3);
+    const float4* xb = reinterpret_cast<const float4*>(data + (s0 - sh));
+    float4* y4 = reinterpret_cast<float4*>(y);
+    const int n = L / 4;
+    switch (sh) {   // uniform
+      case 0: gather_row4<0>(xb, y4, n, fold); break;
+      case 1: gather_row4<1>(xb, y4, n, fold); break;
+      case 2: gather_row4<2>(xb, y4, n, fold); break;
+      default: gather_row4<3>(xb, y4, n, fold); break;
+    }
+  } else {
+    const float* x = data + s0;
+#pragma unroll 4
+    for (int t = threadIdx.x; t < L; t += 256) {
+      float v = x[t];
+      for (int q = 1; q < fold; q++) v += x[t + q * L];
+      y[t] = v;
+    }
   }
 }
 
@@ -524,12 +573,18 @@ hipError_t launch_cc_onebit(float* x, size_t n, hipStream_t st) {
 
 hipError_t launch_cc_gather(const float* data, int nsta, int npts, int L, int fold, int steplen,
                             int w0, int cw, float* seg, hipStream_t st) {
-  if (L % 4 == 0 && npts % 4 == 0 && steplen % 4 == 0)
-    hipLaunchKernelGGL(k_cc_gather<4>, dim3(cw, nsta), dim3(256), 0, st, data, npts, L, fold,
-                       steplen, w0, nsta, seg);
+  const long long rows = (long long)cw * nsta, per = (rows + 7) / 8;
+  if (rows == 0) return hipSuccess;
+  if (8 * per > 0x7fffffffLL) return hipErrorInvalidValue;
+  // V = 4 needs 16-byte rows of seg and an array that ends on the 16-byte
+  // grid (the shifted loads read whole aligned 16 bytes)
+  if (L % 4 == 0 && (size_t)nsta * npts % 4 == 0 && ((uintptr_t)data & 15) == 0 &&
+      ((uintptr_t)seg & 15) == 0)
+    hipLaunchKernelGGL(k_cc_gather<4>, dim3((unsigned)(8 * per)), dim3(256), 0, st, data, npts, L,
+                       fold, steplen, w0, cw, nsta, seg);
   else
-    hipLaunchKernelGGL(k_cc_gather<1>, dim3(cw, nsta), dim3(256), 0, st, data, npts, L, fold,
-                       steplen, w0, nsta, seg);
+    hipLaunchKernelGGL(k_cc_gather<1>, dim3((unsigned)(8 * per)), dim3(256), 0, st, data, npts, L,
+                       fold, steplen, w0, cw, nsta, seg);
   return hipGetLastError();
 }
 

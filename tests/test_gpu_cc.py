@@ -165,6 +165,29 @@ def test_cc_non_finite_samples_follow_the_reference(ccfj):
     assert err <= TOL, err
 
 
+@pytest.mark.parametrize("nsta,npts", [(4, 5001), (5, 5001), (4, 5000)])
+def test_cc_window_starts_off_the_16_byte_grid(ccfj, nsta, npts):
+    """90% overlap of 1200-sample windows gives a step of 119 samples (the
+    reference's int(fftlen * (1 - overlaprate))), so window starts fall at
+    every offset mod 4, and with npts = 5001 so do the station rows: the
+    gather's shifted 16-byte loads (nsta * npts a multiple of 4) and its
+    one-sample path (5 x 5001), folded by the bin stride 3, against the
+    oracle."""
+    fftlen, fstride = 1200, 3
+    nf = (fftlen // 2) // fstride
+    x, se = _case(9, nsta, npts, gaps=True)
+    pairs = ccfj.GetStationPairs(nsta)
+    steplen = int(fftlen * (1 - 0.9))
+    assert steplen % 2 == 1
+    got = ccfj.CC(npts, nsta, nf, fftlen, pairs, se, x.reshape(-1).copy(), overlaprate=0.9,
+                  fstride=fstride)
+    want, count = C.cross_correlate(x.reshape(-1), nsta, npts, nf, fftlen, steplen, pairs, se,
+                                    fstride=fstride)
+    assert np.all(count > 0)
+    err = np.max(np.abs(got - want)) / np.max(np.abs(want))
+    assert err <= TOL, err
+
+
 @pytest.mark.parametrize("fftlen,fstride", [(1200, 3), (1000, 3), (1024, 4)])
 def test_cc_bin_stride_fold_and_plain(ccfj, fftlen, fstride):
     """A bin stride that divides the window folds it before a shorter rFFT
