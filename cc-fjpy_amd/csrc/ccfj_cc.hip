@@ -113,7 +113,8 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
   if (accumulate) {
     if ((e = fjk::launch_cc_interleave(re, im, (size_t)npairs * nf, dacc, st)) != hipSuccess)
       return hip_fail(e, "initial sums");
-  } else if ((e = fjk::launch_fill((float*)dacc, 2 * (size_t)npairs * nf, 0.0f, st)) != hipSuccess) {
+  } else if (!(gram && nshifts > 0) &&   // the Gram form stores every pair on the first chunk
+             (e = fjk::launch_fill((float*)dacc, 2 * (size_t)npairs * nf, 0.0f, st)) != hipSuccess) {
     return hip_fail(e, "fill kernel");
   }
   for (long w0 = 0; w0 < nshifts; w0 += cw) {
@@ -147,7 +148,7 @@ int cc_run(ccfj_ctx* ctx, int nsta, int npts, int nf, int fstride, int fftlen, i
       }
       if (e == hipSuccess)
         e = fjk::launch_cc_pairs_t(dS, whiten, duse, pairs, npairs, nsta, nf, (int)w0, n, ddup,
-                                   dflag, dacc, st);
+                                   dflag, w0 == 0 && !accumulate, dacc, st);
       if (e != hipSuccess) return hip_fail(e, "Gram pair stage");
     } else {
       if ((e = fjk::launch_cc_spec(dspec, n * nsta, nbins, zhalf, nf, bstride, whiten, dS, st)) !=

@@ -25,15 +25,18 @@ hipError_t launch_cc_pair_map(const int* pairs, int npairs, int nsta, int ntile,
                               unsigned char* dup, unsigned char* tiles, hipStream_t st);
 // whiten: the spectra are whitened (|x| <= 1), and the product takes split
 // fp16 MFMAs (k_cc_gram_split); else FP32 ones (k_cc_gram).  first: acc
-// holds zeros, which the split form stores over instead of reading
+// is not initialised; the listed pairs are stored instead of added to
 hipError_t launch_cc_gram(const void* S2, int nsta, int nf, int cw, int ntile, int whiten,
                           const unsigned char* tiles, const int* pidx, const int* nonfinite,
                           int first, void* acc, hipStream_t st);
-// whiten: S2 holds the split form of launch_cc_spec_t
+// whiten: S2 holds the split form of launch_cc_spec_t.  first: acc is
+// not initialised; stores every pair that launch_cc_gram (first) leaves
+// (duplicates, all pairs of a chunk with a non-finite value, zeros for a
+// pair naming a station outside [0, nsta))
 hipError_t launch_cc_pairs_t(const void* S2, int whiten, const unsigned char* use,
                              const int* pairs, int npairs, int nsta, int nf, int w0, int cw,
-                             const unsigned char* dup, const int* nonfinite, void* acc,
-                             hipStream_t st);
+                             const unsigned char* dup, const int* nonfinite, int first,
+                             void* acc, hipStream_t st);
 hipError_t launch_cc_fill_i32(int* x, size_t n, int v, hipStream_t st);
 hipError_t launch_cc_interleave(const float* re, const float* im, size_t n, void* acc,
                                 hipStream_t st);
