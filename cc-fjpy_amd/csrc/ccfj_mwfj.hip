@@ -143,14 +143,16 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
     if ((rc = ccfj_internal::ctx_scratch(ctx, 5, sizeof(float) * nf * Kp * d.nrhs, &dB))) return rc;
     const size_t tb = align256(sizeof(float) * (size_t)nwin * nf * nc);
     const size_t mb = align256(sizeof(float) * (size_t)nwin * nf);
-    if ((rc = ccfj_internal::ctx_scratch(ctx, 6, tb + mb + sizeof(float) * (size_t)nf, &dT)))
+    const size_t pb = align256(sizeof(float) * (size_t)fjk::mwfj_groups(nc) * nwin * nf);
+    if ((rc = ccfj_internal::ctx_scratch(ctx, 6, tb + mb + pb + sizeof(float) * (size_t)nf, &dT)))
       return rc;
     e = fjk::launch_fill((float*)dB, nf * Kp * d.nrhs, 0.0f, st);
     if (e != hipSuccess) return hip_fail(e, "fill kernel launch");
     d.B = (float*)dB;
     d.T = (float*)dT;
     d.M = (float*)((char*)dT + tb);
-    d.bmax = (unsigned int*)((char*)dT + tb + mb);
+    d.Mp = (float*)((char*)dT + tb + mb);
+    d.bmax = (unsigned int*)((char*)dT + tb + mb + pb);
     e = fjk::launch_fill((float*)d.bmax, nf, 0.0f, st);   // bits 0: max |B| = 0
     if (e != hipSuccess) return hip_fail(e, "fill kernel launch");
   } else {

@@ -39,9 +39,10 @@ hipError_t launch_fj_linseg(int mode, const Problem& p, hipStream_t st);
 hipError_t launch_finish(const float* T, float* m, int nb, int nf, int nc, float* out,
                          hipStream_t st);
 // the same normalised transpose into float64 (MWFJ's output type); m is
-// required (nb x nf floats of workspace)
+// required (nb x nf floats): the column maxima are formed into it when
+// rowmax, else it holds them already
 hipError_t launch_finish_f64(const float* T, float* m, int nb, int nf, int nc, double* out,
-                             hipStream_t st);
+                             bool rowmax, hipStream_t st);
 // p[0..n) = v (a kernel, so that a captured graph holds no memset node)
 hipError_t launch_fill(float* p, size_t n, float v, hipStream_t st);
 

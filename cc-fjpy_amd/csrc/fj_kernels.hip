@@ -193,11 +193,13 @@ hipError_t launch_finish(const float* T, float* m, int nb, int nf, int nc, float
 }
 
 hipError_t launch_finish_f64(const float* T, float* m, int nb, int nf, int nc, double* out,
-                             hipStream_t st) {
+                             bool rowmax, hipStream_t st) {
   if (nb <= 0 || nf <= 0 || nc <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rowmax, dim3(nf, nb), dim3(256), 0, st, T, nf, nc, m);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+  if (rowmax) {
+    hipLaunchKernelGGL(k_rowmax, dim3(nf, nb), dim3(256), 0, st, T, nf, nc, m);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_finish<double>, dim3((nc + 63) / 64, (nf + 63) / 64, nb), dim3(256), 0,
                      st, T, (const float*)m, nf, nc, out);
   return hipGetLastError();

@@ -28,6 +28,7 @@ struct MwfjDev {
   float* B;            // [nf][Kp][nrhs] coefficients (itype 0)
   float* T;            // [nwin][nf][nc] |O| before normalisation (itype 0)
   float* M;            // [nwin][nf] column maxima (itype 0)
+  float* Mp;           // [mwfj_groups(nc)][nwin][nf] their partials per 32 velocities
   unsigned int* bmax;  // [nf] bits of max |B| per frequency (itype 0; zeroed before
                        // the coefficient kernels, which raise it by atomicMax)
   double* out;         // [nwin][nc][nf]
@@ -47,6 +48,8 @@ hipError_t launch_mwfj_support(const MwfjDev& d, unsigned long long* sum, hipStr
 hipError_t launch_mwfj_stations(const MwfjDev& d, double* rr, float* ir, hipStream_t st);
 hipError_t launch_mwfj_coef(const MwfjDev& d, int w0, int cw, hipStream_t st);
 hipError_t launch_mwfj_mfma(const MwfjDev& d, hipStream_t st);
+// velocity groups of 32 of the contraction kernel's grid (partial maxima)
+int mwfj_groups(int nc);
 hipError_t launch_mwfj_finish(const MwfjDev& d, hipStream_t st);
 hipError_t launch_mwfj_stage(const MwfjDev& d, int wl, float* re, float* im, hipStream_t st);
 hipError_t launch_to_double(const float* a, size_t n, double* out, hipStream_t st);
