@@ -146,9 +146,9 @@ int mwfj_run(ccfj_ctx* ctx, int itype, int func, int nr, int npts, int nwin, int
     const size_t pb = align256(sizeof(float) * (size_t)fjk::mwfj_groups(nc) * nwin * nf);
     if ((rc = ccfj_internal::ctx_scratch(ctx, 6, tb + mb + pb + sizeof(float) * (size_t)nf, &dT)))
       return rc;
-    e = fjk::launch_fill((float*)dB, nf * Kp * d.nrhs, 0.0f, st);
-    if (e != hipSuccess) return hip_fail(e, "fill kernel launch");
     d.B = (float*)dB;
+    e = fjk::launch_mwfj_bpad(d, st);   // the coefficient kernels write the rest
+    if (e != hipSuccess) return hip_fail(e, "padding kernel launch");
     d.T = (float*)dT;
     d.M = (float*)((char*)dT + tb);
     d.Mp = (float*)((char*)dT + tb + mb);

@@ -46,6 +46,9 @@ hipError_t launch_mwfj_rowbad(const MwfjDev& d, int* rowbad, hipStream_t st);
 // sum of the windows' support lengths into *sum (device; zeroed by the caller)
 hipError_t launch_mwfj_support(const MwfjDev& d, unsigned long long* sum, hipStream_t st);
 hipError_t launch_mwfj_stations(const MwfjDev& d, double* rr, float* ir, hipStream_t st);
+// zeros in the padding of B (rows nr..half-1, columns 2 nwin..nrhs-1) that
+// launch_mwfj_coef leaves; nothing to do when there is none
+hipError_t launch_mwfj_bpad(const MwfjDev& d, hipStream_t st);
 hipError_t launch_mwfj_coef(const MwfjDev& d, int w0, int cw, hipStream_t st);
 hipError_t launch_mwfj_mfma(const MwfjDev& d, hipStream_t st);
 // velocity groups of 32 of the contraction kernel's grid (partial maxima)

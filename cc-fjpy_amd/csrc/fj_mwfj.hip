@@ -31,6 +31,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "fj_lane.h"
 #include "fj_kernels.h"
 #include "fj_mwfj.h"
@@ -750,6 +752,48 @@ hipError_t launch_mwfj_support(const MwfjDev& d, unsigned long long* sum, hipStr
 hipError_t launch_mwfj_stations(const MwfjDev& d, double* rr, float* ir, hipStream_t st) {
   hipLaunchKernelGGL(k_mwfj_stations, dim3((d.half + 255) / 256), dim3(256), 0, st, d.r, d.nr,
                      d.half, rr, ir);
+  return hipGetLastError();
+}
+
+// Zeros in the padding of B [nf][Kp][nrhs] that k_mwfj_coef never writes:
+// rows nr .. half-1 of each basis half (all columns), then columns
+// 2 nwin .. nrhs-1 of rows below nr; e runs over the first set, then the
+// second.  k_mwfj_coef writes every other entry on each call.
+__global__ __launch_bounds__(256) void k_mwfj_bpad(float* __restrict__ B, int nf, int nh,
+                                                   int half, int nr, int nrhs, int ncols,
+                                                   size_t nrowpad, size_t ntotal) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < ntotal;
+       e += (size_t)gridDim.x * blockDim.x) {
+    size_t row, col;
+    if (e < nrowpad) {   // (f, h, s in [nr, half), col in [0, nrhs))
+      col = e % nrhs;
+      size_t q = e / nrhs;
+      const size_t s = nr + q % (size_t)(half - nr);
+      q /= (size_t)(half - nr);
+      const size_t h = q % nh, f = q / nh;
+      row = f * (size_t)nh * half + h * half + s;
+    } else {             // (f, h, s in [0, nr), col in [ncols, nrhs))
+      const size_t e2 = e - nrowpad;
+      col = ncols + e2 % (size_t)(nrhs - ncols);
+      size_t q = e2 / (size_t)(nrhs - ncols);
+      const size_t s = q % nr;
+      q /= nr;
+      const size_t h = q % nh, f = q / nh;
+      row = f * (size_t)nh * half + h * half + s;
+    }
+    B[row * nrhs + col] = 0.0f;
+  }
+}
+
+hipError_t launch_mwfj_bpad(const MwfjDev& d, hipStream_t st) {
+  const int nh = d.func ? 2 : 1, ncols = 2 * d.nwin;
+  const size_t nrowpad = (size_t)d.nf * nh * (d.half - d.nr) * d.nrhs;
+  const size_t ncolpad = (size_t)d.nf * nh * d.nr * (d.nrhs - ncols);
+  const size_t ntotal = nrowpad + ncolpad;
+  if (ntotal == 0) return hipSuccess;   // no padding: every entry is written by k_mwfj_coef
+  const unsigned nb = (unsigned)std::min<size_t>((ntotal + 255) / 256, 16384);
+  hipLaunchKernelGGL(k_mwfj_bpad, dim3(nb), dim3(256), 0, st, d.B, d.nf, nh, d.half, d.nr,
+                     d.nrhs, ncols, nrowpad, ntotal);
   return hipGetLastError();
 }
 
