@@ -206,10 +206,18 @@ __global__ void k_mwfj_twiddles(int n, double2* __restrict__ tw) {
 __global__ __launch_bounds__(256) void k_mwfj_rowbad(const double* __restrict__ u, int npts,
                                                      int* __restrict__ rowbad) {
   const int s = blockIdx.x;
-  int any = 0;
-  for (int t = threadIdx.x; t < npts; t += 256) {
-    const double v = u[(size_t)s * npts + t];
-    any |= !(v - v == 0.0);   // NaN or inf
+  const double* row = u + (size_t)s * npts;
+  int any = 0;   // v - v != 0: NaN or inf
+  if ((npts & 1) == 0 && ((uintptr_t)u & 15) == 0) {   // uniform: 16-byte loads
+    const double2* r2 = reinterpret_cast<const double2*>(row);
+#pragma unroll 8
+    for (int t = threadIdx.x; t < npts / 2; t += 256) {
+      const double2 v = r2[t];
+      any |= !(v.x - v.x == 0.0) | !(v.y - v.y == 0.0);
+    }
+  } else {
+#pragma unroll 8
+    for (int t = threadIdx.x; t < npts; t += 256) any |= !(row[t] - row[t] == 0.0);
   }
   any = __syncthreads_or(any);
   if (threadIdx.x == 0) rowbad[s] = any;
