@@ -584,19 +584,41 @@ __global__ void k_cc_interleave(const float* __restrict__ re, const float* __res
 // ncfsr/ncfsi[p][j] = acc[p][j] / count[p] where count > 0, else acc
 // (the untouched running sums, as the reference leaves them)
 // With im == nullptr, re holds interleaved complex [npairs][nf][2] instead.
-__global__ void k_cc_finish(const float2* __restrict__ acc, const float* __restrict__ count,
-                            int nf, float* __restrict__ re, float* __restrict__ im) {
-  const int p = blockIdx.x;
-  const int j = blockIdx.y * blockDim.x + threadIdx.x;
-  if (j >= nf) return;
-  const float n = count[p];
-  const float2 v = acc[(size_t)p * nf + j];
-  const float a = n > 0.0f ? v.x / n : v.x, b = n > 0.0f ? v.y / n : v.y;
-  if (im) {
-    re[(size_t)p * nf + j] = a;
-    im[(size_t)p * nf + j] = b;
+// re/im[k] (or interleaved re[2k], re[2k+1]) = acc[k] / count[k / nf]
+// (acc[k] itself when the count is 0), k over the flat [npairs][nf]; V = 4
+// consecutive values a thread with 16-byte accesses (n a multiple of 4,
+// the outputs 16-byte aligned), else one.  A block per (pair, 256 bins)
+// made 80k blocks for 19,900 pairs x 1000 bins (0.071 ms; flat, 0.064).
+template <int V>
+__global__ __launch_bounds__(256) void k_cc_finish(const float2* __restrict__ acc,
+                                                   const float* __restrict__ count, int nf,
+                                                   size_t n, float* __restrict__ re,
+                                                   float* __restrict__ im) {
+  const size_t k0 = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) * V;
+  if (k0 >= n) return;
+  float a[V], b[V];
+#pragma unroll
+  for (int u = 0; u < V; u++) {
+    const size_t k = k0 + u;
+    const float c = count[k / (size_t)nf];
+    const float2 v = acc[k];
+    a[u] = c > 0.0f ? v.x / c : v.x;
+    b[u] = c > 0.0f ? v.y / c : v.y;
+  }
+  if (V == 4) {
+    if (im) {
+      *reinterpret_cast<float4*>(re + k0) = make_float4(a[0], a[1], a[2], a[3]);
+      *reinterpret_cast<float4*>(im + k0) = make_float4(b[0], b[1], b[2], b[3]);
+    } else {
+      float4* o = reinterpret_cast<float4*>(re + 2 * k0);
+      o[0] = make_float4(a[0], b[0], a[1], b[1]);
+      o[1] = make_float4(a[2], b[2], a[3], b[3]);
+    }
+  } else if (im) {
+    re[k0] = a[0];
+    im[k0] = b[0];
   } else {
-    reinterpret_cast<float2*>(re)[(size_t)p * nf + j] = make_float2(a, b);
+    reinterpret_cast<float2*>(re)[k0] = make_float2(a[0], b[0]);
   }
 }
 
@@ -717,8 +739,19 @@ hipError_t launch_cc_interleave(const float* re, const float* im, size_t n, void
 
 hipError_t launch_cc_finish(const void* acc, const float* count, int npairs, int nf, float* re,
                             float* im, hipStream_t st) {
-  hipLaunchKernelGGL(k_cc_finish, dim3(npairs, (nf + 255) / 256), dim3(256), 0, st,
-                     (const float2*)acc, count, nf, re, im);
+  const size_t n = (size_t)npairs * nf;
+  if (n == 0) return hipSuccess;
+  const bool v4 = n % 4 == 0 && ((uintptr_t)re & 15) == 0 && ((uintptr_t)im & 15) == 0 &&
+                  ((uintptr_t)acc & 15) == 0;
+  const size_t threads = v4 ? n / 4 : n;
+  const size_t nb = (threads + 255) / 256;
+  if (nb > 0x7fffffffULL) return hipErrorInvalidValue;
+  if (v4)
+    hipLaunchKernelGGL(k_cc_finish<4>, dim3((unsigned)nb), dim3(256), 0, st, (const float2*)acc,
+                       count, nf, n, re, im);
+  else
+    hipLaunchKernelGGL(k_cc_finish<1>, dim3((unsigned)nb), dim3(256), 0, st, (const float2*)acc,
+                       count, nf, n, re, im);
   return hipGetLastError();
 }
 
